@@ -1,0 +1,108 @@
+"""Drop-in mirror of the ``Burgers`` module of ExactClosure.jl (src/Burgers.jl)
+for the batched DNS-aided LES path.
+
+``BurgersBatch`` replaces the serial loop of run_dns_aided_les
+(src/Burgers.jl:267-328): many instances of uneven size and compression run
+in one persistent launch (optionally replayed from a captured hipGraph).
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+
+from . import _abi
+
+
+def spacing(L, n):
+    return L / n
+
+
+def tophat_weights(comp):
+    """tophat_weights (src/Burgers.jl:158-162) for odd comp; even comp uses the
+    trapezoid over comp+1 faces (the reference would index past its array)."""
+    if comp % 2:
+        return comp // 2, np.full(comp, 1.0 / comp)
+    w = np.full(comp + 1, 1.0 / comp)
+    w[0] = w[-1] = 0.5 / comp
+    return comp // 2, w
+
+
+def gaussian_weights(h, delta, nsigma=3):
+    """gaussian_weights (src/Burgers.jl:146-156)."""
+    s2 = (delta / h) ** 2 / 12
+    R = int(np.rint(nsigma * math.sqrt(s2)))
+    if delta < 1e-15:
+        return R, np.ones(1)
+    w = np.exp(-np.arange(-R, R + 1) ** 2 / (2 * s2))
+    return R, w / w.sum()
+
+
+def build_doublekernel(F, G):
+    """build_doublekernel (src/Burgers.jl:217-233)."""
+    I, f = F
+    J, g = G
+    return I + J, np.convolve(f, g)
+
+
+class BurgersBatch:
+    """count instances: n[i] points, compression c[i], RK4 with the exact
+    closure and a no-model LES, FP32 or FP64 state, FP64 commutator."""
+
+    def __init__(self, n, c, visc=5e-4, L=2 * math.pi, cfl=0.3, kmax=10, seed=1, dtype="f32"):
+        self.ctx = _abi.context(0)
+        self.n = np.asarray(n, dtype=np.int32)
+        self.c = np.asarray(c, dtype=np.int32)
+        self.count = len(self.n)
+        self.L = L
+        self.visc = visc
+        code = _abi.EC_F64 if dtype in ("f64", "float64") else _abi.EC_F32
+        h = _abi.c_void_p()
+        nn = (ctypes.c_int * self.count)(*self.n.tolist())
+        cc = (ctypes.c_int * self.count)(*self.c.tolist())
+        self.ctx.call("ec_burgers_batch_create", code, self.count, nn, cc, L, visc, cfl, kmax, seed,
+                      ctypes.byref(h))
+        self.h = h
+
+    def set_threads(self, threads):
+        self.ctx.call("ec_burgers_batch_set_threads", self.h, threads)
+
+    def run(self, nsteps, use_graph=True):
+        self.ctx.call("ec_burgers_batch_run", self.h, nsteps, int(bool(use_graph)))
+
+    def get(self, i, which="dns"):
+        w = {"dns": 0, "swapfil": 1, "nomodel": 2}[which]
+        m = int(self.n[i]) if w == 0 else int(self.n[i] // self.c[i])
+        out = np.empty(m, dtype=np.float64)
+        self.ctx.call("ec_burgers_batch_get", self.h, i, w, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), m)
+        return out
+
+    def dt(self, i):
+        d = ctypes.c_double()
+        rc = self.ctx.lib.ec_burgers_batch_dt(self.h, i, ctypes.byref(d))
+        self.ctx.check(rc, "ec_burgers_batch_dt")
+        return d.value
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.ec_burgers_batch_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def baseline_batch(count=4096, seed=0):
+    """The batched-Burgers workload: sizes {2048, 4096, 8192, 16384} split
+    40/30/20/10 %, compressions drawn from {4, 8, 16, 32, 64}."""
+    rng = np.random.default_rng(seed)
+    counts = [int(round(count * f)) for f in (0.4, 0.3, 0.2)]
+    counts.append(count - sum(counts))
+    n = np.concatenate([np.full(k, s) for k, s in zip(counts, (2048, 4096, 8192, 16384))])
+    c = rng.choice([4, 8, 16, 32, 64], size=count)
+    perm = rng.permutation(count)
+    return n[perm].astype(np.int32), c[perm].astype(np.int32)

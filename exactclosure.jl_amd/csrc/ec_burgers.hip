@@ -1,0 +1,474 @@
+// Batched 1D viscous Burgers DNS-aided LES (src/Burgers.jl:89-142, 350-445).
+//
+// The reference advances one instance at a time with ~12 tiny launches per
+// step (src/Burgers.jl:307-324, 395-439), so it is launch-latency bound.
+// Here one persistent launch advances every instance of a heterogeneous
+// batch: workgroups pull instances from an atomic work queue ordered by
+// size (largest first), keep the stage state of the DNS and of the two LES
+// variants in LDS, keep the RK4 base state and accumulator in registers,
+// and evaluate the exact-closure commutator tau = Pc(s) - s_H(V u) in FP64 at
+// every stage.  The launch can be captured once into a hipGraph and replayed.
+#include "ec_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+
+namespace {
+
+struct BInst {
+    int n, c, off, offl;
+};
+
+__device__ __forceinline__ uint64_t sm64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ULL;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    return x ^ (x >> 31);
+}
+
+// Initial condition: sum_{k=1}^{kmax} (2/k) cos(2 pi (k i / n + phase_k)), the
+// explicit form of irfft of modes with amplitude 1/k and uniform phase
+// (randomfield, src/Burgers.jl:195-202).
+template <typename T>
+__global__ void k_burgers_ic(const BInst *__restrict__ inst, int count, int kmax, uint64_t seed0, T *__restrict__ U) {
+    const int b = blockIdx.y;
+    if (b >= count) return;
+    const BInst I = inst[b];
+    const uint64_t seed = seed0 + (uint64_t)b;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < I.n; i += gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int k = 1; k <= kmax; k++) {
+            const double ph = (double)(sm64(seed ^ sm64((uint64_t)k)) >> 11) * 0x1.0p-53;
+            const long m = ((long)k * i) % I.n;
+            s += 2.0 / k * cos(2.0 * M_PI * ((double)m / I.n + ph));
+        }
+        U[I.off + i] = (T)s;
+    }
+}
+
+// Volume-average filter of the DNS face values at coarse face J (odd c:
+// top-hat of c faces; even c: trapezoid over c+1 faces).
+template <typename T>
+__device__ __forceinline__ double vavg(const T *u, int n, int c, int J) {
+    const int f = c * (J + 1) - 1;
+    const int h = c / 2;
+    double s = 0.0;
+    if (c & 1) {
+        for (int t = -h; t <= h; t++) {
+            int m = f + t;
+            m = m < 0 ? m + n : (m >= n ? m - n : m);
+            s += (double)u[m];
+        }
+        return s / c;
+    }
+    for (int t = -h + 1; t <= h - 1; t++) {
+        int m = f + t;
+        m = m < 0 ? m + n : (m >= n ? m - n : m);
+        s += (double)u[m];
+    }
+    int a = f - h, b = f + h;
+    a = a < 0 ? a + n : (a >= n ? a - n : a);
+    b = b < 0 ? b + n : (b >= n ? b - n : b);
+    return (s + 0.5 * ((double)u[a] + (double)u[b])) / c;
+}
+
+template <typename T>
+__global__ void k_burgers_les_init(const BInst *__restrict__ inst, int count, const T *__restrict__ U,
+                                   T *__restrict__ UL, T *__restrict__ UM) {
+    const int b = blockIdx.y;
+    if (b >= count) return;
+    const BInst I = inst[b];
+    const int nH = I.n / I.c;
+    for (int J = blockIdx.x * blockDim.x + threadIdx.x; J < nH; J += gridDim.x * blockDim.x) {
+        const T v = (T)vavg(U + I.off, I.n, I.c, J);
+        UL[I.offl + J] = v;
+        UM[I.offl + J] = v;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T bstress(T a, T b, T visc, T ih) {  // stress (src/Burgers.jl:89-94)
+    return -((a + b) * (a + b)) / 8 + visc * (b - a) * ih;
+}
+
+__global__ void k_reset_counter(int *counter) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] = 0;
+}
+
+template <typename T, int PT>
+__global__ void __launch_bounds__(1024) k_burgers_persistent(const BInst *__restrict__ inst, const int *__restrict__ order,
+                                                            int count, int *__restrict__ counter, T *__restrict__ U,
+                                                            T *__restrict__ UL, T *__restrict__ UM,
+                                                            double *__restrict__ dts, double L, double visc,
+                                                            double cfl, int nsteps) {
+    extern __shared__ char smem_raw[];
+    __shared__ int s_idx;
+    __shared__ double s_red[16];
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    constexpr int PTL = PT > 1 ? PT / 2 : 1;
+    for (;;) {
+        if (tid == 0) s_idx = atomicAdd(counter, 1);
+        __syncthreads();
+        const int w = s_idx;
+        __syncthreads();
+        if (w >= count) break;
+        const int id = order[w];
+        const BInst I = inst[id];
+        const int n = I.n, c = I.c, nH = n / c;
+        T *us = reinterpret_cast<T *>(smem_raw);
+        T *ul = us + n;
+        T *um = ul + nH;
+        T *tau = um + nH;
+        const double h = L / n, H = L / nH;
+        const T ih = (T)(1.0 / h), iH = (T)(1.0 / H), tv = (T)visc;
+
+        T u0[PT], acc[PT];
+        T l0[PTL], lacc[PTL], m0[PTL], macc[PTL];
+        double umax = 0.0;
+#pragma unroll
+        for (int q = 0; q < PT; q++) {
+            const int i = tid + q * nthr;
+            if (i < n) {
+                u0[q] = U[I.off + i];
+                us[i] = u0[q];
+                umax = fmax(umax, fabs((double)u0[q]));
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < PTL; q++) {
+            const int J = tid + q * nthr;
+            if (J < nH) {
+                l0[q] = UL[I.offl + J];
+                m0[q] = UM[I.offl + J];
+                ul[J] = l0[q];
+                um[J] = m0[q];
+            }
+        }
+        // time step from the initial state: cfl * min(h / max|u|, h^2 / visc)
+        // (cfl, src/Burgers.jl:97-101), fixed for the instance
+        for (int o = 32; o > 0; o >>= 1) umax = fmax(umax, __shfl_xor(umax, o, 64));
+        if ((tid & 63) == 0) s_red[tid >> 6] = umax;
+        __syncthreads();
+        double dt = dts[id];
+        if (dt <= 0.0) {
+            double m = 0.0;
+            for (int k = 0; k < (nthr >> 6); k++) m = fmax(m, s_red[k]);
+            dt = cfl * fmin(h / m, h * h / visc);
+        }
+        __syncthreads();
+        if (tid == 0) dts[id] = dt;
+        const T tdt = (T)dt;
+
+        for (int step = 0; step < nsteps; step++) {
+#pragma unroll
+            for (int st = 0; st < 4; st++) {
+                // --- closure commutator in FP64: tau = Pc(s) - s_H(V u)
+                for (int J = tid; J < nH; J += nthr) {
+                    const double vb = vavg(us, n, c, J);
+                    const double va = vavg(us, n, c, J == 0 ? nH - 1 : J - 1);
+                    const int f = c * (J + 1) - 1;
+                    const int m0i = f - c / 2;
+                    auto sfine = [&](int m) {
+                        const int mm = m < 0 ? m + n : (m >= n ? m - n : m);
+                        const int ml = mm == 0 ? n - 1 : mm - 1;
+                        const double a = (double)us[ml], b = (double)us[mm];
+                        return -((a + b) * (a + b)) / 8 + visc * (b - a) / h;
+                    };
+                    const double fsu = (c & 1) ? sfine(m0i) : 0.5 * (sfine(m0i) + sfine(m0i + 1));
+                    const double svu = -((va + vb) * (va + vb)) / 8 + visc * (vb - va) / H;
+                    tau[J] = (T)(fsu - svu);
+                }
+                // --- DNS right-hand side (rhs!, src/Burgers.jl:103-109)
+                T kd[PT];
+#pragma unroll
+                for (int q = 0; q < PT; q++) {
+                    const int i = tid + q * nthr;
+                    if (i < n) {
+                        const T a = us[i == 0 ? n - 1 : i - 1], b = us[i], cc = us[i + 1 == n ? 0 : i + 1];
+                        kd[q] = (bstress(b, cc, tv, ih) - bstress(a, b, tv, ih)) * ih;
+                    }
+                }
+                __syncthreads();
+                // --- LES right-hand sides: exact closure and no model
+                T kl[PTL], km[PTL];
+#pragma unroll
+                for (int q = 0; q < PTL; q++) {
+                    const int J = tid + q * nthr;
+                    if (J < nH) {
+                        const int Jm = J == 0 ? nH - 1 : J - 1, Jp = J + 1 == nH ? 0 : J + 1;
+                        const T s0 = bstress(ul[Jm], ul[J], tv, iH) + tau[J];
+                        const T s1 = bstress(ul[J], ul[Jp], tv, iH) + tau[Jp];
+                        kl[q] = (s1 - s0) * iH;
+                        km[q] = (bstress(um[J], um[Jp], tv, iH) - bstress(um[Jm], um[J], tv, iH)) * iH;
+                    }
+                }
+                __syncthreads();
+                // --- RK4 bookkeeping (rk4_timestep!, src/Burgers.jl:120-142)
+                const T bw = (st == 0 || st == 3) ? T(1) : T(2);
+                const T a_next = st == 2 ? tdt : tdt / 2;
+#pragma unroll
+                for (int q = 0; q < PT; q++) {
+                    const int i = tid + q * nthr;
+                    if (i < n) {
+                        acc[q] = st == 0 ? kd[q] : acc[q] + bw * kd[q];
+                        if (st < 3) {
+                            us[i] = u0[q] + a_next * kd[q];
+                        } else {
+                            u0[q] = u0[q] + tdt / 6 * acc[q];
+                            us[i] = u0[q];
+                        }
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < PTL; q++) {
+                    const int J = tid + q * nthr;
+                    if (J < nH) {
+                        lacc[q] = st == 0 ? kl[q] : lacc[q] + bw * kl[q];
+                        macc[q] = st == 0 ? km[q] : macc[q] + bw * km[q];
+                        if (st < 3) {
+                            ul[J] = l0[q] + a_next * kl[q];
+                            um[J] = m0[q] + a_next * km[q];
+                        } else {
+                            l0[q] = l0[q] + tdt / 6 * lacc[q];
+                            m0[q] = m0[q] + tdt / 6 * macc[q];
+                            ul[J] = l0[q];
+                            um[J] = m0[q];
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < PT; q++) {
+            const int i = tid + q * nthr;
+            if (i < n) U[I.off + i] = u0[q];
+        }
+#pragma unroll
+        for (int q = 0; q < PTL; q++) {
+            const int J = tid + q * nthr;
+            if (J < nH) {
+                UL[I.offl + J] = l0[q];
+                UM[I.offl + J] = m0[q];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T>
+__global__ void k_burgers_rhs(int n, T ih, T visc, const T *__restrict__ u, T *__restrict__ du) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const T a = u[i == 0 ? n - 1 : i - 1], b = u[i], c = u[i + 1 == n ? 0 : i + 1];
+    du[i] = (bstress(b, c, visc, ih) - bstress(a, b, visc, ih)) * ih;
+}
+
+}  // namespace
+
+struct ec_burgers_batch {
+    int dtype = EC_F32;
+    int count = 0;
+    double L = 0, visc = 0, cfl = 0;
+    int threads = 512;
+    int nmax = 0, nHmax = 0;
+    std::vector<BInst> host_inst;
+    BInst *inst = nullptr;
+    int *order = nullptr;
+    int *counter = nullptr;
+    void *U = nullptr, *UL = nullptr, *UM = nullptr;
+    double *dts = nullptr;
+    std::map<std::pair<int, int>, hipGraphExec_t> graphs;  // (nsteps, threads) -> graph
+};
+
+static int pt_for(int nmax, int threads) {
+    int pt = 1;
+    while (pt * threads < nmax) pt *= 2;
+    return pt;
+}
+
+template <typename T>
+static size_t batch_lds(const ec_burgers_batch *b) {
+    return sizeof(T) * ((size_t)b->nmax + 3 * (size_t)b->nHmax);
+}
+
+template <typename T, int PT>
+static int launch_pt(ec_ctx *ctx, ec_burgers_batch *b, int nsteps, int blocks, size_t lds) {
+    EC_HIP(ctx, hipFuncSetAttribute((const void *)k_burgers_persistent<T, PT>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    k_burgers_persistent<T, PT><<<blocks, b->threads, lds, ctx->stream>>>(
+        b->inst, b->order, b->count, b->counter, (T *)b->U, (T *)b->UL, (T *)b->UM, b->dts, b->L, b->visc, b->cfl,
+        nsteps);
+    EC_CHECK_LAUNCH(ctx, "k_burgers_persistent");
+    return EC_OK;
+}
+
+template <typename T>
+static int launch_run(ec_ctx *ctx, ec_burgers_batch *b, int nsteps) {
+    const size_t lds = batch_lds<T>(b);
+    int dev_cus = 0;
+    EC_HIP(ctx, hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    const int per_cu = std::max(1, (int)((160 * 1024) / std::max<size_t>(lds + 1024, 1)));
+    const int blocks = std::min(b->count, dev_cus * std::min(per_cu, std::max(1, 2048 / b->threads)));
+    k_reset_counter<<<1, 64, 0, ctx->stream>>>(b->counter);
+    EC_CHECK_LAUNCH(ctx, "k_reset_counter");
+    switch (pt_for(b->nmax, b->threads)) {
+        case 1: return launch_pt<T, 1>(ctx, b, nsteps, blocks, lds);
+        case 2: return launch_pt<T, 2>(ctx, b, nsteps, blocks, lds);
+        case 4: return launch_pt<T, 4>(ctx, b, nsteps, blocks, lds);
+        case 8: return launch_pt<T, 8>(ctx, b, nsteps, blocks, lds);
+        case 16: return launch_pt<T, 16>(ctx, b, nsteps, blocks, lds);
+        case 32: return launch_pt<T, 32>(ctx, b, nsteps, blocks, lds);
+        case 64: return launch_pt<T, 64>(ctx, b, nsteps, blocks, lds);
+        default: return ec_fail(ctx, EC_E_UNSUPPORTED, "instance size %d too large for %d threads", b->nmax, b->threads);
+    }
+}
+
+extern "C" int ec_burgers_batch_create(ec_ctx *ctx, int dtype, int count, const int *n, const int *c, double L,
+                                       double visc, double cfl, int kmax, uint64_t seed, ec_burgers_batch **out) {
+    EC_REQUIRE(ctx, ctx && out && n && c && count > 0, "ec_burgers_batch_create: bad argument");
+    *out = nullptr;
+    EC_REQUIRE(ctx, dtype == EC_F32 || dtype == EC_F64, "bad dtype");
+    ec_burgers_batch *b = new ec_burgers_batch();
+    b->dtype = dtype;
+    b->count = count;
+    b->L = L;
+    b->visc = visc;
+    b->cfl = cfl;
+    size_t tot = 0, totl = 0;
+    for (int i = 0; i < count; i++) {
+        if (n[i] < 4 || c[i] < 2 || n[i] % c[i] != 0 || n[i] / c[i] < 2) {
+            delete b;
+            return ec_fail(ctx, EC_E_INVALID, "instance %d: n=%d, c=%d (need c >= 2 dividing n, n/c >= 2)", i, n[i], c[i]);
+        }
+        b->host_inst.push_back(BInst{n[i], c[i], (int)tot, (int)totl});
+        tot += n[i];
+        totl += n[i] / c[i];
+        b->nmax = std::max(b->nmax, n[i]);
+        b->nHmax = std::max(b->nHmax, n[i] / c[i]);
+    }
+    const size_t bt = dtype == EC_F64 ? 8 : 4;
+    const size_t lds = dtype == EC_F64 ? batch_lds<double>(b) : batch_lds<float>(b);
+    if (lds > 160 * 1024 - 256) {
+        delete b;
+        return ec_fail(ctx, EC_E_UNSUPPORTED, "batch needs %zu B of LDS per instance (limit 160 KiB)", lds);
+    }
+    // largest instances first (longest-processing-time order for the queue)
+    std::vector<int> order(count);
+    for (int i = 0; i < count; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int bb) { return n[a] > n[bb]; });
+    hipError_t e = hipMalloc(&b->inst, sizeof(BInst) * count);
+    if (e == hipSuccess) e = hipMalloc(&b->order, sizeof(int) * count);
+    if (e == hipSuccess) e = hipMalloc(&b->counter, sizeof(int) * 64);
+    if (e == hipSuccess) e = hipMalloc(&b->U, bt * tot);
+    if (e == hipSuccess) e = hipMalloc(&b->UL, bt * totl);
+    if (e == hipSuccess) e = hipMalloc(&b->UM, bt * totl);
+    if (e == hipSuccess) e = hipMalloc(&b->dts, sizeof(double) * count);
+    if (e == hipSuccess) e = hipMemcpy(b->inst, b->host_inst.data(), sizeof(BInst) * count, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->order, order.data(), sizeof(int) * count, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(b->dts, 0, sizeof(double) * count);
+    if (e != hipSuccess) {
+        ec_burgers_batch_destroy(ctx, b);
+        return ec_fail(ctx, EC_E_OOM, "ec_burgers_batch_create: %s", hipGetErrorString(e));
+    }
+    dim3 grid(std::max(1, (b->nmax + 255) / 256), count);
+    if (dtype == EC_F64) {
+        k_burgers_ic<double><<<grid, 256, 0, ctx->stream>>>(b->inst, count, kmax, seed, (double *)b->U);
+        k_burgers_les_init<double><<<grid, 256, 0, ctx->stream>>>(b->inst, count, (const double *)b->U, (double *)b->UL,
+                                                                  (double *)b->UM);
+    } else {
+        k_burgers_ic<float><<<grid, 256, 0, ctx->stream>>>(b->inst, count, kmax, seed, (float *)b->U);
+        k_burgers_les_init<float><<<grid, 256, 0, ctx->stream>>>(b->inst, count, (const float *)b->U, (float *)b->UL,
+                                                                 (float *)b->UM);
+    }
+    EC_CHECK_LAUNCH(ctx, "k_burgers_ic");
+    *out = b;
+    return EC_OK;
+}
+
+extern "C" int ec_burgers_batch_destroy(ec_ctx *ctx, ec_burgers_batch *b) {
+    if (!b) return EC_OK;
+    if (ctx) hipStreamSynchronize(ctx->stream);
+    for (auto &kv : b->graphs) hipGraphExecDestroy(kv.second);
+    hipFree(b->inst);
+    hipFree(b->order);
+    hipFree(b->counter);
+    hipFree(b->U);
+    hipFree(b->UL);
+    hipFree(b->UM);
+    hipFree(b->dts);
+    delete b;
+    return EC_OK;
+}
+
+extern "C" int ec_burgers_batch_set_threads(ec_ctx *ctx, ec_burgers_batch *b, int threads) {
+    EC_REQUIRE(ctx, ctx && b, "null argument");
+    EC_REQUIRE(ctx, threads == 128 || threads == 256 || threads == 512 || threads == 1024, "threads=%d", threads);
+    EC_REQUIRE(ctx, pt_for(b->nmax, threads) <= 64, "threads=%d too few for n=%d", threads, b->nmax);
+    b->threads = threads;
+    return EC_OK;
+}
+
+extern "C" int ec_burgers_batch_run(ec_ctx *ctx, ec_burgers_batch *b, int nsteps, int use_graph) {
+    EC_REQUIRE(ctx, ctx && b, "null argument");
+    EC_REQUIRE(ctx, nsteps >= 0, "nsteps=%d", nsteps);
+    if (nsteps == 0) return EC_OK;
+    auto run = [&]() { return b->dtype == EC_F64 ? launch_run<double>(ctx, b, nsteps) : launch_run<float>(ctx, b, nsteps); };
+    if (!use_graph) return run();
+    auto key = std::make_pair(nsteps, b->threads);
+    auto it = b->graphs.find(key);
+    if (it == b->graphs.end()) {
+        // attribute setup outside the capture
+        int rc = EC_OK;
+        hipGraph_t g;
+        EC_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        rc = run();
+        hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        if (rc) return rc;
+        if (e != hipSuccess) return ec_fail(ctx, EC_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
+        hipGraphExec_t ge;
+        e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) return ec_fail(ctx, EC_E_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+        it = b->graphs.emplace(key, ge).first;
+    }
+    EC_HIP(ctx, hipGraphLaunch(it->second, ctx->stream));
+    return EC_OK;
+}
+
+extern "C" int ec_burgers_batch_get(ec_ctx *ctx, ec_burgers_batch *b, int instance, int which, double *out, int cap) {
+    EC_REQUIRE(ctx, ctx && b && out, "null argument");
+    EC_REQUIRE(ctx, instance >= 0 && instance < b->count, "instance %d out of range", instance);
+    EC_REQUIRE(ctx, which >= 0 && which <= 2, "which=%d", which);
+    const BInst &I = b->host_inst[instance];
+    const int len = which == 0 ? I.n : I.n / I.c;
+    EC_REQUIRE(ctx, cap >= len, "output capacity %d < %d", cap, len);
+    const size_t bt = b->dtype == EC_F64 ? 8 : 4;
+    const char *base = (const char *)(which == 0 ? b->U : (which == 1 ? b->UL : b->UM));
+    const size_t off = which == 0 ? I.off : I.offl;
+    std::vector<char> tmp(bt * len);
+    EC_HIP(ctx, hipMemcpyAsync(tmp.data(), base + bt * off, bt * len, hipMemcpyDeviceToHost, ctx->stream));
+    EC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < len; i++)
+        out[i] = b->dtype == EC_F64 ? ((const double *)tmp.data())[i] : (double)((const float *)tmp.data())[i];
+    return EC_OK;
+}
+
+extern "C" int ec_burgers_batch_dt(ec_burgers_batch *b, int instance, double *dt) {
+    if (!b || !dt || instance < 0 || instance >= b->count) return EC_E_INVALID;
+    if (hipMemcpy(dt, b->dts + instance, sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) return EC_E_HIP;
+    return EC_OK;
+}
+
+extern "C" int ec_burgers_rhs(ec_ctx *ctx, int dtype, int n, double L, double visc, const void *u, void *du) {
+    EC_REQUIRE(ctx, ctx && u && du && n >= 2, "ec_burgers_rhs: bad argument");
+    const double ih = n / L;
+    if (dtype == EC_F64)
+        k_burgers_rhs<double><<<(n + 255) / 256, 256, 0, ctx->stream>>>(n, ih, visc, (const double *)u, (double *)du);
+    else
+        k_burgers_rhs<float><<<(n + 255) / 256, 256, 0, ctx->stream>>>(n, (float)ih, (float)visc, (const float *)u,
+                                                                       (float *)du);
+    EC_CHECK_LAUNCH(ctx, "k_burgers_rhs");
+    return EC_OK;
+}

@@ -1,0 +1,643 @@
+// Hand-written 3D real FFT and the FFT Poisson projection (poissonsolver /
+// project!, src/N.jl:202-307) for gfx950.
+//
+// Layout follows FFTW's rfft of a column-major array: the half spectrum is
+// (nh = n/2+1, n, n) complex with kx fastest.  Five passes per projection:
+//   X  r2c  : two real x-lines packed as one complex line, LDS Stockham FFT,
+//             untangled into two half spectra (contiguous reads and writes)
+//   Y  fwd  : tiles of TK consecutive kx columns, whole y extent staged in LDS
+//   Z  fwd + Poisson scale + zero mode + Z inverse, one LDS residency
+//   Y  inv
+//   X  c2r  : Hermitian rebuild of two lines, one complex inverse, split
+// Twiddles and sin^2 eigenvalue tables are computed on the host in FP64.
+#include "ec_internal.h"
+
+#include <cmath>
+
+namespace {
+
+struct RadixPlan {
+    int nrad;
+    int r[16];
+};
+
+constexpr int FFT_E = 16;  // maximum complex elements per thread per line
+
+// ---- radix butterflies: in-place DFT of size R with kernel exp(sign*2*pi*i/R) ----
+template <typename T>
+__device__ __forceinline__ cplx<T> mul_i(cplx<T> z, int sign) {  // (sign*i) * z
+    return sign > 0 ? cplx<T>{-z.y, z.x} : cplx<T>{z.y, -z.x};
+}
+
+template <typename T, int R>
+struct Bfly;
+
+template <typename T>
+struct Bfly<T, 2> {
+    __device__ static void run(cplx<T> *v, int) {
+        cplx<T> a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    }
+};
+
+template <typename T>
+struct Bfly<T, 4> {
+    __device__ static void run(cplx<T> *v, int sign) {
+        cplx<T> a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+        cplx<T> b0 = cadd(v[1], v[3]), b1 = mul_i(csub(v[1], v[3]), sign);
+        v[0] = cadd(a0, b0);
+        v[2] = csub(a0, b0);
+        v[1] = cadd(a1, b1);
+        v[3] = csub(a1, b1);
+    }
+};
+
+template <typename T>
+struct Bfly<T, 8> {
+    __device__ static void run(cplx<T> *v, int sign) {
+        cplx<T> e[4] = {v[0], v[2], v[4], v[6]};
+        cplx<T> o[4] = {v[1], v[3], v[5], v[7]};
+        Bfly<T, 4>::run(e, sign);
+        Bfly<T, 4>::run(o, sign);
+        const T c = (T)0.70710678118654752440;
+        // o[k] *= exp(sign*2*pi*i*k/8)
+        o[1] = cplx<T>{c * (o[1].x - sign * o[1].y), c * (o[1].y + sign * o[1].x)};
+        o[2] = mul_i(o[2], sign);
+        o[3] = cplx<T>{-c * (o[3].x + sign * o[3].y), c * (sign * o[3].x - o[3].y)};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            v[k] = cadd(e[k], o[k]);
+            v[k + 4] = csub(e[k], o[k]);
+        }
+    }
+};
+
+template <typename T, int R>
+struct BflyGeneric {
+    __device__ static void run(cplx<T> *v, int sign) {
+        // cos / sin of 2*pi*m/R for m < R
+        double cs[R], sn[R];
+#pragma unroll
+        for (int m = 0; m < R; m++) {
+            cs[m] = R == 3 ? (m == 0 ? 1.0 : -0.5)
+                           : (R == 5 ? (m == 0 ? 1.0 : (m == 1 || m == 4) ? 0.30901699437494742410 : -0.80901699437494742410)
+                                     : 0.0);
+            sn[m] = R == 3 ? (m == 0 ? 0.0 : (m == 1 ? 0.86602540378443864676 : -0.86602540378443864676))
+                           : (R == 5 ? (m == 0 ? 0.0
+                                               : m == 1 ? 0.95105651629515357212
+                                               : m == 2 ? 0.58778525229247312917
+                                               : m == 3 ? -0.58778525229247312917
+                                                        : -0.95105651629515357212)
+                                     : 0.0);
+        }
+        cplx<T> out[R];
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+            cplx<T> acc = v[0];
+#pragma unroll
+            for (int r = 1; r < R; r++) {
+                const int m = (r * q) % R;
+                const cplx<T> w{(T)cs[m], (T)(sign * sn[m])};
+                acc = cadd(acc, cmul(v[r], w));
+            }
+            out[q] = acc;
+        }
+#pragma unroll
+        for (int q = 0; q < R; q++) v[q] = out[q];
+    }
+};
+template <typename T>
+struct Bfly<T, 3> : BflyGeneric<T, 3> {};
+template <typename T>
+struct Bfly<T, 5> : BflyGeneric<T, 5> {};
+
+// One Stockham stage over a line held in LDS at lds[e * ls], e in [0, n).
+template <typename T, int R, int MAXB>
+__device__ __forceinline__ void fft_stage(cplx<T> *lds, int ls, int n, int t, int nthr, int Ns, int sign,
+                                          const cplx<T> *__restrict__ tw) {
+    const int nR = n / R;
+    const int tstep = n / (Ns * R);
+    cplx<T> v[MAXB][R];
+#pragma unroll
+    for (int q = 0; q < MAXB; q++) {
+        const int b = t + q * nthr;
+        if (b < nR) {
+            const int jm = b % Ns;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                cplx<T> x = lds[(b + r * nR) * ls];
+                if (r > 0 && jm > 0) {
+                    cplx<T> w = tw[r * jm * tstep];
+                    if (sign > 0) w.y = -w.y;
+                    x = cmul(x, w);
+                }
+                v[q][r] = x;
+            }
+            Bfly<T, R>::run(v[q], sign);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MAXB; q++) {
+        const int b = t + q * nthr;
+        if (b < nR) {
+            const int jm = b % Ns;
+            const int idxD = (b - jm) * R + jm;
+#pragma unroll
+            for (int r = 0; r < R; r++) lds[(idxD + r * Ns) * ls] = v[q][r];
+        }
+    }
+    __syncthreads();
+}
+
+// Full line FFT; every thread of the block must call it (it contains barriers).
+template <typename T>
+__device__ void lds_fft(cplx<T> *lds, int ls, int n, int t, int nthr, int sign, const cplx<T> *__restrict__ tw,
+                        const RadixPlan &rp) {
+    int Ns = 1;
+    for (int s = 0; s < rp.nrad; s++) {
+        const int R = rp.r[s];
+        switch (R) {
+            case 8: fft_stage<T, 8, (FFT_E + 7) / 8>(lds, ls, n, t, nthr, Ns, sign, tw); break;
+            case 4: fft_stage<T, 4, (FFT_E + 3) / 4>(lds, ls, n, t, nthr, Ns, sign, tw); break;
+            case 2: fft_stage<T, 2, (FFT_E + 1) / 2>(lds, ls, n, t, nthr, Ns, sign, tw); break;
+            case 3: fft_stage<T, 3, (FFT_E + 2) / 3>(lds, ls, n, t, nthr, Ns, sign, tw); break;
+            default: fft_stage<T, 5, (FFT_E + 4) / 5>(lds, ls, n, t, nthr, Ns, sign, tw); break;
+        }
+        Ns *= R;
+    }
+}
+
+// ---- X pass, forward: pairs of real lines -> half spectra ----
+template <typename T>
+__global__ void k_fft_x_r2c(const T *__restrict__ in, cplx<T> *__restrict__ out, int n, int nh, long nlines,
+                            RadixPlan rp, const cplx<T> *__restrict__ tw, int nthr) {
+    extern __shared__ char smem_raw[];
+    cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
+    const int pair = threadIdx.x / nthr, t = threadIdx.x % nthr;
+    const int P = blockDim.x / nthr;
+    const long L0 = 2 * ((long)blockIdx.x * P + pair);
+    const bool v0 = L0 < nlines, v1 = L0 + 1 < nlines;
+    cplx<T> *line = lds + (size_t)pair * n;
+    const T *a = in + L0 * n, *b = in + (L0 + 1) * n;
+    for (int e = t; e < n; e += nthr) line[e] = cplx<T>{v0 ? a[e] : T(0), v1 ? b[e] : T(0)};
+    __syncthreads();
+    lds_fft(line, 1, n, t, nthr, -1, tw, rp);
+    cplx<T> *oa = out + L0 * nh, *ob = out + (L0 + 1) * nh;
+    for (int k = t; k < nh; k += nthr) {
+        const cplx<T> z = line[k];
+        const cplx<T> zc = line[k == 0 ? 0 : n - k];
+        // A = (Z_k + conj Z_{n-k}) / 2, B = (Z_k - conj Z_{n-k}) / (2i)
+        const cplx<T> A{(z.x + zc.x) / 2, (z.y - zc.y) / 2};
+        const cplx<T> B{(z.y + zc.y) / 2, (zc.x - z.x) / 2};
+        if (v0) oa[k] = A;
+        if (v1) ob[k] = B;
+    }
+}
+
+// ---- X pass, inverse: two half spectra -> two real lines (c2r semantics) ----
+template <typename T>
+__global__ void k_fft_x_c2r(const cplx<T> *__restrict__ in, T *__restrict__ out, int n, int nh, long nlines,
+                            RadixPlan rp, const cplx<T> *__restrict__ tw, int nthr, T scale) {
+    extern __shared__ char smem_raw[];
+    cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
+    const int pair = threadIdx.x / nthr, t = threadIdx.x % nthr;
+    const int P = blockDim.x / nthr;
+    const long L0 = 2 * ((long)blockIdx.x * P + pair);
+    const bool v0 = L0 < nlines, v1 = L0 + 1 < nlines;
+    cplx<T> *line = lds + (size_t)pair * n;
+    const cplx<T> *ia = in + L0 * nh, *ib = in + (L0 + 1) * nh;
+    const int nyq = (n % 2 == 0) ? n / 2 : -1;
+    for (int e = t; e < n; e += nthr) {
+        cplx<T> A{T(0), T(0)}, B{T(0), T(0)};
+        if (e < nh) {
+            if (v0) A = ia[e];
+            if (v1) B = ib[e];
+            if (e == 0 || e == nyq) {  // self-conjugate bins: c2r ignores their imaginary part
+                A.y = T(0);
+                B.y = T(0);
+            }
+        } else {
+            if (v0) A = ia[n - e];
+            if (v1) B = ib[n - e];
+            A.y = -A.y;
+            B.y = -B.y;
+        }
+        line[e] = cplx<T>{A.x - B.y, A.y + B.x};
+    }
+    __syncthreads();
+    lds_fft(line, 1, n, t, nthr, +1, tw, rp);
+    T *oa = out + L0 * n, *ob = out + (L0 + 1) * n;
+    for (int e = t; e < n; e += nthr) {
+        const cplx<T> z = line[e];
+        if (v0) oa[e] = z.x * scale;
+        if (v1) ob[e] = z.y * scale;
+    }
+}
+
+// ---- Y / Z passes over tiles of TK consecutive kx columns ----
+// MODE 0: one transform with `sign`.  MODE 1 (Z pass only): forward, Poisson
+// scale, zero mode, inverse.
+template <typename T, int MODE>
+__global__ void k_fft_cols(cplx<T> *__restrict__ data, int n, int ncols, long elem_stride, long outer_stride,
+                           int TK, RadixPlan rp, const cplx<T> *__restrict__ tw, int sign,
+                           const T *__restrict__ eig, T poisson_a, T scale) {
+    extern __shared__ char smem_raw[];
+    cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
+    const int col0 = blockIdx.x * TK;
+    const int outer = blockIdx.y;
+    const int c = threadIdx.x % TK, t = threadIdx.x / TK;
+    const int nthr = blockDim.x / TK;
+    cplx<T> *base = data + col0 + (long)outer * outer_stride;
+    const int total = n * TK;
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+        const int cc = idx % TK, e = idx / TK;
+        lds[idx] = (col0 + cc < ncols) ? base[cc + e * elem_stride] : cplx<T>{T(0), T(0)};
+    }
+    __syncthreads();
+    if (MODE == 0) {
+        lds_fft(lds + c, TK, n, t, nthr, sign, tw, rp);
+    } else {
+        lds_fft(lds + c, TK, n, t, nthr, -1, tw, rp);
+        // Z pass: column = kx, outer = ky, element = kz.
+        const int kx = col0 + c;
+        const T sxy = (kx < ncols ? eig[kx] : T(0)) + eig[outer];
+        for (int e = t; e < n; e += nthr) {
+            const T denom = poisson_a * (sxy + eig[e]);
+            const bool zero_mode = (kx == 0 && outer == 0 && e == 0);
+            const T f = zero_mode ? T(0) : scale / denom;
+            cplx<T> z = lds[e * TK + c];
+            z.x *= f;
+            z.y *= f;
+            lds[e * TK + c] = z;
+        }
+        __syncthreads();
+        lds_fft(lds + c, TK, n, t, nthr, +1, tw, rp);
+    }
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+        const int cc = idx % TK, e = idx / TK;
+        if (col0 + cc < ncols) base[cc + e * elem_stride] = lds[idx];
+    }
+}
+
+// ---- energy-shell histogram and rescale for randomfield (src/N.jl:375-449) ----
+template <typename T>
+__global__ void k_shell_hist(const cplx<T> *__restrict__ spec, int n, int nh, int kmax, int nshell, double inv_n3,
+                             double *__restrict__ hist) {
+    extern __shared__ double sh[];
+    for (int s = threadIdx.x; s < nshell; s += blockDim.x) sh[s] = 0.0;
+    __syncthreads();
+    const size_t total = (size_t)nh * n * n;
+    for (size_t I = blockIdx.x * (size_t)blockDim.x + threadIdx.x; I < total; I += (size_t)gridDim.x * blockDim.x) {
+        const int kx = (int)(I % nh);
+        const int iy = (int)((I / nh) % n), iz = (int)(I / ((size_t)nh * n));
+        const long ky = iy <= (n - 1) / 2 ? iy : iy - n;  // wavenumber_int (src/N.jl:32)
+        const long kz = iz <= (n - 1) / 2 ? iz : iz - n;
+        const long kk = (long)kx * kx + ky * ky + kz * kz;
+        long s = (long)sqrt((double)kk);
+        while (s * s > kk) s--;
+        while ((s + 1) * (s + 1) <= kk) s++;
+        const double re = (double)spec[I].x * inv_n3, im = (double)spec[I].y * inv_n3;
+        const double w = (kx >= 1 && kx <= kmax - 1) ? 2.0 : 1.0;
+        if (s < nshell) atomicAdd(&sh[s], w * (re * re + im * im) / 2);
+    }
+    __syncthreads();
+    for (int s = threadIdx.x; s < nshell; s += blockDim.x)
+        if (sh[s] != 0.0) atomicAdd(&hist[s], sh[s]);
+}
+
+template <typename T>
+__global__ void k_shell_scale(cplx<T> *__restrict__ spec, int n, int nh, int nshell, const double *__restrict__ factor,
+                              double extra) {
+    const size_t total = (size_t)nh * n * n;
+    for (size_t I = blockIdx.x * (size_t)blockDim.x + threadIdx.x; I < total; I += (size_t)gridDim.x * blockDim.x) {
+        const int kx = (int)(I % nh);
+        const int iy = (int)((I / nh) % n), iz = (int)(I / ((size_t)nh * n));
+        const long ky = iy <= (n - 1) / 2 ? iy : iy - n;
+        const long kz = iz <= (n - 1) / 2 ? iz : iz - n;
+        const long kk = (long)kx * kx + ky * ky + kz * kz;
+        long s = (long)sqrt((double)kk);
+        while (s * s > kk) s--;
+        while ((s + 1) * (s + 1) <= kk) s++;
+        const double f = (s < nshell ? factor[s] : 0.0) * extra;
+        spec[I].x = (T)((double)spec[I].x * f);
+        spec[I].y = (T)((double)spec[I].y * f);
+    }
+}
+
+bool make_plan(int n, RadixPlan &rp) {
+    rp.nrad = 0;
+    int m = n;
+    while (m % 8 == 0) { rp.r[rp.nrad++] = 8; m /= 8; }
+    while (m % 4 == 0) { rp.r[rp.nrad++] = 4; m /= 4; }
+    while (m % 2 == 0) { rp.r[rp.nrad++] = 2; m /= 2; }
+    while (m % 3 == 0) { rp.r[rp.nrad++] = 3; m /= 3; }
+    while (m % 5 == 0) { rp.r[rp.nrad++] = 5; m /= 5; }
+    return m == 1 && rp.nrad <= 16;
+}
+
+struct XCfg {
+    int nthr, P;
+    size_t lds;
+};
+
+template <typename T>
+XCfg xcfg(int n) {
+    XCfg c;
+    c.nthr = (n + FFT_E - 1) / FFT_E;
+    if (c.nthr < 1) c.nthr = 1;
+    c.P = 256 / c.nthr;
+    if (c.P < 1) c.P = 1;
+    while (c.P > 1 && (size_t)c.P * n * sizeof(cplx<T>) > 65536) c.P /= 2;
+    c.lds = (size_t)c.P * n * sizeof(cplx<T>);
+    return c;
+}
+
+struct CCfg {
+    int TK, nthr;
+    size_t lds;
+};
+
+template <typename T>
+CCfg ccfg(int n) {
+    CCfg c;
+    c.TK = sizeof(T) == 4 ? 16 : 8;
+    while (c.TK > 1 && (size_t)n * c.TK * sizeof(cplx<T>) > 128 * 1024) c.TK /= 2;
+    c.nthr = (n + FFT_E - 1) / FFT_E;
+    if (c.nthr < 1) c.nthr = 1;
+    while (c.TK * c.nthr > 1024 && c.TK > 1) c.TK /= 2;
+    c.lds = (size_t)n * c.TK * sizeof(cplx<T>);
+    return c;
+}
+
+template <typename T>
+int set_lds_limits(ec_ctx *ctx) {
+    const int maxlds = 160 * 1024;
+    EC_HIP(ctx, hipFuncSetAttribute((const void *)k_fft_cols<T, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, maxlds));
+    EC_HIP(ctx, hipFuncSetAttribute((const void *)k_fft_cols<T, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, maxlds));
+    EC_HIP(ctx, hipFuncSetAttribute((const void *)k_fft_x_r2c<T>, hipFuncAttributeMaxDynamicSharedMemorySize, maxlds));
+    EC_HIP(ctx, hipFuncSetAttribute((const void *)k_fft_x_c2r<T>, hipFuncAttributeMaxDynamicSharedMemorySize, maxlds));
+    return EC_OK;
+}
+
+template <typename T>
+int r2c_x(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out) {
+    RadixPlan rp;
+    make_plan(ps->n, rp);
+    const int n = ps->n, nh = ps->nh;
+    const long nlines = (long)n * n;
+    XCfg c = xcfg<T>(n);
+    const long npairs = (nlines + 1) / 2;
+    const int blocks = (int)((npairs + c.P - 1) / c.P);
+    ec_prof_scope pr(ctx, "fft_x_r2c", (double)sizeof(T) * (ec_cube(n) + 2.0 * nh * n * n));
+    k_fft_x_r2c<T><<<blocks, c.P * c.nthr, c.lds, ctx->stream>>>((const T *)in, (cplx<T> *)out, n, nh, nlines, rp,
+                                                                  (const cplx<T> *)ps->twiddle, c.nthr);
+    EC_CHECK_LAUNCH(ctx, "k_fft_x_r2c");
+    return EC_OK;
+}
+
+template <typename T>
+int c2r_x(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out, double scale) {
+    RadixPlan rp;
+    make_plan(ps->n, rp);
+    const int n = ps->n, nh = ps->nh;
+    const long nlines = (long)n * n;
+    XCfg c = xcfg<T>(n);
+    const long npairs = (nlines + 1) / 2;
+    const int blocks = (int)((npairs + c.P - 1) / c.P);
+    ec_prof_scope pr(ctx, "fft_x_c2r", (double)sizeof(T) * (ec_cube(n) + 2.0 * nh * n * n));
+    k_fft_x_c2r<T><<<blocks, c.P * c.nthr, c.lds, ctx->stream>>>((const cplx<T> *)in, (T *)out, n, nh, nlines, rp,
+                                                                  (const cplx<T> *)ps->twiddle, c.nthr, (T)scale);
+    EC_CHECK_LAUNCH(ctx, "k_fft_x_c2r");
+    return EC_OK;
+}
+
+// axis 1 (y) or 2 (z); mode 1 = Poisson (z only)
+template <typename T>
+int cols(ec_ctx *ctx, ec_poisson *ps, void *spec, int axis, int sign, int mode) {
+    RadixPlan rp;
+    make_plan(ps->n, rp);
+    const int n = ps->n, nh = ps->nh;
+    CCfg c = ccfg<T>(n);
+    const long elem_stride = axis == 1 ? (long)nh : (long)nh * n;
+    const long outer_stride = axis == 1 ? (long)nh * n : (long)nh;
+    dim3 grid((nh + c.TK - 1) / c.TK, n);
+    const double h = ps->l / n;
+    const T a = (T)(-4.0 / (h * h));
+    const T scale = (T)(1.0 / ((double)n * n * n));
+    const char *name = mode ? "fft_z_poisson" : (axis == 1 ? "fft_y" : "fft_z");
+    ec_prof_scope pr(ctx, name, 2.0 * sizeof(T) * 2.0 * nh * n * n);
+    if (mode)
+        k_fft_cols<T, 1><<<grid, c.TK * c.nthr, c.lds, ctx->stream>>>((cplx<T> *)spec, n, nh, elem_stride, outer_stride,
+                                                                       c.TK, rp, (const cplx<T> *)ps->twiddle, sign,
+                                                                       (const T *)ps->eig, a, scale);
+    else
+        k_fft_cols<T, 0><<<grid, c.TK * c.nthr, c.lds, ctx->stream>>>((cplx<T> *)spec, n, nh, elem_stride, outer_stride,
+                                                                       c.TK, rp, (const cplx<T> *)ps->twiddle, sign,
+                                                                       (const T *)ps->eig, a, scale);
+    EC_CHECK_LAUNCH(ctx, "k_fft_cols");
+    return EC_OK;
+}
+
+template <typename T>
+int pressure_t(ec_ctx *ctx, ec_poisson *ps, void *const u[3]) {
+    const int n = ps->n;
+    int rc = ec_ns_divergence(ctx, ps->dtype, n, ps->l, u, ps->p);
+    if (rc) return rc;
+    if ((rc = r2c_x<T>(ctx, ps, ps->p, ps->spec))) return rc;
+    if ((rc = cols<T>(ctx, ps, ps->spec, 1, -1, 0))) return rc;
+    if ((rc = cols<T>(ctx, ps, ps->spec, 2, -1, 1))) return rc;
+    if ((rc = cols<T>(ctx, ps, ps->spec, 1, +1, 0))) return rc;
+    return c2r_x<T>(ctx, ps, ps->spec, ps->p, 1.0);
+}
+
+template <typename T>
+int project_t(ec_ctx *ctx, ec_poisson *ps, void *const u[3]) {
+    const int n = ps->n;
+    int rc = ec_ns_divergence(ctx, ps->dtype, n, ps->l, u, ps->p);
+    if (rc) return rc;
+    if ((rc = r2c_x<T>(ctx, ps, ps->p, ps->spec))) return rc;
+    if ((rc = cols<T>(ctx, ps, ps->spec, 1, -1, 0))) return rc;
+    if ((rc = cols<T>(ctx, ps, ps->spec, 2, -1, 1))) return rc;
+    if ((rc = cols<T>(ctx, ps, ps->spec, 1, +1, 0))) return rc;
+    if ((rc = c2r_x<T>(ctx, ps, ps->spec, ps->p, 1.0))) return rc;
+    return ec_ns_pressuregradient(ctx, ps->dtype, n, ps->l, u, ps->p);
+}
+
+}  // namespace
+
+extern "C" int ec_poisson_create(ec_ctx *ctx, int dtype, int n, double l, ec_poisson **out) {
+    EC_REQUIRE(ctx, ctx && out, "ec_poisson_create: null argument");
+    *out = nullptr;
+    EC_REQUIRE(ctx, dtype == EC_F32 || dtype == EC_F64, "ec_poisson_create: bad dtype %d", dtype);
+    EC_REQUIRE(ctx, n >= 2, "ec_poisson_create: n=%d", n);
+    RadixPlan rp;
+    if (!make_plan(n, rp))
+        return ec_fail(ctx, EC_E_UNSUPPORTED, "FFT length %d: only factors 2, 3, 5 are supported", n);
+    EC_REQUIRE(ctx, (n + FFT_E - 1) / FFT_E <= 1024, "FFT length %d too large", n);
+    const size_t bt = dtype == EC_F64 ? 8 : 4;
+    if ((size_t)n * 1 * 2 * bt > 128 * 1024)
+        return ec_fail(ctx, EC_E_UNSUPPORTED, "FFT length %d exceeds the LDS column budget", n);
+    ec_poisson *ps = new ec_poisson();
+    ps->n = n;
+    ps->dtype = dtype;
+    ps->l = l;
+    ps->nh = n / 2 + 1;
+    const size_t nspec = (size_t)ps->nh * n * n;
+    hipError_t e = hipMalloc(&ps->spec, nspec * 2 * bt);
+    if (e == hipSuccess) e = hipMalloc(&ps->p, ec_cube(n) * bt);
+    if (e == hipSuccess) e = hipMalloc(&ps->twiddle, (size_t)n * 2 * bt);
+    if (e == hipSuccess) e = hipMalloc(&ps->eig, (size_t)n * bt);
+    if (e != hipSuccess) {
+        hipFree(ps->spec);
+        hipFree(ps->p);
+        hipFree(ps->twiddle);
+        hipFree(ps->eig);
+        delete ps;
+        return ec_fail(ctx, EC_E_OOM, "ec_poisson_create(n=%d): %s", n, hipGetErrorString(e));
+    }
+    std::vector<double> tw(2 * (size_t)n), eg((size_t)n);
+    for (int t = 0; t < n; t++) {
+        // exp(-2 pi i t / n) from the folded angle for accuracy
+        const double a = 2.0 * M_PI * (double)t / (double)n;
+        tw[2 * t] = cos(a);
+        tw[2 * t + 1] = -sin(a);
+        const double s = sin(M_PI * (double)t / (double)n);
+        eg[t] = s * s;
+    }
+    int rc = EC_OK;
+    if (dtype == EC_F64) {
+        if (hipMemcpy(ps->twiddle, tw.data(), tw.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(ps->eig, eg.data(), eg.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+            rc = EC_E_HIP;
+        if (rc == EC_OK) rc = set_lds_limits<double>(ctx);
+    } else {
+        std::vector<float> twf(tw.begin(), tw.end()), egf(eg.begin(), eg.end());
+        if (hipMemcpy(ps->twiddle, twf.data(), twf.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(ps->eig, egf.data(), egf.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+            rc = EC_E_HIP;
+        if (rc == EC_OK) rc = set_lds_limits<float>(ctx);
+    }
+    if (rc != EC_OK) {
+        ec_poisson_destroy(ctx, ps);
+        return rc == EC_E_HIP ? ec_fail(ctx, rc, "ec_poisson_create: table upload failed") : rc;
+    }
+    *out = ps;
+    return EC_OK;
+}
+
+extern "C" int ec_poisson_destroy(ec_ctx *ctx, ec_poisson *ps) {
+    if (!ps) return EC_OK;
+    if (ctx) hipStreamSynchronize(ctx->stream);
+    hipFree(ps->spec);
+    hipFree(ps->p);
+    hipFree(ps->twiddle);
+    hipFree(ps->eig);
+    delete ps;
+    return EC_OK;
+}
+
+extern "C" void *ec_poisson_pressure(ec_poisson *ps) { return ps ? ps->p : nullptr; }
+
+extern "C" int ec_ns_project(ec_ctx *ctx, ec_poisson *ps, void *const u[3]) {
+    EC_REQUIRE(ctx, ctx && ps && u, "ec_ns_project: null argument");
+    return ps->dtype == EC_F64 ? project_t<double>(ctx, ps, u) : project_t<float>(ctx, ps, u);
+}
+
+extern "C" int ec_ns_pressure_solve(ec_ctx *ctx, ec_poisson *ps, void *const u[3]) {
+    EC_REQUIRE(ctx, ctx && ps && u, "ec_ns_pressure_solve: null argument");
+    return ps->dtype == EC_F64 ? pressure_t<double>(ctx, ps, u) : pressure_t<float>(ctx, ps, u);
+}
+
+extern "C" int ec_fft_r2c(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out) {
+    EC_REQUIRE(ctx, ctx && ps && in && out, "ec_fft_r2c: null argument");
+    int rc;
+    if (ps->dtype == EC_F64) {
+        if ((rc = r2c_x<double>(ctx, ps, in, out))) return rc;
+        if ((rc = cols<double>(ctx, ps, out, 1, -1, 0))) return rc;
+        return cols<double>(ctx, ps, out, 2, -1, 0);
+    }
+    if ((rc = r2c_x<float>(ctx, ps, in, out))) return rc;
+    if ((rc = cols<float>(ctx, ps, out, 1, -1, 0))) return rc;
+    return cols<float>(ctx, ps, out, 2, -1, 0);
+}
+
+// Inverse, normalised by 1/n^3 (ldiv!).  Works in place on a copy held in
+// the solver's spectrum buffer unless `in` already is that buffer.
+extern "C" int ec_fft_c2r(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out) {
+    EC_REQUIRE(ctx, ctx && ps && in && out, "ec_fft_c2r: null argument");
+    const size_t bt = ps->dtype == EC_F64 ? 8 : 4;
+    const size_t nspec = (size_t)ps->nh * ps->n * ps->n;
+    if (in != ps->spec) EC_HIP(ctx, hipMemcpyAsync(ps->spec, in, nspec * 2 * bt, hipMemcpyDeviceToDevice, ctx->stream));
+    const double scale = 1.0 / ((double)ps->n * ps->n * ps->n);
+    int rc;
+    if (ps->dtype == EC_F64) {
+        if ((rc = cols<double>(ctx, ps, ps->spec, 2, +1, 0))) return rc;
+        if ((rc = cols<double>(ctx, ps, ps->spec, 1, +1, 0))) return rc;
+        return c2r_x<double>(ctx, ps, ps->spec, out, scale);
+    }
+    if ((rc = cols<float>(ctx, ps, ps->spec, 2, +1, 0))) return rc;
+    if ((rc = cols<float>(ctx, ps, ps->spec, 1, +1, 0))) return rc;
+    return c2r_x<float>(ctx, ps, ps->spec, out, scale);
+}
+
+static double profile_eval(int profile, double param, int k) {
+    if (profile == 0) return pow((double)k, 4) * exp(-2 * pow((double)k / param, 2));
+    return k > 0 ? pow((double)k, param) : 0.0;
+}
+
+template <typename T>
+static int randomfield_t(ec_ctx *ctx, ec_poisson *ps, int profile, double param, double totalenergy, uint64_t seed,
+                         void *const u[3]) {
+    const int n = ps->n, nh = ps->nh, kmax = n / 2;
+    const size_t N = ec_cube(n);
+    int rc;
+    for (int c = 0; c < 3; c++)
+        if ((rc = ec_randn(ctx, ps->dtype, N, seed, (uint64_t)c * N, u[c]))) return rc;
+    if ((rc = ec_ns_project(ctx, ps, u))) return rc;
+    const int kdiag = (int)floor(sqrt(3.0) * kmax);
+    const int nshell = kdiag + 1;
+    double *dhist = nullptr;
+    EC_HIP(ctx, hipMallocAsync((void **)&dhist, sizeof(double) * 2 * nshell, ctx->stream));
+    EC_HIP(ctx, hipMemsetAsync(dhist, 0, sizeof(double) * 2 * nshell, ctx->stream));
+    const double inv_n3 = 1.0 / ((double)n * n * n);
+    const size_t nspec = (size_t)nh * n * n;
+    const int blocks = (int)std::min<size_t>((nspec + 255) / 256, 4096);
+    for (int c = 0; c < 3; c++) {
+        if ((rc = ec_fft_r2c(ctx, ps, u[c], ps->spec))) return rc;
+        k_shell_hist<T><<<blocks, 256, sizeof(double) * nshell, ctx->stream>>>((const cplx<T> *)ps->spec, n, nh, kmax,
+                                                                                nshell, inv_n3, dhist);
+        EC_CHECK_LAUNCH(ctx, "k_shell_hist");
+    }
+    std::vector<double> hist(nshell), factor(nshell);
+    EC_HIP(ctx, hipMemcpyAsync(hist.data(), dhist, sizeof(double) * nshell, hipMemcpyDeviceToHost, ctx->stream));
+    EC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    double totalprofile = 0.0;
+    for (int k = 0; k <= kdiag; k++) totalprofile += profile_eval(profile, param, k);
+    for (int k = 0; k <= kdiag; k++) {
+        const double E0 = totalenergy * profile_eval(profile, param, k) / totalprofile;
+        factor[k] = sqrt(E0 / hist[k]);
+    }
+    EC_HIP(ctx, hipMemcpyAsync(dhist + nshell, factor.data(), sizeof(double) * nshell, hipMemcpyHostToDevice, ctx->stream));
+    for (int c = 0; c < 3; c++) {
+        if ((rc = ec_fft_r2c(ctx, ps, u[c], ps->spec))) return rc;
+        // rfft / n^3, times the shell factor; the inverse then runs unnormalised
+        k_shell_scale<T><<<blocks, 256, 0, ctx->stream>>>((cplx<T> *)ps->spec, n, nh, nshell, dhist + nshell, inv_n3);
+        EC_CHECK_LAUNCH(ctx, "k_shell_scale");
+        if ((rc = cols<T>(ctx, ps, ps->spec, 2, +1, 0))) return rc;
+        if ((rc = cols<T>(ctx, ps, ps->spec, 1, +1, 0))) return rc;
+        if ((rc = c2r_x<T>(ctx, ps, ps->spec, u[c], 1.0))) return rc;
+    }
+    EC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    EC_HIP(ctx, hipFreeAsync(dhist, ctx->stream));
+    return ec_ns_project(ctx, ps, u);
+}
+
+extern "C" int ec_ns_randomfield(ec_ctx *ctx, ec_poisson *ps, int profile, double param, double totalenergy,
+                                 uint64_t seed, void *const u[3]) {
+    EC_REQUIRE(ctx, ctx && ps && u, "ec_ns_randomfield: null argument");
+    EC_REQUIRE(ctx, profile == 0 || profile == 1, "ec_ns_randomfield: bad profile %d", profile);
+    return ps->dtype == EC_F64 ? randomfield_t<double>(ctx, ps, profile, param, totalenergy, seed, u)
+                               : randomfield_t<float>(ctx, ps, profile, param, totalenergy, seed, u);
+}

@@ -1,0 +1,116 @@
+// Internal definitions shared by the HIP translation units of libexactclosure.
+// gfx950 (CDNA4) only: 64-wide wavefronts, 160 KiB LDS per CU.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/exactclosure.h"
+
+struct ec_prof_rec {
+    const char *name;
+    hipEvent_t start, stop;
+    double bytes;  // algorithmic bytes moved by this launch
+};
+
+struct ec_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    char err[1024] = {0};
+    // Reduction workspace (device partials + pinned host result).
+    double *red_partial = nullptr;
+    double *red_host = nullptr;
+    int red_cap = 0;
+    // Per-launch timing, enabled by ec_profile_begin.
+    bool prof = false;
+    std::vector<ec_prof_rec> prof_recs;
+    std::vector<hipEvent_t> event_pool;
+    size_t event_next = 0;
+};
+
+// ---- error handling: no C++ exception crosses the C ABI ----
+int ec_fail(ec_ctx *ctx, int code, const char *fmt, ...);
+
+#define EC_HIP(ctx, call)                                                                     \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return ec_fail((ctx), EC_E_HIP, "%s failed: %s (%s:%d)", #call,                   \
+                           hipGetErrorString(e_), __FILE__, __LINE__);                        \
+    } while (0)
+
+#define EC_CHECK_LAUNCH(ctx, name)                                                            \
+    do {                                                                                      \
+        hipError_t e_ = hipGetLastError();                                                    \
+        if (e_ != hipSuccess)                                                                 \
+            return ec_fail((ctx), EC_E_HIP, "launch of %s failed: %s", (name),                \
+                           hipGetErrorString(e_));                                            \
+    } while (0)
+
+#define EC_REQUIRE(ctx, cond, ...)                                                            \
+    do {                                                                                      \
+        if (!(cond)) return ec_fail((ctx), EC_E_INVALID, __VA_ARGS__);                        \
+    } while (0)
+
+// ---- launch timing (HIP events on the context stream) ----
+// Returns the record index or -1 when profiling is off.
+int ec_prof_start(ec_ctx *ctx, const char *name, double bytes);
+void ec_prof_stop(ec_ctx *ctx, int rec);
+
+struct ec_prof_scope {
+    ec_ctx *ctx;
+    int rec;
+    ec_prof_scope(ec_ctx *c, const char *name, double bytes) : ctx(c), rec(ec_prof_start(c, name, bytes)) {}
+    ~ec_prof_scope() { ec_prof_stop(ctx, rec); }
+};
+
+// ---- complex helpers ----
+template <typename T>
+struct cplx {
+    T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cadd(cplx<T> a, cplx<T> b) {
+    return {a.x + b.x, a.y + b.y};
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T> b) {
+    return {a.x - b.x, a.y - b.y};
+}
+
+// periodic neighbours without division (|shift| == 1)
+__device__ __forceinline__ int ec_up(int i, int n) { return i + 1 == n ? 0 : i + 1; }
+__device__ __forceinline__ int ec_dn(int i, int n) { return i == 0 ? n - 1 : i - 1; }
+
+static inline size_t ec_cube(int n) { return (size_t)n * (size_t)n * (size_t)n; }
+
+// Poisson solver state (FFT plan, spectrum and pressure buffers).
+struct ec_poisson {
+    int n = 0;
+    int dtype = EC_F64;
+    double l = 1.0;
+    int nh = 0;
+    void *spec = nullptr;     // (nh, n, n) complex of dtype
+    void *p = nullptr;        // n^3 pressure of dtype
+    void *twiddle = nullptr;  // exp(-2 pi i t / n), t in [0, n), complex of dtype
+    void *eig = nullptr;      // sin^2(pi k / n), k in [0, n), real of dtype
+    int nrad = 0;
+    int radix[32] = {0};
+    bool own_p = true;
+};
+
+// Internal entry points shared between translation units.
+int ec_fft_r2c_impl(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out);
+int ec_fft_c2r_impl(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out);
+int ec_fft_cols_impl(ec_ctx *ctx, ec_poisson *ps, void *spec, int axis, int sign, int poisson_mode);

@@ -1,0 +1,374 @@
+// Staggered-grid Navier-Stokes stencils: stress, tensor divergence, fused
+// right-hand side, divergence, pressure gradient and the RK4 stage update.
+// Bandwidth-bound: one thread per cell, x fastest so every wave reads 64
+// consecutive values of each face array; the neighbour loads hit L1/L2.
+#include "ec_internal.h"
+
+namespace {
+
+struct Grid3 {
+    dim3 block, grid;
+};
+
+Grid3 grid3(int n) {
+    Grid3 g;
+    int bx = n >= 64 ? 64 : n;
+    int by = 256 / bx;
+    if (by > n) by = n;
+    g.block = dim3(bx, by, 1);
+    g.grid = dim3((n + bx - 1) / bx, (n + by - 1) / by, n);
+    return g;
+}
+
+struct Cell {
+    int i, j, k, im, ip, jm, jp, km, kp;
+    size_t n1, n2;
+    __device__ __forceinline__ size_t at(int a, int b, int c) const { return (size_t)a + n1 * b + n2 * c; }
+};
+
+__device__ __forceinline__ bool cell_of(int n, Cell &c) {
+    c.i = blockIdx.x * blockDim.x + threadIdx.x;
+    c.j = blockIdx.y * blockDim.y + threadIdx.y;
+    c.k = blockIdx.z;
+    if (c.i >= n || c.j >= n) return false;
+    c.im = ec_dn(c.i, n);
+    c.ip = ec_up(c.i, n);
+    c.jm = ec_dn(c.j, n);
+    c.jp = ec_up(c.j, n);
+    c.km = ec_dn(c.k, n);
+    c.kp = ec_up(c.k, n);
+    c.n1 = (size_t)n;
+    c.n2 = (size_t)n * n;
+    return true;
+}
+
+// stress(u, visc, i, i) and stress(u, visc, i, j) of src/N.jl:106-118 from
+// the right/left neighbour values.
+template <typename T>
+__device__ __forceinline__ T s_diag(T r, T l, T visc, T ih) {
+    const T e = (r + l) / 2;
+    const T d = (r - l) * ih;
+    return e * e - visc * (d + d);
+}
+template <typename T>
+__device__ __forceinline__ T s_off(T ar, T al, T br, T bl, T visc, T ih) {
+    return ((ar + al) / 2) * ((br + bl) / 2) - visc * ((ar - al) * ih + (br - bl) * ih);
+}
+
+template <typename T>
+__global__ void k_stresstensor(int n, T ih, T visc, const T *__restrict__ u1, const T *__restrict__ u2,
+                               const T *__restrict__ u3, T *__restrict__ s11, T *__restrict__ s22,
+                               T *__restrict__ s33, T *__restrict__ s12, T *__restrict__ s23,
+                               T *__restrict__ s31) {
+    Cell c;
+    if (!cell_of(n, c)) return;
+    const size_t I = c.at(c.i, c.j, c.k);
+    s11[I] = s_diag(u1[I], u1[c.at(c.im, c.j, c.k)], visc, ih);
+    s22[I] = s_diag(u2[I], u2[c.at(c.i, c.jm, c.k)], visc, ih);
+    s33[I] = s_diag(u3[I], u3[c.at(c.i, c.j, c.km)], visc, ih);
+    s12[I] = s_off(u1[c.at(c.i, c.jp, c.k)], u1[I], u2[c.at(c.ip, c.j, c.k)], u2[I], visc, ih);
+    s23[I] = s_off(u2[c.at(c.i, c.j, c.kp)], u2[I], u3[c.at(c.i, c.jp, c.k)], u3[I], visc, ih);
+    s31[I] = s_off(u3[c.at(c.ip, c.j, c.k)], u3[I], u1[c.at(c.i, c.j, c.kp)], u1[I], visc, ih);
+}
+
+template <typename T>
+__global__ void k_tensordivergence(int n, T ih, const T *__restrict__ s11, const T *__restrict__ s22,
+                                   const T *__restrict__ s33, const T *__restrict__ s12,
+                                   const T *__restrict__ s23, const T *__restrict__ s31,
+                                   T *__restrict__ du1, T *__restrict__ du2, T *__restrict__ du3, int doadd) {
+    Cell c;
+    if (!cell_of(n, c)) return;
+    const size_t I = c.at(c.i, c.j, c.k);
+    T a1 = doadd ? du1[I] : T(0);
+    a1 = a1 - (s11[c.at(c.ip, c.j, c.k)] - s11[I]) * ih - (s12[I] - s12[c.at(c.i, c.jm, c.k)]) * ih -
+         (s31[I] - s31[c.at(c.i, c.j, c.km)]) * ih;
+    T a2 = doadd ? du2[I] : T(0);
+    a2 = a2 - (s12[I] - s12[c.at(c.im, c.j, c.k)]) * ih - (s22[c.at(c.i, c.jp, c.k)] - s22[I]) * ih -
+         (s23[I] - s23[c.at(c.i, c.j, c.km)]) * ih;
+    T a3 = doadd ? du3[I] : T(0);
+    a3 = a3 - (s31[I] - s31[c.at(c.im, c.j, c.k)]) * ih - (s23[I] - s23[c.at(c.i, c.jm, c.k)]) * ih -
+         (s33[c.at(c.i, c.j, c.kp)] - s33[I]) * ih;
+    du1[I] = a1;
+    du2[I] = a2;
+    du3[I] = a3;
+}
+
+struct Ptr9 {
+    const void *p[9];
+};
+
+template <typename T, typename S>
+__global__ void k_tensordivergence_nonsym(int n, T ih, Ptr9 sp, T *__restrict__ du1, T *__restrict__ du2,
+                                          T *__restrict__ du3, int doadd) {
+    Cell c;
+    if (!cell_of(n, c)) return;
+    const size_t I = c.at(c.i, c.j, c.k);
+    T *du[3] = {du1, du2, du3};
+    const size_t nb_up[3] = {c.at(c.ip, c.j, c.k), c.at(c.i, c.jp, c.k), c.at(c.i, c.j, c.kp)};
+    const size_t nb_dn[3] = {c.at(c.im, c.j, c.k), c.at(c.i, c.jm, c.k), c.at(c.i, c.j, c.km)};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        T acc = doadd ? du[a][I] : T(0);
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            const S *s = (const S *)sp.p[a + 3 * b];
+            const S diff = a == b ? s[nb_up[b]] - s[I] : s[I] - s[nb_dn[b]];
+            acc = acc - (T)(diff * (S)ih);
+        }
+        du[a][I] = acc;
+    }
+}
+
+// Fused stresstensor + tensordivergence!: 27 neighbour loads, 13 stresses,
+// nothing materialised.
+template <typename T>
+__global__ void k_rhs(int n, T ih, T visc, const T *__restrict__ u1, const T *__restrict__ u2,
+                      const T *__restrict__ u3, T *__restrict__ du1, T *__restrict__ du2,
+                      T *__restrict__ du3, int doadd) {
+    Cell c;
+    if (!cell_of(n, c)) return;
+    const int i = c.i, j = c.j, k = c.k;
+    const size_t I = c.at(i, j, k);
+    // u1 neighbourhood
+    const T u1_0 = u1[I], u1_xp = u1[c.at(c.ip, j, k)], u1_xm = u1[c.at(c.im, j, k)];
+    const T u1_yp = u1[c.at(i, c.jp, k)], u1_ym = u1[c.at(i, c.jm, k)];
+    const T u1_zp = u1[c.at(i, j, c.kp)], u1_zm = u1[c.at(i, j, c.km)];
+    const T u1_xm_yp = u1[c.at(c.im, c.jp, k)], u1_xm_zp = u1[c.at(c.im, j, c.kp)];
+    // u2 neighbourhood
+    const T u2_0 = u2[I], u2_xp = u2[c.at(c.ip, j, k)], u2_xm = u2[c.at(c.im, j, k)];
+    const T u2_yp = u2[c.at(i, c.jp, k)], u2_ym = u2[c.at(i, c.jm, k)];
+    const T u2_zp = u2[c.at(i, j, c.kp)], u2_zm = u2[c.at(i, j, c.km)];
+    const T u2_xp_ym = u2[c.at(c.ip, c.jm, k)], u2_ym_zp = u2[c.at(i, c.jm, c.kp)];
+    // u3 neighbourhood
+    const T u3_0 = u3[I], u3_xp = u3[c.at(c.ip, j, k)], u3_xm = u3[c.at(c.im, j, k)];
+    const T u3_yp = u3[c.at(i, c.jp, k)], u3_ym = u3[c.at(i, c.jm, k)];
+    const T u3_zp = u3[c.at(i, j, c.kp)], u3_zm = u3[c.at(i, j, c.km)];
+    const T u3_xp_zm = u3[c.at(c.ip, j, c.km)], u3_yp_zm = u3[c.at(i, c.jp, c.km)];
+
+    const T s11_p = s_diag(u1_xp, u1_0, visc, ih), s11_0 = s_diag(u1_0, u1_xm, visc, ih);
+    const T s22_p = s_diag(u2_yp, u2_0, visc, ih), s22_0 = s_diag(u2_0, u2_ym, visc, ih);
+    const T s33_p = s_diag(u3_zp, u3_0, visc, ih), s33_0 = s_diag(u3_0, u3_zm, visc, ih);
+    const T s12_0 = s_off(u1_yp, u1_0, u2_xp, u2_0, visc, ih);
+    const T s12_ym = s_off(u1_0, u1_ym, u2_xp_ym, u2_ym, visc, ih);
+    const T s12_xm = s_off(u1_xm_yp, u1_xm, u2_0, u2_xm, visc, ih);
+    const T s31_0 = s_off(u3_xp, u3_0, u1_zp, u1_0, visc, ih);
+    const T s31_zm = s_off(u3_xp_zm, u3_zm, u1_0, u1_zm, visc, ih);
+    const T s31_xm = s_off(u3_0, u3_xm, u1_xm_zp, u1_xm, visc, ih);
+    const T s23_0 = s_off(u2_zp, u2_0, u3_yp, u3_0, visc, ih);
+    const T s23_zm = s_off(u2_0, u2_zm, u3_yp_zm, u3_zm, visc, ih);
+    const T s23_ym = s_off(u2_ym_zp, u2_ym, u3_0, u3_ym, visc, ih);
+
+    T a1 = doadd ? du1[I] : T(0);
+    a1 = a1 - (s11_p - s11_0) * ih - (s12_0 - s12_ym) * ih - (s31_0 - s31_zm) * ih;
+    T a2 = doadd ? du2[I] : T(0);
+    a2 = a2 - (s12_0 - s12_xm) * ih - (s22_p - s22_0) * ih - (s23_0 - s23_zm) * ih;
+    T a3 = doadd ? du3[I] : T(0);
+    a3 = a3 - (s31_0 - s31_xm) * ih - (s23_0 - s23_ym) * ih - (s33_p - s33_0) * ih;
+    du1[I] = a1;
+    du2[I] = a2;
+    du3[I] = a3;
+}
+
+template <typename T>
+__global__ void k_divergence(int n, T ih, const T *__restrict__ u1, const T *__restrict__ u2,
+                             const T *__restrict__ u3, T *__restrict__ d) {
+    Cell c;
+    if (!cell_of(n, c)) return;
+    const size_t I = c.at(c.i, c.j, c.k);
+    d[I] = (u1[I] - u1[c.at(c.im, c.j, c.k)]) * ih + (u2[I] - u2[c.at(c.i, c.jm, c.k)]) * ih +
+           (u3[I] - u3[c.at(c.i, c.j, c.km)]) * ih;
+}
+
+template <typename T>
+__global__ void k_pressuregradient(int n, T ih, T *__restrict__ u1, T *__restrict__ u2, T *__restrict__ u3,
+                                   const T *__restrict__ p) {
+    Cell c;
+    if (!cell_of(n, c)) return;
+    const size_t I = c.at(c.i, c.j, c.k);
+    const T p0 = p[I];
+    u1[I] -= (p[c.at(c.ip, c.j, c.k)] - p0) * ih;
+    u2[I] -= (p[c.at(c.i, c.jp, c.k)] - p0) * ih;
+    u3[I] -= (p[c.at(c.i, c.j, c.kp)] - p0) * ih;
+}
+
+struct Ptr3 {
+    void *p[3];
+};
+
+// RK4 stage: k = du - grad p (the projected right-hand side), then the
+// tableau of rk4_timestep! (src/Burgers.jl:120-142).
+template <typename T, int STAGE>
+__global__ void k_rk4_update(int n, T ih, T dt, Ptr3 u0p, Ptr3 usp, Ptr3 dup, const T *__restrict__ p,
+                             Ptr3 accp) {
+    Cell c;
+    if (!cell_of(n, c)) return;
+    const size_t I = c.at(c.i, c.j, c.k);
+    const T p0 = p[I];
+    const T gp[3] = {(p[c.at(c.ip, c.j, c.k)] - p0) * ih, (p[c.at(c.i, c.jp, c.k)] - p0) * ih,
+                     (p[c.at(c.i, c.j, c.kp)] - p0) * ih};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const T kk = ((const T *)dup.p[a])[I] - gp[a];
+        T *u0 = (T *)u0p.p[a];
+        T *acc = (T *)accp.p[a];
+        if (STAGE == 0) {
+            acc[I] = kk;
+            ((T *)usp.p[a])[I] = u0[I] + (dt / 2) * kk;
+        } else if (STAGE == 1) {
+            acc[I] = acc[I] + 2 * kk;
+            ((T *)usp.p[a])[I] = u0[I] + (dt / 2) * kk;
+        } else if (STAGE == 2) {
+            acc[I] = acc[I] + 2 * kk;
+            ((T *)usp.p[a])[I] = u0[I] + dt * kk;
+        } else {
+            u0[I] = u0[I] + (dt / 6) * (acc[I] + kk);
+        }
+    }
+}
+
+}  // namespace
+
+#define EC_CHECK_FIELD(ctx, n) EC_REQUIRE(ctx, (n) >= 2, "grid size n=%d must be >= 2", (n))
+
+extern "C" int ec_ns_stresstensor(ec_ctx *ctx, int dtype, int n, double l, double visc, void *const u[3],
+                                  void *const s[6]) {
+    EC_REQUIRE(ctx, ctx && u && s, "ec_ns_stresstensor: null argument");
+    EC_CHECK_FIELD(ctx, n);
+    Grid3 g = grid3(n);
+    const double ih = n / l;
+    ec_prof_scope ps(ctx, "stresstensor", 9.0 * ec_cube(n) * (dtype == EC_F64 ? 8 : 4));
+    if (dtype == EC_F64)
+        k_stresstensor<double><<<g.grid, g.block, 0, ctx->stream>>>(
+            n, ih, visc, (const double *)u[0], (const double *)u[1], (const double *)u[2], (double *)s[0],
+            (double *)s[1], (double *)s[2], (double *)s[3], (double *)s[4], (double *)s[5]);
+    else
+        k_stresstensor<float><<<g.grid, g.block, 0, ctx->stream>>>(
+            n, (float)ih, (float)visc, (const float *)u[0], (const float *)u[1], (const float *)u[2],
+            (float *)s[0], (float *)s[1], (float *)s[2], (float *)s[3], (float *)s[4], (float *)s[5]);
+    EC_CHECK_LAUNCH(ctx, "k_stresstensor");
+    return EC_OK;
+}
+
+extern "C" int ec_ns_tensordivergence(ec_ctx *ctx, int dtype, int n, double l, void *const s[6],
+                                      void *const du[3], int doadd) {
+    EC_REQUIRE(ctx, ctx && s && du, "ec_ns_tensordivergence: null argument");
+    EC_CHECK_FIELD(ctx, n);
+    Grid3 g = grid3(n);
+    const double ih = n / l;
+    if (dtype == EC_F64)
+        k_tensordivergence<double><<<g.grid, g.block, 0, ctx->stream>>>(
+            n, ih, (const double *)s[0], (const double *)s[1], (const double *)s[2], (const double *)s[3],
+            (const double *)s[4], (const double *)s[5], (double *)du[0], (double *)du[1], (double *)du[2], doadd);
+    else
+        k_tensordivergence<float><<<g.grid, g.block, 0, ctx->stream>>>(
+            n, (float)ih, (const float *)s[0], (const float *)s[1], (const float *)s[2], (const float *)s[3],
+            (const float *)s[4], (const float *)s[5], (float *)du[0], (float *)du[1], (float *)du[2], doadd);
+    EC_CHECK_LAUNCH(ctx, "k_tensordivergence");
+    return EC_OK;
+}
+
+extern "C" int ec_ns_tensordivergence_nonsym(ec_ctx *ctx, int dtype, int n, double l, int sigma_dtype,
+                                             void *const s9[9], void *const du[3], int doadd) {
+    EC_REQUIRE(ctx, ctx && s9 && du, "ec_ns_tensordivergence_nonsym: null argument");
+    EC_CHECK_FIELD(ctx, n);
+    Grid3 g = grid3(n);
+    const double ih = n / l;
+    Ptr9 sp;
+    for (int q = 0; q < 9; q++) sp.p[q] = s9[q];
+    if (dtype == EC_F64 && sigma_dtype == EC_F64)
+        k_tensordivergence_nonsym<double, double><<<g.grid, g.block, 0, ctx->stream>>>(
+            n, ih, sp, (double *)du[0], (double *)du[1], (double *)du[2], doadd);
+    else if (dtype == EC_F32 && sigma_dtype == EC_F64)
+        k_tensordivergence_nonsym<float, double><<<g.grid, g.block, 0, ctx->stream>>>(
+            n, (float)ih, sp, (float *)du[0], (float *)du[1], (float *)du[2], doadd);
+    else if (dtype == EC_F32 && sigma_dtype == EC_F32)
+        k_tensordivergence_nonsym<float, float><<<g.grid, g.block, 0, ctx->stream>>>(
+            n, (float)ih, sp, (float *)du[0], (float *)du[1], (float *)du[2], doadd);
+    else
+        return ec_fail(ctx, EC_E_UNSUPPORTED, "nonsym divergence: FP32 sigma into FP64 du");
+    EC_CHECK_LAUNCH(ctx, "k_tensordivergence_nonsym");
+    return EC_OK;
+}
+
+extern "C" int ec_ns_rhs(ec_ctx *ctx, int dtype, int n, double l, double visc, void *const u[3],
+                         void *const du[3], int doadd) {
+    EC_REQUIRE(ctx, ctx && u && du, "ec_ns_rhs: null argument");
+    EC_CHECK_FIELD(ctx, n);
+    Grid3 g = grid3(n);
+    const double ih = n / l;
+    const double b = dtype == EC_F64 ? 8 : 4;
+    ec_prof_scope ps(ctx, "ns_rhs", (doadd ? 9.0 : 6.0) * ec_cube(n) * b);
+    if (dtype == EC_F64)
+        k_rhs<double><<<g.grid, g.block, 0, ctx->stream>>>(n, ih, visc, (const double *)u[0], (const double *)u[1],
+                                                           (const double *)u[2], (double *)du[0], (double *)du[1],
+                                                           (double *)du[2], doadd);
+    else
+        k_rhs<float><<<g.grid, g.block, 0, ctx->stream>>>(n, (float)ih, (float)visc, (const float *)u[0],
+                                                          (const float *)u[1], (const float *)u[2], (float *)du[0],
+                                                          (float *)du[1], (float *)du[2], doadd);
+    EC_CHECK_LAUNCH(ctx, "k_rhs");
+    return EC_OK;
+}
+
+extern "C" int ec_ns_divergence(ec_ctx *ctx, int dtype, int n, double l, void *const u[3], void *d) {
+    EC_REQUIRE(ctx, ctx && u && d, "ec_ns_divergence: null argument");
+    EC_CHECK_FIELD(ctx, n);
+    Grid3 g = grid3(n);
+    const double ih = n / l;
+    ec_prof_scope ps(ctx, "divergence", 4.0 * ec_cube(n) * (dtype == EC_F64 ? 8 : 4));
+    if (dtype == EC_F64)
+        k_divergence<double><<<g.grid, g.block, 0, ctx->stream>>>(n, ih, (const double *)u[0], (const double *)u[1],
+                                                                  (const double *)u[2], (double *)d);
+    else
+        k_divergence<float><<<g.grid, g.block, 0, ctx->stream>>>(n, (float)ih, (const float *)u[0],
+                                                                 (const float *)u[1], (const float *)u[2], (float *)d);
+    EC_CHECK_LAUNCH(ctx, "k_divergence");
+    return EC_OK;
+}
+
+extern "C" int ec_ns_pressuregradient(ec_ctx *ctx, int dtype, int n, double l, void *const u[3], const void *p) {
+    EC_REQUIRE(ctx, ctx && u && p, "ec_ns_pressuregradient: null argument");
+    EC_CHECK_FIELD(ctx, n);
+    Grid3 g = grid3(n);
+    const double ih = n / l;
+    ec_prof_scope ps(ctx, "pressuregradient", 7.0 * ec_cube(n) * (dtype == EC_F64 ? 8 : 4));
+    if (dtype == EC_F64)
+        k_pressuregradient<double><<<g.grid, g.block, 0, ctx->stream>>>(n, ih, (double *)u[0], (double *)u[1],
+                                                                        (double *)u[2], (const double *)p);
+    else
+        k_pressuregradient<float><<<g.grid, g.block, 0, ctx->stream>>>(n, (float)ih, (float *)u[0], (float *)u[1],
+                                                                       (float *)u[2], (const float *)p);
+    EC_CHECK_LAUNCH(ctx, "k_pressuregradient");
+    return EC_OK;
+}
+
+template <typename T>
+static int rk4_launch(ec_ctx *ctx, int n, double l, int stage, double dt, void *const u0[3], void *const us[3],
+                      void *const du[3], const void *p, void *const acc[3]) {
+    Grid3 g = grid3(n);
+    const T ih = (T)(n / l);
+    Ptr3 U0{{u0[0], u0[1], u0[2]}}, US{{us ? us[0] : nullptr, us ? us[1] : nullptr, us ? us[2] : nullptr}},
+        DU{{du[0], du[1], du[2]}}, AC{{acc[0], acc[1], acc[2]}};
+    switch (stage) {
+        case 0: k_rk4_update<T, 0><<<g.grid, g.block, 0, ctx->stream>>>(n, ih, (T)dt, U0, US, DU, (const T *)p, AC); break;
+        case 1: k_rk4_update<T, 1><<<g.grid, g.block, 0, ctx->stream>>>(n, ih, (T)dt, U0, US, DU, (const T *)p, AC); break;
+        case 2: k_rk4_update<T, 2><<<g.grid, g.block, 0, ctx->stream>>>(n, ih, (T)dt, U0, US, DU, (const T *)p, AC); break;
+        default: k_rk4_update<T, 3><<<g.grid, g.block, 0, ctx->stream>>>(n, ih, (T)dt, U0, US, DU, (const T *)p, AC); break;
+    }
+    EC_CHECK_LAUNCH(ctx, "k_rk4_update");
+    return EC_OK;
+}
+
+extern "C" int ec_ns_rk4_update(ec_ctx *ctx, int dtype, int n, double l, int stage, double dt, void *const u0[3],
+                                void *const us[3], void *const du[3], const void *p, void *const acc[3]) {
+    EC_REQUIRE(ctx, ctx && u0 && du && p && acc, "ec_ns_rk4_update: null argument");
+    EC_REQUIRE(ctx, stage >= 0 && stage <= 3, "ec_ns_rk4_update: stage %d out of range", stage);
+    EC_REQUIRE(ctx, stage == 3 || us, "ec_ns_rk4_update: stage state required before the last stage");
+    EC_CHECK_FIELD(ctx, n);
+    const double b = dtype == EC_F64 ? 8 : 4;
+    // reads p, du(3), u0(3), acc(3 unless stage 0); writes us(3) + acc(3), or u0(3) at stage 3
+    const double units = stage == 0 ? 13 : (stage == 3 ? 13 : 16);
+    ec_prof_scope ps(ctx, "rk4_update", units * ec_cube(n) * b);
+    return dtype == EC_F64 ? rk4_launch<double>(ctx, n, l, stage, dt, u0, us, du, p, acc)
+                           : rk4_launch<float>(ctx, n, l, stage, dt, u0, us, du, p, acc);
+}

@@ -1,0 +1,152 @@
+"""DNS-aided LES with classical RK4 and the exact closure at every stage.
+
+Host orchestration of the hot path (the role of dnsaid_project,
+src/N.jl:1239-1332, and dns_aid_volavg / dns_aid_surface,
+src/NavierStokes.jl:141-468) over libexactclosure:
+
+per stage s of an RK4 step (tableau of rk4_timestep!, src/Burgers.jl:120-142,
+projection at every stage as in step_wray3!, src/N.jl:320-346):
+  1. du = -div sigma(u_s)                         ec_ns_rhs (fused stencil)
+  2. p  = L^-1 div du                              ec_ns_pressure_solve (FFT)
+  3. for every LES grid: V u_s and S r, r = sigma(u_s) + p I (FP64)
+                                                    ec_ns_closure
+     ubar = P_H V u_s (FP64 LES projection), tau = S r - sigma_H(ubar) (FP64)
+     du_les = -div_H sigma(u_les) - div^ns_H tau, p_les = L_H^-1 div du_les
+  4. k = du - grad p; RK4 update of DNS and LES     ec_ns_rk4_update
+
+With the volume-average filter the LES reproduces P_H V u_dns to rounding
+(the exactness invariant of navierstokes.jl:33).  Fields run in `dtype`
+(FP32 or FP64); the closure commutator runs in FP64.
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _abi
+from . import navierstokes as NS
+from ._abi import P3, ptr, ptrs
+
+
+def _vec(n, dtype):
+    return [torch.empty((n, n, n), dtype=dtype, device="cuda") for _ in range(3)]
+
+
+class LESGrid:
+    def __init__(self, g_dns, nbar, kind, dtype):
+        self.g = NS.Grid(g_dns.l, nbar)
+        self.kind = kind
+        self.dtype = dtype
+        self.cset = NS.ClosureSet(g_dns, nbar, kind)
+        self.ps = NS.PoissonSolver(self.g, dtype)
+        self.ps64 = self.ps if dtype == torch.float64 else NS.PoissonSolver(self.g, torch.float64)
+        self.u0 = _vec(nbar, dtype)
+        self.us = _vec(nbar, dtype)
+        self.du = _vec(nbar, dtype)
+        self.acc = _vec(nbar, dtype)
+
+
+class DNSAidedLES:
+    """State and buffers of one DNS with exact-closure LES grids.
+
+    sets: list of (nbar, kind) with kind 'va' (volume average) or 'fa'
+    (face average).  Memory at n = 1024 FP32: 12 DNS fields + p + half
+    spectrum ~ 60 GB."""
+
+    def __init__(self, g_dns, visc, sets, dtype=torch.float32):
+        self.g = g_dns
+        self.visc = float(visc)
+        self.dtype = dtype
+        self.ctx = _abi.context(0)
+        self.dt_code = _abi.EC_F64 if dtype == torch.float64 else _abi.EC_F32
+        n = g_dns.n
+        self.ps = NS.PoissonSolver(g_dns, dtype)
+        self.u0 = _vec(n, dtype)
+        self.us = _vec(n, dtype)
+        self.du = _vec(n, dtype)
+        self.acc = _vec(n, dtype)
+        self.les = [LESGrid(g_dns, nbar, kind, dtype) for nbar, kind in sets]
+
+    # -- state -------------------------------------------------------------
+    def set_state(self, u):
+        for a in range(3):
+            self.u0[a].copy_(u[a].data if isinstance(u[a], NS.Field) else u[a])
+        self.init_les()
+
+    def init_les(self):
+        """LES fields start from P_H V u (apply_DHpi, src/N.jl:1228-1237)."""
+        self._closure(self.u0)
+        for L in self.les:
+            ub = L.cset.ubar
+            self.ctx.call("ec_ns_project", L.ps64.h, ptrs(ub, P3))
+            for a in range(3):
+                L.u0[a].copy_(ub[a])
+
+    def filtered_dns(self):
+        """P_H V u of the current DNS state for every LES grid (FP64 tensors)."""
+        self._closure(self.u0)
+        out = []
+        for L in self.les:
+            ub = [x.clone() for x in L.cset.ubar]
+            self.ctx.call("ec_ns_project", L.ps64.h, ptrs(ub, P3))
+            out.append(ub)
+        return out
+
+    def _closure(self, u, with_p=False):
+        sets = [L.cset for L in self.les]
+        if not sets:
+            return
+        p = self.ps.p_ptr
+        arr = (_abi.ClosureSet * len(sets))(*[s.struct() for s in sets])
+        self.ctx.call("ec_ns_closure", self.dt_code, self.g.n, self.g.l, self.visc, ptrs(u, P3),
+                      _abi.c_void_p(p), len(sets), arr)
+
+    # -- one RK4 step --------------------------------------------------------
+    def step(self, dt):
+        c = self.ctx
+        n, l = self.g.n, self.g.l
+        for st in range(4):
+            u = self.u0 if st == 0 else self.us
+            c.call("ec_ns_rhs", self.dt_code, n, l, self.visc, ptrs(u, P3), ptrs(self.du, P3), 0)
+            c.call("ec_ns_pressure_solve", self.ps.h, ptrs(self.du, P3))
+            if self.les:
+                self._closure(u)
+            for L in self.les:
+                ub = L.cset.ubar
+                c.call("ec_ns_project", L.ps64.h, ptrs(ub, P3))
+                tau = NS.commutator_(L.cset, ub, self.visc)
+                ul = L.u0 if st == 0 else L.us
+                gl = L.g
+                c.call("ec_ns_rhs", self.dt_code, gl.n, gl.l, self.visc, ptrs(ul, P3), ptrs(L.du, P3), 0)
+                c.call("ec_ns_tensordivergence_nonsym", self.dt_code, gl.n, gl.l, _abi.EC_F64,
+                       ptrs(tau, _abi.P9), ptrs(L.du, P3), 1)
+                c.call("ec_ns_pressure_solve", L.ps.h, ptrs(L.du, P3))
+                c.call("ec_ns_rk4_update", self.dt_code, gl.n, gl.l, st, dt, ptrs(L.u0, P3), ptrs(L.us, P3),
+                       ptrs(L.du, P3), _abi.c_void_p(L.ps.p_ptr), ptrs(L.acc, P3))
+            c.call("ec_ns_rk4_update", self.dt_code, n, l, st, dt, ptrs(self.u0, P3), ptrs(self.us, P3),
+                   ptrs(self.du, P3), _abi.c_void_p(self.ps.p_ptr), ptrs(self.acc, P3))
+
+    def run(self, dt, nsteps):
+        for _ in range(nsteps):
+            self.step(dt)
+
+    # -- diagnostics -------------------------------------------------------
+    def divergence_norm(self):
+        """h * ||div_h u||_2 / ||u||_2 of the DNS state, FP64 reductions."""
+        d = torch.empty_like(self.u0[0])
+        self.ctx.call("ec_ns_divergence", self.dt_code, self.g.n, self.g.l, ptrs(self.u0, P3), ptr(d))
+        num = NS.sumsq(d)
+        den = sum(NS.sumsq(x) for x in self.u0)
+        h = self.g.l / self.g.n
+        return h * (num ** 0.5) / (den ** 0.5)
+
+    def les_errors(self):
+        """compute_errors of every LES grid against P_H V u_dns."""
+        ref = self.filtered_dns()
+        out = []
+        for L, r in zip(self.les, ref):
+            s = 0.0
+            for a in range(3):
+                la = L.u0[a].double()
+                s += float(((la - r[a]) ** 2).sum() / (r[a] ** 2).sum())
+            out.append((s / 3) ** 0.5)
+        return out

@@ -1,0 +1,493 @@
+"""Drop-in mirror of the ``NavierStokes`` module of ExactClosure.jl (src/N.jl).
+
+Same function names and argument meaning as the reference; Julia's ``!``
+suffix becomes a trailing underscore (``project!`` -> ``project_``).  Every
+loop runs in libexactclosure on the GPU; fields are torch tensors used only
+as device memory.  There is no CPU fallback: without the library or a GPU
+the calls raise.
+
+Layout: a ``Field`` holds a contiguous tensor of shape (n, n, n) indexed
+[k, j, i] -- x fastest, identical in memory to a Julia ``Array{T,3}``.
+``Field.numpy()`` returns the Julia-ordered [i, j, k] view on the host.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _abi
+from ._abi import P3, P6, P9, ptr, ptrs
+
+# --------------------------------------------------------------------------
+# Grid and fields (src/N.jl:22-92)
+# --------------------------------------------------------------------------
+
+
+@dataclass(frozen=True)
+class Grid:
+    """Grid{D}(l, n) (src/N.jl:22-32); D = 3."""
+
+    l: float
+    n: int
+    D: int = 3
+
+
+def spacing(g):
+    return g.l / g.n
+
+
+def dim(g):
+    return g.D
+
+
+def wavenumber_int(g, i):
+    """src/N.jl:32 with the reference's 1-based index i."""
+    return i - 1 - (0 if i <= (g.n + 1) >> 1 else g.n)
+
+
+class Field:
+    """Scalar field on a grid (src/N.jl:35-43), device storage."""
+
+    def __init__(self, grid, data=None, dtype=None):
+        import torch
+
+        self.grid = grid
+        if data is None:
+            dtype = dtype or torch.float64
+            data = torch.zeros((grid.n,) * 3, dtype=dtype, device="cuda")
+        assert data.is_contiguous() and tuple(data.shape) == (grid.n,) * 3
+        self.data = data
+
+    @property
+    def dtype(self):
+        return self.data.dtype
+
+    def numpy(self):
+        """Host copy in Julia index order [i, j, k] (Fortran-ordered float64)."""
+        return np.asfortranarray(self.data.double().cpu().numpy().transpose(2, 1, 0))
+
+    @staticmethod
+    def from_numpy(grid, a, dtype=None):
+        """Field from a Julia-ordered [i, j, k] host array."""
+        import torch
+
+        t = torch.from_numpy(np.ascontiguousarray(np.asarray(a).transpose(2, 1, 0)))
+        return Field(grid, t.to(device="cuda", dtype=dtype or torch.float64).contiguous())
+
+    def __repr__(self):
+        return f"Field({self.grid}, {self.dtype})"
+
+
+def vectorfield(g, dtype=None):
+    return tuple(Field(g, dtype=dtype) for _ in range(3))
+
+
+def tensorfield(g, dtype=None):
+    return tuple(Field(g, dtype=dtype) for _ in range(6))
+
+
+def _dt(f):
+    return _abi.dtype_code(f.data)
+
+
+def _ctx():
+    return _abi.context(0)
+
+
+# --------------------------------------------------------------------------
+# Operators (src/N.jl:94-307)
+# --------------------------------------------------------------------------
+
+
+def stresstensor(u, visc):
+    """Materialised stresstensor(u, visc): (11, 22, 33, 12, 23, 31) (src/N.jl:94-118).
+
+    The reference returns lazy fields; tensordivergence_ fuses the same
+    evaluation without materialising (see rhs_)."""
+    g = u[0].grid
+    s = tuple(Field(g, dtype=u[0].dtype) for _ in range(6))
+    _ctx().call("ec_ns_stresstensor", _dt(u[0]), g.n, g.l, visc, ptrs([x.data for x in u], P3),
+                ptrs([x.data for x in s], P6))
+    return s
+
+
+def tensordivergence_(du, sigma, doadd):
+    """tensordivergence!(du, sigma, doadd) (src/N.jl:120-160)."""
+    g = du[0].grid
+    _ctx().call("ec_ns_tensordivergence", _dt(du[0]), g.n, g.l, ptrs([x.data for x in sigma], P6),
+                ptrs([x.data for x in du], P3), int(bool(doadd)))
+    return du
+
+
+def tensordivergence_nonsym_(du, sigma9, doadd):
+    """tensordivergence_nonsym!(du, sigma, doadd) (src/N.jl:162-200); sigma may be FP64."""
+    g = du[0].grid
+    sd = _abi.dtype_code(sigma9[0].data if isinstance(sigma9[0], Field) else sigma9[0])
+    data = [x.data if isinstance(x, Field) else x for x in sigma9]
+    _ctx().call("ec_ns_tensordivergence_nonsym", _dt(du[0]), g.n, g.l, sd, ptrs(data, P9),
+                ptrs([x.data for x in du], P3), int(bool(doadd)))
+    return du
+
+
+def rhs_(du, u, visc, doadd=False):
+    """tensordivergence!(du, stresstensor(u, visc), doadd) fused in one kernel."""
+    g = u[0].grid
+    _ctx().call("ec_ns_rhs", _dt(u[0]), g.n, g.l, visc, ptrs([x.data for x in u], P3),
+                ptrs([x.data for x in du], P3), int(bool(doadd)))
+    return du
+
+
+def divergence_(d, u):
+    """divergence!(d, u) (src/N.jl:222-238)."""
+    g = d.grid
+    _ctx().call("ec_ns_divergence", _dt(d), g.n, g.l, ptrs([x.data for x in u], P3), ptr(d.data))
+    return d
+
+
+def pressuregradient_(u, p):
+    """pressuregradient!(u, p) (src/N.jl:240-254)."""
+    g = p.grid
+    _ctx().call("ec_ns_pressuregradient", _dt(p), g.n, g.l, ptrs([x.data for x in u], P3), ptr(p.data))
+    return u
+
+
+class PoissonSolver:
+    """poissonsolver(grid, backend) (src/N.jl:202-220): FFT plan, spectrum and p."""
+
+    def __init__(self, grid, dtype=None):
+        import torch
+
+        dtype = dtype or torch.float64
+        self.grid = grid
+        self.torch_dtype = dtype
+        self.ctx = _ctx()
+        h = _abi.c_void_p()
+        code = _abi.EC_F64 if dtype == torch.float64 else _abi.EC_F32
+        self.ctx.call("ec_poisson_create", code, grid.n, grid.l, _abi.ctypes.byref(h))
+        self.h = h
+        n = grid.n
+        # p aliases the solver's own pressure buffer (poisson.p of the reference)
+        self._p_ptr = self.ctx.lib.ec_poisson_pressure(h)
+        self._ptensor = None
+
+    @property
+    def p(self):
+        """The solver's pressure as a Field (a copy synchronised with the stream)."""
+        import torch
+
+        n = self.grid.n
+        t = torch.empty((n, n, n), dtype=self.torch_dtype, device="cuda")
+        self.ctx.call("ec_d2d", _abi.c_void_p(t.data_ptr()), _abi.c_void_p(self._p_ptr),
+                      t.numel() * t.element_size())
+        return Field(self.grid, t)
+
+    @property
+    def p_ptr(self):
+        return self._p_ptr
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.ec_poisson_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def poissonsolver(grid, dtype=None):
+    return PoissonSolver(grid, dtype)
+
+
+def project_(u, poisson):
+    """project!(u, poisson) (src/N.jl:256-307); the pressure stays in poisson.p."""
+    _ctx().call("ec_ns_project", poisson.h, ptrs([x.data for x in u], P3))
+    return u
+
+
+def rfft(x, poisson):
+    """plan_rfft * x (unnormalised), shape (n, n, n/2+1) complex tensor [kz, ky, kx]."""
+    import torch
+
+    n = poisson.grid.n
+    cdt = torch.complex128 if x.dtype == torch.float64 else torch.complex64
+    out = torch.empty((n, n, n // 2 + 1), dtype=cdt, device="cuda")
+    _ctx().call("ec_fft_r2c", poisson.h, ptr(x), ptr(out))
+    return out
+
+
+def irfft(X, poisson):
+    """ldiv!(x, plan, X): normalised inverse."""
+    import torch
+
+    n = poisson.grid.n
+    rdt = torch.float64 if X.dtype == torch.complex128 else torch.float32
+    out = torch.empty((n, n, n), dtype=rdt, device="cuda")
+    _ctx().call("ec_fft_c2r", poisson.h, ptr(X), ptr(out))
+    return out
+
+
+def propose_timestep(u, visc, cfl):
+    """propose_timestep (src/N.jl:348-356), FP64 device reduction."""
+    g = u[0].grid
+    h = spacing(g)
+    umax = max(maxabs(x) for x in u)
+    return min(cfl * h / umax, cfl * h ** 2 / visc / (2 * 3))
+
+
+def maxabs(f):
+    out = _abi.c_double()
+    t = f.data if isinstance(f, Field) else f
+    _ctx().call("ec_reduce_maxabs", _abi.dtype_code(t), t.numel(), ptr(t), _abi.ctypes.byref(out))
+    return out.value
+
+
+def sumsq(f):
+    out = _abi.c_double()
+    t = f.data if isinstance(f, Field) else f
+    _ctx().call("ec_reduce_sumsq", _abi.dtype_code(t), t.numel(), ptr(t), _abi.ctypes.byref(out))
+    return out.value
+
+
+def diffsq(a, b):
+    out = _abi.c_double()
+    ta = a.data if isinstance(a, Field) else a
+    tb = b.data if isinstance(b, Field) else b
+    _ctx().call("ec_reduce_diffsq", _abi.dtype_code(ta), ta.numel(), ptr(ta), ptr(tb), _abi.ctypes.byref(out))
+    return out.value
+
+
+def peak_profile(k, kpeak):
+    """peak_profile (src/N.jl:366)."""
+    return k ** 4 * math.exp(-2 * (k / kpeak) ** 2)
+
+
+def randomfield(grid, poisson, profile="peak", param=5.0, totalenergy=0.5, seed=123, dtype=None):
+    """randomfield (src/N.jl:394-476) with a counter-based generator.
+
+    profile "peak": peak_profile(k; kpeak=param); "power": (k > 0) k^param."""
+    u = vectorfield(grid, dtype=dtype or poisson.torch_dtype)
+    pid = {"peak": 0, "power": 1}[profile]
+    _ctx().call("ec_ns_randomfield", poisson.h, pid, param, totalenergy, seed,
+                ptrs([x.data for x in u], P3))
+    return u
+
+
+def step_forwardeuler_(u, du, poisson, visc, dt):
+    """step_forwardeuler! (src/N.jl:309-318)."""
+    rhs_(du, u, visc, False)
+    for a, b in zip(u, du):
+        _axpy(dt, b, a)
+    return project_(u, poisson)
+
+
+def step_wray3_(u, du, u0, poisson, visc, dt):
+    """step_wray3! (src/N.jl:320-346)."""
+    a = (8 / 15, 5 / 12, 3 / 4)
+    b = (1 / 4, 0 / 1)
+    for x, x0 in zip(u, u0):
+        x0.data.copy_(x.data)
+    for i in range(3):
+        rhs_(du, u, visc, False)
+        for d, x, x0 in zip(du, u, u0):
+            if i != 0:
+                x.data.copy_(x0.data)
+            _axpy(a[i] * dt, d, x)
+        project_(u, poisson)
+        if i != 2:
+            for d, x0 in zip(du, u0):
+                _axpy(b[i] * dt, d, x0)
+    return u
+
+
+def _axpy(a, x, y):
+    _ctx().call("ec_axpy", _dt(y), y.data.numel(), a, ptr(x.data), ptr(y.data))
+
+
+# --------------------------------------------------------------------------
+# Filters (src/N.jl:592-698)
+# --------------------------------------------------------------------------
+
+
+def tophat(grid, compression):
+    """tophat (src/N.jl:592-597): odd compression, weights 1/c."""
+    assert compression % 2 == 1, "tophat needs odd compression (trapezoid_face covers even)"
+    return np.full(compression, 1.0 / compression)
+
+
+def trapezoid_face(compression):
+    """Face-centred volume-average weights for any compression: odd c is the
+    top-hat; even c spans c+1 faces with half weight at both ends."""
+    c = compression
+    if c % 2:
+        return np.full(c, 1.0 / c)
+    w = np.full(c + 1, 1.0 / c)
+    w[0] = w[-1] = 0.5 / c
+    return w
+
+
+def gaussian(grid, delta, nsigma):
+    """gaussian (src/N.jl:599-607); delta = 0 gives [1] (the reference's NaN
+    case, handled as src/Burgers.jl:149-150 does)."""
+    if delta < 1e-15:
+        return np.ones(1)
+    h = spacing(grid)
+    sigma = delta / math.sqrt(12)
+    r = int(np.rint(nsigma * sigma / h))
+    x = h * np.arange(-r, r + 1)
+    w = np.exp(-x ** 2 / (2 * sigma ** 2))
+    return w / w.sum()
+
+
+def composekernel(F, G):
+    """composekernel (src/N.jl:609-625): full discrete convolution."""
+    return np.convolve(np.asarray(F, dtype=np.float64), np.asarray(G, dtype=np.float64))
+
+
+class KernelProduct:
+    """kernelproduct(grid, kernels) (src/N.jl:627-637), kept separable."""
+
+    def __init__(self, kernels):
+        self.kernels = [np.ascontiguousarray(k, dtype=np.float64) for k in kernels]
+
+    def dense(self):
+        w1, w2, w3 = self.kernels
+        return w1[:, None, None] * w2[None, :, None] * w3[None, None, :]
+
+
+def kernelproduct(grid, kernels):
+    return KernelProduct(kernels)
+
+
+def _quantity(source, pieces, out):
+    q = _abi.FilterQuantity()
+    q.source = _abi.SRC[source]
+    keep = []
+    for d, (w, off) in enumerate(pieces):
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        keep.append(w)
+        q.len[d] = len(w)
+        q.off[d] = int(off)
+        q.w[d] = w.ctypes.data_as(_abi.ctypes.POINTER(_abi.c_double))
+    q.out = out.data_ptr()
+    return q, keep
+
+
+def filter_coarse(u, p, gbar, visc, quantities):
+    """Evaluate lazycoarsegrain(lazyfilter(src, kernel)) for several sources.
+
+    quantities: list of (source, [(w, off)] * 3) with offsets relative to the
+    coarse face index c*(J+1)-1.  Returns FP64 tensors (nbar, nbar, nbar)."""
+    import torch
+
+    g = u[0].grid
+    outs = []
+    qs = []
+    keep = []
+    for src, pieces in quantities:
+        o = torch.empty((gbar.n,) * 3, dtype=torch.float64, device="cuda")
+        q, k = _quantity(src, pieces, o)
+        outs.append(o)
+        qs.append(q)
+        keep.append(k)
+    arr = (_abi.FilterQuantity * len(qs))(*qs)
+    pp = ptr(p.data) if p is not None else None
+    _ctx().call("ec_filter_coarse", _dt(u[0]), g.n, gbar.n, g.l, visc, ptrs([x.data for x in u], P3),
+                pp, len(qs), arr)
+    return outs
+
+
+def coarse_offset(compression, stag, R):
+    """Offset of a (2R+1)-tap centred kernel at the coarse-grain point
+    c*J - !stag*(c/2) (coarsegrain!, src/N.jl:670-684), relative to c*(J+1)-1."""
+    return -R - (0 if stag else compression // 2)
+
+
+def applyfilter_coarsegrain(ubar, u, kernel, stag):
+    """coarsegrain!(ubar, lazyfilter(u, kernel), stag) fused (src/N.jl:655-684)."""
+    g = u.grid
+    c = g.n // ubar.grid.n
+    pieces = [(k, coarse_offset(c, s, len(k) // 2)) for k, s in zip(kernel.kernels, stag)]
+    dummy = (u, u, u)
+    (out,) = filter_coarse(dummy, None, ubar.grid, 0.0, [("u1", pieces)])
+    ubar.data.copy_(out.to(ubar.dtype))
+    return ubar
+
+
+def coarsegrain_(ubar, u, stag):
+    """coarsegrain!(ubar, u, stag) (src/N.jl:670-684): point decimation."""
+    one = KernelProduct([np.ones(1)] * 3)
+    return applyfilter_coarsegrain(ubar, u, one, stag)
+
+
+def get_kernels(g_dns, g_les, delta_ratio, nsigma):
+    """get_kernels (src/N.jl:1008-1020) as separable factors."""
+    comp = g_dns.n // g_les.n
+    FH = tophat(g_dns, comp)
+    FD = gaussian(g_dns, delta_ratio * spacing(g_les), nsigma)
+    FDH = composekernel(FH, FD)
+    GDH = KernelProduct([FDH] * 3)
+    GDHi = tuple(KernelProduct([FD if i == j else FDH for j in range(3)]) for i in range(3))
+    return GDH, GDHi
+
+
+# --------------------------------------------------------------------------
+# Exact closure (src/N.jl:999-1237)
+# --------------------------------------------------------------------------
+
+NONSYM = ["11", "21", "31", "12", "22", "32", "13", "23", "33"]
+
+
+class ClosureSet:
+    """One LES grid of the closure: kind 'va' (volume average) or 'fa' (face
+    average).  Holds the FP64 outputs ubar (3) and S r (9, nonsym order)."""
+
+    def __init__(self, g_dns, nbar, kind="va"):
+        import torch
+
+        self.g_les = Grid(g_dns.l, nbar)
+        self.kind = kind
+        self.ubar = [torch.empty((nbar,) * 3, dtype=torch.float64, device="cuda") for _ in range(3)]
+        self.flux = [torch.empty((nbar,) * 3, dtype=torch.float64, device="cuda") for _ in range(9)]
+        self.tau = [torch.empty((nbar,) * 3, dtype=torch.float64, device="cuda") for _ in range(9)]
+
+    def struct(self):
+        s = _abi.ClosureSet()
+        s.nbar = self.g_les.n
+        s.kind = 0 if self.kind == "va" else 1
+        for a in range(3):
+            s.ubar[a] = self.ubar[a].data_ptr()
+        for m in range(9):
+            s.flux[m] = self.flux[m].data_ptr()
+        return s
+
+
+def closure_(sets, u, p, visc):
+    """Filtered velocity V u and face-filtered fluxes S r, r = sigma(u) + p I
+    (get_r, get_rDHi, apply_DHpi: src/N.jl:1022-1073, 1228-1237), every set in
+    one call, FP64 outputs."""
+    g = u[0].grid
+    arr = (_abi.ClosureSet * len(sets))(*[s.struct() for s in sets])
+    pp = ptr(p.data) if isinstance(p, Field) else _abi.c_void_p(p)
+    _ctx().call("ec_ns_closure", _dt(u[0]), g.n, g.l, visc, ptrs([x.data for x in u], P3), pp, len(sets), arr)
+    return sets
+
+
+def commutator_(cset, ubar, visc):
+    """tau = S r - sigma_H(ubar) in FP64 (the closure of step_DHpi_classic_flux_div!,
+    src/N.jl:1189-1206, with sigma_les cancelling)."""
+    g = cset.g_les
+    _ctx().call("ec_ns_commutator", g.n, g.l, visc, ptrs(ubar, P3), ptrs(cset.flux, P9), ptrs(cset.tau, P9))
+    return cset.tau
+
+
+def compute_errors(ubar, uref):
+    """compute_errors (src/N.jl:1334-1349) for one LES field, FP64 reductions."""
+    s = 0.0
+    for a, b in zip(ubar, uref):
+        s += diffsq(a, b) / sumsq(b)
+    return math.sqrt(s / 3)

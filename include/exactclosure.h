@@ -1,0 +1,204 @@
+/*
+ * libexactclosure -- MI355X (gfx950) implementation of the DNS-aided LES hot
+ * path of ExactClosure.jl behind a plain C ABI.
+ *
+ * The reference has no FFI: its seam is Julia dispatch on the array type
+ * inside Field/LazyField (src/N.jl:35-59) with every loop behind
+ * myforeachindex (src/N.jl:16, src/Burgers.jl:21) and the FFT behind
+ * plan_rfft/mul!/ldiv! (src/N.jl:217,267,301).  Each entry point below names
+ * the reference function it replaces; a Julia `ccall` (or Python ctypes)
+ * binding passes device pointers from ec_alloc.
+ *
+ * Conventions
+ *  - Every function returns 0 (EC_OK) or a negative EC_E_* code; the message
+ *    is available from ec_last_error(ctx).  No C++ exception crosses the ABI.
+ *  - One context = one device + one HIP stream; calls are stream-ordered and
+ *    asynchronous, except reductions and ec_d2h which synchronise.  A context
+ *    must not be used from two host threads at once.
+ *  - Fields: n^3 values, x fastest (idx = i + n*(j + n*k)), periodic with no
+ *    ghost cells, one buffer per vector/tensor component (the layout of a
+ *    Julia Array{T,3}).  dtype is EC_F32 or EC_F64.
+ *  - Vector arguments (u[3], du[3], sigma[6|9]) are arrays of device pointers
+ *    on the host.  Closure outputs are always FP64.
+ *  - The caller owns field buffers; the library owns contexts, Poisson
+ *    solvers (FFT plans, spectra) and Burgers batches.
+ */
+#ifndef EXACTCLOSURE_H
+#define EXACTCLOSURE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EC_OK 0
+#define EC_E_INVALID (-1)
+#define EC_E_HIP (-2)
+#define EC_E_OOM (-3)
+#define EC_E_UNSUPPORTED (-4)
+
+#define EC_F32 0
+#define EC_F64 1
+
+typedef struct ec_ctx ec_ctx;
+typedef struct ec_poisson ec_poisson;
+typedef struct ec_burgers_batch ec_burgers_batch;
+
+/* ---- context (replaces defaultbackend(), src/N.jl:19, src/Burgers.jl:19) ---- */
+/* stream: the hipStream_t every call of this context is ordered on (NULL: the
+ * device's default stream). */
+int ec_create(ec_ctx **ctx, int device, void *stream);
+int ec_destroy(ec_ctx *ctx);
+const char *ec_last_error(ec_ctx *ctx);
+int ec_sync(ec_ctx *ctx);
+int ec_version(void);
+
+/* ---- memory (replaces KernelAbstractions.zeros/allocate and Adapt moves) ---- */
+int ec_alloc(ec_ctx *ctx, size_t bytes, void **ptr);
+int ec_free(ec_ctx *ctx, void *ptr);
+int ec_h2d(ec_ctx *ctx, void *dst, const void *src, size_t bytes);
+int ec_d2h(ec_ctx *ctx, void *dst, const void *src, size_t bytes);
+int ec_d2d(ec_ctx *ctx, void *dst, const void *src, size_t bytes);
+int ec_zero(ec_ctx *ctx, void *dst, size_t bytes);
+/* element-wise y = a*x + y (LinearAlgebra.axpy!, src/N.jl:337,1161,1322) */
+int ec_axpy(ec_ctx *ctx, int dtype, size_t count, double a, const void *x, void *y);
+/* dtype conversion copy (FP64 <-> FP32) */
+int ec_convert(ec_ctx *ctx, int dst_dtype, void *dst, int src_dtype, const void *src, size_t count);
+
+/* ---- per-launch timing with HIP events on the context stream ---- */
+int ec_profile_begin(ec_ctx *ctx);
+/* Writes up to cap records: name (static string), milliseconds, algorithmic bytes. */
+int ec_profile_end(ec_ctx *ctx, const char **names, double *ms, double *bytes, int cap, int *count);
+
+/* ---- Navier-Stokes operators (src/N.jl) ---- */
+/* stresstensor (src/N.jl:94-118) materialised: sigma[6] = 11,22,33,12,23,31 */
+int ec_ns_stresstensor(ec_ctx *ctx, int dtype, int n, double l, double visc,
+                       void *const u[3], void *const sigma[6]);
+/* tensordivergence! (src/N.jl:120-160) */
+int ec_ns_tensordivergence(ec_ctx *ctx, int dtype, int n, double l, void *const sigma[6],
+                           void *const du[3], int doadd);
+/* tensordivergence_nonsym! (src/N.jl:162-200); sigma9 order 11,21,31,12,22,32,13,23,33.
+ * sigma_dtype may differ from dtype (FP64 closure fluxes into FP32 fields). */
+int ec_ns_tensordivergence_nonsym(ec_ctx *ctx, int dtype, int n, double l, int sigma_dtype,
+                                  void *const sigma9[9], void *const du[3], int doadd);
+/* stresstensor + tensordivergence!(du, sigma, doadd) fused: du = doadd*du - div sigma(u)
+ * (src/N.jl:310-311, 1306-1307) */
+int ec_ns_rhs(ec_ctx *ctx, int dtype, int n, double l, double visc, void *const u[3],
+              void *const du[3], int doadd);
+/* divergence! (src/N.jl:222-238) */
+int ec_ns_divergence(ec_ctx *ctx, int dtype, int n, double l, void *const u[3], void *d);
+/* pressuregradient! (src/N.jl:240-254) */
+int ec_ns_pressuregradient(ec_ctx *ctx, int dtype, int n, double l, void *const u[3],
+                           const void *p);
+
+/* ---- FFT Poisson projection (poissonsolver / project!, src/N.jl:202-220,256-307) ---- */
+int ec_poisson_create(ec_ctx *ctx, int dtype, int n, double l, ec_poisson **ps);
+int ec_poisson_destroy(ec_ctx *ctx, ec_poisson *ps);
+/* The solver's pressure buffer (poisson.p), valid until destroy. */
+void *ec_poisson_pressure(ec_poisson *ps);
+/* project!(u, poisson): leaves the pressure in ec_poisson_pressure(ps) */
+int ec_ns_project(ec_ctx *ctx, ec_poisson *ps, void *const u[3]);
+/* The first half of project!: divergence! and the FFT Poisson solve, leaving p in
+ * ec_poisson_pressure(ps) and u unchanged (the gradient is then applied by
+ * ec_ns_pressuregradient or fused into ec_ns_rk4_update). */
+int ec_ns_pressure_solve(ec_ctx *ctx, ec_poisson *ps, void *const u[3]);
+/* plan_rfft * x (unnormalised) and ldiv! (normalised by 1/n^3); spectrum (n/2+1, n, n) */
+int ec_fft_r2c(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out);
+int ec_fft_c2r(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out);
+
+/* ---- reductions (maximum(abs, .), sum(abs2, .), dot; src/N.jl:352,1346).  FP64 result,
+ *      synchronising. ---- */
+int ec_reduce_maxabs(ec_ctx *ctx, int dtype, size_t count, const void *x, double *out);
+int ec_reduce_sumsq(ec_ctx *ctx, int dtype, size_t count, const void *x, double *out);
+int ec_reduce_diffsq(ec_ctx *ctx, int dtype, size_t count, const void *x, const void *y,
+                     double *out);
+
+/* ---- random initial field (randomfield, src/N.jl:394-476) ----
+ * profile 0: k^4 exp(-2 (k/param)^2) (peak_profile, src/N.jl:366);
+ * profile 1: (k > 0) k^param (src/N.jl:1245).  Counter-based normal variates. */
+int ec_ns_randomfield(ec_ctx *ctx, ec_poisson *ps, int profile, double param,
+                      double totalenergy, uint64_t seed, void *const u[3]);
+int ec_randn(ec_ctx *ctx, int dtype, size_t count, uint64_t seed, uint64_t offset, void *out);
+
+/* ---- separable filters and closure (src/N.jl:592-698, 999-1135) ----
+ * A quantity is one source evaluated on the fine grid, filtered with three 1D
+ * weight vectors and sampled at the coarse points:
+ *   out[J] = sum_t w1[t1] w2[t2] w3[t3] src[c*(J_d+1)-1 + off_d + t_d]
+ * Sources: EC_SRC_U1..U3 (velocity), EC_SRC_S11..S31 (stress sigma(u, visc)),
+ * EC_SRC_R11..R33 (sigma_aa + p), EC_SRC_P (pressure). */
+#define EC_SRC_U1 0
+#define EC_SRC_U2 1
+#define EC_SRC_U3 2
+#define EC_SRC_S11 3
+#define EC_SRC_S22 4
+#define EC_SRC_S33 5
+#define EC_SRC_S12 6
+#define EC_SRC_S23 7
+#define EC_SRC_S31 8
+#define EC_SRC_R11 9
+#define EC_SRC_R22 10
+#define EC_SRC_R33 11
+#define EC_SRC_P 12
+
+typedef struct {
+    int source;
+    int len[3];
+    int off[3];
+    const double *w[3]; /* host pointers, copied at call time */
+    void *out;          /* FP64 device array, nbar^3 */
+} ec_filter_quantity;
+
+/* lazycoarsegrain(lazyfilter(.)) of many quantities in one launch
+ * (get_rDHi :1038, get_sigDH :1088, apply_DHpi :1228) */
+int ec_filter_coarse(ec_ctx *ctx, int dtype, int n, int nbar, double l, double visc,
+                     void *const u[3], const void *p, int nq, const ec_filter_quantity *q);
+
+/* Volume-average (kind 0) or face-average (kind 1) closure of one or more LES
+ * grids from one DNS state, in FP64:
+ *   ubar[s][3]  filtered velocity V u (not yet projected)
+ *   flux[s][9]  S r with r = sigma(u) + p I, nonsym order 11,21,31,12,22,32,13,23,33
+ * Any compression c = n / nbar (odd: top-hat; even: trapezoid / two-point). */
+typedef struct {
+    int nbar;
+    int kind;
+    void *ubar[3];
+    void *flux[9];
+} ec_closure_set;
+int ec_ns_closure(ec_ctx *ctx, int dtype, int n, double l, double visc, void *const u[3],
+                  const void *p, int nsets, const ec_closure_set *sets);
+/* tau = flux - sigma_H(ubar) in FP64 on the LES grid (the closure commutator) */
+int ec_ns_commutator(ec_ctx *ctx, int nbar, double l, double visc, void *const ubar[3],
+                     void *const flux9[9], void *const tau9[9]);
+
+/* ---- RK4 stage update fused with the pressure gradient ----
+ * k = du - grad p.  stage 0..2: acc (= or +=) b_s k; us = u0 + a_{s+1} dt k.
+ * stage 3: u0 += dt/6 (acc + k).  (rk4_timestep! tableau, src/Burgers.jl:120-142,
+ * with the projection of step_wray3!, src/N.jl:320-346, at every stage.) */
+int ec_ns_rk4_update(ec_ctx *ctx, int dtype, int n, double l, int stage, double dt,
+                     void *const u0[3], void *const us[3], void *const du[3], const void *p,
+                     void *const acc[3]);
+
+/* ---- batched 1D Burgers DNS-aided LES (src/Burgers.jl:89-142,350-445) ----
+ * count independent instances, instance i: n[i] points, compression c[i],
+ * state FP32 or FP64, exact-closure commutator in FP64.  One persistent
+ * launch advances every instance nsteps RK4 steps (optionally replayed from a
+ * captured hipGraph). */
+int ec_burgers_batch_create(ec_ctx *ctx, int dtype, int count, const int *n, const int *c,
+                            double L, double visc, double cfl, int kmax, uint64_t seed,
+                            ec_burgers_batch **batch);
+int ec_burgers_batch_destroy(ec_ctx *ctx, ec_burgers_batch *batch);
+int ec_burgers_batch_set_threads(ec_ctx *ctx, ec_burgers_batch *batch, int threads);
+int ec_burgers_batch_run(ec_ctx *ctx, ec_burgers_batch *batch, int nsteps, int use_graph);
+/* which: 0 DNS, 1 exact-closure LES, 2 no-model LES; FP64 host output */
+int ec_burgers_batch_get(ec_ctx *ctx, ec_burgers_batch *batch, int instance, int which,
+                         double *out, int cap);
+int ec_burgers_batch_dt(ec_burgers_batch *batch, int instance, double *dt);
+/* Single-instance operators for parity tests (src/Burgers.jl:89-142). */
+int ec_burgers_rhs(ec_ctx *ctx, int dtype, int n, double L, double visc, const void *u, void *du);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

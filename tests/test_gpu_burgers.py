@@ -1,0 +1,87 @@
+"""GPU parity of the batched Burgers DNS-aided LES persistent kernel against the
+FP64 CPU checker (per instance), graph replay against eager launches, and the
+exactness of the swap-filter LES (src/Burgers.jl:1087-1089 identity)."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def _small_batch():
+    n = [64, 128, 96, 256, 48, 512, 60]
+    c = [4, 8, 3, 16, 6, 64, 5]
+    return n, c
+
+
+def test_burgers_rhs(gpu, O):
+    n, L, visc = 200, 2 * np.pi, 5e-4
+    u = O.burgers_ic(n, 3)
+    ud = torch.from_numpy(u).cuda()
+    du = torch.empty_like(ud)
+    gpu._abi.context().call("ec_burgers_rhs", 1, n, L, visc, ud.data_ptr(), du.data_ptr())
+    assert rel(du.cpu().numpy(), O.burgers_rhs(u, visc, L)) < 1e-12
+
+
+@pytest.mark.parametrize("threads", [128, 256, 1024])
+def test_batch_fp64_matches_checker(gpu, O, threads):
+    B = gpu.burgers
+    n, c = _small_batch()
+    visc, L, seed, steps = 5e-3, 2 * np.pi, 11, 40
+    b = B.BurgersBatch(n, c, visc=visc, L=L, cfl=0.3, seed=seed, dtype="f64")
+    b.set_threads(threads)
+    b.run(steps, use_graph=False)
+    for i in range(len(n)):
+        u0 = O.burgers_ic(n[i], seed + i)
+        dt = b.dt(i)
+        assert abs(dt - 0.3 * O.burgers_cfl(u0, visc, L)) < 1e-15 * dt
+        u, us, um = O.burgers_dnsaid_rk4(u0, c[i], visc, dt, steps, L)
+        assert rel(b.get(i, "dns"), u) < 1e-12
+        assert rel(b.get(i, "swapfil"), us) < 1e-11
+        assert rel(b.get(i, "nomodel"), um) < 1e-11
+        # exactness: the swap-filter LES is the filtered DNS
+        _, w = B.tophat_weights(c[i])
+        vu = O.burgers_filter_coarse(u, n[i] // c[i], w, -(c[i] // 2))
+        assert rel(b.get(i, "swapfil"), vu) < 1e-12
+
+
+def test_batch_fp32_and_graph_replay(gpu, O):
+    B = gpu.burgers
+    n, c = _small_batch()
+    visc, L, seed = 5e-4, 2 * np.pi, 3
+    eager = B.BurgersBatch(n, c, visc=visc, L=L, seed=seed, dtype="f32")
+    graph = B.BurgersBatch(n, c, visc=visc, L=L, seed=seed, dtype="f32")
+    for _ in range(3):
+        eager.run(25, use_graph=False)
+        graph.run(25, use_graph=True)  # captured once, replayed
+        torch.cuda.synchronize()
+        for i in range(len(n)):
+            for w in ("dns", "swapfil", "nomodel"):
+                assert np.array_equal(eager.get(i, w), graph.get(i, w))
+    for i in range(len(n)):
+        u0 = O.burgers_ic(n[i], seed + i)
+        u, us, um = O.burgers_dnsaid_rk4(u0, c[i], visc, eager.dt(i), 75, L)
+        assert rel(eager.get(i, "dns"), u) < 1e-4
+        assert rel(eager.get(i, "swapfil"), us) < 1e-4
+
+
+def test_baseline_batch_shape(gpu):
+    B = gpu.burgers
+    n, c = B.baseline_batch(4096)
+    assert len(n) == 4096 and int(n.sum()) == 1638 * 2048 + 1229 * 4096 + 819 * 8192 + 410 * 16384
+    b = B.BurgersBatch(n[:64], c[:64], dtype="f32")
+    b.run(2)
+    torch.cuda.synchronize()
+    assert np.isfinite(b.get(0, "dns")).all()
+
+
+def test_invalid_batch_rejected(gpu):
+    B = gpu.burgers
+    with pytest.raises(gpu._abi.ECError):
+        B.BurgersBatch([100], [3])  # 3 does not divide 100
+    with pytest.raises(gpu._abi.ECError):
+        B.BurgersBatch([32768], [4], dtype="f64")  # exceeds LDS

@@ -1,0 +1,218 @@
+"""GPU parity of the Navier-Stokes operators and the FFT projection against the
+FP64 CPU checker (same seeded inputs).  FP64 kernels must agree to ~1e-12
+relative; FP32 kernels to the FP32 tolerances written at each test."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [8, 12, 15, 16, 32]
+
+
+def dev(a, dtype=torch.float64):
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(a).transpose(2, 1, 0))).to("cuda", dtype).contiguous()
+
+
+def host(t):
+    return np.asfortranarray(t.double().cpu().numpy().transpose(2, 1, 0))
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def rand_u(O, n, seed):
+    rng = np.random.default_rng(seed)
+    return [np.asfortranarray(rng.standard_normal((n, n, n))) for _ in range(3)]
+
+
+def P(ts, k):
+    from ctypes import c_void_p
+    return (c_void_p * k)(*[t.data_ptr() for t in ts])
+
+
+@pytest.mark.parametrize("n", SIZES)
+def test_rhs_fp64(gpu, O, n):
+    u = rand_u(O, n, n)
+    visc, l = 0.013, 1.0
+    ref = O.rhs(u, visc, l)
+    ud = [dev(x) for x in u]
+    du = [torch.empty_like(x) for x in ud]
+    ctx = gpu._abi.context()
+    ctx.call("ec_ns_rhs", 1, n, l, visc, P(ud, 3), P(du, 3), 0)
+    torch.cuda.synchronize()
+    for a in range(3):
+        assert rel(host(du[a]), ref[a]) < 1e-12
+    # doadd: du = du - div sigma
+    ctx.call("ec_ns_rhs", 1, n, l, visc, P(ud, 3), P(du, 3), 1)
+    for a in range(3):
+        assert rel(host(du[a]), 2 * ref[a]) < 1e-12
+
+
+@pytest.mark.parametrize("n", [16, 32])
+def test_rhs_fp32(gpu, O, n):
+    u = rand_u(O, n, 100 + n)
+    visc, l = 0.01, 1.0
+    ref = O.rhs(u, visc, l)
+    ud = [dev(x, torch.float32) for x in u]
+    du = [torch.empty_like(x) for x in ud]
+    gpu._abi.context().call("ec_ns_rhs", 0, n, l, visc, P(ud, 3), P(du, 3), 0)
+    for a in range(3):
+        # FP32 rounding of inputs and a ~40-term stencil: 1e-5 relative to the field max
+        assert rel(host(du[a]), ref[a]) < 2e-5
+
+
+@pytest.mark.parametrize("n", [8, 12])
+def test_stress_and_tensordivergence(gpu, O, n):
+    u = rand_u(O, n, 7 * n)
+    visc = 0.02
+    s_ref = O.stresstensor(u, visc)
+    ud = [dev(x) for x in u]
+    s = [torch.empty_like(ud[0]) for _ in range(6)]
+    ctx = gpu._abi.context()
+    ctx.call("ec_ns_stresstensor", 1, n, 1.0, visc, P(ud, 3), P(s, 6))
+    for a in range(6):
+        assert rel(host(s[a]), s_ref[a]) < 1e-13
+    du = [torch.empty_like(ud[0]) for _ in range(3)]
+    ctx.call("ec_ns_tensordivergence", 1, n, 1.0, P(s, 6), P(du, 3), 0)
+    ref = O.tensordivergence(s_ref)
+    for a in range(3):
+        assert rel(host(du[a]), ref[a]) < 1e-12
+    # non-symmetric with FP64 sigma into FP32 and FP64 du
+    rng = np.random.default_rng(n)
+    s9 = [np.asfortranarray(rng.standard_normal((n, n, n))) for _ in range(9)]
+    ref9 = O.tensordivergence_nonsym(s9, du=[x.copy(order="F") for x in ref], doadd=True)
+    s9d = [dev(x) for x in s9]
+    ctx.call("ec_ns_tensordivergence_nonsym", 1, n, 1.0, 1, P(s9d, 9), P(du, 3), 1)
+    for a in range(3):
+        assert rel(host(du[a]), ref9[a]) < 1e-12
+    du32 = [dev(x, torch.float32) for x in ref]
+    ctx.call("ec_ns_tensordivergence_nonsym", 0, n, 1.0, 1, P(s9d, 9), P(du32, 3), 1)
+    for a in range(3):
+        assert rel(host(du32[a]), ref9[a]) < 1e-5
+
+
+@pytest.mark.parametrize("n", SIZES)
+def test_divergence_gradient(gpu, O, n):
+    u = rand_u(O, n, 3 * n)
+    ud = [dev(x) for x in u]
+    d = torch.empty_like(ud[0])
+    ctx = gpu._abi.context()
+    ctx.call("ec_ns_divergence", 1, n, 1.0, P(ud, 3), d.data_ptr())
+    assert rel(host(d), O.divergence(u)) < 1e-13
+    p = np.asfortranarray(np.random.default_rng(n).standard_normal((n, n, n)))
+    ref = O.pressuregradient([x.copy(order="F") for x in u], p)
+    ctx.call("ec_ns_pressuregradient", 1, n, 1.0, P(ud, 3), dev(p).data_ptr())
+    for a in range(3):
+        assert rel(host(ud[a]), ref[a]) < 1e-13
+
+
+@pytest.mark.parametrize("n", SIZES + [48, 64])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_fft_roundtrip_and_spectrum(gpu, O, n, prec):
+    NS = gpu.navierstokes
+    dt = torch.float64 if prec == "f64" else torch.float32
+    g = NS.Grid(1.0, n)
+    ps = NS.PoissonSolver(g, dt)
+    x = np.asfortranarray(np.random.default_rng(n).standard_normal((n, n, n)))
+    X = NS.rfft(dev(x, dt), ps)
+    ref = np.fft.rfftn(x, axes=(2, 1, 0))  # [kx, ky, kz]
+    got = X.cpu().numpy().transpose(2, 1, 0)
+    tol = 1e-12 if prec == "f64" else 3e-6
+    assert np.abs(got - ref).max() / np.abs(ref).max() < tol
+    back = NS.irfft(X, ps)
+    assert rel(host(back), x) < tol
+
+
+@pytest.mark.parametrize("n", SIZES + [64])
+def test_project_fp64(gpu, O, n):
+    NS = gpu.navierstokes
+    g = NS.Grid(1.0, n)
+    ps = NS.PoissonSolver(g, torch.float64)
+    u = rand_u(O, n, 11 * n)
+    uref = [x.copy(order="F") for x in u]
+    pref = O.project(uref)
+    ud = [NS.Field(g, dev(x)) for x in u]
+    NS.project_(ud, ps)
+    for a in range(3):
+        assert rel(ud[a].numpy(), uref[a]) < 1e-12
+    assert rel(ps.p.numpy(), pref) < 1e-11
+    d = O.divergence([x.numpy() for x in ud])
+    assert np.abs(d).max() < 1e-10 * n
+
+
+@pytest.mark.parametrize("n", [32, 64])
+def test_project_fp32_divergence(gpu, O, n):
+    NS = gpu.navierstokes
+    g = NS.Grid(1.0, n)
+    ps = NS.PoissonSolver(g, torch.float32)
+    u = rand_u(O, n, 5 * n)
+    ud = [NS.Field(g, dev(x, torch.float32)) for x in u]
+    NS.project_(ud, ps)
+    uh = [x.numpy() for x in ud]
+    d = O.divergence(uh)
+    h = 1.0 / n
+    norm = h * np.sqrt((d ** 2).sum()) / np.sqrt(sum((x ** 2).sum() for x in uh))
+    assert norm < 1e-6  # the FP32 divergence gate of the workload
+    uref = [x.copy(order="F") for x in u]
+    O.project(uref)
+    for a in range(3):
+        assert rel(uh[a], uref[a]) < 1e-5
+
+
+@pytest.mark.parametrize("n", [8, 16, 32])
+def test_randomfield(gpu, O, n):
+    NS = gpu.navierstokes
+    g = NS.Grid(1.0, n)
+    ps = NS.PoissonSolver(g, torch.float64)
+    u = NS.randomfield(g, ps, "peak", 5.0, 0.5, seed=123)
+    ref = O.randomfield(n, seed=123)
+    for a in range(3):
+        assert rel(u[a].numpy(), ref[a]) < 1e-10
+    e = sum(NS.sumsq(x) for x in u) / 2 / n ** 3
+    assert abs(e - 0.5) < 1e-6
+
+
+def test_reductions(gpu):
+    NS = gpu.navierstokes
+    x = torch.randn(1000003, dtype=torch.float64, device="cuda")
+    y = torch.randn(1000003, dtype=torch.float64, device="cuda")
+    assert abs(NS.maxabs(x) - x.abs().max().item()) == 0
+    assert abs(NS.sumsq(x) - (x * x).sum().item()) < 1e-9 * (x * x).sum().item()
+    assert abs(NS.diffsq(x, y) - ((x - y) ** 2).sum().item()) < 1e-9 * ((x - y) ** 2).sum().item()
+    assert NS.sumsq(x[:0]) == 0.0
+    xf = x.float()
+    assert abs(NS.maxabs(xf) - xf.abs().max().item()) == 0
+
+
+@pytest.mark.parametrize("stage", [0, 1, 2, 3])
+def test_rk4_update(gpu, O, stage):
+    n, dt = 12, 0.01
+    rng = np.random.default_rng(stage)
+    f = lambda: np.asfortranarray(rng.standard_normal((n, n, n)))
+    u0, us, du, acc = [f() for _ in range(3)], [f() for _ in range(3)], [f() for _ in range(3)], [f() for _ in range(3)]
+    p = f()
+    k = O.pressuregradient([x.copy(order="F") for x in du], p)
+    b = (1, 2, 2, 1)[stage]
+    a = (0.5, 0.5, 1.0)[stage] if stage < 3 else None
+    T = [[dev(x) for x in v] for v in (u0, us, du, acc)]
+    gpu._abi.context().call("ec_ns_rk4_update", 1, n, 1.0, stage, dt, P(T[0], 3), P(T[1], 3), P(T[2], 3),
+                            dev(p).data_ptr(), P(T[3], 3))
+    for c in range(3):
+        acc_new = k[c] if stage == 0 else acc[c] + b * k[c]
+        if stage < 3:
+            assert rel(host(T[3][c]), acc_new) < 1e-13
+            assert rel(host(T[1][c]), u0[c] + a * dt * k[c]) < 1e-13
+        else:
+            assert rel(host(T[0][c]), u0[c] + dt / 6 * acc_new) < 1e-13
+
+
+def test_invalid_arguments_raise(gpu):
+    NS = gpu.navierstokes
+    ctx = gpu._abi.context()
+    with pytest.raises(gpu._abi.ECError):
+        NS.PoissonSolver(NS.Grid(1.0, 7 * 11), torch.float64)  # factor 7 unsupported
+    with pytest.raises(gpu._abi.ECError):
+        P3 = gpu._abi.P3
+        ctx.call("ec_ns_rk4_update", 1, 8, 1.0, 5, 0.1, P3(), P3(), P3(), None, P3())

@@ -1,0 +1,296 @@
+"""Pins of the FP64 CPU checker (oracle/) by what the mathematics fixes.
+
+The reference holds no golden vectors and cannot run here (Julia is absent),
+so every oracle function is pinned by checks chosen so a plausible mistake
+(dropped term, wrong sign or index, transposed operand) fails one of them:
+
+  FFT          numpy.fft (rfftn / fft) and a brute-force DFT
+  stress/RHS   viscous part against the spectral Laplacian eigenvalues
+               (-4/h^2 sum sin^2(pi k/n)); inviscid energy conservation of the
+               divergence-form convection for div-free u; momentum
+               conservation; second-order convergence to the continuous
+               Taylor-Green right-hand side
+  projection   div_h(P u) = 0; P idempotent; P(grad phi) = 0; eigenvalue
+               closed form for one Fourier mode
+  filters      constants preserved; box transfer function on a Fourier mode;
+               the commutation identity V(-div_h r) = -div_H^ns (S r) for a
+               random tensor r (which fixes every offset of S)
+  closure      exactness of the classic+flux+div LES (navierstokes.jl:33:
+               5.7e-15) under forward Euler and RK4, odd and even c, with and
+               without the Gaussian; error ordering of the five variants
+               (navierstokes.jl:30-34)
+  randomfield  divergence-free, total energy, shell spectrum follows the
+               profile (src/N.jl:441-449)
+  Burgers      RK4 4th-order convergence; swap-filter exactness; the
+               identity classic + flux + div = swap (src/Burgers.jl:1087-1089)
+"""
+import math
+
+import numpy as np
+import pytest
+
+
+def rnd(n, seed, k=3):
+    rng = np.random.default_rng(seed)
+    return [np.asfortranarray(rng.standard_normal((n, n, n))) for _ in range(k)]
+
+
+# ---------------------------------------------------------------- FFT
+@pytest.mark.parametrize("n", [4, 6, 8, 9, 10, 12, 15, 16, 20, 24])
+def test_fft_matches_numpy(O, n):
+    x = rnd(n, n, 1)[0]
+    X = O.rfft3(x)
+    ref = np.fft.rfftn(x, axes=(2, 1, 0))
+    assert np.abs(X - ref).max() < 1e-12 * np.abs(ref).max()
+    assert np.abs(O.irfft3(X) - x).max() < 1e-13
+    z = np.random.default_rng(n).standard_normal(n) + 1j * np.random.default_rng(n + 1).standard_normal(n)
+    assert np.abs(O.fft1d(z) - np.fft.fft(z)).max() < 1e-12 * n
+    assert np.abs(O.fft1d(z, +1) - n * np.fft.ifft(z)).max() < 1e-12 * n
+
+
+def test_fft_brute_force_dft(O):
+    n = 5
+    x = rnd(n, 1, 1)[0]
+    X = O.rfft3(x)
+    idx = np.arange(n)
+    for kx in range(n // 2 + 1):
+        for ky in range(n):
+            for kz in (0, 3):
+                ph = np.exp(-2j * np.pi * (kx * idx[:, None, None] + ky * idx[None, :, None] + kz * idx[None, None, :]) / n)
+                assert abs(X[kx, ky, kz] - (x * ph).sum()) < 1e-12
+
+
+# ---------------------------------------------------------------- RHS
+def spectral_laplacian(f):
+    n = f.shape[0]
+    F = np.fft.fftn(f)
+    s = np.sin(np.pi * np.fft.fftfreq(n) * n / n) ** 2
+    lam = -4 * n ** 2 * (s[:, None, None] + s[None, :, None] + s[None, None, :])
+    return np.real(np.fft.ifftn(F * lam))
+
+
+def test_rhs_viscous_part_is_laplacian(O):
+    n, visc, eps = 12, 0.7, 1e-7
+    u = rnd(n, 2)
+    O.project(u)
+    du = O.rhs([eps * x for x in u], visc)
+    for a in range(3):
+        lap = spectral_laplacian(u[a])
+        # du = visc * lap(u) + O(eps^2) convection for a div-free field
+        assert np.abs(du[a] / eps - visc * lap).max() < 1e-5 * np.abs(visc * lap).max()
+
+
+def test_rhs_inviscid_energy_conservation(O):
+    n = 16
+    u = O.randomfield(n, seed=9)
+    du = O.rhs(u, 0.0)
+    e = sum(float((a * b).sum()) for a, b in zip(u, du))
+    scale = sum(float(np.abs(a * b).sum()) for a, b in zip(u, du))
+    assert abs(e) < 1e-13 * scale
+    # and momentum: sum of each component vanishes (periodic flux form)
+    for a in range(3):
+        assert abs(du[a].sum()) < 1e-12 * np.abs(du[a]).sum()
+    # a wrong stencil loses the invariant: a shifted stress breaks it
+    s = O.stresstensor(u, 0.0)
+    s_bad = list(s)
+    s_bad[3] = np.roll(s[3], 1, axis=0)
+    du_bad = O.tensordivergence(s_bad)
+    e_bad = sum(float((a * b).sum()) for a, b in zip(u, du_bad))
+    assert abs(e_bad) > 1e-6 * scale
+
+
+def tg_rhs_error(O, n, visc):
+    h = 1.0 / n
+    x = np.arange(n) * h
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    c = h / 2
+    # Taylor-Green velocity sampled at the face centres
+    u1 = np.sin(2 * np.pi * (X + h)) * np.cos(2 * np.pi * (Y + c)) * np.cos(2 * np.pi * (Z + c))
+    u2 = -np.cos(2 * np.pi * (X + c)) * np.sin(2 * np.pi * (Y + h)) * np.cos(2 * np.pi * (Z + c))
+    u3 = 0 * u1
+    du = O.rhs([np.asfortranarray(u1), np.asfortranarray(u2), np.asfortranarray(u3)], visc)
+    def exact(px, py, pz):
+        s, co = np.sin, np.cos
+        k = 2 * np.pi
+        U = s(k * px) * co(k * py) * co(k * pz)
+        V = -co(k * px) * s(k * py) * co(k * pz)
+        Ux = k * co(k * px) * co(k * py) * co(k * pz); Uy = -k * s(k * px) * s(k * py) * co(k * pz)
+        Vx = k * s(k * px) * s(k * py) * co(k * pz); Vy = -k * co(k * px) * co(k * py) * co(k * pz)
+        lapU, lapV = -3 * k * k * U, -3 * k * k * V
+        # -div(u u) + visc lap u for the div-free TG field: -(u.grad)u + visc lap u
+        return -(U * Ux + V * Uy) + visc * lapU, -(U * Vx + V * Vy) + visc * lapV
+    e1, _ = exact(X + h, Y + c, Z + c)
+    _, e2 = exact(X + c, Y + h, Z + c)
+    return max(np.abs(du[0] - e1).max(), np.abs(du[1] - e2).max())
+
+
+def test_rhs_second_order_taylor_green(O):
+    e = [tg_rhs_error(O, n, 0.01) for n in (16, 32, 64)]
+    r1, r2 = e[0] / e[1], e[1] / e[2]
+    assert 3.5 < r1 < 4.5 and 3.7 < r2 < 4.3, e
+
+
+# ---------------------------------------------------------------- projection
+@pytest.mark.parametrize("n", [8, 9, 16])
+def test_projection_properties(O, n):
+    u = rnd(n, 3 * n)
+    O.project(u)
+    assert np.abs(O.divergence(u)).max() < 1e-11 * n
+    v = [x.copy(order="F") for x in u]
+    O.project(v)
+    assert max(np.abs(a - b).max() for a, b in zip(u, v)) < 1e-13
+    phi = rnd(n, 7, 1)[0]
+    g = [np.asfortranarray(np.zeros_like(phi)) for _ in range(3)]
+    O.pressuregradient(g, -phi)  # g = +grad phi (forward differences)
+    O.project(g)
+    assert max(np.abs(x).max() for x in g) < 1e-11 * n
+
+
+def test_projection_single_mode_pressure(O):
+    n = 16
+    h = 1.0 / n
+    i = np.arange(n)
+    k = 3
+    phi = np.asfortranarray(np.cos(2 * np.pi * k * i / n)[:, None, None] * np.ones((n, n, n)))
+    u = [np.asfortranarray(np.zeros((n, n, n))) for _ in range(3)]
+    O.pressuregradient(u, -phi)  # u = grad phi
+    p = O.project(u)
+    # L p = div u = L phi  =>  p = phi (zero-mean part)
+    assert np.abs(p - phi).max() < 1e-12
+
+
+# ---------------------------------------------------------------- filters
+@pytest.mark.parametrize("c", [2, 3, 4, 5])
+def test_filters_constants_and_transfer(O, c):
+    n = 6 * c
+    for name, (src, fs) in O.closure_filters(c, "va").items():
+        assert all(abs(w.sum() - 1) < 1e-15 for w, _ in fs), name
+    # box of cells on a Fourier mode along y: transfer sin(pi k c/n)/(c sin(pi k/n))
+    k = 2
+    j = np.arange(n)
+    f = np.asfortranarray(np.ones((n, n, n)) * np.cos(2 * np.pi * k * (j + 0.5) / n)[None, :, None])
+    out = O.filter_coarse(f, n // c, O._point(), O._box(c), O._point())
+    J = np.arange(n // c)
+    centre = (c * J + c / 2) / n
+    expect = np.sin(np.pi * k * c / n) / (c * np.sin(np.pi * k / n)) * np.cos(2 * np.pi * k * centre)
+    assert np.abs(out[0, :, 0] - expect).max() < 1e-13
+
+
+@pytest.mark.parametrize("c", [2, 3, 4, 5])
+def test_commutation_identity(O, c):
+    """V(-div_h r) = -div_H^ns (S r) to rounding for a random symmetric tensor r."""
+    n = 5 * c if c % 2 else 4 * c
+    nb = n // c
+    rng = np.random.default_rng(c)
+    s = [np.asfortranarray(rng.standard_normal((n, n, n))) for _ in range(6)]  # 11,22,33,12,23,31
+    p = np.asfortranarray(rng.standard_normal((n, n, n)))
+    rfull = [s[0] + p, s[1] + p, s[2] + p, s[3], s[4], s[5]]
+    du = O.tensordivergence(rfull)
+    F = O.closure_filters(c, "va")
+    V = [O.filter_coarse(du[a], nb, *F[f"ubar{a + 1}"][1]) for a in range(3)]
+    src = {"r11": rfull[0], "r22": rfull[1], "r33": rfull[2], "s12": s[3], "s23": s[4], "s31": s[5]}
+    Sr = [O.filter_coarse(src[F["rDHi" + ab][0]], nb, *F["rDHi" + ab][1]) for ab in O.NONSYM_ORDER]
+    rhs = O.tensordivergence_nonsym(Sr)
+    for a in range(3):
+        assert np.abs(V[a] - rhs[a]).max() < 1e-12 * np.abs(V[a]).max()
+
+
+# ---------------------------------------------------------------- closure exactness
+@pytest.mark.parametrize("n,nl,dr", [(15, 5, 0.0), (15, 5, 2.0), (16, 4, 0.0), (24, 8, 2.0)])
+def test_exactness_forward_euler(O, n, nl, dr):
+    u = O.randomfield(n, seed=2, profile="power", param=-5 / 3, totalenergy=1.0)
+    les = O.dnsaid_project_fe(u, nl, 2e-3, 4, 1e-3, delta_ratio=dr)
+    assert O.compute_errors(les["u_cfd"], les["u_les"]) < 1e-13
+
+
+def test_error_ordering_odd_c(O):
+    """Same ordering as the reference's pins (navierstokes.jl:30-34):
+    no-model > classic > classic+flux > symmetrised > exact ~ 1e-15."""
+    n, nl = 30, 6
+    u = O.randomfield(n, seed=0, profile="power", param=-5 / 3, totalenergy=1.0)
+    les = O.dnsaid_project_fe(u, nl, 2e-4, 8, 2e-3, delta_ratio=2.0)
+    e = {k: O.compute_errors(les[k], les["u_les"]) for k in ("u_nomo", "u_c", "u_cf", "u_cfd_symm", "u_cfd")}
+    assert e["u_nomo"] > e["u_c"] > e["u_cf"] > e["u_cfd_symm"] > 1e3 * e["u_cfd"]
+    assert e["u_cfd"] < 1e-13
+
+
+@pytest.mark.parametrize("sets", [[(4, "va")], [(3, "va")], [(2, "va"), (4, "va")]])
+def test_exactness_rk4(O, sets):
+    n = 12
+    u = O.randomfield(n, seed=4)
+    u2, les, _ = O.dnsaid_rk4([x.copy(order="F") for x in u], sets, 1e-3, 2e-3, 3)
+    for (c, kind), l in zip(sets, les):
+        ref = [O.filter_coarse(u2[a], n // c, *O.closure_filters(c, kind)[f"ubar{a + 1}"][1]) for a in range(3)]
+        O.project(ref)
+        assert O.compute_errors(l, ref) < 1e-13
+
+
+# ---------------------------------------------------------------- randomfield
+def test_randomfield_spectrum(O):
+    n = 32
+    u = O.randomfield(n, seed=123, profile="peak", param=5.0, totalenergy=0.5)
+    assert np.abs(O.divergence(u)).max() < 1e-10
+    E = sum((x ** 2).sum() for x in u) / 2 / n ** 3
+    assert abs(E - 0.5) < 1e-3
+    # shell energies follow the profile k^4 exp(-2 (k/5)^2)
+    spec = np.zeros(n)
+    kk = np.fft.fftfreq(n) * n
+    K = np.sqrt(kk[:, None, None] ** 2 + kk[None, :, None] ** 2 + kk[None, None, :] ** 2)
+    for x in u:
+        X = np.fft.fftn(x) / n ** 3
+        np.add.at(spec, np.floor(K).astype(int).ravel() % n, (np.abs(X) ** 2 / 2).ravel())
+    kd = int(np.floor(np.sqrt(3) * n / 2))
+    prof = np.array([k ** 4 * np.exp(-2 * (k / 5) ** 2) for k in range(kd + 1)])
+    target = 0.5 * prof / prof.sum()
+    for k in range(1, 12):
+        assert abs(spec[k] - target[k]) < 0.02 * target[k] + 1e-6, (k, spec[k], target[k])
+
+
+def test_randn_moments(O):
+    x = O.randn(400000, 5)
+    assert abs(x.mean()) < 5e-3 and abs(x.std() - 1) < 5e-3
+    assert np.array_equal(O.randn(10, 5, 7), x[7:17])
+
+
+# ---------------------------------------------------------------- Burgers
+def test_burgers_rk4_fourth_order(O):
+    n, visc, L = 256, 0.05, 2 * np.pi
+    u0 = O.burgers_ic(n, 1, kmax=3)
+    T = 0.2
+    ref = O.burgers_rk4(u0, visc, T / 640, 640, L)
+    errs = [np.abs(O.burgers_rk4(u0, visc, T / m, m, L) - ref).max() for m in (20, 40, 80)]
+    assert 14 < errs[0] / errs[1] < 18 and 14 < errs[1] / errs[2] < 18, errs
+
+
+@pytest.mark.parametrize("c", [3, 4, 5, 8, 64])
+def test_burgers_swap_exactness_and_identity(O, c):
+    n, visc, L = 64 * c if c < 64 else 4096, 5e-4, 2 * np.pi
+    u0 = O.burgers_ic(n, 2)
+    dt = 0.3 * O.burgers_cfl(u0, visc, L)
+    u, us, um = O.burgers_dnsaid_rk4(u0, c, visc, dt, 20, L)
+    w = np.full(c + (c % 2 == 0), 1.0 / c)
+    if c % 2 == 0:
+        w[0] = w[-1] = 0.5 / c
+    vu = O.burgers_filter_coarse(u, n // c, w, -(c // 2))
+    assert np.abs(us - vu).max() < 1e-13 * np.abs(vu).max()
+    assert np.abs(um - vu).max() > 1e-6
+
+
+@pytest.mark.parametrize("c", [2, 3, 4, 7, 8])
+def test_burgers_commutation(O, c):
+    """V(delta_coll s) = delta_coll_H(Pc s) for any s: the identity that makes the
+    swap-filter closure exact (src/Burgers.jl:1087-1089 decomposition)."""
+    n, L = 40 * c, 2 * np.pi
+    h, H = L / n, L * c / n
+    s = np.random.default_rng(c).standard_normal(n)
+    ds = (np.roll(s, -1) - s) / h
+    if c % 2:
+        wv, wp = np.full(c, 1.0 / c), np.ones(1)
+    else:
+        wv = np.full(c + 1, 1.0 / c)
+        wv[0] = wv[-1] = 0.5 / c
+        wp = np.array([0.5, 0.5])
+    lhs = O.burgers_filter_coarse(ds, n // c, wv, -(c // 2))
+    ps = O.burgers_filter_coarse(s, n // c, wp, -(c // 2))
+    rhs = (np.roll(ps, -1) - ps) / H
+    assert np.abs(lhs - rhs).max() < 1e-12 * np.abs(lhs).max()
