@@ -1,0 +1,210 @@
+"""DNS-aided LES benchmark on one MI355X.
+
+Workload: 3D incompressible Navier-Stokes DNS at 1024^3 (FP32 fields,
+viscosity 1/8000), classical RK4, exact closure evaluated at every stage for
+four LES filter sets (volume average at 64^3, 128^3, 256^3 and face average at
+256^3), each LES grid advanced with its closure.  Closure commutators and
+energy reductions run in FP64.  Initial field: random, spectrum
+k^4 exp(-2 (k/5)^2), energy 1/2, seed 123, projected divergence-free.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 1024] [--dtype f32]
+
+Prints one JSON line.  value = DNS-aided LES RK4 steps per second.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+
+METRIC = "DNS-aided LES steps/s + HBM GB/s at 1024³ FP32; closure rel-err vs FP64"
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1024)
+    ap.add_argument("--dtype", default="f32", choices=["f32", "f64"])
+    ap.add_argument("--sets", default="16:va,8:va,4:va,4:fa", help="compression:kind list")
+    ap.add_argument("--visc", type=float, default=1.0 / 8000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-closure-check", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=256)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a plain 1 GiB device copy first (HBM counter calibration)")
+    ap.add_argument("--kernels", action="store_true", help="print the per-kernel table to stderr")
+    ap.add_argument("--closure-tiles", default=None, help="tx,ty,tz block targets of the closure kernel")
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(n=256, nsteps=1):
+    """The FP64 C oracle (OpenMP, all host threads) on the 256^3 DNS-aided RK4
+    step with the 32^3 volume-average closure; steps/s scaled to 1024^3 by N."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+
+    visc, dt = 1.0 / 2000, 1e-3
+    u = O.randomfield(n, seed=123)
+    t0 = time.perf_counter()
+    O.dnsaid_rk4(u, [(8, "va")], visc, dt, nsteps)
+    t = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    sps = nsteps / t
+    return {"value": sps * (n / 1024) ** 3, "unit": "steps/s (1024^3-equivalent)", "cores": threads,
+            "kind": "port",
+            "sample": f"{nsteps} RK4 DNS-aided step at {n}^3 FP64 with the 32^3 closure: {t:.2f} s "
+                      f"({sps:.4f} steps/s at {n}^3), scaled by (n/1024)^3"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); the multi-GPU sharding is "
+              "described in DESIGN.md and not part of this build", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    sys.path.insert(0, ROOT)
+    import __graft_entry__ as ge
+
+    ec = ge.load_package()
+    NS, D, abi = ec.navierstokes, ec.dnsaid, ec._abi
+    n = a.n
+    dtype = torch.float32 if a.dtype == "f32" else torch.float64
+    sets = []
+    for tok in a.sets.split(","):
+        c, kind = tok.split(":")
+        sets.append((n // int(c), kind))
+    g = NS.Grid(1.0, n)
+
+    if a.copy_calibration:
+        x = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        del x, y
+
+    # initial field in FP64, then the run's precision
+    t0 = time.perf_counter()
+    ps64 = NS.PoissonSolver(g, torch.float64)
+    u64 = NS.randomfield(g, ps64, "peak", 5.0, 0.5, seed=123)
+    dt = NS.propose_timestep(u64, a.visc, 0.4)
+    u_init = [x.data.to(dtype) for x in u64]
+    del u64, ps64
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    log(f"[bench] IC {n}^3 ready in {time.perf_counter() - t0:.1f}s, dt={dt:.3e}")
+
+    drv = D.DNSAidedLES(g, a.visc, sets, dtype=dtype)
+    if a.closure_tiles:
+        drv.ctx.call("ec_closure_set_tiles", *[int(x) for x in a.closure_tiles.split(",")])
+    drv.set_state(u_init)
+    del u_init
+    for _ in range(a.warmup):
+        drv.step(dt)
+    torch.cuda.synchronize()
+    log(f"[bench] warmup {a.warmup} steps done")
+
+    ctx = drv.ctx
+    ctx.call("ec_profile_begin")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        drv.step(dt)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    cap = 100000
+    names = (abi.ctypes.c_char_p * cap)()
+    ms = (abi.c_double * cap)()
+    by = (abi.c_double * cap)()
+    cnt = abi.c_int()
+    ctx.call("ec_profile_end", names, ms, by, cap, abi.ctypes.byref(cnt))
+    m = min(cnt.value, cap)
+    per = {}
+    for i in range(m):
+        k = names[i].decode()
+        e = per.setdefault(k, [0, 0.0, 0.0])
+        e[0] += 1
+        e[1] += ms[i]
+        e[2] += by[i]
+    total_bytes = sum(v[2] for v in per.values()) / a.steps
+    ms_step = 1000 * t / a.steps
+    dom = max(per.items(), key=lambda kv: kv[1][1])
+    dname, (dcount, dms, dbytes) = dom
+    d_avg_ms = dms / dcount
+    d_achieved = (dbytes / dcount) / (d_avg_ms * 1e-3) / 1e9
+    if a.kernels:
+        for k, (c, tms, b) in sorted(per.items(), key=lambda kv: -kv[1][1]):
+            log(f"  {k:22s} launches {c:6d}  total {tms:9.2f} ms  avg {tms / c:8.3f} ms  "
+                f"{(b / c) / (tms / c * 1e-3) / 1e9:8.1f} GB/s")
+    log(f"[bench] {a.steps} steps in {t:.3f}s: {ms_step:.1f} ms/step, algorithmic {total_bytes / 1e9:.1f} GB/step")
+
+    closure_err = None
+    les_err = drv.les_errors()
+    div = drv.divergence_norm()
+    if not a.no_closure_check:
+        taus32 = drv.closure_taus()
+        u_state = [x.double() for x in drv.u0]
+        del drv
+        torch.cuda.empty_cache()
+        d64 = D.DNSAidedLES(g, a.visc, sets, dtype=torch.float64)
+        taus64 = d64.closure_taus(u_state)
+        num = den = 0.0
+        for s32, s64 in zip(taus32, taus64):
+            for x, y in zip(s32, s64):
+                num += float(((x - y) ** 2).sum())
+                den += float((y ** 2).sum())
+        closure_err = math.sqrt(num / den)
+        del d64
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        log("[bench] CPU baseline (oracle, OpenMP) ...")
+        cpu = cpu_baseline(a.cpu_n)
+
+    out = {
+        "metric": METRIC,
+        "value": a.steps / t,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 fields + fp64 closure" if a.dtype == "f32" else "fp64",
+        "data": "synthetic: random divergence-free field, spectrum k^4 exp(-2(k/5)^2), E=1/2, seed 123",
+        "config": {"workload": f"DNS-aided LES, NS {n}^3, RK4, nu={a.visc:.3g}, dt={dt:.3e} "
+                               f"(propose_timestep, CFL 0.4), closure sets "
+                               + ", ".join(f"{nb}^3 {k}" for nb, k in sets),
+                   "grid": n, "les": [f"{nb}:{k}" for nb, k in sets], "parallelism": "single GPU"},
+        "hbm_gbs": total_bytes / (t / a.steps) / 1e9,
+        "hbm_frac_step": total_bytes / (t / a.steps) / 1e9 / PEAK_HBM_GBS,
+        "closure_relerr_vs_fp64": closure_err,
+        "divergence": div,
+        "les_error_vs_filtered_dns": les_err,
+        "roofline": {"bound": "hbm", "kernel": dname, "achieved": d_achieved, "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": d_achieved / PEAK_HBM_GBS,
+                     "avg_ms": d_avg_ms, "bytes_per_launch": dbytes / dcount, "traffic": None},
+        "kernels": {k: {"launches": c, "ms": round(tms, 3), "gbs": round((b / c) / (tms / c * 1e-3) / 1e9, 1)}
+                    for k, (c, tms, b) in per.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -85,6 +85,7 @@ _SIGS = {
                                  c_int, ctypes.POINTER(FilterQuantity)]),
     "ec_ns_closure": (c_int, [c_void_p, c_int, c_int, c_double, c_double, P3, c_void_p, c_int,
                               ctypes.POINTER(ClosureSet)]),
+    "ec_closure_set_tiles": (c_int, [c_void_p, c_int, c_int, c_int]),
     "ec_ns_commutator": (c_int, [c_void_p, c_int, c_double, c_double, P3, P9, P9]),
     "ec_ns_rk4_update": (c_int, [c_void_p, c_int, c_int, c_double, c_int, c_double, P3, P3, P3,
                                  c_void_p, P3]),

@@ -96,8 +96,8 @@ __global__ void k_reset_counter(int *counter) {
     if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] = 0;
 }
 
-template <typename T, int PT>
-__global__ void __launch_bounds__(1024) k_burgers_persistent(const BInst *__restrict__ inst, const int *__restrict__ order,
+template <typename T, int PT, int NT>
+__global__ void __launch_bounds__(NT) k_burgers_persistent(const BInst *__restrict__ inst, const int *__restrict__ order,
                                                             int count, int *__restrict__ counter, T *__restrict__ U,
                                                             T *__restrict__ UL, T *__restrict__ UM,
                                                             double *__restrict__ dts, double L, double visc,
@@ -280,6 +280,7 @@ struct ec_burgers_batch {
     void *U = nullptr, *UL = nullptr, *UM = nullptr;
     double *dts = nullptr;
     std::map<std::pair<int, int>, hipGraphExec_t> graphs;  // (nsteps, threads) -> graph
+    hipStream_t cap_stream = nullptr;                      // private stream used for capture
 };
 
 static int pt_for(int nmax, int threads) {
@@ -293,35 +294,48 @@ static size_t batch_lds(const ec_burgers_batch *b) {
     return sizeof(T) * ((size_t)b->nmax + 3 * (size_t)b->nHmax);
 }
 
-template <typename T, int PT>
-static int launch_pt(ec_ctx *ctx, ec_burgers_batch *b, int nsteps, int blocks, size_t lds) {
-    EC_HIP(ctx, hipFuncSetAttribute((const void *)k_burgers_persistent<T, PT>,
+template <typename T, int PT, int NT>
+static int launch_pt(ec_ctx *ctx, ec_burgers_batch *b, int nsteps, int blocks, size_t lds, hipStream_t st) {
+    EC_HIP(ctx, hipFuncSetAttribute((const void *)k_burgers_persistent<T, PT, NT>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    k_burgers_persistent<T, PT><<<blocks, b->threads, lds, ctx->stream>>>(
+    k_burgers_persistent<T, PT, NT><<<blocks, NT, lds, st>>>(
         b->inst, b->order, b->count, b->counter, (T *)b->U, (T *)b->UL, (T *)b->UM, b->dts, b->L, b->visc, b->cfl,
         nsteps);
     EC_CHECK_LAUNCH(ctx, "k_burgers_persistent");
     return EC_OK;
 }
 
+// Points per thread PT (a power of two with PT * NT >= the largest instance);
+// PT * NT <= 32768 covers instances up to 16384 points at every block size.
+template <typename T, int NT>
+static int dispatch_pt(ec_ctx *ctx, ec_burgers_batch *b, int nsteps, int blocks, size_t lds, hipStream_t st) {
+    const int pt = pt_for(b->nmax, NT);
+    if (pt * NT > 32768) return ec_fail(ctx, EC_E_UNSUPPORTED, "instance size %d too large for %d threads", b->nmax, NT);
+    switch (pt) {
+        case 1: return launch_pt<T, 1, NT>(ctx, b, nsteps, blocks, lds, st);
+        case 2: return launch_pt<T, 2, NT>(ctx, b, nsteps, blocks, lds, st);
+        case 4: return launch_pt<T, 4, NT>(ctx, b, nsteps, blocks, lds, st);
+        case 8: return launch_pt<T, 8, NT>(ctx, b, nsteps, blocks, lds, st);
+        case 16: return launch_pt<T, 16, NT>(ctx, b, nsteps, blocks, lds, st);
+        case 32: return launch_pt<T, (32 * NT <= 32768 ? 32 : 1), NT>(ctx, b, nsteps, blocks, lds, st);
+        default: return launch_pt<T, (64 * NT <= 32768 ? 64 : 1), NT>(ctx, b, nsteps, blocks, lds, st);
+    }
+}
+
 template <typename T>
-static int launch_run(ec_ctx *ctx, ec_burgers_batch *b, int nsteps) {
+static int launch_run(ec_ctx *ctx, ec_burgers_batch *b, int nsteps, hipStream_t st) {
     const size_t lds = batch_lds<T>(b);
     int dev_cus = 0;
     EC_HIP(ctx, hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
     const int per_cu = std::max(1, (int)((160 * 1024) / std::max<size_t>(lds + 1024, 1)));
     const int blocks = std::min(b->count, dev_cus * std::min(per_cu, std::max(1, 2048 / b->threads)));
-    k_reset_counter<<<1, 64, 0, ctx->stream>>>(b->counter);
+    k_reset_counter<<<1, 64, 0, st>>>(b->counter);
     EC_CHECK_LAUNCH(ctx, "k_reset_counter");
-    switch (pt_for(b->nmax, b->threads)) {
-        case 1: return launch_pt<T, 1>(ctx, b, nsteps, blocks, lds);
-        case 2: return launch_pt<T, 2>(ctx, b, nsteps, blocks, lds);
-        case 4: return launch_pt<T, 4>(ctx, b, nsteps, blocks, lds);
-        case 8: return launch_pt<T, 8>(ctx, b, nsteps, blocks, lds);
-        case 16: return launch_pt<T, 16>(ctx, b, nsteps, blocks, lds);
-        case 32: return launch_pt<T, 32>(ctx, b, nsteps, blocks, lds);
-        case 64: return launch_pt<T, 64>(ctx, b, nsteps, blocks, lds);
-        default: return ec_fail(ctx, EC_E_UNSUPPORTED, "instance size %d too large for %d threads", b->nmax, b->threads);
+    switch (b->threads) {
+        case 128: return dispatch_pt<T, 128>(ctx, b, nsteps, blocks, lds, st);
+        case 256: return dispatch_pt<T, 256>(ctx, b, nsteps, blocks, lds, st);
+        case 512: return dispatch_pt<T, 512>(ctx, b, nsteps, blocks, lds, st);
+        default: return dispatch_pt<T, 1024>(ctx, b, nsteps, blocks, lds, st);
     }
 }
 
@@ -391,6 +405,7 @@ extern "C" int ec_burgers_batch_destroy(ec_ctx *ctx, ec_burgers_batch *b) {
     if (!b) return EC_OK;
     if (ctx) hipStreamSynchronize(ctx->stream);
     for (auto &kv : b->graphs) hipGraphExecDestroy(kv.second);
+    if (b->cap_stream) hipStreamDestroy(b->cap_stream);
     hipFree(b->inst);
     hipFree(b->order);
     hipFree(b->counter);
@@ -405,7 +420,7 @@ extern "C" int ec_burgers_batch_destroy(ec_ctx *ctx, ec_burgers_batch *b) {
 extern "C" int ec_burgers_batch_set_threads(ec_ctx *ctx, ec_burgers_batch *b, int threads) {
     EC_REQUIRE(ctx, ctx && b, "null argument");
     EC_REQUIRE(ctx, threads == 128 || threads == 256 || threads == 512 || threads == 1024, "threads=%d", threads);
-    EC_REQUIRE(ctx, pt_for(b->nmax, threads) <= 64, "threads=%d too few for n=%d", threads, b->nmax);
+    EC_REQUIRE(ctx, pt_for(b->nmax, threads) * threads <= 32768, "threads=%d: n=%d needs more points per thread than supported", threads, b->nmax);
     b->threads = threads;
     return EC_OK;
 }
@@ -414,17 +429,21 @@ extern "C" int ec_burgers_batch_run(ec_ctx *ctx, ec_burgers_batch *b, int nsteps
     EC_REQUIRE(ctx, ctx && b, "null argument");
     EC_REQUIRE(ctx, nsteps >= 0, "nsteps=%d", nsteps);
     if (nsteps == 0) return EC_OK;
-    auto run = [&]() { return b->dtype == EC_F64 ? launch_run<double>(ctx, b, nsteps) : launch_run<float>(ctx, b, nsteps); };
-    if (!use_graph) return run();
+    auto run = [&](hipStream_t st) {
+        return b->dtype == EC_F64 ? launch_run<double>(ctx, b, nsteps, st) : launch_run<float>(ctx, b, nsteps, st);
+    };
+    if (!use_graph) return run(ctx->stream);
     auto key = std::make_pair(nsteps, b->threads);
     auto it = b->graphs.find(key);
     if (it == b->graphs.end()) {
-        // attribute setup outside the capture
-        int rc = EC_OK;
+        // Captured on a private stream (the legacy default stream cannot be
+        // captured); the instantiated graph is then launched on the context
+        // stream, ordered with the caller's other work.
+        if (!b->cap_stream) EC_HIP(ctx, hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
         hipGraph_t g;
-        EC_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        rc = run();
-        hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        EC_HIP(ctx, hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+        int rc = run(b->cap_stream);
+        hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
         if (rc) return rc;
         if (e != hipSuccess) return ec_fail(ctx, EC_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
         hipGraphExec_t ge;

@@ -220,6 +220,9 @@ Piece point() { return {{1.0}, 0}; }
 extern "C" int ec_ns_closure(ec_ctx *ctx, int dtype, int n, double l, double visc, void *const u[3], const void *p,
                              int nsets, const ec_closure_set *sets) {
     EC_REQUIRE(ctx, ctx && u && p && (nsets == 0 || sets), "ec_ns_closure: null argument");
+    // one plane-marching launch for all sets when they fit the tiled scheme
+    const int trc = ec_closure_tiled(ctx, dtype, n, l, visc, u, p, nsets, sets);
+    if (trc != EC_E_UNSUPPORTED) return trc;
     static const int rsrc[3][3] = {{EC_SRC_R11, EC_SRC_S12, EC_SRC_S31},
                                    {EC_SRC_S12, EC_SRC_R22, EC_SRC_S23},
                                    {EC_SRC_S31, EC_SRC_S23, EC_SRC_R33}};

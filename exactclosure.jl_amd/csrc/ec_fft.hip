@@ -326,6 +326,239 @@ __global__ void k_shell_scale(cplx<T> *__restrict__ spec, int n, int nh, int nsh
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Power-of-two sizes: the whole radix plan, every stride and the tile width
+// are compile-time constants, so stages unroll with shift/mask index math and
+// no plan array lives in memory.  n = 2^m is split into radix-8 stages and a
+// final radix-4 or radix-2 stage; each thread owns 16 elements of a line.
+constexpr int ct_radix(int m) { return m % 8 == 0 ? 8 : (m % 4 == 0 ? 4 : 2); }
+
+// LDS position of line element e: one padding slot after every 8 elements,
+// which spreads the strided Stockham scatters over all banks.
+__device__ __forceinline__ constexpr int pad8(int e) { return e + (e >> 3); }
+constexpr int padded_len(int n) { return n + n / 8; }
+
+template <typename T, int N, int NS, int NTHR, int LS, int SIGN>
+__device__ __forceinline__ void ct_stages(cplx<T> *lds, int t, const cplx<T> *__restrict__ tw) {
+    if constexpr (NS < N) {
+        constexpr int R = ct_radix(N / NS);
+        constexpr int NR = N / R;
+        constexpr int B = (NR + NTHR - 1) / NTHR;
+        constexpr int TSTEP = N / (NS * R);
+        cplx<T> v[B][R];
+#pragma unroll
+        for (int q = 0; q < B; q++) {
+            const int b = t + q * NTHR;
+            if (NR % NTHR == 0 || b < NR) {
+                const int jm = b & (NS - 1);
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    cplx<T> x = lds[pad8(b + r * NR) * LS];
+                    if (NS > 1 && r > 0) {
+                        cplx<T> w = tw[r * jm * TSTEP];
+                        if (SIGN > 0) w.y = -w.y;
+                        x = cmul(x, w);
+                    }
+                    v[q][r] = x;
+                }
+                Bfly<T, R>::run(v[q], SIGN);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < B; q++) {
+            const int b = t + q * NTHR;
+            if (NR % NTHR == 0 || b < NR) {
+                const int jm = b & (NS - 1);
+                const int idxD = (b - jm) * R + jm;
+#pragma unroll
+                for (int r = 0; r < R; r++) lds[pad8(idxD + r * NS) * LS] = v[q][r];
+            }
+        }
+        __syncthreads();
+        ct_stages<T, N, NS * R, NTHR, LS, SIGN>(lds, t, tw);
+    }
+}
+
+template <typename T, int N, int NTHR>
+__global__ void __launch_bounds__(256) k_fft_x_r2c_ct(const T *__restrict__ in, cplx<T> *__restrict__ out, long nlines,
+                                                      const cplx<T> *__restrict__ tw) {
+    extern __shared__ char smem_raw[];
+    cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
+    constexpr int NH = N / 2 + 1;
+    const int pair = threadIdx.x / NTHR, t = threadIdx.x % NTHR;
+    const int P = blockDim.x / NTHR;
+    const long L0 = 2 * ((long)blockIdx.x * P + pair);
+    const bool v0 = L0 < nlines, v1 = L0 + 1 < nlines;
+    cplx<T> *line = lds + (size_t)pair * padded_len(N);
+    const T *a = in + L0 * N, *b = in + (L0 + 1) * N;
+#pragma unroll
+    for (int e = t; e < N; e += NTHR) line[pad8(e)] = cplx<T>{v0 ? a[e] : T(0), v1 ? b[e] : T(0)};
+    __syncthreads();
+    ct_stages<T, N, 1, NTHR, 1, -1>(line, t, tw);
+    cplx<T> *oa = out + L0 * NH, *ob = out + (L0 + 1) * NH;
+    for (int k = t; k < NH; k += NTHR) {
+        const cplx<T> z = line[pad8(k)];
+        const cplx<T> zc = line[pad8((N - k) & (N - 1))];
+        const cplx<T> A{(z.x + zc.x) / 2, (z.y - zc.y) / 2};
+        const cplx<T> Bv{(z.y + zc.y) / 2, (zc.x - z.x) / 2};
+        if (v0) oa[k] = A;
+        if (v1) ob[k] = Bv;
+    }
+}
+
+template <typename T, int N, int NTHR>
+__global__ void __launch_bounds__(256) k_fft_x_c2r_ct(const cplx<T> *__restrict__ in, T *__restrict__ out, long nlines,
+                                                      const cplx<T> *__restrict__ tw, T scale) {
+    extern __shared__ char smem_raw[];
+    cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
+    constexpr int NH = N / 2 + 1;
+    const int pair = threadIdx.x / NTHR, t = threadIdx.x % NTHR;
+    const int P = blockDim.x / NTHR;
+    const long L0 = 2 * ((long)blockIdx.x * P + pair);
+    const bool v0 = L0 < nlines, v1 = L0 + 1 < nlines;
+    cplx<T> *line = lds + (size_t)pair * padded_len(N);
+    const cplx<T> *ia = in + L0 * NH, *ib = in + (L0 + 1) * NH;
+#pragma unroll
+    for (int e = t; e < N; e += NTHR) {
+        const int src = e < NH ? e : N - e;
+        cplx<T> A = v0 ? ia[src] : cplx<T>{T(0), T(0)};
+        cplx<T> B = v1 ? ib[src] : cplx<T>{T(0), T(0)};
+        if (e == 0 || e == N / 2) {  // self-conjugate bins: c2r ignores their imaginary part
+            A.y = T(0);
+            B.y = T(0);
+        } else if (e >= NH) {
+            A.y = -A.y;
+            B.y = -B.y;
+        }
+        line[pad8(e)] = cplx<T>{A.x - B.y, A.y + B.x};
+    }
+    __syncthreads();
+    ct_stages<T, N, 1, NTHR, 1, +1>(line, t, tw);
+    T *oa = out + L0 * N, *ob = out + (L0 + 1) * N;
+#pragma unroll
+    for (int e = t; e < N; e += NTHR) {
+        const cplx<T> z = line[pad8(e)];
+        if (v0) oa[e] = z.x * scale;
+        if (v1) ob[e] = z.y * scale;
+    }
+}
+
+// Y / Z passes: TK consecutive kx columns, the whole column staged in LDS
+// (element e of column c at lds[e * TK + c]: consecutive lanes take
+// consecutive columns, so both global rows and LDS rows are contiguous).
+template <typename T, int N, int TK, int NTHR, int MODE, int SIGN>
+__global__ void __launch_bounds__(TK *NTHR) k_fft_cols_ct(cplx<T> *__restrict__ data, int ncols, long elem_stride,
+                                                         long outer_stride, const cplx<T> *__restrict__ tw,
+                                                         const T *__restrict__ eig, T poisson_a, T scale) {
+    extern __shared__ char smem_raw[];
+    cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
+    const int col0 = blockIdx.x * TK;
+    const int outer = blockIdx.y;
+    const int c = threadIdx.x % TK, t = threadIdx.x / TK;
+    cplx<T> *base = data + col0 + (long)outer * outer_stride;
+    const bool full = col0 + TK <= ncols;
+    constexpr int TOTAL = N * TK;
+#pragma unroll 4
+    for (int idx = threadIdx.x; idx < TOTAL; idx += TK * NTHR) {
+        const int cc = idx % TK, e = idx / TK;
+        lds[pad8(e) * TK + cc] = (full || col0 + cc < ncols) ? base[cc + e * elem_stride] : cplx<T>{T(0), T(0)};
+    }
+    __syncthreads();
+    if (MODE == 0) {
+        ct_stages<T, N, 1, NTHR, TK, SIGN>(lds + c, t, tw);
+    } else {
+        ct_stages<T, N, 1, NTHR, TK, -1>(lds + c, t, tw);
+        const int kx = col0 + c;
+        const T sxy = (kx < ncols ? eig[kx] : T(0)) + eig[outer];
+#pragma unroll
+        for (int e = t; e < N; e += NTHR) {
+            const T denom = poisson_a * (sxy + eig[e]);
+            const bool zero_mode = (kx == 0 && outer == 0 && e == 0);
+            const T f = zero_mode ? T(0) : scale / denom;
+            cplx<T> z = lds[pad8(e) * TK + c];
+            z.x *= f;
+            z.y *= f;
+            lds[pad8(e) * TK + c] = z;
+        }
+        __syncthreads();
+        ct_stages<T, N, 1, NTHR, TK, +1>(lds + c, t, tw);
+    }
+#pragma unroll 4
+    for (int idx = threadIdx.x; idx < TOTAL; idx += TK * NTHR) {
+        const int cc = idx % TK, e = idx / TK;
+        if (full || col0 + cc < ncols) base[cc + e * elem_stride] = lds[pad8(e) * TK + cc];
+    }
+}
+
+template <typename T, int N>
+int r2c_x_ct(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out) {
+    constexpr int NTHR = N / 16;
+    const int P = 256 / NTHR;
+    const size_t lds = (size_t)P * padded_len(N) * sizeof(cplx<T>);
+    const long nlines = (long)N * N;
+    const int blocks = (int)(((nlines + 1) / 2 + P - 1) / P);
+    EC_HIP(ctx, hipFuncSetAttribute((const void *)k_fft_x_r2c_ct<T, N, NTHR>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    k_fft_x_r2c_ct<T, N, NTHR><<<blocks, P * NTHR, lds, ctx->stream>>>((const T *)in, (cplx<T> *)out, nlines,
+                                                                        (const cplx<T> *)ps->twiddle);
+    EC_CHECK_LAUNCH(ctx, "k_fft_x_r2c_ct");
+    return EC_OK;
+}
+
+template <typename T, int N>
+int c2r_x_ct(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out, double scale) {
+    constexpr int NTHR = N / 16;
+    const int P = 256 / NTHR;
+    const size_t lds = (size_t)P * padded_len(N) * sizeof(cplx<T>);
+    const long nlines = (long)N * N;
+    const int blocks = (int)(((nlines + 1) / 2 + P - 1) / P);
+    EC_HIP(ctx, hipFuncSetAttribute((const void *)k_fft_x_c2r_ct<T, N, NTHR>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    k_fft_x_c2r_ct<T, N, NTHR><<<blocks, P * NTHR, lds, ctx->stream>>>((const cplx<T> *)in, (T *)out, nlines,
+                                                                        (const cplx<T> *)ps->twiddle, (T)scale);
+    EC_CHECK_LAUNCH(ctx, "k_fft_x_c2r_ct");
+    return EC_OK;
+}
+
+template <typename T, int N, int MODE, int SIGN>
+int cols_ct(ec_ctx *ctx, ec_poisson *ps, void *spec, int axis) {
+    constexpr int NTHR = N / 16;
+    constexpr int TK0 = sizeof(T) == 4 ? 16 : 8;
+    constexpr int TK = (TK0 * NTHR > 1024) ? 1024 / NTHR : TK0;
+    const int nh = ps->nh;
+    const long elem_stride = axis == 1 ? (long)nh : (long)nh * N;
+    const long outer_stride = axis == 1 ? (long)nh * N : (long)nh;
+    dim3 grid((nh + TK - 1) / TK, N);
+    const size_t lds = (size_t)padded_len(N) * TK * sizeof(cplx<T>);
+    const double h = ps->l / N;
+    EC_HIP(ctx, hipFuncSetAttribute((const void *)k_fft_cols_ct<T, N, TK, NTHR, MODE, SIGN>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    k_fft_cols_ct<T, N, TK, NTHR, MODE, SIGN><<<grid, TK * NTHR, lds, ctx->stream>>>(
+        (cplx<T> *)spec, nh, elem_stride, outer_stride, (const cplx<T> *)ps->twiddle, (const T *)ps->eig,
+        (T)(-4.0 / (h * h)), (T)(1.0 / ((double)N * N * N)));
+    EC_CHECK_LAUNCH(ctx, "k_fft_cols_ct");
+    return EC_OK;
+}
+
+template <typename T, int N>
+int cols_ct_any(ec_ctx *ctx, ec_poisson *ps, void *spec, int axis, int sign, int mode) {
+    if (mode) return cols_ct<T, N, 1, -1>(ctx, ps, spec, axis);
+    return sign < 0 ? cols_ct<T, N, 0, -1>(ctx, ps, spec, axis) : cols_ct<T, N, 0, +1>(ctx, ps, spec, axis);
+}
+
+#define EC_FFT_CT_SWITCH(n, CALL) \
+    switch (n) {                  \
+        case 32: CALL(32);        \
+        case 64: CALL(64);        \
+        case 128: CALL(128);      \
+        case 256: CALL(256);      \
+        case 512: CALL(512);      \
+        case 1024: CALL(1024);    \
+        default: break;           \
+    }
+
 bool make_plan(int n, RadixPlan &rp) {
     rp.nrad = 0;
     int m = n;
@@ -383,6 +616,10 @@ int set_lds_limits(ec_ctx *ctx) {
 
 template <typename T>
 int r2c_x(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out) {
+    ec_prof_scope pr(ctx, "fft_x_r2c", (double)sizeof(T) * (ec_cube(ps->n) + 2.0 * ps->nh * ps->n * ps->n));
+#define EC_CALL(N) return r2c_x_ct<T, N>(ctx, ps, in, out)
+    EC_FFT_CT_SWITCH(ps->n, EC_CALL)
+#undef EC_CALL
     RadixPlan rp;
     make_plan(ps->n, rp);
     const int n = ps->n, nh = ps->nh;
@@ -390,7 +627,6 @@ int r2c_x(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out) {
     XCfg c = xcfg<T>(n);
     const long npairs = (nlines + 1) / 2;
     const int blocks = (int)((npairs + c.P - 1) / c.P);
-    ec_prof_scope pr(ctx, "fft_x_r2c", (double)sizeof(T) * (ec_cube(n) + 2.0 * nh * n * n));
     k_fft_x_r2c<T><<<blocks, c.P * c.nthr, c.lds, ctx->stream>>>((const T *)in, (cplx<T> *)out, n, nh, nlines, rp,
                                                                   (const cplx<T> *)ps->twiddle, c.nthr);
     EC_CHECK_LAUNCH(ctx, "k_fft_x_r2c");
@@ -399,6 +635,10 @@ int r2c_x(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out) {
 
 template <typename T>
 int c2r_x(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out, double scale) {
+    ec_prof_scope pr(ctx, "fft_x_c2r", (double)sizeof(T) * (ec_cube(ps->n) + 2.0 * ps->nh * ps->n * ps->n));
+#define EC_CALL(N) return c2r_x_ct<T, N>(ctx, ps, in, out, scale)
+    EC_FFT_CT_SWITCH(ps->n, EC_CALL)
+#undef EC_CALL
     RadixPlan rp;
     make_plan(ps->n, rp);
     const int n = ps->n, nh = ps->nh;
@@ -406,7 +646,6 @@ int c2r_x(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out, double scale) 
     XCfg c = xcfg<T>(n);
     const long npairs = (nlines + 1) / 2;
     const int blocks = (int)((npairs + c.P - 1) / c.P);
-    ec_prof_scope pr(ctx, "fft_x_c2r", (double)sizeof(T) * (ec_cube(n) + 2.0 * nh * n * n));
     k_fft_x_c2r<T><<<blocks, c.P * c.nthr, c.lds, ctx->stream>>>((const cplx<T> *)in, (T *)out, n, nh, nlines, rp,
                                                                   (const cplx<T> *)ps->twiddle, c.nthr, (T)scale);
     EC_CHECK_LAUNCH(ctx, "k_fft_x_c2r");
@@ -416,6 +655,11 @@ int c2r_x(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out, double scale) 
 // axis 1 (y) or 2 (z); mode 1 = Poisson (z only)
 template <typename T>
 int cols(ec_ctx *ctx, ec_poisson *ps, void *spec, int axis, int sign, int mode) {
+    const char *name = mode ? "fft_z_poisson" : (axis == 1 ? "fft_y" : "fft_z");
+    ec_prof_scope pr(ctx, name, 2.0 * sizeof(T) * 2.0 * ps->nh * ps->n * ps->n);
+#define EC_CALL(N) return cols_ct_any<T, N>(ctx, ps, spec, axis, sign, mode)
+    EC_FFT_CT_SWITCH(ps->n, EC_CALL)
+#undef EC_CALL
     RadixPlan rp;
     make_plan(ps->n, rp);
     const int n = ps->n, nh = ps->nh;
@@ -426,8 +670,6 @@ int cols(ec_ctx *ctx, ec_poisson *ps, void *spec, int axis, int sign, int mode) 
     const double h = ps->l / n;
     const T a = (T)(-4.0 / (h * h));
     const T scale = (T)(1.0 / ((double)n * n * n));
-    const char *name = mode ? "fft_z_poisson" : (axis == 1 ? "fft_y" : "fft_z");
-    ec_prof_scope pr(ctx, name, 2.0 * sizeof(T) * 2.0 * nh * n * n);
     if (mode)
         k_fft_cols<T, 1><<<grid, c.TK * c.nthr, c.lds, ctx->stream>>>((cplx<T> *)spec, n, nh, elem_stride, outer_stride,
                                                                        c.TK, rp, (const cplx<T> *)ps->twiddle, sign,
@@ -476,6 +718,15 @@ extern "C" int ec_poisson_create(ec_ctx *ctx, int dtype, int n, double l, ec_poi
     if (!make_plan(n, rp))
         return ec_fail(ctx, EC_E_UNSUPPORTED, "FFT length %d: only factors 2, 3, 5 are supported", n);
     EC_REQUIRE(ctx, (n + FFT_E - 1) / FFT_E <= 1024, "FFT length %d too large", n);
+    {
+        // every butterfly of every stage must fall on a thread slot
+        const int nthr = std::max(1, (n + FFT_E - 1) / FFT_E);
+        for (int s = 0; s < rp.nrad; s++) {
+            const int R = rp.r[s];
+            const int need = (n / R + nthr - 1) / nthr, have = (FFT_E + R - 1) / R;
+            if (need > have) return ec_fail(ctx, EC_E_UNSUPPORTED, "FFT length %d: radix-%d stage not covered", n, R);
+        }
+    }
     const size_t bt = dtype == EC_F64 ? 8 : 4;
     if ((size_t)n * 1 * 2 * bt > 128 * 1024)
         return ec_fail(ctx, EC_E_UNSUPPORTED, "FFT length %d exceeds the LDS column budget", n);

@@ -40,9 +40,11 @@ int ec_fail(ec_ctx *ctx, int code, const char *fmt, ...);
 #define EC_HIP(ctx, call)                                                                     \
     do {                                                                                      \
         hipError_t e_ = (call);                                                               \
-        if (e_ != hipSuccess)                                                                 \
+        if (e_ != hipSuccess) {                                                               \
+            (void)hipGetLastError(); /* do not leave the error for the next launch check */   \
             return ec_fail((ctx), EC_E_HIP, "%s failed: %s (%s:%d)", #call,                   \
                            hipGetErrorString(e_), __FILE__, __LINE__);                        \
+        }                                                                                     \
     } while (0)
 
 #define EC_CHECK_LAUNCH(ctx, name)                                                            \
@@ -111,6 +113,8 @@ struct ec_poisson {
 };
 
 // Internal entry points shared between translation units.
-int ec_fft_r2c_impl(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out);
-int ec_fft_c2r_impl(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out);
-int ec_fft_cols_impl(ec_ctx *ctx, ec_poisson *ps, void *spec, int axis, int sign, int poisson_mode);
+
+
+
+int ec_closure_tiled(ec_ctx *ctx, int dtype, int n, double l, double visc, void *const u[3], const void *p,
+                     int nsets, const ec_closure_set *sets);

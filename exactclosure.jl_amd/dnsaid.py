@@ -139,6 +139,22 @@ class DNSAidedLES:
         h = self.g.l / self.g.n
         return h * (num ** 0.5) / (den ** 0.5)
 
+    def closure_taus(self, u=None):
+        """Closure commutator tau (9 FP64 components per LES grid) of state u
+        (default: the current state), without advancing anything."""
+        c = self.ctx
+        u = self.u0 if u is None else u
+        n, l = self.g.n, self.g.l
+        c.call("ec_ns_rhs", self.dt_code, n, l, self.visc, ptrs(u, P3), ptrs(self.du, P3), 0)
+        c.call("ec_ns_pressure_solve", self.ps.h, ptrs(self.du, P3))
+        self._closure(u)
+        out = []
+        for L in self.les:
+            ub = L.cset.ubar
+            c.call("ec_ns_project", L.ps64.h, ptrs(ub, P3))
+            out.append([t.clone() for t in NS.commutator_(L.cset, ub, self.visc)])
+        return out
+
     def les_errors(self):
         """compute_errors of every LES grid against P_H V u_dns."""
         ref = self.filtered_dns()

@@ -168,6 +168,10 @@ typedef struct {
 } ec_closure_set;
 int ec_ns_closure(ec_ctx *ctx, int dtype, int n, double l, double visc, void *const u[3],
                   const void *p, int nsets, const ec_closure_set *sets);
+/* Target block size (fine cells x, y, z) of the one-launch closure kernel; the
+ * library picks the largest multiples of every compression not above them.
+ * Set by the tile autotuner; no counterpart in the reference. */
+int ec_closure_set_tiles(ec_ctx *ctx, int tx, int ty, int tz);
 /* tau = flux - sigma_H(ubar) in FP64 on the LES grid (the closure commutator) */
 int ec_ns_commutator(ec_ctx *ctx, int nbar, double l, double visc, void *const ubar[3],
                      void *const flux9[9], void *const tau9[9]);
