@@ -72,16 +72,60 @@ __device__ __forceinline__ int wrapn(int i, int n) {
     return i < 0 ? i + n : i;
 }
 
-template <typename T, int MAXS, int PRE, int MAXX>
+
+// Sum of L values p[0], p[stride], ..., unrolled with four independent
+// accumulators so all LDS loads issue back to back.
+template <int L, typename V>
+__device__ __forceinline__ double usum(const V *p, int stride) {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+#pragma unroll
+    for (int t = 0; t < L; t++) {
+        const double v = (double)p[t * stride];
+        if ((t & 3) == 0) a0 += v;
+        else if ((t & 3) == 1) a1 += v;
+        else if ((t & 3) == 2) a2 += v;
+        else a3 += v;
+    }
+    return (a0 + a1) + (a2 + a3);
+}
+
+// we * (p[lo] + p[hi]) + wi * (p[lo+1] + ... + p[hi-1]); lo == hi: wi * p[lo].
+// Interior lengths of the box / face / centre pieces at power-of-two
+// compressions are unrolled; any other length takes the loop.
+template <typename V>
+__device__ __forceinline__ double piece_sum(const V *p, int stride, int lo, int hi, double wi, double we) {
+    if (lo == hi) return wi * (double)p[lo * stride];
+    const V *q = p + (lo + 1) * stride;
+    double in;
+    switch (hi - lo - 1) {
+        case 0: in = 0.0; break;
+        case 1: in = usum<1>(q, stride); break;
+        case 2: in = usum<2>(q, stride); break;
+        case 3: in = usum<3>(q, stride); break;
+        case 6: in = usum<6>(q, stride); break;
+        case 7: in = usum<7>(q, stride); break;
+        case 14: in = usum<14>(q, stride); break;
+        case 15: in = usum<15>(q, stride); break;
+        case 30: in = usum<30>(q, stride); break;
+        case 31: in = usum<31>(q, stride); break;
+        default: {
+            in = 0.0;
+            for (int t = 0; t < hi - lo - 1; t++) in += (double)q[t * stride];
+        }
+    }
+    return we * ((double)p[lo * stride] + (double)p[hi * stride]) + wi * in;
+}
+
+template <typename T, int MAXS, int MAXX, int LW, int NP>
 __global__ void __launch_bounds__(512) k_closure_tiled(const T *__restrict__ u1, const T *__restrict__ u2,
                                                       const T *__restrict__ u3, const T *__restrict__ p,
                                                       const XD *__restrict__ xd, const SD *__restrict__ sd, int n,
                                                       int TX, int TY, int ZC, int SX, int SY, int UX, int UY,
-                                                      int NXP, int nslots, int hmax, int SST, double ih,
-                                                      double visc) {
+                                                      int NXP, int nslots, int hmax, int SST, int SXP, int SYP,
+                                                      int4 ibase, int4 rows, int4 cbase, double ih, double visc) {
     extern __shared__ double smem_d[];
     double *XP = smem_d;
-    XD *XDs = reinterpret_cast<XD *>(XP + (size_t)SY * NXP);
+    XD *XDs = reinterpret_cast<XD *>(XP + (size_t)SYP * NXP);
     T *U = reinterpret_cast<T *>(XDs + NXP);
     T *PS = U + 9 * UX * UY;
     T *S = PS + SX * SY;  // nine source planes, stride SST (odd: planes start on different banks)
@@ -92,56 +136,37 @@ __global__ void __launch_bounds__(512) k_closure_tiled(const T *__restrict__ u1,
     const int nfetch = 3 * plane_u + plane_s;
 
     for (int j = tid; j < NXP; j += nthr) XDs[j] = xd[j];
-    // This thread's share of every plane fetch, decomposed once: source array,
-    // (x, y) offset in the global field and destination in LDS (>= 0: ring
-    // offset of a velocity component, < 0: -1 - index into the p tile).
-    int f_src[PRE], f_xy[PRE], f_lds[PRE];
-#pragma unroll
-    for (int i = 0; i < PRE; i++) {
-        const int e = tid + i * nthr;
-        f_src[i] = -1;
-        if (e < nfetch) {
-            if (e < 3 * plane_u) {
-                const int comp = e / plane_u, r = e - comp * plane_u;
-                const int ly = r / UX, lx = r - ly * UX;
-                f_src[i] = comp;
-                f_xy[i] = wrapn(x0 - 1 + lx, n) + n * wrapn(y0 - 1 + ly, n);
-                f_lds[i] = e;
-            } else {
-                const int r = e - 3 * plane_u;
-                const int ly = r / SX, lx = r - ly * SX;
-                f_src[i] = 3;
-                f_xy[i] = wrapn(x0 + lx, n) + n * wrapn(y0 + ly, n);
-                f_lds[i] = -1 - r;
-            }
-        }
-    }
-    // x-partial items of this thread: (row, partial)
-    int x_row[MAXX], x_j[MAXX];
-#pragma unroll
-    for (int m = 0; m < MAXX; m++) {
-        const int idx = tid + m * nthr;
-        x_row[m] = idx / NXP;
-        x_j[m] = idx - x_row[m] * NXP;
-    }
-    const int nxitems = SY * NXP;
+    // a / b for the small non-negative integers here (a < 2^16): the float
+    // product is never within 2^-10 of an integer boundary, so truncation is exact
+    const float r_pu = 1.0f / plane_u, r_ux = 1.0f / UX, r_sx = 1.0f / SX;
+    auto q = [](int a, float rb) { return (int)(((float)a + 0.5f) * rb); };
+    // x-partial items: the columns of set s cover rows [0, rows.s) only
+    // (TY + c/2: the rows its y pieces reach); set s starts at item ibase.s
+    // and column cbase.s
+    const int nxitems = ibase.w + rows.w * (NXP - cbase.w);
+
+    // fetch layout: LW lanes along x, rows ly0, ly0 + RP, ... (NP passes); the
+    // three velocity components share one row offset per pass
+    constexpr int RP = 512 / LW;
+    const int lx = tid % LW, ly0 = tid / LW;
+    const size_t Xu = (size_t)wrapn(x0 - 1 + lx, n), Xs = (size_t)wrapn(x0 + lx, n);
+    const bool in_ux = lx < UX, in_sx = lx < SX;
 
     // planes z0-1, z0, z0+1 into ring slots 0, 1, 2 and p(z0)
 #pragma unroll
-    for (int sl = 0; sl < 3; sl++) {
-        const size_t kz = (size_t)wrapn(z0 - 1 + sl, n) * n2;
-        const size_t kp = (size_t)wrapn(z0, n) * n2;
+    for (int i = 0; i < NP; i++) {
+        const int ly = ly0 + i * RP;
+        if (in_ux && ly < UY) {
+            const size_t rowoff = Xu + nn * (size_t)wrapn(y0 - 1 + ly, n);
 #pragma unroll
-        for (int i = 0; i < PRE; i++) {
-            const int fs = f_src[i];
-            if (fs < 0) continue;
-            if (fs < 3) {
-                const T *src = fs == 0 ? u1 : (fs == 1 ? u2 : u3);
-                U[sl * 3 * plane_u + f_lds[i]] = src[f_xy[i] + kz];
-            } else if (sl == 2) {
-                PS[-1 - f_lds[i]] = p[f_xy[i] + kp];
+            for (int sl = 0; sl < 3; sl++) {
+                const size_t off = rowoff + n2 * (size_t)wrapn(z0 - 1 + sl, n);
+                U[(sl * 3 + 0) * plane_u + ly * UX + lx] = u1[off];
+                U[(sl * 3 + 1) * plane_u + ly * UX + lx] = u2[off];
+                U[(sl * 3 + 2) * plane_u + ly * UX + lx] = u3[off];
             }
         }
+        if (in_sx && ly < SY) PS[ly * SX + lx] = p[Xs + nn * (size_t)wrapn(y0 + ly, n) + n2 * (size_t)wrapn(z0, n)];
     }
     __syncthreads();
 
@@ -153,18 +178,22 @@ __global__ void __launch_bounds__(512) k_closure_tiled(const T *__restrict__ u1,
     for (int k = z0; k <= kend; ++k) {
         const int lk = k - (z0 - 1);
         const int sm = (lk - 1) % 3, s0 = lk % 3, sp = (lk + 1) % 3;
-        // prefetch plane k+2 and p(k+1); stored into slot sm / the p tile at the
-        // end of this iteration, so the whole plane's work hides the latency
-        T pre[PRE];
+        // prefetch plane k+2 and p(k+1) into registers; they land in slot sm
+        // (plane k-1) and the p tile after the slot phase, so the whole plane's
+        // work covers the memory latency
+        T pre[4 * NP];
         {
             const size_t ku = (size_t)wrapn(k + 2, n) * n2, kp = (size_t)wrapn(k + 1, n) * n2;
 #pragma unroll
-            for (int i = 0; i < PRE; i++) {
-                const int fs = f_src[i];
-                if (fs >= 0) {
-                    const T *src = fs == 0 ? u1 : (fs == 1 ? u2 : (fs == 2 ? u3 : p));
-                    pre[i] = src[f_xy[i] + (fs < 3 ? ku : kp)];
+            for (int i = 0; i < NP; i++) {
+                const int ly = ly0 + i * RP;
+                if (in_ux && ly < UY) {
+                    const size_t off = Xu + nn * (size_t)wrapn(y0 - 1 + ly, n) + ku;
+                    pre[0 * NP + i] = u1[off];
+                    pre[1 * NP + i] = u2[off];
+                    pre[2 * NP + i] = u3[off];
                 }
+                if (in_sx && ly < SY) pre[3 * NP + i] = p[Xs + nn * (size_t)wrapn(y0 + ly, n) + kp];
             }
         }
         // ---- sources at plane k
@@ -182,35 +211,36 @@ __global__ void __launch_bounds__(512) k_closure_tiled(const T *__restrict__ u1,
             const double e2 = (u2c + u2ym) / 2, d2 = (u2c - u2ym) * ih;
             const double e3 = (u3c + u3zm) / 2, d3 = (u3c - u3zm) * ih;
             const double pp = (double)PS[idx];
-            S[0 * SST + idx] = (T)u1c;
-            S[1 * SST + idx] = (T)u2c;
-            S[2 * SST + idx] = (T)u3c;
-            S[3 * SST + idx] = (T)(e1 * e1 - visc * (d1 + d1) + pp);
-            S[4 * SST + idx] = (T)(e2 * e2 - visc * (d2 + d2) + pp);
-            S[5 * SST + idx] = (T)(e3 * e3 - visc * (d3 + d3) + pp);
-            S[6 * SST + idx] =
+            const int so = ly * SXP + lx;  // padded position in the source planes
+            S[0 * SST + so] = (T)u1c;
+            S[1 * SST + so] = (T)u2c;
+            S[2 * SST + so] = (T)u3c;
+            S[3 * SST + so] = (T)(e1 * e1 - visc * (d1 + d1) + pp);
+            S[4 * SST + so] = (T)(e2 * e2 - visc * (d2 + d2) + pp);
+            S[5 * SST + so] = (T)(e3 * e3 - visc * (d3 + d3) + pp);
+            S[6 * SST + so] =
                 (T)(((u1yp + u1c) / 2) * ((u2xp + u2c) / 2) - visc * ((u1yp - u1c) * ih + (u2xp - u2c) * ih));
-            S[7 * SST + idx] =
+            S[7 * SST + so] =
                 (T)(((u2zp + u2c) / 2) * ((u3yp + u3c) / 2) - visc * ((u2zp - u2c) * ih + (u3yp - u3c) * ih));
-            S[8 * SST + idx] =
+            S[8 * SST + so] =
                 (T)(((u3xp + u3c) / 2) * ((u1zp + u1c) / 2) - visc * ((u3xp - u3c) * ih + (u1zp - u1c) * ih));
         }
         __syncthreads();
         // ---- x-partials for every row of the source tile
-#pragma unroll
+#pragma unroll 1
         for (int m = 0; m < MAXX; m++) {
-            if (tid + m * nthr < nxitems) {
-                const XD d = XDs[x_j[m]];
-                const T *srow = S + d.soff + x_row[m] * SX;  // d.soff = source * SST
-                double sum;
-                if (d.lo == d.hi) {
-                    sum = d.wi * (double)srow[d.lo];
-                } else {
-                    double in = 0.0;
-                    for (int t = d.lo + 1; t < d.hi; t++) in += (double)srow[t];
-                    sum = d.we * ((double)srow[d.lo] + (double)srow[d.hi]) + d.wi * in;
-                }
-                XP[tid + m * nthr] = sum;
+            const int it = tid + m * nthr;
+            if (it < nxitems) {
+                // rows fastest: the lanes of a wave share one (source, piece) column
+                int b = 0, R = rows.x, cb = 0;
+                if (it >= ibase.y) { b = ibase.y; R = rows.y; cb = cbase.y; }
+                if (it >= ibase.z) { b = ibase.z; R = rows.z; cb = cbase.z; }
+                if (it >= ibase.w) { b = ibase.w; R = rows.w; cb = cbase.w; }
+                const int jl = q(it - b, 1.0f / R), row = it - b - jl * R;
+                const int j = cb + jl;
+                const XD d = XDs[j];
+                const T *srow = S + d.soff + row * SXP;  // d.soff = source * SST
+                XP[j * SYP + row] = piece_sum(srow, 1, d.lo, d.hi, d.wi, d.we);
             }
         }
         __syncthreads();
@@ -220,15 +250,8 @@ __global__ void __launch_bounds__(512) k_closure_tiled(const T *__restrict__ u1,
             const int sl = tid + m * nthr;
             if (sl < nslots) {
                 const SD d = sd[sl];
-                const double *xcol = XP + d.xcol;
-                double P;
-                if (d.ylo == d.yhi) {
-                    P = d.wiy * xcol[d.ylo * NXP];
-                } else {
-                    double in = 0.0;
-                    for (int r = d.ylo + 1; r < d.yhi; r++) in += xcol[r * NXP];
-                    P = d.wey * (xcol[d.ylo * NXP] + xcol[d.yhi * NXP]) + d.wiy * in;
-                }
+                const double *xcol = XP + (size_t)d.xcol * SYP;
+                const double P = piece_sum(xcol, 1, d.ylo, d.yhi, d.wiy, d.wey);
                 const int c = d.c;
                 const int Kmin = z0 / c, Kmax = (z0 + ZC) / c;
                 const int K1 = k / c;
@@ -256,14 +279,16 @@ __global__ void __launch_bounds__(512) k_closure_tiled(const T *__restrict__ u1,
                 }
             }
         }
-        // land the prefetch: slot sm (plane k-1) and the p tile were last read by the sources of k
+        // land the prefetch (slot sm and the p tile were last read by the sources of k)
 #pragma unroll
-        for (int i = 0; i < PRE; i++) {
-            const int fl = f_lds[i];
-            if (f_src[i] >= 0) {
-                if (fl >= 0) U[sm * 3 * plane_u + fl] = pre[i];
-                else PS[-1 - fl] = pre[i];
+        for (int i = 0; i < NP; i++) {
+            const int ly = ly0 + i * RP;
+            if (in_ux && ly < UY) {
+                U[(sm * 3 + 0) * plane_u + ly * UX + lx] = pre[0 * NP + i];
+                U[(sm * 3 + 1) * plane_u + ly * UX + lx] = pre[1 * NP + i];
+                U[(sm * 3 + 2) * plane_u + ly * UX + lx] = pre[2 * NP + i];
             }
+            if (in_sx && ly < SY) PS[ly * SX + lx] = pre[3 * NP + i];
         }
         __syncthreads();
     }
@@ -307,16 +332,16 @@ const void *table_on_device(ec_ctx *ctx, const std::vector<char> &bytes, int *rc
     return d;
 }
 
-template <typename T, int MAXS, int PRE, int MAXX>
+template <typename T, int MAXS, int MAXX, int LW, int NP>
 int launch_one(ec_ctx *ctx, dim3 grid, int threads, size_t lds, const void *const u[3], const void *p, const XD *xd,
                const SD *sd, int n, int TX, int TY, int ZC, int SX, int SY, int UX, int UY, int NXP, int nslots,
-               int hmax, double ih, double visc) {
-    const int SST = ((SX * SY + 63) / 64) * 64 + 1;
-    EC_HIP(ctx, hipFuncSetAttribute((const void *)k_closure_tiled<T, MAXS, PRE, MAXX>,
+               int hmax, int4 ibase, int4 rows, int4 cbase, double ih, double visc) {
+    const int SXP = SX | 1, SYP = SY | 1, SST = (SXP * SY) | 1;
+    EC_HIP(ctx, hipFuncSetAttribute((const void *)k_closure_tiled<T, MAXS, MAXX, LW, NP>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    k_closure_tiled<T, MAXS, PRE, MAXX><<<grid, threads, lds, ctx->stream>>>(
+    k_closure_tiled<T, MAXS, MAXX, LW, NP><<<grid, threads, lds, ctx->stream>>>(
         (const T *)u[0], (const T *)u[1], (const T *)u[2], (const T *)p, xd, sd, n, TX, TY, ZC, SX, SY, UX, UY, NXP,
-        nslots, hmax, SST, ih, visc);
+        nslots, hmax, SST, SXP, SYP, ibase, rows, cbase, ih, visc);
     EC_CHECK_LAUNCH(ctx, "k_closure_tiled");
     return EC_OK;
 }
@@ -324,14 +349,18 @@ int launch_one(ec_ctx *ctx, dim3 grid, int threads, size_t lds, const void *cons
 template <typename T, int MAXS>
 int launch_tiles(ec_ctx *ctx, dim3 grid, int threads, size_t lds, const void *const u[3], const void *p, const XD *xd,
                  const SD *sd, int n, int TX, int TY, int ZC, int SX, int SY, int UX, int UY, int NXP, int nslots,
-                 int hmax, double ih, double visc) {
-    const int pre = (3 * UX * UY + SX * SY + threads - 1) / threads;
-    const int mx = (SY * NXP + threads - 1) / threads;
-#define EC_ARGS ctx, grid, threads, lds, u, p, xd, sd, n, TX, TY, ZC, SX, SY, UX, UY, NXP, nslots, hmax, ih, visc
-    if (pre <= 8 && mx <= 8) return launch_one<T, MAXS, 8, 8>(EC_ARGS);
-    if (pre <= 8 && mx <= 16) return launch_one<T, MAXS, 8, 16>(EC_ARGS);
-    if (pre <= 16 && mx <= 8) return launch_one<T, MAXS, 16, 8>(EC_ARGS);
-    if (pre <= 16 && mx <= 16) return launch_one<T, MAXS, 16, 16>(EC_ARGS);
+                 int hmax, int4 ibase, int4 rows, int4 cbase, int nitems, double ih, double visc) {
+    const int mx = (nitems + threads - 1) / threads;
+#define EC_ARGS \
+    ctx, grid, threads, lds, u, p, xd, sd, n, TX, TY, ZC, SX, SY, UX, UY, NXP, nslots, hmax, ibase, rows, cbase, ih, visc
+    if (threads != 512 || mx > 16) return EC_E_UNSUPPORTED;
+    const int LW = UX <= 32 ? 32 : 64;
+    if (UX > 64) return EC_E_UNSUPPORTED;
+    const int np = (UY + 512 / LW - 1) / (512 / LW);
+    if (LW == 32 && np <= 2) return launch_one<T, MAXS, 16, 32, 2>(EC_ARGS);
+    if (LW == 32 && np <= 3) return launch_one<T, MAXS, 16, 32, 3>(EC_ARGS);
+    if (LW == 64 && np <= 4) return launch_one<T, MAXS, 16, 64, 4>(EC_ARGS);
+    if (LW == 64 && np <= 6) return launch_one<T, MAXS, 16, 64, 6>(EC_ARGS);
 #undef EC_ARGS
     return EC_E_UNSUPPORTED;
 }
@@ -353,9 +382,11 @@ int tiled_launch(ec_ctx *ctx, int n, double l, double visc, void *const u[3], co
         const int ZC = largest_tile(n, L, zc_target);
         if (!TX || !TY || !ZC) return EC_E_UNSUPPORTED;
         const int SX = TX + hmax, SY = TY + hmax, UX = SX + 2, UY = SY + 2;
-        const int sst = ((SX * SY + 63) / 64) * 64 + 1;  // source plane stride (kernel uses the same)
+        const int sxp = SX | 1, syp = SY | 1, sst = (sxp * SY) | 1;  // the kernel's padded strides
         std::vector<XD> xds;
         std::vector<SD> sds;
+        int ib[4] = {0, 0, 0, 0}, rw[4] = {1, 1, 1, 1}, cbs[4] = {0, 0, 0, 0};
+        int nitems = 0;
         for (int s = 0; s < nsets; s++) {
             const ec_closure_set &S = sets[s];
             const int c = n / S.nbar;
@@ -391,15 +422,19 @@ int tiled_launch(ec_ctx *ctx, int n, double l, double visc, void *const u[3], co
                 qt[q] = found;
             }
             const int xp_off = (int)xds.size();
-            for (int Ix = 0; Ix < TX / c; Ix++)
-                for (int t = 0; t < ntypes; t++) {
+            ib[s] = nitems;
+            rw[s] = std::min(SY, TY + c / 2);
+            cbs[s] = xp_off;
+            nitems += rw[s] * ntypes * (TX / c);
+            for (int t = 0; t < ntypes; t++)
+                for (int Ix = 0; Ix < TX / c; Ix++) {
                     const Piece pc = make_piece(tpc[t], c);
                     const int f = c * (Ix + 1) - 1;
                     xds.push_back(XD{tsrc[t] * sst, f + pc.lo, f + pc.hi, 0, pc.wi, pc.we});
                 }
-            for (int Iy = 0; Iy < TY / c; Iy++)
-                for (int Ix = 0; Ix < TX / c; Ix++)
-                    for (int q = 0; q < 12; q++) {
+            for (int q = 0; q < 12; q++)
+                for (int Iy = 0; Iy < TY / c; Iy++)
+                    for (int Ix = 0; Ix < TX / c; Ix++) {
                         const Piece py = make_piece(qp[q][1], c), pz = make_piece(qp[q][2], c);
                         const int fy = c * (Iy + 1) - 1;
                         SD d;
@@ -409,7 +444,7 @@ int tiled_launch(ec_ctx *ctx, int n, double l, double visc, void *const u[3], co
                         d.wey = py.we;
                         d.wiz = pz.wi;
                         d.wez = pz.we;
-                        d.xcol = xp_off + Ix * ntypes + qt[q];
+                        d.xcol = xp_off + qt[q] * (TX / c) + Ix;
                         d.ylo = fy + py.lo;
                         d.yhi = fy + py.hi;
                         d.zlo = pz.lo;
@@ -422,13 +457,21 @@ int tiled_launch(ec_ctx *ctx, int n, double l, double visc, void *const u[3], co
                     }
         }
         const int NXP = (int)xds.size(), nslots = (int)sds.size();
-        const size_t lds = (size_t)SY * NXP * 8 + (size_t)NXP * sizeof(XD) +
+        const size_t lds = (size_t)syp * NXP * 8 + (size_t)NXP * sizeof(XD) +
                            bt * (9 * (size_t)UX * UY + (size_t)SX * SY + 9 * (size_t)sst);
         const int threads = 512;
         const int maxs = (nslots + threads - 1) / threads;
         const int pre = (3 * UX * UY + SX * SY + threads - 1) / threads;
-        const int mx = (SY * NXP + threads - 1) / threads;
+        for (int s = nsets; s < 4; s++) {  // unused sets: empty ranges past the end
+            ib[s] = nitems;
+            rw[s] = 1;
+            cbs[s] = (int)xds.size();
+        }
+        const int mx = (nitems + threads - 1) / threads;
         if (lds <= 152 * 1024 && maxs <= 2 && pre <= 16 && mx <= 16) {
+            const int4 ibase = make_int4(ib[0], ib[1], ib[2], ib[3]);
+            const int4 rows = make_int4(rw[0], rw[1], rw[2], rw[3]);
+            const int4 cbase = make_int4(cbs[0], cbs[1], cbs[2], cbs[3]);
             int rc = EC_OK;
             std::vector<char> xb((char *)xds.data(), (char *)(xds.data() + xds.size()));
             std::vector<char> sb((char *)sds.data(), (char *)(sds.data() + sds.size()));
@@ -441,9 +484,9 @@ int tiled_launch(ec_ctx *ctx, int n, double l, double visc, void *const u[3], co
             const double ih = n / l;
             if (maxs <= 1)
                 return launch_tiles<T, 1>(ctx, grid, threads, lds, u, p, dxd, dsd, n, TX, TY, ZC, SX, SY, UX, UY, NXP,
-                                          nslots, hmax, ih, visc);
+                                          nslots, hmax, ibase, rows, cbase, nitems, ih, visc);
             return launch_tiles<T, 2>(ctx, grid, threads, lds, u, p, dxd, dsd, n, TX, TY, ZC, SX, SY, UX, UY, NXP,
-                                      nslots, hmax, ih, visc);
+                                      nslots, hmax, ibase, rows, cbase, nitems, ih, visc);
         }
         tx_target = std::max(L, tx_target / 2);
         ty_target = std::max(L, ty_target / 2);
@@ -453,7 +496,7 @@ int tiled_launch(ec_ctx *ctx, int n, double l, double visc, void *const u[3], co
 
 }  // namespace
 
-static int g_tile_x = 32, g_tile_y = 16, g_tile_z = 64;
+static int g_tile_x = 16, g_tile_y = 16, g_tile_z = 64;
 
 extern "C" int ec_closure_set_tiles(ec_ctx *ctx, int tx, int ty, int tz) {
     EC_REQUIRE(ctx, ctx, "null context");
