@@ -41,7 +41,7 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain 1 GiB device copy first (HBM counter calibration)")
     ap.add_argument("--kernels", action="store_true", help="print the per-kernel table to stderr")
-    ap.add_argument("--closure-tiles", default=None, help="tx,ty,tz block targets of the closure kernel")
+    ap.add_argument("--closure-tiles", default="16,16,128", help="tx,ty,tz block targets of the closure kernel")
     return ap.parse_args()
 
 
@@ -49,7 +49,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n=256, nsteps=1):
+def cpu_baseline(n=256, nsteps=5):
     """The FP64 C oracle (OpenMP, all host threads) on the 256^3 DNS-aided RK4
     step with the 32^3 volume-average closure; steps/s scaled to 1024^3 by N."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -64,7 +64,7 @@ def cpu_baseline(n=256, nsteps=1):
     sps = nsteps / t
     return {"value": sps * (n / 1024) ** 3, "unit": "steps/s (1024^3-equivalent)", "cores": threads,
             "kind": "port",
-            "sample": f"{nsteps} RK4 DNS-aided step at {n}^3 FP64 with the 32^3 closure: {t:.2f} s "
+            "sample": f"{nsteps} RK4 DNS-aided steps at {n}^3 FP64 with the 32^3 closure: {t:.2f} s "
                       f"({sps:.4f} steps/s at {n}^3), scaled by (n/1024)^3"}
 
 
@@ -155,12 +155,12 @@ def main():
     les_err = drv.les_errors()
     div = drv.divergence_norm()
     if not a.no_closure_check:
-        taus32 = drv.closure_taus()
+        taus32 = drv.closure_terms()
         u_state = [x.double() for x in drv.u0]
         del drv
         torch.cuda.empty_cache()
         d64 = D.DNSAidedLES(g, a.visc, sets, dtype=torch.float64)
-        taus64 = d64.closure_taus(u_state)
+        taus64 = d64.closure_terms(u_state)
         num = den = 0.0
         for s32, s64 in zip(taus32, taus64):
             for x, y in zip(s32, s64):

@@ -20,7 +20,7 @@ EC_F32 = 0
 EC_F64 = 1
 
 SRC = {"u1": 0, "u2": 1, "u3": 2, "s11": 3, "s22": 4, "s33": 5, "s12": 6, "s23": 7,
-       "s31": 8, "r11": 9, "r22": 10, "r33": 11, "p": 12}
+       "s31": 8, "r11": 9, "r22": 10, "r33": 11, "p": 12, "k1": 13, "k2": 14, "k3": 15}
 
 c_int, c_double, c_size_t, c_void_p = ctypes.c_int, ctypes.c_double, ctypes.c_size_t, ctypes.c_void_p
 c_u64 = ctypes.c_uint64
@@ -38,6 +38,10 @@ class ECError(RuntimeError):
 class FilterQuantity(ctypes.Structure):
     _fields_ = [("source", c_int), ("len", c_int * 3), ("off", c_int * 3),
                 ("w", ctypes.POINTER(c_double) * 3), ("out", c_void_p)]
+
+
+class VelSet(ctypes.Structure):
+    _fields_ = [("nbar", c_int), ("kind", c_int), ("out", c_void_p * 3)]
 
 
 class ClosureSet(ctypes.Structure):
@@ -59,6 +63,7 @@ _SIGS = {
     "ec_zero": (c_int, [c_void_p, c_void_p, c_size_t]),
     "ec_axpy": (c_int, [c_void_p, c_int, c_size_t, c_double, c_void_p, c_void_p]),
     "ec_convert": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_size_t]),
+    "ec_axpy2": (c_int, [c_void_p, c_int, c_size_t, c_double, c_int, c_void_p, c_void_p]),
     "ec_profile_begin": (c_int, [c_void_p]),
     "ec_profile_end": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(c_double),
                               ctypes.POINTER(c_double), c_int, ctypes.POINTER(c_int)]),
@@ -86,6 +91,8 @@ _SIGS = {
     "ec_ns_closure": (c_int, [c_void_p, c_int, c_int, c_double, c_double, P3, c_void_p, c_int,
                               ctypes.POINTER(ClosureSet)]),
     "ec_closure_set_tiles": (c_int, [c_void_p, c_int, c_int, c_int]),
+    "ec_ns_filter_vel": (c_int, [c_void_p, c_int, c_int, c_double, P3, c_void_p, c_int, ctypes.POINTER(VelSet)]),
+    "ec_filter_vel_set_tiles": (c_int, [c_void_p, c_int, c_int]),
     "ec_ns_commutator": (c_int, [c_void_p, c_int, c_double, c_double, P3, P9, P9]),
     "ec_ns_rk4_update": (c_int, [c_void_p, c_int, c_int, c_double, c_int, c_double, P3, P3, P3,
                                  c_void_p, P3]),

@@ -73,6 +73,9 @@ struct Src {
             case EC_SRC_R11: return stress(0, 0, i, j, k) + at(p, i, j, k);
             case EC_SRC_R22: return stress(1, 1, i, j, k) + at(p, i, j, k);
             case EC_SRC_R33: return stress(2, 2, i, j, k) + at(p, i, j, k);
+            case EC_SRC_K1: return at(u[0], i, j, k) - (at(p, i + 1 == n ? 0 : i + 1, j, k) - at(p, i, j, k)) * ih;
+            case EC_SRC_K2: return at(u[1], i, j, k) - (at(p, i, j + 1 == n ? 0 : j + 1, k) - at(p, i, j, k)) * ih;
+            case EC_SRC_K3: return at(u[2], i, j, k) - (at(p, i, j, k + 1 == n ? 0 : k + 1) - at(p, i, j, k)) * ih;
             default: return at(p, i, j, k);
         }
     }
@@ -175,7 +178,7 @@ static int filter_launch(ec_ctx *ctx, int n, int nbar, double l, double visc, vo
                 if (rc) return rc;
             }
             EC_REQUIRE(ctx, src.out, "filter quantity %d: null output", q0 + m);
-            EC_REQUIRE(ctx, src.source >= 0 && src.source <= EC_SRC_P, "filter quantity %d: bad source", q0 + m);
+            EC_REQUIRE(ctx, src.source >= 0 && src.source <= EC_SRC_K3, "filter quantity %d: bad source", q0 + m);
             EC_REQUIRE(ctx, (src.source < EC_SRC_R11) || p, "pressure source without pressure field");
             d.out = (double *)src.out;
         }

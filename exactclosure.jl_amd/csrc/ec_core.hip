@@ -134,6 +134,29 @@ extern "C" int ec_axpy(ec_ctx *ctx, int dtype, size_t count, double a, const voi
     return EC_OK;
 }
 
+template <typename Y, typename X>
+__global__ void k_axpy2(size_t count, double a, const X *__restrict__ x, Y *__restrict__ y) {
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < count; i += stride) y[i] = (Y)((double)y[i] + a * (double)x[i]);
+}
+
+extern "C" int ec_axpy2(ec_ctx *ctx, int ydtype, size_t count, double a, int xdtype, const void *x, void *y) {
+    EC_REQUIRE(ctx, ctx && (count == 0 || (x && y)), "ec_axpy2: null argument");
+    if (count == 0) return EC_OK;
+    const int blocks = (int)std::min<size_t>((count + 255) / 256, 1 << 20);
+    if (ydtype == EC_F32 && xdtype == EC_F64)
+        k_axpy2<float, double><<<blocks, 256, 0, ctx->stream>>>(count, a, (const double *)x, (float *)y);
+    else if (ydtype == EC_F64 && xdtype == EC_F64)
+        k_axpy2<double, double><<<blocks, 256, 0, ctx->stream>>>(count, a, (const double *)x, (double *)y);
+    else if (ydtype == EC_F64 && xdtype == EC_F32)
+        k_axpy2<double, float><<<blocks, 256, 0, ctx->stream>>>(count, a, (const float *)x, (double *)y);
+    else
+        k_axpy2<float, float><<<blocks, 256, 0, ctx->stream>>>(count, a, (const float *)x, (float *)y);
+    EC_CHECK_LAUNCH(ctx, "k_axpy2");
+    return EC_OK;
+}
+
 template <typename D, typename S>
 __global__ void k_convert(size_t count, D *__restrict__ dst, const S *__restrict__ src) {
     size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;

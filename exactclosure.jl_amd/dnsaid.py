@@ -17,6 +17,14 @@ projection at every stage as in step_wray3!, src/N.jl:320-346):
 With the volume-average filter the LES reproduces P_H V u_dns to rounding
 (the exactness invariant of navierstokes.jl:33).  Fields run in `dtype`
 (FP32 or FP64); the closure commutator runs in FP64.
+
+closure="forcing" (the default) evaluates the same closure in divergence
+form: V(k) = -div_H^ns (S r) by the commutation identity of the filter pair,
+so step 3 becomes one filter launch of k = du - grad p for all LES grids,
+v = V u is carried through the RK4 stages in FP64, and the closure forcing
+m = V(k) - F_H(P_H v) (the commutator, FP64) drives
+du_les = F_H(u_les) + m.  closure="flux" keeps the nine-component flux of
+step 3 above.
 """
 from __future__ import annotations
 
@@ -32,11 +40,21 @@ def _vec(n, dtype):
 
 
 class LESGrid:
-    def __init__(self, g_dns, nbar, kind, dtype):
+    def __init__(self, g_dns, nbar, kind, dtype, closure="forcing"):
         self.g = NS.Grid(g_dns.l, nbar)
         self.kind = kind
         self.dtype = dtype
-        self.cset = NS.ClosureSet(g_dns, nbar, kind)
+        self.cset = NS.ClosureSet(g_dns, nbar, kind) if closure == "flux" else None
+        f64 = torch.float64
+        if closure == "forcing":
+            self.v0 = _vec(nbar, f64)    # V u at the step start (unprojected)
+            self.vs = _vec(nbar, f64)    # V u at the current stage
+            self.vacc = _vec(nbar, f64)
+            self.g_k = _vec(nbar, f64)   # V k of the current stage
+            self.ubar = _vec(nbar, f64)  # P_H V u
+            self.fh = _vec(nbar, f64)    # F_H(ubar)
+            self.m = _vec(nbar, f64)     # closure forcing V k - F_H(ubar)
+            self.zero64 = torch.zeros((nbar,) * 3, dtype=f64, device="cuda")
         self.ps = NS.PoissonSolver(self.g, dtype)
         self.ps64 = self.ps if dtype == torch.float64 else NS.PoissonSolver(self.g, torch.float64)
         self.u0 = _vec(nbar, dtype)
@@ -52,7 +70,9 @@ class DNSAidedLES:
     (face average).  Memory at n = 1024 FP32: 12 DNS fields + p + half
     spectrum ~ 60 GB."""
 
-    def __init__(self, g_dns, visc, sets, dtype=torch.float32):
+    def __init__(self, g_dns, visc, sets, dtype=torch.float32, closure="forcing"):
+        assert closure in ("forcing", "flux")
+        self.closure = closure
         self.g = g_dns
         self.visc = float(visc)
         self.dtype = dtype
@@ -64,7 +84,7 @@ class DNSAidedLES:
         self.us = _vec(n, dtype)
         self.du = _vec(n, dtype)
         self.acc = _vec(n, dtype)
-        self.les = [LESGrid(g_dns, nbar, kind, dtype) for nbar, kind in sets]
+        self.les = [LESGrid(g_dns, nbar, kind, dtype, closure) for nbar, kind in sets]
 
     # -- state -------------------------------------------------------------
     def set_state(self, u):
@@ -72,8 +92,33 @@ class DNSAidedLES:
             self.u0[a].copy_(u[a].data if isinstance(u[a], NS.Field) else u[a])
         self.init_les()
 
+    def _vel_sets(self, outs):
+        arr = (_abi.VelSet * len(self.les))()
+        for i, (L, o) in enumerate(zip(self.les, outs)):
+            arr[i].nbar = L.g.n
+            arr[i].kind = 0 if L.kind == "va" else 1
+            for a in range(3):
+                arr[i].out[a] = o[a].data_ptr()
+        return arr
+
+    def _filter_vel(self, v, p_ptr, outs):
+        """outs[s][a] = V_s(v_a - delta_a p) for every LES grid, one launch."""
+        if not self.les:
+            return
+        self.ctx.call("ec_ns_filter_vel", self.dt_code, self.g.n, self.g.l, ptrs(v, P3),
+                      _abi.c_void_p(p_ptr) if p_ptr else None, len(self.les), self._vel_sets(outs))
+
     def init_les(self):
         """LES fields start from P_H V u (apply_DHpi, src/N.jl:1228-1237)."""
+        if self.closure == "forcing":
+            self._filter_vel(self.u0, None, [L.v0 for L in self.les])
+            for L in self.les:
+                for a in range(3):
+                    L.ubar[a].copy_(L.v0[a])
+                self.ctx.call("ec_ns_project", L.ps64.h, ptrs(L.ubar, P3))
+                for a in range(3):
+                    L.u0[a].copy_(L.ubar[a])
+            return
         self._closure(self.u0)
         for L in self.les:
             ub = L.cset.ubar
@@ -83,6 +128,12 @@ class DNSAidedLES:
 
     def filtered_dns(self):
         """P_H V u of the current DNS state for every LES grid (FP64 tensors)."""
+        if self.closure == "forcing":
+            outs = [[torch.empty_like(L.v0[0]) for _ in range(3)] for L in self.les]
+            self._filter_vel(self.u0, None, outs)
+            for L, ub in zip(self.les, outs):
+                self.ctx.call("ec_ns_project", L.ps64.h, ptrs(ub, P3))
+            return outs
         self._closure(self.u0)
         out = []
         for L in self.les:
@@ -102,6 +153,8 @@ class DNSAidedLES:
 
     # -- one RK4 step --------------------------------------------------------
     def step(self, dt):
+        if self.closure == "forcing":
+            return self._step_forcing(dt)
         c = self.ctx
         n, l = self.g.n, self.g.l
         for st in range(4):
@@ -125,6 +178,44 @@ class DNSAidedLES:
             c.call("ec_ns_rk4_update", self.dt_code, n, l, st, dt, ptrs(self.u0, P3), ptrs(self.us, P3),
                    ptrs(self.du, P3), _abi.c_void_p(self.ps.p_ptr), ptrs(self.acc, P3))
 
+    def _les_forcing(self, L, st):
+        """m = V k - F_H(P_H v) in FP64 for the stage state of LES grid L."""
+        c = self.ctx
+        gl = L.g
+        v = L.v0 if st == 0 else L.vs
+        for a in range(3):
+            L.ubar[a].copy_(v[a])
+        c.call("ec_ns_project", L.ps64.h, ptrs(L.ubar, P3))
+        c.call("ec_ns_rhs", _abi.EC_F64, gl.n, gl.l, self.visc, ptrs(L.ubar, P3), ptrs(L.fh, P3), 0)
+        for a in range(3):
+            L.m[a].copy_(L.g_k[a])
+            c.call("ec_axpy", _abi.EC_F64, L.m[a].numel(), -1.0, ptr(L.fh[a]), ptr(L.m[a]))
+        return L.m
+
+    def _step_forcing(self, dt):
+        c = self.ctx
+        n, l = self.g.n, self.g.l
+        for st in range(4):
+            u = self.u0 if st == 0 else self.us
+            c.call("ec_ns_rhs", self.dt_code, n, l, self.visc, ptrs(u, P3), ptrs(self.du, P3), 0)
+            c.call("ec_ns_pressure_solve", self.ps.h, ptrs(self.du, P3))
+            self._filter_vel(self.du, self.ps.p_ptr, [L.g_k for L in self.les])
+            for L in self.les:
+                gl = L.g
+                m = self._les_forcing(L, st)
+                ul = L.u0 if st == 0 else L.us
+                c.call("ec_ns_rhs", self.dt_code, gl.n, gl.l, self.visc, ptrs(ul, P3), ptrs(L.du, P3), 0)
+                for a in range(3):
+                    c.call("ec_axpy2", self.dt_code, L.du[a].numel(), 1.0, _abi.EC_F64, ptr(m[a]), ptr(L.du[a]))
+                c.call("ec_ns_pressure_solve", L.ps.h, ptrs(L.du, P3))
+                c.call("ec_ns_rk4_update", self.dt_code, gl.n, gl.l, st, dt, ptrs(L.u0, P3), ptrs(L.us, P3),
+                       ptrs(L.du, P3), _abi.c_void_p(L.ps.p_ptr), ptrs(L.acc, P3))
+                # v = V u follows the DNS through the same tableau (V k is already projected)
+                c.call("ec_ns_rk4_update", _abi.EC_F64, gl.n, gl.l, st, dt, ptrs(L.v0, P3), ptrs(L.vs, P3),
+                       ptrs(L.g_k, P3), ptr(L.zero64), ptrs(L.vacc, P3))
+            c.call("ec_ns_rk4_update", self.dt_code, n, l, st, dt, ptrs(self.u0, P3), ptrs(self.us, P3),
+                   ptrs(self.du, P3), _abi.c_void_p(self.ps.p_ptr), ptrs(self.acc, P3))
+
     def run(self, dt, nsteps):
         for _ in range(nsteps):
             self.step(dt)
@@ -138,6 +229,21 @@ class DNSAidedLES:
         den = sum(NS.sumsq(x) for x in self.u0)
         h = self.g.l / self.g.n
         return h * (num ** 0.5) / (den ** 0.5)
+
+    def closure_terms(self, u=None):
+        """The FP64 closure of state u (default: the current state) for every
+        LES grid, without advancing anything: the forcing m = V k - F_H(P_H V u)
+        (3 components) with closure="forcing", the flux tau (9) with "flux"."""
+        if self.closure == "flux":
+            return self.closure_taus(u)
+        c = self.ctx
+        u = self.u0 if u is None else u
+        n, l = self.g.n, self.g.l
+        c.call("ec_ns_rhs", self.dt_code, n, l, self.visc, ptrs(u, P3), ptrs(self.du, P3), 0)
+        c.call("ec_ns_pressure_solve", self.ps.h, ptrs(self.du, P3))
+        self._filter_vel(self.du, self.ps.p_ptr, [L.g_k for L in self.les])
+        self._filter_vel(u, None, [L.v0 for L in self.les])
+        return [[t.clone() for t in self._les_forcing(L, 0)] for L in self.les]
 
     def closure_taus(self, u=None):
         """Closure commutator tau (9 FP64 components per LES grid) of state u

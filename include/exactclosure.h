@@ -64,6 +64,8 @@ int ec_d2d(ec_ctx *ctx, void *dst, const void *src, size_t bytes);
 int ec_zero(ec_ctx *ctx, void *dst, size_t bytes);
 /* element-wise y = a*x + y (LinearAlgebra.axpy!, src/N.jl:337,1161,1322) */
 int ec_axpy(ec_ctx *ctx, int dtype, size_t count, double a, const void *x, void *y);
+/* y = a*x + y with x and y of different dtypes (FP64 closure terms into FP32 fields) */
+int ec_axpy2(ec_ctx *ctx, int ydtype, size_t count, double a, int xdtype, const void *x, void *y);
 /* dtype conversion copy (FP64 <-> FP32) */
 int ec_convert(ec_ctx *ctx, int dst_dtype, void *dst, int src_dtype, const void *src, size_t count);
 
@@ -141,6 +143,9 @@ int ec_randn(ec_ctx *ctx, int dtype, size_t count, uint64_t seed, uint64_t offse
 #define EC_SRC_R22 10
 #define EC_SRC_R33 11
 #define EC_SRC_P 12
+#define EC_SRC_K1 13 /* u_a - delta_a p: the projected tendency when u = du */
+#define EC_SRC_K2 14
+#define EC_SRC_K3 15
 
 typedef struct {
     int source;
@@ -168,6 +173,22 @@ typedef struct {
 } ec_closure_set;
 int ec_ns_closure(ec_ctx *ctx, int dtype, int n, double l, double visc, void *const u[3],
                   const void *p, int nsets, const ec_closure_set *sets);
+/* Velocity-type filter of one or more LES grids in one launch:
+ *   out[s][a] = V_s(v_a - delta_a p)   (p may be NULL: V_s v_a)
+ * V_s is the LES velocity filter of set s (kind 0 volume average, 1 face
+ * average; apply_DHpi, src/N.jl:1228-1237, for any compression).  With v the
+ * DNS right-hand side and p its pressure, V(k) equals -div_H^ns (S r) of
+ * get_rDHi (src/N.jl:1038-1073), the exact closure forcing in divergence form.
+ * FP64 outputs. */
+typedef struct {
+    int nbar;
+    int kind;
+    void *out[3];
+} ec_vel_set;
+int ec_ns_filter_vel(ec_ctx *ctx, int dtype, int n, double l, void *const v[3], const void *p,
+                     int nsets, const ec_vel_set *sets);
+/* Target block size (x = y, z) of ec_ns_filter_vel; set by the tile autotuner. */
+int ec_filter_vel_set_tiles(ec_ctx *ctx, int tx, int tz);
 /* Target block size (fine cells x, y, z) of the one-launch closure kernel; the
  * library picks the largest multiples of every compression not above them.
  * Set by the tile autotuner; no counterpart in the reference. */

@@ -445,7 +445,7 @@ def les_rhs_exact(u_les, tau, visc, l=1.0):
     return du, p
 
 
-def dnsaid_rk4(u, sets, visc, dt, nsteps, l=1.0, les=None):
+def dnsaid_rk4(u, sets, visc, dt, nsteps, l=1.0, les=None, closure="flux"):
     """DNS-aided LES with classical RK4 and the exact closure at every stage.
 
     sets: list of (c, kind).  For each set an LES field starts from
@@ -453,6 +453,10 @@ def dnsaid_rk4(u, sets, visc, dt, nsteps, l=1.0, les=None):
     tau = S r(u_s) - sigma_H(P_H V u_s), r = sigma + p I with p from the DNS
     right-hand-side projection (src/N.jl:1305-1319).  Returns (u, les, taus) with
     taus the closure of the last stage evaluated.
+
+    closure="forcing": the same closure in divergence form, the LES driven by
+    F_H(u_les) + m with m = V(k) - F_H(P_H V u_s), k the projected DNS
+    right-hand side (taus then holds the three components of m).
     """
     n = u[0].shape[0]
     names = ["ubar1", "ubar2", "ubar3"] + ["rDHi" + ab for ab in NONSYM_ORDER]
@@ -482,6 +486,19 @@ def dnsaid_rk4(u, sets, visc, dt, nsteps, l=1.0, les=None):
             taus = []
             kl = []
             for si, (c, kind) in enumerate(sets):
+                if closure == "forcing":
+                    F = closure_filters(c, kind)
+                    ub = [filter_coarse(_f(u[a]), n // c, *F[f"ubar{a + 1}"][1]) for a in range(3)]
+                    project(ub, l)
+                    vk = [filter_coarse(_f(k[a]), n // c, *F[f"ubar{a + 1}"][1]) for a in range(3)]
+                    fh = rhs(ub, visc, l)
+                    m = [vk[a] - fh[a] for a in range(3)]
+                    taus.append(m)
+                    kk = rhs(les[si], visc, l)
+                    kk = [kk[a] + m[a] for a in range(3)]
+                    project(kk, l)
+                    kl.append(kk)
+                    continue
                 q, ub = filtered(u, p, c, kind)
                 tau = commutator(q, ub, visc, l)
                 taus.append(tau)

@@ -28,6 +28,55 @@ def test_closure_quantities(gpu, O, n, c, kind):
         assert rel(host(cs.flux[m]), ref["rDHi" + ab]) < 1e-12
 
 
+@pytest.mark.parametrize("n,sets", [(16, [(4, "va")]), (15, [(5, "va"), (3, "fa")]), (12, [(2, "va"), (6, "fa")]),
+                                    (32, [(16, "va"), (8, "va"), (4, "va"), (4, "fa")])])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_filter_vel(gpu, O, n, sets, prec):
+    """V(u) and V(k), k = du - grad p, against the checker's separable filters;
+    and V(k) = -div_H^ns (S r) (the flux form) on the same data."""
+    NS = gpu.navierstokes
+    dt = torch.float64 if prec == "f64" else torch.float32
+    tol = 1e-12 if prec == "f64" else 2e-6
+    rng = np.random.default_rng(n)
+    v = [np.asfortranarray(rng.standard_normal((n, n, n))) for _ in range(3)]
+    p = np.asfortranarray(rng.standard_normal((n, n, n)))
+    k = O.pressuregradient([x.copy(order="F") for x in v], p)
+    ctx = gpu._abi.context()
+    vd = [dev(x, dt) for x in v]
+    outs = []
+    arr = (gpu._abi.VelSet * len(sets))()
+    for i, (c, kind) in enumerate(sets):
+        o = [torch.empty((n // c,) * 3, dtype=torch.float64, device="cuda") for _ in range(3)]
+        outs.append(o)
+        arr[i].nbar = n // c
+        arr[i].kind = 0 if kind == "va" else 1
+        for a in range(3):
+            arr[i].out[a] = o[a].data_ptr()
+    P3 = gpu._abi.P3
+    code = 1 if prec == "f64" else 0
+    for with_p in (False, True):
+        ctx.call("ec_ns_filter_vel", code, n, 1.0, P3(*[x.data_ptr() for x in vd]),
+                 dev(p, dt).data_ptr() if with_p else None, len(sets), arr)
+        for (c, kind), o in zip(sets, outs):
+            F = O.closure_filters(c, kind)
+            for a in range(3):
+                ref = O.filter_coarse((k if with_p else v)[a], n // c, *F[f"ubar{a + 1}"][1])
+                assert rel(host(o[a]), ref) < tol, (c, kind, a, with_p)
+    # commutation: V(k) for k = -div r equals -div^ns_H (S r) for the volume average
+    if prec == "f64":
+        u = O.randomfield(n, seed=1)
+        for c, kind in sets:
+            if kind != "va":
+                continue
+            q = O.closure_quantities(u, p, 0.01, n // c, c, "va")
+            du = O.rhs(u, 0.01)
+            kk = O.pressuregradient([x.copy(order="F") for x in du], p)
+            lhs = [O.filter_coarse(kk[a], n // c, *O.closure_filters(c, "va")[f"ubar{a + 1}"][1]) for a in range(3)]
+            rhs = O.tensordivergence_nonsym([q["rDHi" + ab] for ab in O.NONSYM_ORDER])
+            for a in range(3):
+                assert rel(lhs[a], rhs[a]) < 1e-11
+
+
 def test_filter_coarse_gaussian(gpu, O):
     """lazycoarsegrain(lazyfilter(u, GDH)) with the reference's Gaussian-composed
     kernel at odd compression (get_kernels, src/N.jl:1008-1019)."""
@@ -46,28 +95,31 @@ def test_filter_coarse_gaussian(gpu, O):
     assert rel(pt.numpy(), O.coarsegrain(u[1], n // c, (False, True, False))) < 1e-15
 
 
-@pytest.mark.parametrize("n,sets", [(16, [(4, "va")]), (24, [(3, "va"), (8, "va")]), (16, [(4, "fa"), (2, "va")])])
-def test_dnsaid_rk4_step_fp64(gpu, O, n, sets):
+@pytest.mark.parametrize("closure", ["flux", "forcing"])
+@pytest.mark.parametrize("n,sets", [(16, [(4, "va")]), (24, [(3, "va"), (8, "va")]), (16, [(4, "fa"), (2, "va")]),
+                                    (32, [(16, "va"), (8, "va"), (4, "va"), (4, "fa")])])
+def test_dnsaid_rk4_step_fp64(gpu, O, n, sets, closure):
     NS, D = gpu.navierstokes, gpu.dnsaid
     visc, dt = 1.0 / 500, 2e-3
     g = NS.Grid(1.0, n)
     u = O.randomfield(n, seed=5)
-    drv = D.DNSAidedLES(g, visc, [(n // c, k) for c, k in sets], dtype=torch.float64)
+    drv = D.DNSAidedLES(g, visc, [(n // c, k) for c, k in sets], dtype=torch.float64, closure=closure)
     drv.set_state([dev(x) for x in u])
     drv.run(dt, 2)
-    u_cpu, les_cpu, tau_cpu = O.dnsaid_rk4([x.copy(order="F") for x in u], sets, visc, dt, 2)
+    u_cpu, les_cpu, tau_cpu = O.dnsaid_rk4([x.copy(order="F") for x in u], sets, visc, dt, 2, closure=closure)
     for a in range(3):
         assert rel(host(drv.u0[a]), u_cpu[a]) < 1e-11
     for si, (L, lc) in enumerate(zip(drv.les, les_cpu)):
         for a in range(3):
             assert rel(host(L.u0[a]), lc[a]) < 1e-10
-        for m in range(9):
-            # closure commutator of the last stage evaluated
-            assert rel(host(L.cset.tau[m]), tau_cpu[si][m]) < 1e-9
+        # closure of the last stage evaluated: 9 flux components or 3 forcing components
+        terms = L.cset.tau if closure == "flux" else L.m
+        for m in range(len(terms)):
+            assert rel(host(terms[m]), tau_cpu[si][m]) < 1e-9
     errs = drv.les_errors()
     for (c, k), e in zip(sets, errs):
-        if k == "va":
-            assert e < 1e-12, (c, e)  # exactness invariant
+        if k == "va" or closure == "forcing":
+            assert e < 1e-12, (c, e)  # exactness invariant (the forcing form is exact for any filter)
 
 
 def test_exactness_many_steps_fp64(gpu, O):
@@ -77,14 +129,15 @@ def test_exactness_many_steps_fp64(gpu, O):
     ps = NS.PoissonSolver(g, torch.float64)
     u = NS.randomfield(g, ps, seed=123)
     dt = NS.propose_timestep(u, visc, 0.4)
-    drv = D.DNSAidedLES(g, visc, [(8, "va"), (4, "va"), (16, "va")], dtype=torch.float64)
+    drv = D.DNSAidedLES(g, visc, [(8, "va"), (4, "va"), (16, "va")], dtype=torch.float64, closure="flux")
     drv.set_state(u)
     drv.run(dt, 10)
     assert max(drv.les_errors()) < 1e-12
     assert drv.divergence_norm() < 1e-13
 
 
-def test_fp32_pipeline_closure_error(gpu, O):
+@pytest.mark.parametrize("closure", ["flux", "forcing"])
+def test_fp32_pipeline_closure_error(gpu, O, closure):
     """Closure from the same state: FP32 fields + FP64 commutator vs all-FP64.
     Gate: relative L2 <= 1e-5 over all components and LES points."""
     NS, D = gpu.navierstokes, gpu.dnsaid
@@ -95,22 +148,9 @@ def test_fp32_pipeline_closure_error(gpu, O):
     u32 = [x.data.float() for x in u64]
     res = {}
     for dt, u in ((torch.float32, u32), (torch.float64, [x.double() for x in u32])):
-        drv = D.DNSAidedLES(g, visc, [(16, "va"), (8, "va")], dtype=dt)
+        drv = D.DNSAidedLES(g, visc, [(16, "va"), (8, "va"), (4, "va"), (4, "fa")], dtype=dt, closure=closure)
         drv.set_state(u)
-        c = drv.ctx
-        code = drv.dt_code
-        from ctypes import c_void_p
-        P3 = gpu._abi.P3
-        c.call("ec_ns_rhs", code, n, 1.0, visc, P3(*[x.data_ptr() for x in drv.u0]),
-               P3(*[x.data_ptr() for x in drv.du]), 0)
-        c.call("ec_ns_pressure_solve", drv.ps.h, P3(*[x.data_ptr() for x in drv.du]))
-        drv._closure(drv.u0)
-        taus = []
-        for L in drv.les:
-            ub = L.cset.ubar
-            c.call("ec_ns_project", L.ps64.h, P3(*[x.data_ptr() for x in ub]))
-            taus.append([t.clone() for t in NS.commutator_(L.cset, ub, visc)])
-        res[dt] = taus
+        res[dt] = drv.closure_terms()
     for t32, t64 in zip(res[torch.float32], res[torch.float64]):
         num = sum(float(((a - b) ** 2).sum()) for a, b in zip(t32, t64))
         den = sum(float((b ** 2).sum()) for b in t64)
