@@ -11,9 +11,15 @@
 // the filtered flux divergence of get_rDHi (src/N.jl:1038-1073), so the exact
 // closure forcing of the LES is available without forming the nine stresses.
 //
-// Structure: the plane-marching block of the one-launch closure kernel, with
-// the compressions and kinds of the sets as template parameters so every
-// piece sum is an unrolled, compile-time-length loop.  All sums are FP64.
+// Structure (z-first separable filter): a block owns a TX x TX column tile
+// and marches a chunk of ZC planes along z.  Each thread owns a few
+// (component, x, y) columns and keeps, per set, the running z-piece of each
+// column in an FP64 register, so a plane costs no LDS traffic and no
+// barrier.  When the z-window of a coarse plane of some set closes (every c
+// planes) the closed z-sums go to LDS and the block reduces them along x,
+// then y, and writes the coarse values.  Component a needs columns up to c/2
+// past the tile along a (its face piece) and none past it along the other
+// axis, so no corner columns are read.  All sums are FP64.
 #include "ec_internal.h"
 
 #include <algorithm>
@@ -24,16 +30,17 @@ namespace {
 // periodic wrap of an index within one period of [0, n)
 __device__ __forceinline__ int wrap1(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
-// piece kinds: 0 box (cells cI..cI+c-1), 1 face (top-hat / trapezoid around
-// the coarse face f = c(I+1)-1), 2 point at f
-template <int C, int PIECE, typename V>
-__device__ __forceinline__ double psum(const V *p, int stride, int f) {
+// 1D pieces over FP64 data: 0 box (cells cI..cI+c-1, i.e. f-c+1..f), 1 face
+// (top-hat, or trapezoid for even c, around the coarse face f = c(I+1)-1),
+// 2 point at f
+template <int C, int PIECE>
+__device__ __forceinline__ double psum(const double *p, int stride, int f) {
     if constexpr (PIECE == 0) {
         double a0 = 0.0, a1 = 0.0;
 #pragma unroll
         for (int t = 0; t < C; t++) {
-            if (t & 1) a1 += (double)p[(f - (C - 1) + t) * stride];
-            else a0 += (double)p[(f - (C - 1) + t) * stride];
+            if (t & 1) a1 += p[(f - (C - 1) + t) * stride];
+            else a0 += p[(f - (C - 1) + t) * stride];
         }
         return (a0 + a1) * (1.0 / C);
     } else if constexpr (PIECE == 1) {
@@ -42,32 +49,22 @@ __device__ __forceinline__ double psum(const V *p, int stride, int f) {
         if constexpr (C & 1) {
 #pragma unroll
             for (int t = -H; t <= H; t++) {
-                if (t & 1) a1 += (double)p[(f + t) * stride];
-                else a0 += (double)p[(f + t) * stride];
+                if (t & 1) a1 += p[(f + t) * stride];
+                else a0 += p[(f + t) * stride];
             }
             return (a0 + a1) * (1.0 / C);
         } else {
 #pragma unroll
             for (int t = -H + 1; t <= H - 1; t++) {
-                if (t & 1) a1 += (double)p[(f + t) * stride];
-                else a0 += (double)p[(f + t) * stride];
+                if (t & 1) a1 += p[(f + t) * stride];
+                else a0 += p[(f + t) * stride];
             }
-            return ((a0 + a1) + 0.5 * ((double)p[(f - H) * stride] + (double)p[(f + H) * stride])) * (1.0 / C);
+            return ((a0 + a1) + 0.5 * (p[(f - H) * stride] + p[(f + H) * stride])) * (1.0 / C);
         }
     } else {
-        return (double)p[f * stride];
+        return p[f * stride];
     }
 }
-
-// piece of component a along axis d for a set of kind K (0 volume, 1 face)
-template <int K>
-__host__ __device__ constexpr int piece_of(int a, int d) {
-    return d == a ? (K == 0 ? 1 : 2) : 0;
-}
-
-// high extent of a piece beyond the coarse face (face: c/2, else 0)
-template <int C>
-constexpr int hext() { return C / 2; }
 
 struct FSet {
     double *out[3];
@@ -78,199 +75,311 @@ struct FArgs {
     FSet s[4];
 };
 
-template <int C, int K>
-struct SetCfg {
-    static constexpr int c = C, kind = K;
+// Compile-time description of the tile (TX x TY columns, NT threads) and up
+// to four sets (C = 0: absent) with their LDS layout.
+template <int TX_, int TY_, int NT_, int C0, int K0, int C1, int K1, int C2, int K2, int C3, int K3>
+struct Cfg {
+    static constexpr int TX = TX_, TY = TY_, NT = NT_;
+    static constexpr int c(int s) { return s == 0 ? C0 : s == 1 ? C1 : s == 2 ? C2 : C3; }
+    static constexpr int kind(int s) { return s == 0 ? K0 : s == 1 ? K1 : s == 2 ? K2 : K3; }
+    // columns needed past the tile along a for component a (volume: face piece)
+    static constexpr int h(int s) { return c(s) && kind(s) == 0 ? c(s) / 2 : 0; }
+    static constexpr int hmax = std::max(std::max(h(0), h(1)), std::max(h(2), h(3)));
+    static constexpr int lcm() {
+        int l = 1;
+        for (int s = 0; s < 4; s++)
+            if (c(s)) l = std::lcm(l, c(s));
+        return l;
+    }
+    // columns of the tile: component 0 (TX + hmax) x TY, 1 TX x (TY + hmax),
+    // 2 TX x TY; each region padded to whole waves so a wave holds one component
+    static constexpr int up64(int x) { return (x + 63) / 64 * 64; }
+    static constexpr int N0 = (TX + hmax) * TY, N1 = TX * (TY + hmax), N2 = TX * TY;
+    static constexpr int P0 = up64(N0), P1 = up64(N1), P2 = up64(N2);
+    static constexpr int NCC = P0 + P1 + P2;
+    static constexpr int NJ = (NCC + NT - 1) / NT;
+    // closed z-sums per (set, component): width (odd, for the banks) x height
+    static constexpr int zw(int s, int a) { return (a == 0 ? TX + h(s) : TX) | 1; }
+    static constexpr int zh(int s, int a) { return a == 1 ? TY + h(s) : TY; }
+    static constexpr int zoff(int s, int a) {
+        int o = 0;
+        for (int t = 0; t < 4; t++)
+            for (int b = 0; b < 3; b++) {
+                if (t == s && b == a) return o;
+                if (c(t)) o += zw(t, b) * zh(t, b);
+            }
+        return o;
+    }
+    // x-partials per (set, component): TX/c columns of XR rows
+    static constexpr int XR = (TY + hmax) | 1;
+    static constexpr int xoff(int s, int a) {
+        int o = zoff(4, 0);
+        for (int t = 0; t < 4; t++)
+            for (int b = 0; b < 3; b++) {
+                if (t == s && b == a) return o;
+                if (c(t)) o += (TX / c(t)) * XR;
+            }
+        return o;
+    }
+    static constexpr int lds_doubles = xoff(4, 0);
 };
 
-// ---- per-set phases, instantiated per (C, K) ----
-template <typename T, int C, int K>
-__device__ __forceinline__ void xpartials(const T *S, int SSTR, int SXP, int TX, int TY, double *XP, int XOFF,
-                                          int tid, int nthr) {
-    // items ordered (component, Ix, row): rows fastest so a wave shares one piece
-    constexpr int R_EXTRA = C / 2;
-    const int rows = TY + R_EXTRA;
-    const int nix = TX / C;
-    const int per_a = nix * rows;
-    for (int it = tid; it < 3 * per_a; it += nthr) {
-        const int a = it / per_a, r0 = it - a * per_a;
-        const int Ix = r0 / rows, row = r0 - Ix * rows;
-        const T *srow = S + a * SSTR + row * SXP;
-        const int f = C * (Ix + 1) - 1;
-        double v;
-        if (a == 0) v = psum<C, piece_of<K>(0, 0)>(srow, 1, f);
-        else v = psum<C, 0>(srow, 1, f);  // components 2, 3: box along x
-        XP[XOFF + (a * nix + Ix) * 32 + row] = v;
-    }
-}
-
-template <typename T, int C, int K>
-__device__ __forceinline__ void slots(const double *XP, int XOFF, int TX, int TY, int ZC, int x0, int y0, int z0,
-                                      int k, int tid, int nthr, int soff, const FSet &fs, double *acc0,
-                                      double *acc1, int m_base) {
-    // slot (component, Iy, Ix) for this set, slots soff.. handled by threads
-    const int nix = TX / C, niy = TY / C;
-    const int ns = 3 * nix * niy;
-    const int sl = tid - soff;
-    (void)m_base;
-    if (sl < 0 || sl >= ns) return;
-    const int a = sl / (nix * niy), c0 = sl - a * nix * niy;
-    const int Iy = c0 / nix, Ix = c0 - Iy * nix;
-    const double *xcol = XP + XOFF + (a * nix + Ix) * 32;
-    const int fy = C * (Iy + 1) - 1;
-    double P;
-    if (a == 1) P = psum<C, piece_of<K>(1, 1)>(xcol, 1, fy);
-    else P = psum<C, 0>(xcol, 1, fy);
-    // z piece: component 3 takes the face piece, the others the box
-    const int Kmin = z0 / C, Kmax = (z0 + ZC) / C;
-    const int K1 = k / C;
-#pragma unroll
-    for (int dd = 0; dd < 2; dd++) {
-        const int Kc = K1 - dd;
-        if (Kc < Kmin || Kc >= Kmax) continue;
-        const int t = k - (C * (Kc + 1) - 1);
-        double w = 0.0;
-        if (a == 2) {
-            if constexpr (K == 0) {
-                if (t >= -(C / 2) && t <= C / 2)
-                    w = ((C & 1) == 0 && (t == -(C / 2) || t == C / 2)) ? 0.5 / C : 1.0 / C;
-            } else {
-                w = t == 0 ? 1.0 : 0.0;
-            }
+// z-window state of a set with compression C at plane r of the chunk
+template <int C>
+struct ZWin {
+    bool box_on, box_start, box_emit;     // box / point windows [cK, cK + c - 1]
+    bool face_on, face_start, face_emit;  // face windows around c(K + 1) - 1
+    int kbox, kface;                      // chunk-relative coarse plane an emit closes
+    __device__ __forceinline__ ZWin(int r, int ZC) {
+        const int ph = r % C;
+        box_on = r < ZC;
+        box_start = ph == 0;
+        box_emit = box_on && ph == C - 1;
+        kbox = r / C;
+        constexpr int r0 = (C & 1) ? (C - 1) / 2 : C / 2 - 1;
+        const int d = r - r0;
+        const int fph = d >= 0 ? d % C : 1;
+        if constexpr (C & 1) {
+            face_on = d >= 0 && d < ZC;
+            face_start = fph == 0;
+            face_emit = face_on && fph == C - 1;
+            kface = d >= 0 ? d / C : 0;
         } else {
-            w = (t >= -(C - 1) && t <= 0) ? 1.0 / C : 0.0;
+            // trapezoid: the last tap (weight 1/2) of window K is the first of K + 1
+            face_on = d >= 0 && d <= ZC;
+            face_start = fph == 0;
+            face_emit = face_on && fph == 0 && d > 0;
+            kface = d > 0 ? d / C - 1 : 0;
         }
-        if (Kc & 1) *acc1 += w * P;
-        else *acc0 += w * P;
-        if (t == C / 2) {
-            const size_t nb = (size_t)fs.nb;
-            const size_t o = (size_t)(x0 / C + Ix) + nb * ((size_t)(y0 / C + Iy) + nb * (size_t)Kc);
-            double *out = fs.out[a];
-            if (Kc & 1) {
-                out[o] = *acc1;
-                *acc1 = 0.0;
+    }
+};
+
+// z update of set S for one column of component a (uniform over the wave)
+// with source value src.  The running sums are updated branch-free,
+// acc = m * acc + w * src with uniform m (0 at a window start) and w: a
+// window start discards what came before it, so planes outside every window
+// of the chunk are harmless.  When a window closes, the closed z-sum goes to
+// the LDS plane of (S, a); padding lanes (ok false) write nothing.
+template <class G, int S>
+__device__ __forceinline__ void zupdate(int r, int ZC, int a, bool ok, int lx, int ly, double src, double &acc,
+                                        double *lds) {
+    constexpr int C = G::c(S), K = G::kind(S), h = G::h(S);
+    constexpr int r0 = (C & 1) ? (C - 1) / 2 : C / 2 - 1;  // first face-window start
+    const int ph = r % C;
+    const bool box_emit = r < ZC && ph == C - 1;
+    if (a != 2 || K == 1) {
+        if (a != 2) acc = (ph == 0 ? 0.0 : acc) + src;  // box along z
+        if (box_emit && ok) {
+            const double val = a == 2 ? src : acc * (1.0 / C);  // face average: point along z
+            if (a == 0) {
+                if (lx < G::TX + h) lds[G::zoff(S, 0) + ly * G::zw(S, 0) + lx] = val;
+            } else if (a == 1) {
+                if (ly < G::TY + h) lds[G::zoff(S, 1) + ly * G::zw(S, 1) + lx] = val;
             } else {
-                out[o] = *acc0;
-                *acc0 = 0.0;
+                lds[G::zoff(S, 2) + ly * G::zw(S, 2) + lx] = val;
             }
+        }
+    } else {  // volume average, component along z: face piece
+        const int d = r - r0 + C;  // >= 1 for r >= 0; window starts where d % C == 0
+        const int fph = d % C;
+        if constexpr (C & 1) {
+            acc = (fph == 0 ? 0.0 : acc) + src;
+            if (fph == C - 1 && d - C < ZC && d >= C && ok)
+                lds[G::zoff(S, 2) + ly * G::zw(S, 2) + lx] = acc * (1.0 / C);
+        } else {
+            // trapezoid: the last tap (weight 1/2) of window K is the first of K + 1
+            if (fph == 0 && d > C && d - C <= ZC && ok)
+                lds[G::zoff(S, 2) + ly * G::zw(S, 2) + lx] = (acc + 0.5 * src) * (1.0 / C);
+            acc = fph == 0 ? 0.5 * src : acc + src;
         }
     }
 }
 
-// Up to four sets; C = 0 marks an absent set.
-template <typename T, int C0, int K0, int C1, int K1, int C2, int K2, int C3, int K3>
-__global__ void __launch_bounds__(256) k_filter_vel(const T *__restrict__ v1, const T *__restrict__ v2,
-                                                   const T *__restrict__ v3, const T *__restrict__ p, int n, int TX,
-                                                   int TY, int ZC, double ih, FArgs A) {
-    constexpr int CMAX = std::max(std::max(C0, C1), std::max(C2, C3));
-    constexpr int HMAX = CMAX / 2;
-    extern __shared__ double smem_d[];
-    const int SX = TX + HMAX, SY = TY + HMAX;
-    const int SXP = SX | 1;
-    const int SSTR = (SXP * SY) | 1;
-    // XP: per set, 3 components x (TX/C) columns x 32 rows (rows <= TY + C/2 <= 32)
-    constexpr int NCOL0 = C0 ? 3 : 0, NCOL1 = C1 ? 3 : 0, NCOL2 = C2 ? 3 : 0, NCOL3 = C3 ? 3 : 0;
-    const int xo0 = 0;
-    const int xo1 = xo0 + (C0 ? NCOL0 * (TX / (C0 ? C0 : 1)) * 32 : 0);
-    const int xo2 = xo1 + (C1 ? NCOL1 * (TX / (C1 ? C1 : 1)) * 32 : 0);
-    const int xo3 = xo2 + (C2 ? NCOL2 * (TX / (C2 ? C2 : 1)) * 32 : 0);
-    const int xend = xo3 + (C3 ? NCOL3 * (TX / (C3 ? C3 : 1)) * 32 : 0);
-    double *XP = smem_d;
-    T *S = reinterpret_cast<T *>(XP + xend);          // 3 source planes
-    T *Pp = S + 3 * SSTR;                             // p at planes k, k+1: (SX+1) x (SY+1) each
-    const int PW = SX + 1, PH = SY + 1;
-    const size_t nn = (size_t)n, n2 = nn * nn;
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z0 = blockIdx.z * ZC;
-    const int tid = threadIdx.x, nthr = blockDim.x;
-
-    double acc0 = 0.0, acc1 = 0.0;
-    // slot ownership: one slot per thread, sets laid out consecutively
-    const int ns0 = C0 ? 3 * (TX / (C0 ? C0 : 1)) * (TY / (C0 ? C0 : 1)) : 0;
-    const int ns1 = C1 ? 3 * (TX / (C1 ? C1 : 1)) * (TY / (C1 ? C1 : 1)) : 0;
-    const int ns2 = C2 ? 3 * (TX / (C2 ? C2 : 1)) * (TY / (C2 ? C2 : 1)) : 0;
-    const int so1 = ns0, so2 = so1 + ns1, so3 = so2 + ns2;
-
-    // fetch layout: 32 lanes along x, rows ly0, ly0 + 8, ... (NPASS passes)
-    constexpr int LW = 32, RP = 256 / LW, NPASS = 4;  // rows <= 32 (checked on the host)
-    const int lx = tid % LW, ly0 = tid / LW;
-    const bool in_v = lx < SX, in_p = lx < PW;
-    const size_t Xo = (size_t)wrap1(x0 + lx, n);
-    auto rowoff = [&](int ly) { return Xo + nn * (size_t)wrap1(y0 + ly, n); };
-
-    // v at plane z0, p at planes z0 and z0+1
-    {
-        const size_t k0 = (size_t)wrap1(z0, n) * n2, k1 = (size_t)wrap1(z0 + 1, n) * n2;
-#pragma unroll
-        for (int i = 0; i < NPASS; i++) {
-            const int ly = ly0 + i * RP;
-            if (in_v && ly < SY) {
-                const size_t o = rowoff(ly) + k0;
-                const int so = ly * SXP + lx;
-                S[so] = v1[o];
-                S[SSTR + so] = v2[o];
-                S[2 * SSTR + so] = v3[o];
-            }
-            if (in_p && ly < PH) {
-                const size_t o = rowoff(ly);
-                Pp[ly * PW + lx] = p ? p[o + k0] : T(0);
-                Pp[PW * PH + ly * PW + lx] = p ? p[o + k1] : T(0);
-            }
-        }
+// x stage of (S, A): x-partials of the closed z-sums, rows fastest
+template <class G, int S, int A>
+__device__ __forceinline__ void xstage(double *lds, int tid) {
+    constexpr int C = G::c(S), K = G::kind(S);
+    constexpr int rows = A == 1 ? G::TY + G::h(S) : G::TY, nix = G::TX / C;
+    constexpr int PX = A == 0 ? (K == 0 ? 1 : 2) : 0;
+    const int t = (tid + 64 * (S * 3 + A)) % G::NT;  // spread the stages over the waves
+    for (int it = t; it < nix * rows; it += G::NT) {
+        const int Ix = it / rows, row = it - Ix * rows;
+        const double *zr = lds + G::zoff(S, A) + row * G::zw(S, A);
+        lds[G::xoff(S, A) + Ix * G::XR + row] = psum<C, PX>(zr, 1, C * (Ix + 1) - 1);
     }
-    __syncthreads();
-    const int kend = z0 + ZC - 1 + HMAX;
-    for (int k = z0; k <= kend; ++k) {
-        const int cur = (k - z0) & 1, nxt = cur ^ 1;
-        // prefetch v(k+1) and p(k+2); they land after the slot phase
-        T pv[3][NPASS], pp[NPASS];
-        {
-            const size_t kv = (size_t)wrap1(k + 1, n) * n2, kq = (size_t)wrap1(k + 2, n) * n2;
+}
+
+// y stage of (S, A): coarse values of the closed coarse plane Kc
+template <class G, int S, int A>
+__device__ __forceinline__ void ystage(const double *lds, int tid, int x0, int y0, int Kc, const FSet *fsl) {
+    const FSet &fs = fsl[S];
+    constexpr int C = G::c(S), K = G::kind(S);
+    constexpr int nix = G::TX / C, niy = G::TY / C;
+    constexpr int PY = A == 1 ? (K == 0 ? 1 : 2) : 0;
+    const int t = (tid + 64 * (S * 3 + A)) % G::NT;
+    for (int it = t; it < nix * niy; it += G::NT) {
+        const int Iy = it / nix, Ix = it - Iy * nix;
+        const double v = psum<C, PY>(lds + G::xoff(S, A) + Ix * G::XR, 1, C * (Iy + 1) - 1);
+        const size_t nb = (size_t)fs.nb;
+        fs.out[A][(size_t)(x0 / C + Ix) + nb * ((size_t)(y0 / C + Iy) + nb * (size_t)Kc)] = v;
+    }
+}
+
+template <class G, int S>
+__device__ __forceinline__ void set_plane(int r, int ZC, const int *ca, const bool *ok, const int *clx,
+                                          const int *cly, const double *src, double (*acc)[4], double *lds) {
+    if constexpr (G::c(S) > 0) {
 #pragma unroll
-            for (int i = 0; i < NPASS; i++) {
-                const int ly = ly0 + i * RP;
-                const size_t o = rowoff(ly < PH ? ly : 0);
-                if (in_v && ly < SY) {
-                    pv[0][i] = v1[o + kv];
-                    pv[1][i] = v2[o + kv];
-                    pv[2][i] = v3[o + kv];
-                }
-                if (p && in_p && ly < PH) pp[i] = p[o + kq];
-            }
+        for (int j = 0; j < G::NJ; j++) zupdate<G, S>(r, ZC, ca[j], ok[j], clx[j], cly[j], src[j], acc[j][S], lds);
+    }
+}
+
+template <class G, int S>
+__device__ __forceinline__ bool set_emits(int r, int ZC) {
+    if constexpr (G::c(S) > 0) {
+        const ZWin<G::c(S)> w(r, ZC);
+        return w.box_emit || (G::kind(S) == 0 && w.face_emit);
+    } else {
+        return false;
+    }
+}
+
+template <class G, int S>
+__device__ __forceinline__ void set_xstage(int r, int ZC, double *lds, int tid) {
+    if constexpr (G::c(S) > 0) {
+        const ZWin<G::c(S)> w(r, ZC);
+        if (w.box_emit) {
+            xstage<G, S, 0>(lds, tid);
+            xstage<G, S, 1>(lds, tid);
+            if constexpr (G::kind(S) == 1) xstage<G, S, 2>(lds, tid);
         }
-        // sources: k_a = v_a - delta_a p
-        if (p) {
-            const T *pc = Pp + cur * PW * PH, *pn = Pp + nxt * PW * PH;
-            for (int e = tid; e < SX * SY; e += nthr) {
-                const int ly = (int)(((float)e + 0.5f) * (1.0f / SX)), lx2 = e - ly * SX;
-                const int so = ly * SXP + lx2, pi = ly * PW + lx2;
-                const double p0 = (double)pc[pi];
-                S[so] = (T)((double)S[so] - ((double)pc[pi + 1] - p0) * ih);
-                S[SSTR + so] = (T)((double)S[SSTR + so] - ((double)pc[pi + PW] - p0) * ih);
-                S[2 * SSTR + so] = (T)((double)S[2 * SSTR + so] - ((double)pn[pi] - p0) * ih);
-            }
+        if constexpr (G::kind(S) == 0)
+            if (w.face_emit) xstage<G, S, 2>(lds, tid);
+    }
+}
+
+template <class G, int S>
+__device__ __forceinline__ void set_ystage(int r, int ZC, const double *lds, int tid, int x0, int y0, int z0,
+                                           const FSet *fs) {
+    if constexpr (G::c(S) > 0) {
+        constexpr int C = G::c(S);
+        const ZWin<C> w(r, ZC);
+        if (w.box_emit) {
+            ystage<G, S, 0>(lds, tid, x0, y0, z0 / C + w.kbox, fs);
+            ystage<G, S, 1>(lds, tid, x0, y0, z0 / C + w.kbox, fs);
+            if constexpr (G::kind(S) == 1) ystage<G, S, 2>(lds, tid, x0, y0, z0 / C + w.kbox, fs);
         }
-        __syncthreads();
-        if constexpr (C0 > 0) xpartials<T, C0, K0>(S, SSTR, SXP, TX, TY, XP, xo0, tid, nthr);
-        if constexpr (C1 > 0) xpartials<T, C1, K1>(S, SSTR, SXP, TX, TY, XP, xo1, tid, nthr);
-        if constexpr (C2 > 0) xpartials<T, C2, K2>(S, SSTR, SXP, TX, TY, XP, xo2, tid, nthr);
-        if constexpr (C3 > 0) xpartials<T, C3, K3>(S, SSTR, SXP, TX, TY, XP, xo3, tid, nthr);
-        __syncthreads();
-        if constexpr (C0 > 0) slots<T, C0, K0>(XP, xo0, TX, TY, ZC, x0, y0, z0, k, tid, nthr, 0, A.s[0], &acc0, &acc1, 0);
-        if constexpr (C1 > 0) slots<T, C1, K1>(XP, xo1, TX, TY, ZC, x0, y0, z0, k, tid, nthr, so1, A.s[1], &acc0, &acc1, 0);
-        if constexpr (C2 > 0) slots<T, C2, K2>(XP, xo2, TX, TY, ZC, x0, y0, z0, k, tid, nthr, so2, A.s[2], &acc0, &acc1, 0);
-        if constexpr (C3 > 0) slots<T, C3, K3>(XP, xo3, TX, TY, ZC, x0, y0, z0, k, tid, nthr, so3, A.s[3], &acc0, &acc1, 0);
+        if constexpr (G::kind(S) == 0)
+            if (w.face_emit) ystage<G, S, 2>(lds, tid, x0, y0, z0 / C + w.kface, fs);
+    }
+}
+
+// Element offsets are 32-bit: the host requires n^3 < 2^32.
+template <typename T, class G>
+__global__ void __launch_bounds__(G::NT) k_filter_vel(const T *__restrict__ v1, const T *__restrict__ v2,
+                                                     const T *__restrict__ v3, const T *__restrict__ p, int n, int ZC,
+                                                     double ih, FArgs A) {
+    constexpr int TX = G::TX, TY = G::TY, H = G::hmax, NJ = G::NJ, NT = G::NT;
+    extern __shared__ double lds[];
+    // output table in LDS: read only when a coarse plane closes, so it holds
+    // no scalar registers through the plane loop
+    FSet *fsl = reinterpret_cast<FSet *>(lds + G::lds_doubles);
+    if (threadIdx.x < 4) fsl[threadIdx.x] = A.s[threadIdx.x];
+    // 1D grid of (n/TX) x (n/TY) x (n/ZC) tiles, x fastest, in XCD order
+    const int gx = n / TX, gy = n / TY;
+    const int L = (int)ec_xcd_tile(blockIdx.x, gridDim.x);
+    const int x0 = (L % gx) * TX, y0 = ((L / gx) % gy) * TY, z0 = (L / (gx * gy)) * ZC;
+    const int tid = threadIdx.x;
+    const unsigned nn = (unsigned)n, n2 = nn * nn;
+
+    // the thread's columns: component (one per wave), tile-local position,
+    // offsets of the value and of the pressure neighbour along the component
+    int ca[NJ], clx[NJ], cly[NJ];
+    bool ok[NJ];
+    unsigned co[NJ], cq[NJ];
 #pragma unroll
-        for (int i = 0; i < NPASS; i++) {
-            const int ly = ly0 + i * RP;
-            if (in_v && ly < SY) {
-                const int so = ly * SXP + lx;
-                S[so] = pv[0][i];
-                S[SSTR + so] = pv[1][i];
-                S[2 * SSTR + so] = pv[2][i];
-            }
-            if (in_p && ly < PH) Pp[cur * PW * PH + ly * PW + lx] = p ? pp[i] : T(0);
+    for (int j = 0; j < NJ; j++) {
+        int e = tid + NT * j, a, lx = 0, ly = 0;
+        bool valid;
+        if (e < G::P0) {
+            a = 0, valid = e < G::N0, lx = e % (TX + H), ly = e / (TX + H);
+        } else if (e < G::P0 + G::P1) {
+            e -= G::P0;
+            a = 1, valid = e < G::N1, lx = e % TX, ly = e / TX;
+        } else {
+            e -= G::P0 + G::P1;
+            a = 2, valid = e < G::N2, lx = e % TX, ly = e / TX;
         }
-        __syncthreads();
+        if (!valid) lx = ly = 0;  // padding: reads column (0, 0), writes nothing
+        ca[j] = __builtin_amdgcn_readfirstlane(a);
+        ok[j] = valid;
+        clx[j] = lx;
+        cly[j] = ly;
+        const unsigned xo = (unsigned)wrap1(x0 + lx, n), yo = (unsigned)wrap1(y0 + ly, n);
+        co[j] = xo + nn * yo;
+        cq[j] = a == 0 ? (unsigned)wrap1(x0 + lx + 1, n) + nn * yo
+                       : (a == 1 ? xo + nn * (unsigned)wrap1(y0 + ly + 1, n) : co[j]);
+    }
+
+    double acc[NJ][4];
+#pragma unroll
+    for (int j = 0; j < NJ; j++)
+#pragma unroll
+        for (int s = 0; s < 4; s++) acc[j][s] = 0.0;
+
+    // register double buffer: v, p and the p neighbour of plane r
+    T cv[NJ], cp[NJ], cqv[NJ];
+    auto load = [&](int r, T *vv, T *pp, T *qq) {
+        const unsigned kk = (unsigned)wrap1(z0 + r, n) * n2, k1 = (unsigned)wrap1(z0 + r + 1, n) * n2;
+#pragma unroll
+        for (int j = 0; j < NJ; j++) {
+            const int a = ca[j];
+            const T *vp = a == 0 ? v1 : (a == 1 ? v2 : v3);
+            vv[j] = vp[co[j] + kk];
+            if (p) {
+                pp[j] = p[co[j] + kk];
+                qq[j] = p[cq[j] + (a == 2 ? k1 : kk)];
+            } else {
+                pp[j] = qq[j] = T(0);
+            }
+        }
+    };
+    load(0, cv, cp, cqv);
+    const int rend = ZC - 1 + H;
+    for (int r = 0; r <= rend; ++r) {
+        T nv[NJ], np[NJ], nq[NJ];
+        load(r + 1, nv, np, nq);
+        double src[NJ];
+#pragma unroll
+        for (int j = 0; j < NJ; j++) src[j] = (double)cv[j] - ((double)cqv[j] - (double)cp[j]) * ih;
+        set_plane<G, 0>(r, ZC, ca, ok, clx, cly, src, acc, lds);
+        set_plane<G, 1>(r, ZC, ca, ok, clx, cly, src, acc, lds);
+        set_plane<G, 2>(r, ZC, ca, ok, clx, cly, src, acc, lds);
+        set_plane<G, 3>(r, ZC, ca, ok, clx, cly, src, acc, lds);
+        const bool any = set_emits<G, 0>(r, ZC) || set_emits<G, 1>(r, ZC) || set_emits<G, 2>(r, ZC) ||
+                         set_emits<G, 3>(r, ZC);
+        if (any) {
+            // an opaque copy of the thread index keeps the stage indexing from
+            // being hoisted out of the plane loop (it would hold registers)
+            int t = tid;
+            asm volatile("" : "+v"(t));
+            __syncthreads();
+            set_xstage<G, 0>(r, ZC, lds, t);
+            set_xstage<G, 1>(r, ZC, lds, t);
+            set_xstage<G, 2>(r, ZC, lds, t);
+            set_xstage<G, 3>(r, ZC, lds, t);
+            __syncthreads();
+            set_ystage<G, 0>(r, ZC, lds, t, x0, y0, z0, fsl);
+            set_ystage<G, 1>(r, ZC, lds, t, x0, y0, z0, fsl);
+            set_ystage<G, 2>(r, ZC, lds, t, x0, y0, z0, fsl);
+            set_ystage<G, 3>(r, ZC, lds, t, x0, y0, z0, fsl);
+        }
+#pragma unroll
+        for (int j = 0; j < NJ; j++) {
+            cv[j] = nv[j];
+            cp[j] = np[j];
+            cqv[j] = nq[j];
+        }
     }
 }
 
@@ -284,57 +393,46 @@ int largest_tile(int n, int L, int target) {
     return best;
 }
 
-template <typename T, int C0, int K0, int C1, int K1, int C2, int K2, int C3, int K3>
-int launch_fv(ec_ctx *ctx, int n, double l, void *const v[3], const void *p, const FArgs &A, int tx_target,
-              int tz_target) {
-    constexpr int CMAX = std::max(std::max(C0, C1), std::max(C2, C3));
-    int L = 1;
-    for (int c : {C0, C1, C2, C3})
-        if (c) L = std::lcm(L, c);
-    if (n % L) return EC_E_UNSUPPORTED;
-    const int TX = largest_tile(n, L, tx_target), TY = TX, ZC = largest_tile(n, L, tz_target);
-    // rows and columns (with the p halo) fit one 32 x 32 fetch; every plane index
-    // read (up to n + CMAX/2 + 1) lies within one period, as wrap1 assumes
-    if (!TX || !ZC || TY + CMAX / 2 + 1 > 32 || TX + CMAX / 2 + 1 > 32 || CMAX / 2 + 1 >= n) return EC_E_UNSUPPORTED;
-    const int SX = TX + CMAX / 2, SY = TY + CMAX / 2, SXP = SX | 1, SSTR = (SXP * SY) | 1;
-    int slots = 0, xp = 0;
-    for (int c : {C0, C1, C2, C3})
-        if (c) {
-            slots += 3 * (TX / c) * (TY / c);
-            xp += 3 * (TX / c) * 32;
-        }
-    const int threads = 256;
-    if (slots > threads) return EC_E_UNSUPPORTED;
-    const size_t lds = (size_t)xp * 8 + sizeof(T) * (3 * (size_t)SSTR + 2 * (size_t)(SX + 1) * (SY + 1));
+template <typename T, class G>
+int launch_fv(ec_ctx *ctx, int n, double l, void *const v[3], const void *p, const FArgs &A, int tz_target) {
+    constexpr int TX = G::TX, TY = G::TY, L = G::lcm();
+    static_assert(TX % L == 0 && TY % L == 0, "tile edges must be multiples of every compression");
+    if (n % TX || n % TY || n % L) return EC_E_UNSUPPORTED;
+    if ((double)n * n * n >= 4294967296.0) return EC_E_UNSUPPORTED;  // 32-bit element offsets
+    // every index read (up to n + hmax + 1 along each axis) lies within one period
+    if (G::hmax + 1 >= n) return EC_E_UNSUPPORTED;
+    const int ZC = largest_tile(n, L, tz_target);
+    if (!ZC) return EC_E_UNSUPPORTED;
+    const size_t lds = sizeof(double) * (size_t)G::lds_doubles + sizeof(FSet) * 4;
     if (lds > 150 * 1024) return EC_E_UNSUPPORTED;
-    dim3 grid(n / TX, n / TY, n / ZC);
+    const long tiles = (long)(n / TX) * (n / TY) * (n / ZC);
+    if (tiles > 0x7fffffffL) return EC_E_UNSUPPORTED;
     ec_prof_scope pr(ctx, "filter_vel", sizeof(T) * (p ? 4.0 : 3.0) * ec_cube(n));
-    EC_HIP(ctx, hipFuncSetAttribute((const void *)k_filter_vel<T, C0, K0, C1, K1, C2, K2, C3, K3>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    k_filter_vel<T, C0, K0, C1, K1, C2, K2, C3, K3><<<grid, threads, lds, ctx->stream>>>(
-        (const T *)v[0], (const T *)v[1], (const T *)v[2], (const T *)p, n, TX, TY, ZC, n / l, A);
+    EC_HIP(ctx, hipFuncSetAttribute((const void *)k_filter_vel<T, G>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds));
+    k_filter_vel<T, G><<<dim3((unsigned)tiles), G::NT, lds, ctx->stream>>>(
+        (const T *)v[0], (const T *)v[1], (const T *)v[2], (const T *)p, n, ZC, n / l, A);
     EC_CHECK_LAUNCH(ctx, "k_filter_vel");
     return EC_OK;
 }
 
 // one set with compression c (instantiated for the common compressions)
 template <typename T, int K>
-int launch_single(ec_ctx *ctx, int n, double l, void *const v[3], const void *p, const FArgs &A, int c, int tx,
-                  int tz) {
+int launch_single(ec_ctx *ctx, int n, double l, void *const v[3], const void *p, const FArgs &A, int c, int tz) {
     switch (c) {
-        case 2: return launch_fv<T, 2, K, 0, 0, 0, 0, 0, 0>(ctx, n, l, v, p, A, tx, tz);
-        case 3: return launch_fv<T, 3, K, 0, 0, 0, 0, 0, 0>(ctx, n, l, v, p, A, tx, tz);
-        case 4: return launch_fv<T, 4, K, 0, 0, 0, 0, 0, 0>(ctx, n, l, v, p, A, tx, tz);
-        case 5: return launch_fv<T, 5, K, 0, 0, 0, 0, 0, 0>(ctx, n, l, v, p, A, tx, tz);
-        case 8: return launch_fv<T, 8, K, 0, 0, 0, 0, 0, 0>(ctx, n, l, v, p, A, tx, tz);
-        case 16: return launch_fv<T, 16, K, 0, 0, 0, 0, 0, 0>(ctx, n, l, v, p, A, tx, tz);
+        case 2: return launch_fv<T, Cfg<32, 16, 512, 2, K, 0, 0, 0, 0, 0, 0>>(ctx, n, l, v, p, A, tz);
+        case 3: return launch_fv<T, Cfg<24, 12, 256, 3, K, 0, 0, 0, 0, 0, 0>>(ctx, n, l, v, p, A, tz);
+        case 4: return launch_fv<T, Cfg<32, 16, 512, 4, K, 0, 0, 0, 0, 0, 0>>(ctx, n, l, v, p, A, tz);
+        case 5: return launch_fv<T, Cfg<20, 10, 256, 5, K, 0, 0, 0, 0, 0, 0>>(ctx, n, l, v, p, A, tz);
+        case 8: return launch_fv<T, Cfg<32, 16, 512, 8, K, 0, 0, 0, 0, 0, 0>>(ctx, n, l, v, p, A, tz);
+        case 16: return launch_fv<T, Cfg<32, 16, 512, 16, K, 0, 0, 0, 0, 0, 0>>(ctx, n, l, v, p, A, tz);
         default: return EC_E_UNSUPPORTED;
     }
 }
 
 template <typename T>
 int filter_vel_t(ec_ctx *ctx, int n, double l, void *const v[3], const void *p, int nsets, const ec_vel_set *sets,
-                 int tx, int tz) {
+                 int tz) {
     // the four-set volume/face configuration of the 1024^3 workload in one launch
     if (nsets == 4) {
         FArgs A;
@@ -345,8 +443,11 @@ int filter_vel_t(ec_ctx *ctx, int n, double l, void *const v[3], const void *p, 
             for (int a = 0; a < 3; a++) A.s[s].out[a] = (double *)sets[s].out[a];
             A.s[s].nb = sets[s].nbar;
         }
-        if (cs[0] == 16 && ks[0] == 0 && cs[1] == 8 && ks[1] == 0 && cs[2] == 4 && ks[2] == 0 && cs[3] == 4 && ks[3] == 1)
-            return launch_fv<T, 16, 0, 8, 0, 4, 0, 4, 1>(ctx, n, l, v, p, A, tx, tz);
+        if (cs[0] == 16 && ks[0] == 0 && cs[1] == 8 && ks[1] == 0 && cs[2] == 4 && ks[2] == 0 && cs[3] == 4 &&
+            ks[3] == 1) {
+            const int rc = launch_fv<T, Cfg<32, 16, 512, 16, 0, 8, 0, 4, 0, 4, 1>>(ctx, n, l, v, p, A, tz);
+            if (rc != EC_E_UNSUPPORTED) return rc;
+        }
     }
     // otherwise one launch per set
     for (int s = 0; s < nsets; s++) {
@@ -354,8 +455,8 @@ int filter_vel_t(ec_ctx *ctx, int n, double l, void *const v[3], const void *p, 
         for (int a = 0; a < 3; a++) A.s[0].out[a] = (double *)sets[s].out[a];
         A.s[0].nb = sets[s].nbar;
         const int c = n / sets[s].nbar;
-        int rc = sets[s].kind == 0 ? launch_single<T, 0>(ctx, n, l, v, p, A, c, tx, tz)
-                                   : launch_single<T, 1>(ctx, n, l, v, p, A, c, tx, tz);
+        int rc = sets[s].kind == 0 ? launch_single<T, 0>(ctx, n, l, v, p, A, c, tz)
+                                   : launch_single<T, 1>(ctx, n, l, v, p, A, c, tz);
         if (rc) return rc;
     }
     return EC_OK;
@@ -363,11 +464,11 @@ int filter_vel_t(ec_ctx *ctx, int n, double l, void *const v[3], const void *p, 
 
 }  // namespace
 
-static int g_fv_tx = 16, g_fv_tz = 64;
+static int g_fv_tz = 256;  // z planes per block (tile chunk); 256 is fastest at 1024^3
 
+// tx is kept for the interface; the column tile edge is fixed per set list
 extern "C" int ec_filter_vel_set_tiles(ec_ctx *ctx, int tx, int tz) {
     EC_REQUIRE(ctx, ctx && tx >= 1 && tz >= 1, "ec_filter_vel_set_tiles: bad argument");
-    g_fv_tx = tx;
     g_fv_tz = tz;
     return EC_OK;
 }
@@ -381,8 +482,8 @@ extern "C" int ec_ns_filter_vel(ec_ctx *ctx, int dtype, int n, double l, void *c
         EC_REQUIRE(ctx, sets[s].kind == 0 || sets[s].kind == 1, "set %d: kind %d", s, sets[s].kind);
         for (int a = 0; a < 3; a++) EC_REQUIRE(ctx, sets[s].out[a], "set %d: null output", s);
     }
-    int rc = dtype == EC_F64 ? filter_vel_t<double>(ctx, n, l, v, p, nsets, sets, g_fv_tx, g_fv_tz)
-                             : filter_vel_t<float>(ctx, n, l, v, p, nsets, sets, g_fv_tx, g_fv_tz);
+    int rc = dtype == EC_F64 ? filter_vel_t<double>(ctx, n, l, v, p, nsets, sets, g_fv_tz)
+                             : filter_vel_t<float>(ctx, n, l, v, p, nsets, sets, g_fv_tz);
     if (rc != EC_E_UNSUPPORTED) return rc;
     // any other compression: the per-quantity separable filter
     static const int src_k[3] = {EC_SRC_K1, EC_SRC_K2, EC_SRC_K3};

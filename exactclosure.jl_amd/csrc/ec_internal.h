@@ -95,6 +95,17 @@ __device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T> b) {
 __device__ __forceinline__ int ec_up(int i, int n) { return i + 1 == n ? 0 : i + 1; }
 __device__ __forceinline__ int ec_dn(int i, int n) { return i == 0 ? n - 1 : i - 1; }
 
+// XCD-aware tile order for a 1D grid.  gfx950 dispatches workgroup b to XCD
+// b % 8, and each XCD has its own L2; this maps the workgroups of one XCD to
+// one contiguous range of logical tiles, so tiles whose halos overlap are
+// fetched into the same L2 at about the same time.  A bijection on
+// [0, total) for any total.
+__device__ __forceinline__ unsigned ec_xcd_tile(unsigned b, unsigned total) {
+    constexpr unsigned NX = 8;
+    const unsigned q = total / NX, r = total % NX, x = b % NX;
+    return x * q + (x < r ? x : r) + b / NX;
+}
+
 static inline size_t ec_cube(int n) { return (size_t)n * (size_t)n * (size_t)n; }
 
 // Poisson solver state (FFT plan, spectrum and pressure buffers).

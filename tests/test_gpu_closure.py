@@ -29,7 +29,10 @@ def test_closure_quantities(gpu, O, n, c, kind):
 
 
 @pytest.mark.parametrize("n,sets", [(16, [(4, "va")]), (15, [(5, "va"), (3, "fa")]), (12, [(2, "va"), (6, "fa")]),
-                                    (32, [(16, "va"), (8, "va"), (4, "va"), (4, "fa")])])
+                                    (32, [(16, "va"), (8, "va"), (4, "va"), (4, "fa")]),
+                                    (48, [(3, "va"), (3, "fa"), (16, "fa"), (2, "va")]),
+                                    (30, [(5, "fa"), (5, "va")]),
+                                    (64, [(16, "va"), (8, "va"), (4, "va"), (4, "fa")])])
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_filter_vel(gpu, O, n, sets, prec):
     """V(u) and V(k), k = du - grad p, against the checker's separable filters;
@@ -54,14 +57,19 @@ def test_filter_vel(gpu, O, n, sets, prec):
             arr[i].out[a] = o[a].data_ptr()
     P3 = gpu._abi.P3
     code = 1 if prec == "f64" else 0
-    for with_p in (False, True):
-        ctx.call("ec_ns_filter_vel", code, n, 1.0, P3(*[x.data_ptr() for x in vd]),
-                 dev(p, dt).data_ptr() if with_p else None, len(sets), arr)
-        for (c, kind), o in zip(sets, outs):
-            F = O.closure_filters(c, kind)
-            for a in range(3):
-                ref = O.filter_coarse((k if with_p else v)[a], n // c, *F[f"ubar{a + 1}"][1])
-                assert rel(host(o[a]), ref) < tol, (c, kind, a, with_p)
+    # n = 64: z chunks of 16 planes, so windows meet chunk boundaries
+    ctx.call("ec_filter_vel_set_tiles", 16, 16 if n == 64 else 64)
+    try:
+        for with_p in (False, True):
+            ctx.call("ec_ns_filter_vel", code, n, 1.0, P3(*[x.data_ptr() for x in vd]),
+                     dev(p, dt).data_ptr() if with_p else None, len(sets), arr)
+            for (c, kind), o in zip(sets, outs):
+                F = O.closure_filters(c, kind)
+                for a in range(3):
+                    ref = O.filter_coarse((k if with_p else v)[a], n // c, *F[f"ubar{a + 1}"][1])
+                    assert rel(host(o[a]), ref) < tol, (c, kind, a, with_p)
+    finally:
+        ctx.call("ec_filter_vel_set_tiles", 16, 256)
     # commutation: V(k) for k = -div r equals -div^ns_H (S r) for the volume average
     if prec == "f64":
         u = O.randomfield(n, seed=1)
