@@ -3,7 +3,7 @@
 Modules mirror the reference's call surface:
   NavierStokes  (src/N.jl)        -> navierstokes
   Burgers       (src/Burgers.jl)  -> burgers
-plus dnsaid (the RK4 DNS-aided LES driver).  All compute runs in
+plus dnsaid (the RK4 DNS-aided LES driver) and cnn (the learned closure).  All compute runs in
 libexactclosure.so (hand-written gfx950 HIP kernels behind a C ABI).
 """
 from . import _abi  # noqa: F401
@@ -12,5 +12,6 @@ from . import navierstokes as NavierStokes  # noqa: F401
 from . import burgers  # noqa: F401
 from . import burgers as Burgers  # noqa: F401
 from . import dnsaid  # noqa: F401
+from . import cnn  # noqa: F401
 
 lib = _abi.lib

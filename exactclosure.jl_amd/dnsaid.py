@@ -85,6 +85,9 @@ class DNSAidedLES:
         self.du = _vec(n, dtype)
         self.acc = _vec(n, dtype)
         self.les = [LESGrid(g_dns, nbar, kind, dtype, closure) for nbar, kind in sets]
+        # objects with .stage(st, dt), advanced after every stage of the forcing
+        # form once the stage's exact closure is known (a posteriori model LES)
+        self.stage_hooks = []
 
     # -- state -------------------------------------------------------------
     def set_state(self, u):
@@ -213,6 +216,8 @@ class DNSAidedLES:
                 # v = V u follows the DNS through the same tableau (V k is already projected)
                 c.call("ec_ns_rk4_update", _abi.EC_F64, gl.n, gl.l, st, dt, ptrs(L.v0, P3), ptrs(L.vs, P3),
                        ptrs(L.g_k, P3), ptr(L.zero64), ptrs(L.vacc, P3))
+            for h in self.stage_hooks:
+                h.stage(st, dt)
             c.call("ec_ns_rk4_update", self.dt_code, n, l, st, dt, ptrs(self.u0, P3), ptrs(self.us, P3),
                    ptrs(self.du, P3), _abi.c_void_p(self.ps.p_ptr), ptrs(self.acc, P3))
 

@@ -223,6 +223,29 @@ int ec_burgers_batch_dt(ec_burgers_batch *batch, int instance, double *dt);
 /* Single-instance operators for parity tests (src/Burgers.jl:89-142). */
 int ec_burgers_rhs(ec_ctx *ctx, int dtype, int n, double L, double visc, const void *u, void *du);
 
+/* ---- learned closure: 3D CNN 3 -> 32 -> 32 -> 32 -> 3, 3x3x3 kernels, periodic
+ * padding, GELU; BF16 weights and activations, FP32 accumulation (MFMA), Adam.
+ * No reference counterpart (the reference has no learned closure); the
+ * network maps the filtered LES velocity to the exact closure forcing.
+ * Parameters (FP32 master copy) are one flat array, layer by layer:
+ * W[27][cout][cin] (tap t = 9 (dz+1) + 3 (dy+1) + (dx+1), cross-correlation
+ * y[p] = sum_t W[t] x[p + (dx, dy, dz)]) then b[cout]. ---- */
+typedef struct ec_cnn ec_cnn;
+int ec_cnn_create(ec_ctx *ctx, uint64_t seed, ec_cnn **net);
+int ec_cnn_destroy(ec_ctx *ctx, ec_cnn *net);
+int ec_cnn_num_params(ec_cnn *net);
+/* which 0: parameters, 1: gradients of the last backward; get 1 reads, 0 writes */
+int ec_cnn_params(ec_ctx *ctx, ec_cnn *net, int which, int get, float *host, int count);
+/* u: 3*batch FP64 n^3 device fields (sample s, component a at u[3s+a]), scaled by
+ * in_scale; out (nullable): 3*batch FP64 fields receiving prediction / out_scale.
+ * Keeps the activations for ec_cnn_backward.  n a multiple of 16. */
+int ec_cnn_forward(ec_ctx *ctx, ec_cnn *net, int n, int batch, const void *const *u, double in_scale,
+                   void *const *out, double out_scale);
+/* Mean squared error against target * tgt_scale (3*batch FP64 fields) of the
+ * last forward; overwrites the gradients; loss (nullable, synchronising). */
+int ec_cnn_backward(ec_ctx *ctx, ec_cnn *net, const void *const *target, double tgt_scale, double *loss);
+int ec_cnn_adam(ec_ctx *ctx, ec_cnn *net, double lr, double beta1, double beta2, double eps);
+
 #ifdef __cplusplus
 }
 #endif
