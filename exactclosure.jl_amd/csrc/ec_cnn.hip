@@ -236,21 +236,30 @@ __global__ void __launch_bounds__(256) k_wgrad(const uint16_t *__restrict__ X, c
     }
 }
 
-// bias gradient partials: db[c] = sum_p dY[p][c]
+// bias gradient partials: db[c] = sum_p dY[p][c]; a thread sums one 16-byte
+// chunk (8 channels) of every point it visits
 template <int CY>
 __global__ void __launch_bounds__(256) k_bias_grad(const uint16_t *__restrict__ dY, size_t npts,
                                                    float *__restrict__ bpart) {
-    __shared__ float red[256];
-    const int tid = threadIdx.x, c = tid % CY, lane_p = tid / CY;
-    constexpr int PP = 256 / CY;
-    float acc = 0.0f;
-    for (size_t p = (size_t)blockIdx.x * PP + lane_p; p < npts; p += (size_t)gridDim.x * PP)
-        acc += bf16_float(dY[p * CY + c]);
-    red[tid] = acc;
+    constexpr int NQ = CY / 8, PP = 256 / NQ;  // chunks per point, points per block pass
+    __shared__ float red[PP][CY + 1];
+    const int tid = threadIdx.x, q = tid % NQ, lp = tid / NQ;
+    float acc[8] = {};
+    for (size_t p = (size_t)blockIdx.x * PP + lp; p < npts; p += (size_t)gridDim.x * PP) {
+        const uint4 v = *(const uint4 *)(dY + p * CY + q * 8);
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            acc[2 * k] += __uint_as_float(w[k] << 16);
+            acc[2 * k + 1] += __uint_as_float(w[k] & 0xffff0000u);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) red[lp][q * 8 + k] = acc[k];
     __syncthreads();
     if (tid < CY) {
         float s = 0.0f;
-        for (int k = 0; k < PP; k++) s += red[k * CY + tid];
+        for (int k = 0; k < PP; k++) s += red[k][tid];
         bpart[blockIdx.x * CH + tid] = s;
     }
 }
@@ -469,7 +478,7 @@ extern "C" int ec_cnn_create(ec_ctx *ctx, uint64_t seed, ec_cnn **net) {
             return fail(ec_fail(ctx, EC_E_OOM, "ec_cnn_create: out of device memory"));
     }
     c->wblocks = 512;
-    c->bblocks = 256;
+    c->bblocks = 1024;
     if (hipMalloc(&c->wpart, (size_t)c->wblocks * 27 * CH * CH * 4) || hipMalloc(&c->bpart, c->bblocks * CH * 4) ||
         hipHostMalloc(&c->lpart_h, sizeof(double) * 65536))
         return fail(ec_fail(ctx, EC_E_OOM, "ec_cnn_create: out of memory"));

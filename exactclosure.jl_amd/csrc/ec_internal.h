@@ -94,6 +94,8 @@ __device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T> b) {
 // periodic neighbours without division (|shift| == 1)
 __device__ __forceinline__ int ec_up(int i, int n) { return i + 1 == n ? 0 : i + 1; }
 __device__ __forceinline__ int ec_dn(int i, int n) { return i == 0 ? n - 1 : i - 1; }
+// periodic wrap of an index within one period of [0, n): i in [-n, 2n)
+__device__ __forceinline__ int ec_wrap_any(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
 // XCD-aware tile order for a 1D grid.  gfx950 dispatches workgroup b to XCD
 // b % 8, and each XCD has its own L2; this maps the workgroups of one XCD to

@@ -169,6 +169,107 @@ __global__ void k_rhs(int n, T ih, T visc, const T *__restrict__ u1, const T *__
     du3[I] = a3;
 }
 
+// Fused RHS for large grids, z-marching: a block owns a TX x TY column tile
+// and marches ZC planes.  Planes k-1 .. k+2 of the three velocity components
+// (with a one-cell x/y halo) rotate through a four-slot LDS ring; the loads of
+// plane k+2 are issued before the stencil of plane k and land after it, so
+// every u value is read from HBM once per tile (the plain kernel re-reads the
+// neighbouring planes, about 4x the compulsory traffic).
+template <typename T, int TX, int TY>
+__global__ void __launch_bounds__(256) k_rhs_march(int n, int ZC, T ih, T visc, const T *__restrict__ u1,
+                                                   const T *__restrict__ u2, const T *__restrict__ u3,
+                                                   T *__restrict__ du1, T *__restrict__ du2, T *__restrict__ du3,
+                                                   int doadd) {
+    constexpr int HX = TX + 2, HY = TY + 2, HP = HX * HY, NLD = (3 * HP + 255) / 256;
+    constexpr int CPT = TX * TY / 256;  // cells per thread and plane
+    static_assert(TX * TY % 256 == 0 && 256 % TX == 0, "tile shape");
+    __shared__ T ring[4][3 * HP];
+    const int gx = n / TX, gy = n / TY;
+    const int L = (int)ec_xcd_tile(blockIdx.x, gridDim.x);
+    const int x0 = (L % gx) * TX, y0 = ((L / gx) % gy) * TY, z0 = (L / (gx * gy)) * ZC;
+    const int tid = threadIdx.x;
+    const unsigned nn = (unsigned)n, n2 = nn * nn;
+    // loader: element e = comp * HP + (hy * HX + hx) of a haloed plane
+    unsigned loff[NLD];
+    int lcomp[NLD];
+#pragma unroll
+    for (int j = 0; j < NLD; j++) {
+        const int e = tid + 256 * j;
+        const int comp = e / HP, q = e - comp * HP, hx = q % HX, hy = q / HX;
+        lcomp[j] = e < 3 * HP ? comp : -1;
+        loff[j] = (unsigned)ec_wrap_any(x0 + hx - 1, n) + nn * (unsigned)ec_wrap_any(y0 + hy - 1, n);
+    }
+    auto load = [&](int k, T *r) {
+        const unsigned kk = (unsigned)ec_wrap_any(k, n) * n2;
+#pragma unroll
+        for (int j = 0; j < NLD; j++) {
+            const T *sp = lcomp[j] == 1 ? u2 : (lcomp[j] == 2 ? u3 : u1);
+            r[j] = sp[loff[j] + kk];
+        }
+    };
+    auto store = [&](int slot, const T *r) {
+#pragma unroll
+        for (int j = 0; j < NLD; j++)
+            if (lcomp[j] >= 0) ring[slot][tid + 256 * j] = r[j];
+    };
+    {
+        T r[NLD];
+        load(z0 - 1, r);
+        store(0, r);
+        load(z0, r);
+        store(1, r);
+        load(z0 + 1, r);
+        store(2, r);
+    }
+    __syncthreads();
+    const int cx = tid % TX, cy0 = tid / TX;
+    for (int rr = 0; rr < ZC; ++rr) {
+        const int k = z0 + rr;
+        T nxt[NLD];
+        if (rr + 1 < ZC) load(k + 2, nxt);
+        const T *pm = ring[rr & 3], *p0 = ring[(rr + 1) & 3], *pp = ring[(rr + 2) & 3];
+#pragma unroll
+        for (int m = 0; m < CPT; m++) {
+            const int cy = cy0 + m * (256 / TX);
+            const int o = (cy + 1) * HX + (cx + 1);  // the cell in a haloed plane
+            const T *a0 = p0, *b0 = p0 + HP, *c0 = p0 + 2 * HP;
+            const T u1_0 = a0[o], u1_xp = a0[o + 1], u1_xm = a0[o - 1], u1_yp = a0[o + HX], u1_ym = a0[o - HX];
+            const T u1_xm_yp = a0[o - 1 + HX], u1_zp = pp[o], u1_zm = pm[o], u1_xm_zp = pp[o - 1];
+            const T u2_0 = b0[o], u2_xp = b0[o + 1], u2_xm = b0[o - 1], u2_yp = b0[o + HX], u2_ym = b0[o - HX];
+            const T u2_xp_ym = b0[o + 1 - HX], u2_zp = pp[HP + o], u2_zm = pm[HP + o], u2_ym_zp = pp[HP + o - HX];
+            const T u3_0 = c0[o], u3_xp = c0[o + 1], u3_xm = c0[o - 1], u3_yp = c0[o + HX], u3_ym = c0[o - HX];
+            const T u3_zp = pp[2 * HP + o], u3_zm = pm[2 * HP + o], u3_xp_zm = pm[2 * HP + o + 1];
+            const T u3_yp_zm = pm[2 * HP + o + HX];
+
+            const T s11_p = s_diag(u1_xp, u1_0, visc, ih), s11_0 = s_diag(u1_0, u1_xm, visc, ih);
+            const T s22_p = s_diag(u2_yp, u2_0, visc, ih), s22_0 = s_diag(u2_0, u2_ym, visc, ih);
+            const T s33_p = s_diag(u3_zp, u3_0, visc, ih), s33_0 = s_diag(u3_0, u3_zm, visc, ih);
+            const T s12_0 = s_off(u1_yp, u1_0, u2_xp, u2_0, visc, ih);
+            const T s12_ym = s_off(u1_0, u1_ym, u2_xp_ym, u2_ym, visc, ih);
+            const T s12_xm = s_off(u1_xm_yp, u1_xm, u2_0, u2_xm, visc, ih);
+            const T s31_0 = s_off(u3_xp, u3_0, u1_zp, u1_0, visc, ih);
+            const T s31_zm = s_off(u3_xp_zm, u3_zm, u1_0, u1_zm, visc, ih);
+            const T s31_xm = s_off(u3_0, u3_xm, u1_xm_zp, u1_xm, visc, ih);
+            const T s23_0 = s_off(u2_zp, u2_0, u3_yp, u3_0, visc, ih);
+            const T s23_zm = s_off(u2_0, u2_zm, u3_yp_zm, u3_zm, visc, ih);
+            const T s23_ym = s_off(u2_ym_zp, u2_ym, u3_0, u3_ym, visc, ih);
+
+            const size_t I = (size_t)(x0 + cx) + (size_t)nn * ((size_t)(y0 + cy) + (size_t)nn * (size_t)k);
+            T a1 = doadd ? du1[I] : T(0);
+            a1 = a1 - (s11_p - s11_0) * ih - (s12_0 - s12_ym) * ih - (s31_0 - s31_zm) * ih;
+            T a2 = doadd ? du2[I] : T(0);
+            a2 = a2 - (s12_0 - s12_xm) * ih - (s22_p - s22_0) * ih - (s23_0 - s23_zm) * ih;
+            T a3 = doadd ? du3[I] : T(0);
+            a3 = a3 - (s31_0 - s31_xm) * ih - (s23_0 - s23_ym) * ih - (s33_p - s33_0) * ih;
+            du1[I] = a1;
+            du2[I] = a2;
+            du3[I] = a3;
+        }
+        if (rr + 1 < ZC) store((rr + 3) & 3, nxt);
+        __syncthreads();
+    }
+}
+
 template <typename T>
 __global__ void k_divergence(int n, T ih, const T *__restrict__ u1, const T *__restrict__ u2,
                              const T *__restrict__ u3, T *__restrict__ d) {
@@ -298,6 +399,21 @@ extern "C" int ec_ns_rhs(ec_ctx *ctx, int dtype, int n, double l, double visc, v
     const double ih = n / l;
     const double b = dtype == EC_F64 ? 8 : 4;
     ec_prof_scope ps(ctx, "ns_rhs", (doadd ? 9.0 : 6.0) * ec_cube(n) * b);
+    if (n % 64 == 0 && (double)n * n * n < 4294967296.0) {
+        // z-marching tiles of 64 x 8 columns, chunks of up to 128 planes
+        const int ZC = n % 128 == 0 ? 128 : 64;
+        const unsigned tiles = (unsigned)((n / 64) * (n / 8) * (n / ZC));
+        if (dtype == EC_F64)
+            k_rhs_march<double, 64, 8><<<tiles, 256, 0, ctx->stream>>>(
+                n, ZC, ih, visc, (const double *)u[0], (const double *)u[1], (const double *)u[2], (double *)du[0],
+                (double *)du[1], (double *)du[2], doadd);
+        else
+            k_rhs_march<float, 64, 8><<<tiles, 256, 0, ctx->stream>>>(
+                n, ZC, (float)ih, (float)visc, (const float *)u[0], (const float *)u[1], (const float *)u[2],
+                (float *)du[0], (float *)du[1], (float *)du[2], doadd);
+        EC_CHECK_LAUNCH(ctx, "k_rhs_march");
+        return EC_OK;
+    }
     if (dtype == EC_F64)
         k_rhs<double><<<g.grid, g.block, 0, ctx->stream>>>(n, ih, visc, (const double *)u[0], (const double *)u[1],
                                                            (const double *)u[2], (double *)du[0], (double *)du[1],

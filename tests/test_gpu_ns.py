@@ -50,7 +50,14 @@ def test_rhs_fp64(gpu, O, n):
         assert rel(host(du[a]), 2 * ref[a]) < 1e-12
 
 
-@pytest.mark.parametrize("n", [16, 32])
+@pytest.mark.parametrize("n", [64, 128, 192])
+def test_rhs_marching_tiles_fp64(gpu, O, n):
+    """Grids that take the z-marching kernel (64 x 8 column tiles, chunks of
+    64 or 128 planes): every tile and chunk boundary, with and without doadd."""
+    test_rhs_fp64(gpu, O, n)
+
+
+@pytest.mark.parametrize("n", [16, 32, 128])
 def test_rhs_fp32(gpu, O, n):
     u = rand_u(O, n, 100 + n)
     visc, l = 0.01, 1.0
