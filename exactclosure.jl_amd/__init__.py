@@ -13,5 +13,6 @@ from . import burgers  # noqa: F401
 from . import burgers as Burgers  # noqa: F401
 from . import dnsaid  # noqa: F401
 from . import cnn  # noqa: F401
+from . import autotune  # noqa: F401
 
 lib = _abi.lib

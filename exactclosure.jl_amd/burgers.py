@@ -106,3 +106,36 @@ def baseline_batch(count=4096, seed=0):
     c = rng.choice([4, 8, 16, 32, 64], size=count)
     perm = rng.permutation(count)
     return n[perm].astype(np.int32), c[perm].astype(np.int32)
+
+
+def benchmark(count=4096, nsteps=1000, dtype="f32", tune=True, seed=0):
+    """Time the batched workload: `nsteps` RK4 steps of every instance (DNS,
+    exact-closure LES and no-model LES) in one graph-replayed persistent
+    launch. Returns a dict with DNS point-steps per second."""
+    import torch
+
+    from . import autotune
+
+    n, c = baseline_batch(count, seed)
+    tuned = None
+    if tune:
+        probe = BurgersBatch(n, c, dtype=dtype, seed=seed + 1)
+        tuned = autotune.tune_burgers_threads(autotune.Autotuner(), probe)
+        probe.close()
+    b = BurgersBatch(n, c, dtype=dtype, seed=seed)
+    if tuned:
+        b.set_threads(tuned)
+    b.run(nsteps, use_graph=True)     # capture the graph of this length, warm up
+    torch.cuda.synchronize()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    b.run(nsteps, use_graph=True)
+    t1.record()
+    t1.synchronize()
+    ms = t0.elapsed_time(t1)
+    pts = int(n.sum())
+    les_pts = int((n // c).sum())
+    out = {"instances": count, "points": pts, "les_points": les_pts, "steps": nsteps, "ms": ms,
+           "point_steps_per_s": pts * nsteps / (ms * 1e-3), "threads": tuned, "dtype": dtype}
+    b.close()
+    return out

@@ -171,8 +171,8 @@ __device__ void lds_fft(cplx<T> *lds, int ls, int n, int t, int nthr, int sign, 
 
 // ---- X pass, forward: pairs of real lines -> half spectra ----
 template <typename T>
-__global__ void k_fft_x_r2c(const T *__restrict__ in, cplx<T> *__restrict__ out, int n, int nh, long nlines,
-                            RadixPlan rp, const cplx<T> *__restrict__ tw, int nthr) {
+__global__ void k_fft_x_r2c(const T *__restrict__ in, cplx<T> *__restrict__ out, int n, int nh, int pitch,
+                            long nlines, RadixPlan rp, const cplx<T> *__restrict__ tw, int nthr) {
     extern __shared__ char smem_raw[];
     cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
     const int pair = threadIdx.x / nthr, t = threadIdx.x % nthr;
@@ -184,7 +184,7 @@ __global__ void k_fft_x_r2c(const T *__restrict__ in, cplx<T> *__restrict__ out,
     for (int e = t; e < n; e += nthr) line[e] = cplx<T>{v0 ? a[e] : T(0), v1 ? b[e] : T(0)};
     __syncthreads();
     lds_fft(line, 1, n, t, nthr, -1, tw, rp);
-    cplx<T> *oa = out + L0 * nh, *ob = out + (L0 + 1) * nh;
+    cplx<T> *oa = out + L0 * pitch, *ob = out + (L0 + 1) * pitch;
     for (int k = t; k < nh; k += nthr) {
         const cplx<T> z = line[k];
         const cplx<T> zc = line[k == 0 ? 0 : n - k];
@@ -198,8 +198,8 @@ __global__ void k_fft_x_r2c(const T *__restrict__ in, cplx<T> *__restrict__ out,
 
 // ---- X pass, inverse: two half spectra -> two real lines (c2r semantics) ----
 template <typename T>
-__global__ void k_fft_x_c2r(const cplx<T> *__restrict__ in, T *__restrict__ out, int n, int nh, long nlines,
-                            RadixPlan rp, const cplx<T> *__restrict__ tw, int nthr, T scale) {
+__global__ void k_fft_x_c2r(const cplx<T> *__restrict__ in, T *__restrict__ out, int n, int nh, int pitch,
+                            long nlines, RadixPlan rp, const cplx<T> *__restrict__ tw, int nthr, T scale) {
     extern __shared__ char smem_raw[];
     cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
     const int pair = threadIdx.x / nthr, t = threadIdx.x % nthr;
@@ -207,7 +207,7 @@ __global__ void k_fft_x_c2r(const cplx<T> *__restrict__ in, T *__restrict__ out,
     const long L0 = 2 * ((long)blockIdx.x * P + pair);
     const bool v0 = L0 < nlines, v1 = L0 + 1 < nlines;
     cplx<T> *line = lds + (size_t)pair * n;
-    const cplx<T> *ia = in + L0 * nh, *ib = in + (L0 + 1) * nh;
+    const cplx<T> *ia = in + L0 * pitch, *ib = in + (L0 + 1) * pitch;
     const int nyq = (n % 2 == 0) ? n / 2 : -1;
     for (int e = t; e < n; e += nthr) {
         cplx<T> A{T(0), T(0)}, B{T(0), T(0)};
@@ -283,7 +283,8 @@ __global__ void k_fft_cols(cplx<T> *__restrict__ data, int n, int ncols, long el
 
 // ---- energy-shell histogram and rescale for randomfield (src/N.jl:375-449) ----
 template <typename T>
-__global__ void k_shell_hist(const cplx<T> *__restrict__ spec, int n, int nh, int kmax, int nshell, double inv_n3,
+__global__ void k_shell_hist(const cplx<T> *__restrict__ spec, int n, int nh, int pitch, int kmax, int nshell,
+                             double inv_n3,
                              double *__restrict__ hist) {
     extern __shared__ double sh[];
     for (int s = threadIdx.x; s < nshell; s += blockDim.x) sh[s] = 0.0;
@@ -298,7 +299,8 @@ __global__ void k_shell_hist(const cplx<T> *__restrict__ spec, int n, int nh, in
         long s = (long)sqrt((double)kk);
         while (s * s > kk) s--;
         while ((s + 1) * (s + 1) <= kk) s++;
-        const double re = (double)spec[I].x * inv_n3, im = (double)spec[I].y * inv_n3;
+        const size_t J = (size_t)kx + (size_t)pitch * ((size_t)iy + (size_t)n * iz);
+        const double re = (double)spec[J].x * inv_n3, im = (double)spec[J].y * inv_n3;
         const double w = (kx >= 1 && kx <= kmax - 1) ? 2.0 : 1.0;
         if (s < nshell) atomicAdd(&sh[s], w * (re * re + im * im) / 2);
     }
@@ -308,7 +310,8 @@ __global__ void k_shell_hist(const cplx<T> *__restrict__ spec, int n, int nh, in
 }
 
 template <typename T>
-__global__ void k_shell_scale(cplx<T> *__restrict__ spec, int n, int nh, int nshell, const double *__restrict__ factor,
+__global__ void k_shell_scale(cplx<T> *__restrict__ spec, int n, int nh, int pitch, int nshell,
+                              const double *__restrict__ factor,
                               double extra) {
     const size_t total = (size_t)nh * n * n;
     for (size_t I = blockIdx.x * (size_t)blockDim.x + threadIdx.x; I < total; I += (size_t)gridDim.x * blockDim.x) {
@@ -321,8 +324,9 @@ __global__ void k_shell_scale(cplx<T> *__restrict__ spec, int n, int nh, int nsh
         while (s * s > kk) s--;
         while ((s + 1) * (s + 1) <= kk) s++;
         const double f = (s < nshell ? factor[s] : 0.0) * extra;
-        spec[I].x = (T)((double)spec[I].x * f);
-        spec[I].y = (T)((double)spec[I].y * f);
+        const size_t J = (size_t)kx + (size_t)pitch * ((size_t)iy + (size_t)n * iz);
+        spec[J].x = (T)((double)spec[J].x * f);
+        spec[J].y = (T)((double)spec[J].y * f);
     }
 }
 
@@ -383,7 +387,7 @@ __device__ __forceinline__ void ct_stages(cplx<T> *lds, int t, const cplx<T> *__
 
 template <typename T, int N, int NTHR>
 __global__ void __launch_bounds__(256) k_fft_x_r2c_ct(const T *__restrict__ in, cplx<T> *__restrict__ out, long nlines,
-                                                      const cplx<T> *__restrict__ tw) {
+                                                      int pitch, const cplx<T> *__restrict__ tw) {
     extern __shared__ char smem_raw[];
     cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
     constexpr int NH = N / 2 + 1;
@@ -397,7 +401,7 @@ __global__ void __launch_bounds__(256) k_fft_x_r2c_ct(const T *__restrict__ in, 
     for (int e = t; e < N; e += NTHR) line[pad8(e)] = cplx<T>{v0 ? a[e] : T(0), v1 ? b[e] : T(0)};
     __syncthreads();
     ct_stages<T, N, 1, NTHR, 1, -1>(line, t, tw);
-    cplx<T> *oa = out + L0 * NH, *ob = out + (L0 + 1) * NH;
+    cplx<T> *oa = out + L0 * pitch, *ob = out + (L0 + 1) * pitch;
     for (int k = t; k < NH; k += NTHR) {
         const cplx<T> z = line[pad8(k)];
         const cplx<T> zc = line[pad8((N - k) & (N - 1))];
@@ -410,7 +414,7 @@ __global__ void __launch_bounds__(256) k_fft_x_r2c_ct(const T *__restrict__ in, 
 
 template <typename T, int N, int NTHR>
 __global__ void __launch_bounds__(256) k_fft_x_c2r_ct(const cplx<T> *__restrict__ in, T *__restrict__ out, long nlines,
-                                                      const cplx<T> *__restrict__ tw, T scale) {
+                                                      int pitch, const cplx<T> *__restrict__ tw, T scale) {
     extern __shared__ char smem_raw[];
     cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
     constexpr int NH = N / 2 + 1;
@@ -419,7 +423,7 @@ __global__ void __launch_bounds__(256) k_fft_x_c2r_ct(const cplx<T> *__restrict_
     const long L0 = 2 * ((long)blockIdx.x * P + pair);
     const bool v0 = L0 < nlines, v1 = L0 + 1 < nlines;
     cplx<T> *line = lds + (size_t)pair * padded_len(N);
-    const cplx<T> *ia = in + L0 * NH, *ib = in + (L0 + 1) * NH;
+    const cplx<T> *ia = in + L0 * pitch, *ib = in + (L0 + 1) * pitch;
 #pragma unroll
     for (int e = t; e < N; e += NTHR) {
         const int src = e < NH ? e : N - e;
@@ -493,7 +497,7 @@ __global__ void __launch_bounds__(TK *NTHR) k_fft_cols_ct(cplx<T> *__restrict__ 
 }
 
 template <typename T, int N>
-int r2c_x_ct(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out) {
+int r2c_x_ct(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out, int pitch) {
     constexpr int NTHR = N / 16;
     const int P = 256 / NTHR;
     const size_t lds = (size_t)P * padded_len(N) * sizeof(cplx<T>);
@@ -501,14 +505,14 @@ int r2c_x_ct(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out) {
     const int blocks = (int)(((nlines + 1) / 2 + P - 1) / P);
     EC_HIP(ctx, hipFuncSetAttribute((const void *)k_fft_x_r2c_ct<T, N, NTHR>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    k_fft_x_r2c_ct<T, N, NTHR><<<blocks, P * NTHR, lds, ctx->stream>>>((const T *)in, (cplx<T> *)out, nlines,
+    k_fft_x_r2c_ct<T, N, NTHR><<<blocks, P * NTHR, lds, ctx->stream>>>((const T *)in, (cplx<T> *)out, nlines, pitch,
                                                                         (const cplx<T> *)ps->twiddle);
     EC_CHECK_LAUNCH(ctx, "k_fft_x_r2c_ct");
     return EC_OK;
 }
 
 template <typename T, int N>
-int c2r_x_ct(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out, double scale) {
+int c2r_x_ct(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out, double scale, int pitch) {
     constexpr int NTHR = N / 16;
     const int P = 256 / NTHR;
     const size_t lds = (size_t)P * padded_len(N) * sizeof(cplx<T>);
@@ -516,20 +520,20 @@ int c2r_x_ct(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out, double scal
     const int blocks = (int)(((nlines + 1) / 2 + P - 1) / P);
     EC_HIP(ctx, hipFuncSetAttribute((const void *)k_fft_x_c2r_ct<T, N, NTHR>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    k_fft_x_c2r_ct<T, N, NTHR><<<blocks, P * NTHR, lds, ctx->stream>>>((const cplx<T> *)in, (T *)out, nlines,
+    k_fft_x_c2r_ct<T, N, NTHR><<<blocks, P * NTHR, lds, ctx->stream>>>((const cplx<T> *)in, (T *)out, nlines, pitch,
                                                                         (const cplx<T> *)ps->twiddle, (T)scale);
     EC_CHECK_LAUNCH(ctx, "k_fft_x_c2r_ct");
     return EC_OK;
 }
 
 template <typename T, int N, int MODE, int SIGN>
-int cols_ct(ec_ctx *ctx, ec_poisson *ps, void *spec, int axis) {
+int cols_ct(ec_ctx *ctx, ec_poisson *ps, void *spec, int axis, int pitch) {
     constexpr int NTHR = N / 16;
     constexpr int TK0 = sizeof(T) == 4 ? 16 : 8;
     constexpr int TK = (TK0 * NTHR > 1024) ? 1024 / NTHR : TK0;
     const int nh = ps->nh;
-    const long elem_stride = axis == 1 ? (long)nh : (long)nh * N;
-    const long outer_stride = axis == 1 ? (long)nh * N : (long)nh;
+    const long elem_stride = axis == 1 ? (long)pitch : (long)pitch * N;
+    const long outer_stride = axis == 1 ? (long)pitch * N : (long)pitch;
     dim3 grid((nh + TK - 1) / TK, N);
     const size_t lds = (size_t)padded_len(N) * TK * sizeof(cplx<T>);
     const double h = ps->l / N;
@@ -543,9 +547,10 @@ int cols_ct(ec_ctx *ctx, ec_poisson *ps, void *spec, int axis) {
 }
 
 template <typename T, int N>
-int cols_ct_any(ec_ctx *ctx, ec_poisson *ps, void *spec, int axis, int sign, int mode) {
-    if (mode) return cols_ct<T, N, 1, -1>(ctx, ps, spec, axis);
-    return sign < 0 ? cols_ct<T, N, 0, -1>(ctx, ps, spec, axis) : cols_ct<T, N, 0, +1>(ctx, ps, spec, axis);
+int cols_ct_any(ec_ctx *ctx, ec_poisson *ps, void *spec, int axis, int sign, int mode, int pitch) {
+    if (mode) return cols_ct<T, N, 1, -1>(ctx, ps, spec, axis, pitch);
+    return sign < 0 ? cols_ct<T, N, 0, -1>(ctx, ps, spec, axis, pitch)
+                    : cols_ct<T, N, 0, +1>(ctx, ps, spec, axis, pitch);
 }
 
 #define EC_FFT_CT_SWITCH(n, CALL) \
@@ -615,9 +620,9 @@ int set_lds_limits(ec_ctx *ctx) {
 }
 
 template <typename T>
-int r2c_x(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out) {
+int r2c_x(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out, int pitch) {
     ec_prof_scope pr(ctx, "fft_x_r2c", (double)sizeof(T) * (ec_cube(ps->n) + 2.0 * ps->nh * ps->n * ps->n));
-#define EC_CALL(N) return r2c_x_ct<T, N>(ctx, ps, in, out)
+#define EC_CALL(N) return r2c_x_ct<T, N>(ctx, ps, in, out, pitch)
     EC_FFT_CT_SWITCH(ps->n, EC_CALL)
 #undef EC_CALL
     RadixPlan rp;
@@ -627,16 +632,16 @@ int r2c_x(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out) {
     XCfg c = xcfg<T>(n);
     const long npairs = (nlines + 1) / 2;
     const int blocks = (int)((npairs + c.P - 1) / c.P);
-    k_fft_x_r2c<T><<<blocks, c.P * c.nthr, c.lds, ctx->stream>>>((const T *)in, (cplx<T> *)out, n, nh, nlines, rp,
-                                                                  (const cplx<T> *)ps->twiddle, c.nthr);
+    k_fft_x_r2c<T><<<blocks, c.P * c.nthr, c.lds, ctx->stream>>>((const T *)in, (cplx<T> *)out, n, nh, pitch, nlines,
+                                                                  rp, (const cplx<T> *)ps->twiddle, c.nthr);
     EC_CHECK_LAUNCH(ctx, "k_fft_x_r2c");
     return EC_OK;
 }
 
 template <typename T>
-int c2r_x(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out, double scale) {
+int c2r_x(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out, double scale, int pitch) {
     ec_prof_scope pr(ctx, "fft_x_c2r", (double)sizeof(T) * (ec_cube(ps->n) + 2.0 * ps->nh * ps->n * ps->n));
-#define EC_CALL(N) return c2r_x_ct<T, N>(ctx, ps, in, out, scale)
+#define EC_CALL(N) return c2r_x_ct<T, N>(ctx, ps, in, out, scale, pitch)
     EC_FFT_CT_SWITCH(ps->n, EC_CALL)
 #undef EC_CALL
     RadixPlan rp;
@@ -646,26 +651,26 @@ int c2r_x(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out, double scale) 
     XCfg c = xcfg<T>(n);
     const long npairs = (nlines + 1) / 2;
     const int blocks = (int)((npairs + c.P - 1) / c.P);
-    k_fft_x_c2r<T><<<blocks, c.P * c.nthr, c.lds, ctx->stream>>>((const cplx<T> *)in, (T *)out, n, nh, nlines, rp,
-                                                                  (const cplx<T> *)ps->twiddle, c.nthr, (T)scale);
+    k_fft_x_c2r<T><<<blocks, c.P * c.nthr, c.lds, ctx->stream>>>((const cplx<T> *)in, (T *)out, n, nh, pitch, nlines,
+                                                                  rp, (const cplx<T> *)ps->twiddle, c.nthr, (T)scale);
     EC_CHECK_LAUNCH(ctx, "k_fft_x_c2r");
     return EC_OK;
 }
 
 // axis 1 (y) or 2 (z); mode 1 = Poisson (z only)
 template <typename T>
-int cols(ec_ctx *ctx, ec_poisson *ps, void *spec, int axis, int sign, int mode) {
+int cols(ec_ctx *ctx, ec_poisson *ps, void *spec, int axis, int sign, int mode, int pitch) {
     const char *name = mode ? "fft_z_poisson" : (axis == 1 ? "fft_y" : "fft_z");
     ec_prof_scope pr(ctx, name, 2.0 * sizeof(T) * 2.0 * ps->nh * ps->n * ps->n);
-#define EC_CALL(N) return cols_ct_any<T, N>(ctx, ps, spec, axis, sign, mode)
+#define EC_CALL(N) return cols_ct_any<T, N>(ctx, ps, spec, axis, sign, mode, pitch)
     EC_FFT_CT_SWITCH(ps->n, EC_CALL)
 #undef EC_CALL
     RadixPlan rp;
     make_plan(ps->n, rp);
     const int n = ps->n, nh = ps->nh;
     CCfg c = ccfg<T>(n);
-    const long elem_stride = axis == 1 ? (long)nh : (long)nh * n;
-    const long outer_stride = axis == 1 ? (long)nh * n : (long)nh;
+    const long elem_stride = axis == 1 ? (long)pitch : (long)pitch * n;
+    const long outer_stride = axis == 1 ? (long)pitch * n : (long)pitch;
     dim3 grid((nh + c.TK - 1) / c.TK, n);
     const double h = ps->l / n;
     const T a = (T)(-4.0 / (h * h));
@@ -687,11 +692,12 @@ int pressure_t(ec_ctx *ctx, ec_poisson *ps, void *const u[3]) {
     const int n = ps->n;
     int rc = ec_ns_divergence(ctx, ps->dtype, n, ps->l, u, ps->p);
     if (rc) return rc;
-    if ((rc = r2c_x<T>(ctx, ps, ps->p, ps->spec))) return rc;
-    if ((rc = cols<T>(ctx, ps, ps->spec, 1, -1, 0))) return rc;
-    if ((rc = cols<T>(ctx, ps, ps->spec, 2, -1, 1))) return rc;
-    if ((rc = cols<T>(ctx, ps, ps->spec, 1, +1, 0))) return rc;
-    return c2r_x<T>(ctx, ps, ps->spec, ps->p, 1.0);
+    const int sp = ps->pitch;
+    if ((rc = r2c_x<T>(ctx, ps, ps->p, ps->spec, sp))) return rc;
+    if ((rc = cols<T>(ctx, ps, ps->spec, 1, -1, 0, sp))) return rc;
+    if ((rc = cols<T>(ctx, ps, ps->spec, 2, -1, 1, sp))) return rc;
+    if ((rc = cols<T>(ctx, ps, ps->spec, 1, +1, 0, sp))) return rc;
+    return c2r_x<T>(ctx, ps, ps->spec, ps->p, 1.0, sp);
 }
 
 template <typename T>
@@ -699,11 +705,12 @@ int project_t(ec_ctx *ctx, ec_poisson *ps, void *const u[3]) {
     const int n = ps->n;
     int rc = ec_ns_divergence(ctx, ps->dtype, n, ps->l, u, ps->p);
     if (rc) return rc;
-    if ((rc = r2c_x<T>(ctx, ps, ps->p, ps->spec))) return rc;
-    if ((rc = cols<T>(ctx, ps, ps->spec, 1, -1, 0))) return rc;
-    if ((rc = cols<T>(ctx, ps, ps->spec, 2, -1, 1))) return rc;
-    if ((rc = cols<T>(ctx, ps, ps->spec, 1, +1, 0))) return rc;
-    if ((rc = c2r_x<T>(ctx, ps, ps->spec, ps->p, 1.0))) return rc;
+    const int sp = ps->pitch;
+    if ((rc = r2c_x<T>(ctx, ps, ps->p, ps->spec, sp))) return rc;
+    if ((rc = cols<T>(ctx, ps, ps->spec, 1, -1, 0, sp))) return rc;
+    if ((rc = cols<T>(ctx, ps, ps->spec, 2, -1, 1, sp))) return rc;
+    if ((rc = cols<T>(ctx, ps, ps->spec, 1, +1, 0, sp))) return rc;
+    if ((rc = c2r_x<T>(ctx, ps, ps->spec, ps->p, 1.0, sp))) return rc;
     return ec_ns_pressuregradient(ctx, ps->dtype, n, ps->l, u, ps->p);
 }
 
@@ -735,7 +742,9 @@ extern "C" int ec_poisson_create(ec_ctx *ctx, int dtype, int n, double l, ec_poi
     ps->dtype = dtype;
     ps->l = l;
     ps->nh = n / 2 + 1;
-    const size_t nspec = (size_t)ps->nh * n * n;
+    // rows of whole 128-byte lines for the column passes (FP32 complex: 16 per line)
+    ps->pitch = (ps->nh + 15) / 16 * 16;
+    const size_t nspec = (size_t)ps->pitch * n * n;
     hipError_t e = hipMalloc(&ps->spec, nspec * 2 * bt);
     if (e == hipSuccess) e = hipMalloc(&ps->p, ec_cube(n) * bt);
     if (e == hipSuccess) e = hipMalloc(&ps->twiddle, (size_t)n * 2 * bt);
@@ -803,34 +812,39 @@ extern "C" int ec_ns_pressure_solve(ec_ctx *ctx, ec_poisson *ps, void *const u[3
 
 extern "C" int ec_fft_r2c(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out) {
     EC_REQUIRE(ctx, ctx && ps && in && out, "ec_fft_r2c: null argument");
+    // a caller's buffer is the packed (nh, n, n) spectrum; the solver's own is pitched
+    const int sp = out == ps->spec ? ps->pitch : ps->nh;
     int rc;
     if (ps->dtype == EC_F64) {
-        if ((rc = r2c_x<double>(ctx, ps, in, out))) return rc;
-        if ((rc = cols<double>(ctx, ps, out, 1, -1, 0))) return rc;
-        return cols<double>(ctx, ps, out, 2, -1, 0);
+        if ((rc = r2c_x<double>(ctx, ps, in, out, sp))) return rc;
+        if ((rc = cols<double>(ctx, ps, out, 1, -1, 0, sp))) return rc;
+        return cols<double>(ctx, ps, out, 2, -1, 0, sp);
     }
-    if ((rc = r2c_x<float>(ctx, ps, in, out))) return rc;
-    if ((rc = cols<float>(ctx, ps, out, 1, -1, 0))) return rc;
-    return cols<float>(ctx, ps, out, 2, -1, 0);
+    if ((rc = r2c_x<float>(ctx, ps, in, out, sp))) return rc;
+    if ((rc = cols<float>(ctx, ps, out, 1, -1, 0, sp))) return rc;
+    return cols<float>(ctx, ps, out, 2, -1, 0, sp);
 }
 
 // Inverse, normalised by 1/n^3 (ldiv!).  Works in place on a copy held in
 // the solver's spectrum buffer unless `in` already is that buffer.
 extern "C" int ec_fft_c2r(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out) {
     EC_REQUIRE(ctx, ctx && ps && in && out, "ec_fft_c2r: null argument");
-    const size_t bt = ps->dtype == EC_F64 ? 8 : 4;
-    const size_t nspec = (size_t)ps->nh * ps->n * ps->n;
-    if (in != ps->spec) EC_HIP(ctx, hipMemcpyAsync(ps->spec, in, nspec * 2 * bt, hipMemcpyDeviceToDevice, ctx->stream));
+    const size_t bc = (ps->dtype == EC_F64 ? 8 : 4) * 2;  // bytes per complex
+    // a caller's packed (nh, n, n) spectrum goes into the pitched solver buffer
+    if (in != ps->spec)
+        EC_HIP(ctx, hipMemcpy2DAsync(ps->spec, ps->pitch * bc, in, ps->nh * bc, ps->nh * bc, (size_t)ps->n * ps->n,
+                                     hipMemcpyDeviceToDevice, ctx->stream));
     const double scale = 1.0 / ((double)ps->n * ps->n * ps->n);
+    const int sp = ps->pitch;
     int rc;
     if (ps->dtype == EC_F64) {
-        if ((rc = cols<double>(ctx, ps, ps->spec, 2, +1, 0))) return rc;
-        if ((rc = cols<double>(ctx, ps, ps->spec, 1, +1, 0))) return rc;
-        return c2r_x<double>(ctx, ps, ps->spec, out, scale);
+        if ((rc = cols<double>(ctx, ps, ps->spec, 2, +1, 0, sp))) return rc;
+        if ((rc = cols<double>(ctx, ps, ps->spec, 1, +1, 0, sp))) return rc;
+        return c2r_x<double>(ctx, ps, ps->spec, out, scale, sp);
     }
-    if ((rc = cols<float>(ctx, ps, ps->spec, 2, +1, 0))) return rc;
-    if ((rc = cols<float>(ctx, ps, ps->spec, 1, +1, 0))) return rc;
-    return c2r_x<float>(ctx, ps, ps->spec, out, scale);
+    if ((rc = cols<float>(ctx, ps, ps->spec, 2, +1, 0, sp))) return rc;
+    if ((rc = cols<float>(ctx, ps, ps->spec, 1, +1, 0, sp))) return rc;
+    return c2r_x<float>(ctx, ps, ps->spec, out, scale, sp);
 }
 
 static double profile_eval(int profile, double param, int k) {
@@ -857,8 +871,8 @@ static int randomfield_t(ec_ctx *ctx, ec_poisson *ps, int profile, double param,
     const int blocks = (int)std::min<size_t>((nspec + 255) / 256, 4096);
     for (int c = 0; c < 3; c++) {
         if ((rc = ec_fft_r2c(ctx, ps, u[c], ps->spec))) return rc;
-        k_shell_hist<T><<<blocks, 256, sizeof(double) * nshell, ctx->stream>>>((const cplx<T> *)ps->spec, n, nh, kmax,
-                                                                                nshell, inv_n3, dhist);
+        k_shell_hist<T><<<blocks, 256, sizeof(double) * nshell, ctx->stream>>>((const cplx<T> *)ps->spec, n, nh,
+                                                                                ps->pitch, kmax, nshell, inv_n3, dhist);
         EC_CHECK_LAUNCH(ctx, "k_shell_hist");
     }
     std::vector<double> hist(nshell), factor(nshell);
@@ -874,11 +888,12 @@ static int randomfield_t(ec_ctx *ctx, ec_poisson *ps, int profile, double param,
     for (int c = 0; c < 3; c++) {
         if ((rc = ec_fft_r2c(ctx, ps, u[c], ps->spec))) return rc;
         // rfft / n^3, times the shell factor; the inverse then runs unnormalised
-        k_shell_scale<T><<<blocks, 256, 0, ctx->stream>>>((cplx<T> *)ps->spec, n, nh, nshell, dhist + nshell, inv_n3);
+        k_shell_scale<T><<<blocks, 256, 0, ctx->stream>>>((cplx<T> *)ps->spec, n, nh, ps->pitch, nshell,
+                                                          dhist + nshell, inv_n3);
         EC_CHECK_LAUNCH(ctx, "k_shell_scale");
-        if ((rc = cols<T>(ctx, ps, ps->spec, 2, +1, 0))) return rc;
-        if ((rc = cols<T>(ctx, ps, ps->spec, 1, +1, 0))) return rc;
-        if ((rc = c2r_x<T>(ctx, ps, ps->spec, u[c], 1.0))) return rc;
+        if ((rc = cols<T>(ctx, ps, ps->spec, 2, +1, 0, ps->pitch))) return rc;
+        if ((rc = cols<T>(ctx, ps, ps->spec, 1, +1, 0, ps->pitch))) return rc;
+        if ((rc = c2r_x<T>(ctx, ps, ps->spec, u[c], 1.0, ps->pitch))) return rc;
     }
     EC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     EC_HIP(ctx, hipFreeAsync(dhist, ctx->stream));

@@ -116,7 +116,8 @@ struct ec_poisson {
     int dtype = EC_F64;
     double l = 1.0;
     int nh = 0;
-    void *spec = nullptr;     // (nh, n, n) complex of dtype
+    int pitch = 0;            // row pitch of spec in complex elements: nh rounded up to 16
+    void *spec = nullptr;     // (pitch, n, n) complex of dtype, columns kx < nh used
     void *p = nullptr;        // n^3 pressure of dtype
     void *twiddle = nullptr;  // exp(-2 pi i t / n), t in [0, n), complex of dtype
     void *eig = nullptr;      // sin^2(pi k / n), k in [0, n), real of dtype
