@@ -41,7 +41,7 @@ def parse():
     ap.add_argument("--no-closure-check", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="run a plain 1 GiB device copy first (HBM counter calibration)")
+                    help="run a plain 1 GiB read + 1 GiB write kernel first (HBM counter calibration)")
     ap.add_argument("--kernels", action="store_true", help="print the per-kernel table to stderr")
     ap.add_argument("--closure-tiles", default="16,16,128", help="tx,ty,tz block targets of the closure kernel")
     ap.add_argument("--no-autotune", action="store_true")
@@ -130,9 +130,11 @@ def main():
     g = NS.Grid(1.0, n)
 
     if a.copy_calibration:
-        x = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        # a plain streaming kernel of known traffic for the HBM counters: reads
+        # 1 GiB and writes 1 GiB (element-wise negation of 2^28 floats)
+        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
         y = torch.empty_like(x)
-        y.copy_(x)
+        torch.neg(x, out=y)
         torch.cuda.synchronize()
         del x, y
 
