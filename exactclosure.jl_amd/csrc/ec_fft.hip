@@ -453,13 +453,17 @@ __global__ void __launch_bounds__(256) k_fft_x_c2r_ct(const cplx<T> *__restrict_
 // (element e of column c at lds[e * TK + c]: consecutive lanes take
 // consecutive columns, so both global rows and LDS rows are contiguous).
 template <typename T, int N, int TK, int NTHR, int MODE, int SIGN>
-__global__ void __launch_bounds__(TK *NTHR) k_fft_cols_ct(cplx<T> *__restrict__ data, int ncols, long elem_stride,
-                                                         long outer_stride, const cplx<T> *__restrict__ tw,
-                                                         const T *__restrict__ eig, T poisson_a, T scale) {
+__global__ void __launch_bounds__(TK *NTHR) k_fft_cols_ct(cplx<T> *__restrict__ data, int ncols, int ntiles,
+                                                         long elem_stride, long outer_stride,
+                                                         const cplx<T> *__restrict__ tw, const T *__restrict__ eig,
+                                                         T poisson_a, T scale) {
     extern __shared__ char smem_raw[];
     cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
-    const int col0 = blockIdx.x * TK;
-    const int outer = blockIdx.y;
+    // 1D grid of ntiles column tiles x N outer indices, column tiles fastest, in
+    // XCD order: neighbouring tiles share 128-byte lines in one L2
+    const int L = (int)ec_xcd_tile(blockIdx.x, gridDim.x);
+    const int col0 = (L % ntiles) * TK;
+    const int outer = L / ntiles;
     const int c = threadIdx.x % TK, t = threadIdx.x / TK;
     cplx<T> *base = data + col0 + (long)outer * outer_stride;
     const bool full = col0 + TK <= ncols;
@@ -529,18 +533,20 @@ int c2r_x_ct(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out, double scal
 template <typename T, int N, int MODE, int SIGN>
 int cols_ct(ec_ctx *ctx, ec_poisson *ps, void *spec, int axis, int pitch) {
     constexpr int NTHR = N / 16;
+    // one 128-byte line of complex values per row
     constexpr int TK0 = sizeof(T) == 4 ? 16 : 8;
     constexpr int TK = (TK0 * NTHR > 1024) ? 1024 / NTHR : TK0;
     const int nh = ps->nh;
     const long elem_stride = axis == 1 ? (long)pitch : (long)pitch * N;
     const long outer_stride = axis == 1 ? (long)pitch * N : (long)pitch;
-    dim3 grid((nh + TK - 1) / TK, N);
+    const int ntiles = (nh + TK - 1) / TK;
+    dim3 grid((unsigned)(ntiles * N));
     const size_t lds = (size_t)padded_len(N) * TK * sizeof(cplx<T>);
     const double h = ps->l / N;
     EC_HIP(ctx, hipFuncSetAttribute((const void *)k_fft_cols_ct<T, N, TK, NTHR, MODE, SIGN>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     k_fft_cols_ct<T, N, TK, NTHR, MODE, SIGN><<<grid, TK * NTHR, lds, ctx->stream>>>(
-        (cplx<T> *)spec, nh, elem_stride, outer_stride, (const cplx<T> *)ps->twiddle, (const T *)ps->eig,
+        (cplx<T> *)spec, nh, ntiles, elem_stride, outer_stride, (const cplx<T> *)ps->twiddle, (const T *)ps->eig,
         (T)(-4.0 / (h * h)), (T)(1.0 / ((double)N * N * N)));
     EC_CHECK_LAUNCH(ctx, "k_fft_cols_ct");
     return EC_OK;

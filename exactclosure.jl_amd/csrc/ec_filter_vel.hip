@@ -234,6 +234,97 @@ __device__ __forceinline__ void set_plane(int r, int ZC, const int *ca, const bo
     }
 }
 
+// The four-set list {16 volume, 8 volume, 4 volume, 4 face}: its box windows
+// nest, so one running 4-plane box sum b4 per column serves every set.  At the
+// last plane of 4-window m (phase 3): B = b4 closes the compression-4 box; the
+// 8- and 16-plane boxes are sums of 2 and 4 consecutive B; the 8- and 16-plane
+// trapezoids (ends of weight 1/2 at phase-3 planes) are
+//   face8(K)  = x(2K)/2   + B(2K+1) + B(2K+2)                 - x(2K+2)/2
+//   face16(K) = x(4K+1)/2 + B(4K+2) + ... + B(4K+5)            - x(4K+5)/2
+// with x(m) the plane value at phase 3 of window m.  The compression-4
+// trapezoid does not align with the 4-windows and keeps its own sum.
+template <class G>
+constexpr bool nested_list() {
+    return G::c(0) == 16 && G::kind(0) == 0 && G::c(1) == 8 && G::kind(1) == 0 && G::c(2) == 4 &&
+           G::kind(2) == 0 && G::c(3) == 4 && G::kind(3) == 1;
+}
+
+struct NestState {
+    double b4, s1, s2, s3;  // components 0, 1: b4, box8, box16; component 2: b4, face4, face8, face16
+};
+
+template <class G, int S, int A>
+__device__ __forceinline__ void zput(double *lds, int lx, int ly, double v) {
+    lds[G::zoff(S, A) + ly * G::zw(S, A) + lx] = v;
+}
+
+// component A is needed up to h(S) columns past the tile along A (volume average)
+template <class G, int S, int A>
+__device__ __forceinline__ bool zin(int lx, int ly) {
+    if constexpr (A == 0) return lx < G::TX + G::h(S);
+    else if constexpr (A == 1) return ly < G::TY + G::h(S);
+    else return true;
+}
+
+// components 0 and 1: box along z for every set
+template <class G, int A>
+__device__ __forceinline__ void nested_box(int r, int ZC, bool ok, int lx, int ly, double src, NestState &st,
+                                           double *lds) {
+    const int ph = r & 3, m = r >> 2;
+    st.b4 = (ph == 0 ? 0.0 : st.b4) + src;
+    if (ph == 3) {
+        const double B = st.b4;
+        st.s1 = ((m & 1) == 0 ? 0.0 : st.s1) + B;
+        st.s2 = ((m & 3) == 0 ? 0.0 : st.s2) + B;
+        if (r < ZC && ok) {
+            if (zin<G, 3, A>(lx, ly)) zput<G, 3, A>(lds, lx, ly, B * 0.25);
+            if (zin<G, 2, A>(lx, ly)) zput<G, 2, A>(lds, lx, ly, B * 0.25);
+            if ((m & 1) == 1 && zin<G, 1, A>(lx, ly)) zput<G, 1, A>(lds, lx, ly, st.s1 * 0.125);
+            if ((m & 3) == 3 && zin<G, 0, A>(lx, ly)) zput<G, 0, A>(lds, lx, ly, st.s2 * 0.0625);
+        }
+    }
+}
+
+// component 2: face average point, compression-4 trapezoid on its own sum,
+// compression-8 and -16 trapezoids from the 4-plane boxes
+template <class G>
+__device__ __forceinline__ void nested_face(int r, int ZC, bool ok, int lx, int ly, double src, NestState &st,
+                                            double *lds) {
+    const int ph = r & 3, m = r >> 2;
+    st.b4 = (ph == 0 ? 0.0 : st.b4) + src;
+    // compression-4 trapezoid: windows start at planes 4K + 1, where (r + 3) % 4 == 0
+    if (((r + 3) & 3) == 0) {
+        if (r > 1 && r - 1 <= ZC && ok) zput<G, 2, 2>(lds, lx, ly, (st.s1 + 0.5 * src) * 0.25);
+        st.s1 = 0.5 * src;
+    } else {
+        st.s1 += src;
+    }
+    if (ph == 3) {
+        const double B = st.b4, hx = 0.5 * src;
+        if (r < ZC && ok) zput<G, 3, 2>(lds, lx, ly, src);
+        if ((m & 1) == 0) {
+            if (m > 0 && m <= ZC / 4 && ok) zput<G, 1, 2>(lds, lx, ly, (st.s2 + B - hx) * 0.125);
+            st.s2 = hx;
+        } else {
+            st.s2 += B;
+        }
+        if ((m & 3) == 1) {
+            if (m >= 5 && m <= ZC / 4 + 1 && ok) zput<G, 0, 2>(lds, lx, ly, (st.s3 + B - hx) * 0.0625);
+            st.s3 = hx;
+        } else {
+            st.s3 += B;
+        }
+    }
+}
+
+template <class G>
+__device__ __forceinline__ void zupdate_nested(int r, int ZC, int a, bool ok, int lx, int ly, double src,
+                                               NestState &st, double *lds) {
+    if (a == 0) nested_box<G, 0>(r, ZC, ok, lx, ly, src, st, lds);
+    else if (a == 1) nested_box<G, 1>(r, ZC, ok, lx, ly, src, st, lds);
+    else if (a == 2) nested_face<G>(r, ZC, ok, lx, ly, src, st, lds);
+}
+
 template <class G, int S>
 __device__ __forceinline__ bool set_emits(int r, int ZC) {
     if constexpr (G::c(S) > 0) {
@@ -321,11 +412,16 @@ __global__ void __launch_bounds__(G::NT) k_filter_vel(const T *__restrict__ v1, 
                        : (a == 1 ? xo + nn * (unsigned)wrap1(y0 + ly + 1, n) : co[j]);
     }
 
-    double acc[NJ][4];
+    constexpr bool NEST = nested_list<G>();
+    double acc[NEST ? 1 : NJ][4];
+    NestState nst[NJ];
 #pragma unroll
-    for (int j = 0; j < NJ; j++)
+    for (int j = 0; j < NJ; j++) {
+        nst[j] = NestState{0.0, 0.0, 0.0, 0.0};
+        if constexpr (!NEST)
 #pragma unroll
-        for (int s = 0; s < 4; s++) acc[j][s] = 0.0;
+            for (int s = 0; s < 4; s++) acc[j][s] = 0.0;
+    }
 
     // register double buffer: v, p and the p neighbour of plane r
     T cv[NJ], cp[NJ], cqv[NJ];
@@ -352,10 +448,15 @@ __global__ void __launch_bounds__(G::NT) k_filter_vel(const T *__restrict__ v1, 
         double src[NJ];
 #pragma unroll
         for (int j = 0; j < NJ; j++) src[j] = (double)cv[j] - ((double)cqv[j] - (double)cp[j]) * ih;
-        set_plane<G, 0>(r, ZC, ca, ok, clx, cly, src, acc, lds);
-        set_plane<G, 1>(r, ZC, ca, ok, clx, cly, src, acc, lds);
-        set_plane<G, 2>(r, ZC, ca, ok, clx, cly, src, acc, lds);
-        set_plane<G, 3>(r, ZC, ca, ok, clx, cly, src, acc, lds);
+        if constexpr (NEST) {
+#pragma unroll
+            for (int j = 0; j < NJ; j++) zupdate_nested<G>(r, ZC, ca[j], ok[j], clx[j], cly[j], src[j], nst[j], lds);
+        } else {
+            set_plane<G, 0>(r, ZC, ca, ok, clx, cly, src, acc, lds);
+            set_plane<G, 1>(r, ZC, ca, ok, clx, cly, src, acc, lds);
+            set_plane<G, 2>(r, ZC, ca, ok, clx, cly, src, acc, lds);
+            set_plane<G, 3>(r, ZC, ca, ok, clx, cly, src, acc, lds);
+        }
         const bool any = set_emits<G, 0>(r, ZC) || set_emits<G, 1>(r, ZC) || set_emits<G, 2>(r, ZC) ||
                          set_emits<G, 3>(r, ZC);
         if (any) {
