@@ -73,6 +73,34 @@ struct Bfly<T, 8> {
     }
 };
 
+template <typename T>
+struct Bfly<T, 16> {
+    __device__ static void run(cplx<T> *v, int sign) {
+        cplx<T> e[8], o[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            e[k] = v[2 * k];
+            o[k] = v[2 * k + 1];
+        }
+        Bfly<T, 8>::run(e, sign);
+        Bfly<T, 8>::run(o, sign);
+        // o[k] *= exp(sign*2*pi*i*k/16): (cos, sin) of k*pi/8
+        const T c1 = (T)0.92387953251128675613, s1 = (T)0.38268343236508977173, c2 = (T)0.70710678118654752440;
+        const T cs[8] = {T(1), c1, c2, s1, T(0), -s1, -c2, -c1};
+        const T sn[8] = {T(0), s1, c2, c1, T(1), c1, c2, s1};
+#pragma unroll
+        for (int k = 1; k < 8; k++) {
+            const T sk = sign > 0 ? sn[k] : -sn[k];
+            o[k] = cplx<T>{o[k].x * cs[k] - o[k].y * sk, o[k].y * cs[k] + o[k].x * sk};
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            v[k] = cadd(e[k], o[k]);
+            v[k + 8] = csub(e[k], o[k]);
+        }
+    }
+};
+
 template <typename T, int R>
 struct BflyGeneric {
     __device__ static void run(cplx<T> *v, int sign) {
@@ -336,17 +364,24 @@ __global__ void k_shell_scale(cplx<T> *__restrict__ spec, int n, int nh, int pit
 // are compile-time constants, so stages unroll with shift/mask index math and
 // no plan array lives in memory.  n = 2^m is split into radix-8 stages and a
 // final radix-4 or radix-2 stage; each thread owns 16 elements of a line.
-constexpr int ct_radix(int m) { return m % 8 == 0 ? 8 : (m % 4 == 0 ? 4 : 2); }
+constexpr int ct_radix(int m, int maxr = 16) {
+    return (maxr >= 16 && m % 16 == 0) ? 16 : (m % 8 == 0 ? 8 : (m % 4 == 0 ? 4 : 2));
+}
 
 // LDS position of line element e: one padding slot after every 8 elements,
 // which spreads the strided Stockham scatters over all banks.
-__device__ __forceinline__ constexpr int pad8(int e) { return e + (e >> 3); }
-constexpr int padded_len(int n) { return n + n / 8; }
+#ifndef EC_FFT_PAD_SHIFT
+#define EC_FFT_PAD_SHIFT 5
+#endif
+// LDS padding: one spare element every 2^EC_FFT_PAD_SHIFT elements
+__device__ __forceinline__ constexpr int pad8(int e) { return e + (e >> EC_FFT_PAD_SHIFT); }
+constexpr int padded_len(int n) { return n + (n >> EC_FFT_PAD_SHIFT); }
 
-template <typename T, int N, int NS, int NTHR, int LS, int SIGN>
+// MAXR caps the radix (a 1024-thread block keeps radix 8 to stay within 128 VGPRs)
+template <typename T, int N, int NS, int NTHR, int LS, int SIGN, int MAXR = 16>
 __device__ __forceinline__ void ct_stages(cplx<T> *lds, int t, const cplx<T> *__restrict__ tw) {
     if constexpr (NS < N) {
-        constexpr int R = ct_radix(N / NS);
+        constexpr int R = ct_radix(N / NS, MAXR);
         constexpr int NR = N / R;
         constexpr int B = (NR + NTHR - 1) / NTHR;
         constexpr int TSTEP = N / (NS * R);
@@ -381,7 +416,7 @@ __device__ __forceinline__ void ct_stages(cplx<T> *lds, int t, const cplx<T> *__
             }
         }
         __syncthreads();
-        ct_stages<T, N, NS * R, NTHR, LS, SIGN>(lds, t, tw);
+        ct_stages<T, N, NS * R, NTHR, LS, SIGN, MAXR>(lds, t, tw);
     }
 }
 
@@ -468,16 +503,26 @@ __global__ void __launch_bounds__(TK *NTHR) k_fft_cols_ct(cplx<T> *__restrict__ 
     cplx<T> *base = data + col0 + (long)outer * outer_stride;
     const bool full = col0 + TK <= ncols;
     constexpr int TOTAL = N * TK;
-#pragma unroll 4
-    for (int idx = threadIdx.x; idx < TOTAL; idx += TK * NTHR) {
-        const int cc = idx % TK, e = idx / TK;
-        lds[pad8(e) * TK + cc] = (full || col0 + cc < ncols) ? base[cc + e * elem_stride] : cplx<T>{T(0), T(0)};
+    // all loads of the tile in flight at once (one block per CU at N = 1024)
+    {
+        constexpr int PER = TOTAL / (TK * NTHR);
+        cplx<T> tmp[PER];
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int idx = threadIdx.x + q * TK * NTHR, cc = idx % TK, e = idx / TK;
+            tmp[q] = (full || col0 + cc < ncols) ? base[cc + e * elem_stride] : cplx<T>{T(0), T(0)};
+        }
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int idx = threadIdx.x + q * TK * NTHR, cc = idx % TK, e = idx / TK;
+            lds[pad8(e) * TK + cc] = tmp[q];
+        }
     }
     __syncthreads();
     if (MODE == 0) {
         ct_stages<T, N, 1, NTHR, TK, SIGN>(lds + c, t, tw);
     } else {
-        ct_stages<T, N, 1, NTHR, TK, -1>(lds + c, t, tw);
+        ct_stages<T, N, 1, NTHR, TK, -1, 8>(lds + c, t, tw);
         const int kx = col0 + c;
         const T sxy = (kx < ncols ? eig[kx] : T(0)) + eig[outer];
 #pragma unroll
@@ -491,9 +536,9 @@ __global__ void __launch_bounds__(TK *NTHR) k_fft_cols_ct(cplx<T> *__restrict__ 
             lds[pad8(e) * TK + c] = z;
         }
         __syncthreads();
-        ct_stages<T, N, 1, NTHR, TK, +1>(lds + c, t, tw);
+        ct_stages<T, N, 1, NTHR, TK, +1, 8>(lds + c, t, tw);
     }
-#pragma unroll 4
+#pragma unroll
     for (int idx = threadIdx.x; idx < TOTAL; idx += TK * NTHR) {
         const int cc = idx % TK, e = idx / TK;
         if (full || col0 + cc < ncols) base[cc + e * elem_stride] = lds[pad8(e) * TK + cc];
