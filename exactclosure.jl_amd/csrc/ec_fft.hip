@@ -738,6 +738,18 @@ int cols(ec_ctx *ctx, ec_poisson *ps, void *spec, int axis, int sign, int mode, 
     return EC_OK;
 }
 
+// p <- L^-1 p in place (p = the solver's pressure buffer holding a divergence)
+template <typename T>
+int poisson_fft_t(ec_ctx *ctx, ec_poisson *ps) {
+    int rc;
+    const int sp = ps->pitch;
+    if ((rc = r2c_x<T>(ctx, ps, ps->p, ps->spec, sp))) return rc;
+    if ((rc = cols<T>(ctx, ps, ps->spec, 1, -1, 0, sp))) return rc;
+    if ((rc = cols<T>(ctx, ps, ps->spec, 2, -1, 1, sp))) return rc;
+    if ((rc = cols<T>(ctx, ps, ps->spec, 1, +1, 0, sp))) return rc;
+    return c2r_x<T>(ctx, ps, ps->spec, ps->p, 1.0, sp);
+}
+
 template <typename T>
 int pressure_t(ec_ctx *ctx, ec_poisson *ps, void *const u[3]) {
     const int n = ps->n;
@@ -859,6 +871,11 @@ extern "C" int ec_ns_project(ec_ctx *ctx, ec_poisson *ps, void *const u[3]) {
 extern "C" int ec_ns_pressure_solve(ec_ctx *ctx, ec_poisson *ps, void *const u[3]) {
     EC_REQUIRE(ctx, ctx && ps && u, "ec_ns_pressure_solve: null argument");
     return ps->dtype == EC_F64 ? pressure_t<double>(ctx, ps, u) : pressure_t<float>(ctx, ps, u);
+}
+
+extern "C" int ec_poisson_solve(ec_ctx *ctx, ec_poisson *ps) {
+    EC_REQUIRE(ctx, ctx && ps, "ec_poisson_solve: null argument");
+    return ps->dtype == EC_F64 ? poisson_fft_t<double>(ctx, ps) : poisson_fft_t<float>(ctx, ps);
 }
 
 extern "C" int ec_fft_r2c(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out) {

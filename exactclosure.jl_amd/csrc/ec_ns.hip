@@ -270,6 +270,143 @@ __global__ void __launch_bounds__(256) k_rhs_march(int n, int ZC, T ih, T visc, 
     }
 }
 
+// -div sigma(u) at one cell of a haloed plane triple (pm, p0, pp: planes k-1,
+// k, k+1 of the three components, each component HP values apart, rows HX
+// apart); o is the cell's offset within a plane.
+template <typename T>
+__device__ __forceinline__ void rhs_cell(const T *pm, const T *p0, const T *pp, int o, int HX, int HP, T ih, T visc,
+                                         T &a1, T &a2, T &a3) {
+    const T *b0 = p0 + HP, *c0 = p0 + 2 * HP;
+    const T u1_0 = p0[o], u1_xp = p0[o + 1], u1_xm = p0[o - 1], u1_yp = p0[o + HX], u1_ym = p0[o - HX];
+    const T u1_xm_yp = p0[o - 1 + HX], u1_zp = pp[o], u1_zm = pm[o], u1_xm_zp = pp[o - 1];
+    const T u2_0 = b0[o], u2_xp = b0[o + 1], u2_xm = b0[o - 1], u2_yp = b0[o + HX], u2_ym = b0[o - HX];
+    const T u2_xp_ym = b0[o + 1 - HX], u2_zp = pp[HP + o], u2_zm = pm[HP + o], u2_ym_zp = pp[HP + o - HX];
+    const T u3_0 = c0[o], u3_xp = c0[o + 1], u3_xm = c0[o - 1], u3_yp = c0[o + HX], u3_ym = c0[o - HX];
+    const T u3_zp = pp[2 * HP + o], u3_zm = pm[2 * HP + o], u3_xp_zm = pm[2 * HP + o + 1];
+    const T u3_yp_zm = pm[2 * HP + o + HX];
+    const T s11_p = s_diag(u1_xp, u1_0, visc, ih), s11_0 = s_diag(u1_0, u1_xm, visc, ih);
+    const T s22_p = s_diag(u2_yp, u2_0, visc, ih), s22_0 = s_diag(u2_0, u2_ym, visc, ih);
+    const T s33_p = s_diag(u3_zp, u3_0, visc, ih), s33_0 = s_diag(u3_0, u3_zm, visc, ih);
+    const T s12_0 = s_off(u1_yp, u1_0, u2_xp, u2_0, visc, ih);
+    const T s12_ym = s_off(u1_0, u1_ym, u2_xp_ym, u2_ym, visc, ih);
+    const T s12_xm = s_off(u1_xm_yp, u1_xm, u2_0, u2_xm, visc, ih);
+    const T s31_0 = s_off(u3_xp, u3_0, u1_zp, u1_0, visc, ih);
+    const T s31_zm = s_off(u3_xp_zm, u3_zm, u1_0, u1_zm, visc, ih);
+    const T s31_xm = s_off(u3_0, u3_xm, u1_xm_zp, u1_xm, visc, ih);
+    const T s23_0 = s_off(u2_zp, u2_0, u3_yp, u3_0, visc, ih);
+    const T s23_zm = s_off(u2_0, u2_zm, u3_yp_zm, u3_zm, visc, ih);
+    const T s23_ym = s_off(u2_ym_zp, u2_ym, u3_0, u3_ym, visc, ih);
+    a1 = -(s11_p - s11_0) * ih - (s12_0 - s12_ym) * ih - (s31_0 - s31_zm) * ih;
+    a2 = -(s12_0 - s12_xm) * ih - (s22_p - s22_0) * ih - (s23_0 - s23_zm) * ih;
+    a3 = -(s31_0 - s31_xm) * ih - (s23_0 - s23_ym) * ih - (s33_p - s33_0) * ih;
+}
+
+// Fused RHS and its divergence, z-marching like k_rhs_march: du = -div sigma(u)
+// and d = div_h du (the right-hand side of the pressure Poisson equation).  The
+// planes carry a two-cell halo on the low x/y sides so that the block also
+// evaluates du1 on the column x0 - 1 and du2 on the row y0 - 1; du3 of the
+// previous plane stays in registers (one extra plane z0 - 1 starts the march).
+// du1 / du2 go through a double-buffered LDS exchange, one barrier per plane.
+template <typename T, int TX, int TY>
+__global__ void __launch_bounds__(256) k_rhs_div_march(int n, int ZC, T ih, T visc, const T *__restrict__ u1,
+                                                       const T *__restrict__ u2, const T *__restrict__ u3,
+                                                       T *__restrict__ du1, T *__restrict__ du2, T *__restrict__ du3,
+                                                       T *__restrict__ dout) {
+    constexpr int HX = TX + 3, HY = TY + 3, HP = HX * HY, NLD = (3 * HP + 255) / 256;
+    constexpr int CPT = TX * TY / 256;
+    constexpr int D1N = TY * (TX + 1), D2N = (TY + 1) * TX;
+    static_assert(TX * TY % 256 == 0 && 256 % TX == 0 && TX + TY <= 256, "tile shape");
+    __shared__ T ring[4][3 * HP];
+    __shared__ T dx1[2][D1N], dx2[2][D2N];  // du1 at x in [x0-1, x0+TX), du2 at y in [y0-1, y0+TY)
+    const int gx = n / TX, gy = n / TY;
+    const int L = (int)ec_xcd_tile(blockIdx.x, gridDim.x);
+    const int x0 = (L % gx) * TX, y0 = ((L / gx) % gy) * TY, z0 = (L / (gx * gy)) * ZC;
+    const int tid = threadIdx.x;
+    const unsigned nn = (unsigned)n, n2 = nn * nn;
+
+    unsigned loff[NLD];
+    int lcomp[NLD];
+#pragma unroll
+    for (int j = 0; j < NLD; j++) {
+        const int e = tid + 256 * j;
+        const int comp = e / HP, q = e - comp * HP, hx = q % HX, hy = q / HX;
+        lcomp[j] = e < 3 * HP ? comp : -1;
+        loff[j] = (unsigned)ec_wrap_any(x0 + hx - 2, n) + nn * (unsigned)ec_wrap_any(y0 + hy - 2, n);
+    }
+    auto load = [&](int k, T *r) {
+        const unsigned kk = (unsigned)ec_wrap_any(k, n) * n2;
+#pragma unroll
+        for (int j = 0; j < NLD; j++) {
+            const T *sp = lcomp[j] == 1 ? u2 : (lcomp[j] == 2 ? u3 : u1);
+            r[j] = sp[loff[j] + kk];
+        }
+    };
+    auto store = [&](int slot, const T *r) {
+#pragma unroll
+        for (int j = 0; j < NLD; j++)
+            if (lcomp[j] >= 0) ring[slot][tid + 256 * j] = r[j];
+    };
+    {
+        T r[NLD];
+        load(z0 - 2, r);
+        store(0, r);
+        load(z0 - 1, r);
+        store(1, r);
+        load(z0, r);
+        store(2, r);
+    }
+    __syncthreads();
+    const int cx = tid % TX, cy0 = tid / TX;
+    // halo cell of this thread: column x0 - 1 (tid < TY) or row y0 - 1
+    const bool hcol = tid < TY, hrow = tid >= TY && tid < TY + TX;
+    const int hcx = hcol ? -1 : tid - TY, hcy = hcol ? tid : -1;
+    T a3prev[CPT];
+    for (int rr = -1; rr < ZC; ++rr) {
+        const int k = z0 + rr, q = rr + 2;  // ring plane index of k
+        T nxt[NLD];
+        if (rr + 1 < ZC) load(k + 2, nxt);
+        const T *pm = ring[(q - 1) & 3], *p0 = ring[q & 3], *pp = ring[(q + 1) & 3];
+        T a1[CPT], a2[CPT], a3[CPT];
+#pragma unroll
+        for (int m = 0; m < CPT; m++) {
+            const int cy = cy0 + m * (256 / TX);
+            rhs_cell(pm, p0, pp, (cy + 2) * HX + (cx + 2), HX, HP, ih, visc, a1[m], a2[m], a3[m]);
+        }
+        const int buf = (rr + 2) & 1;
+        if (rr >= 0) {
+#pragma unroll
+            for (int m = 0; m < CPT; m++) {
+                const int cy = cy0 + m * (256 / TX);
+                const size_t I = (size_t)(x0 + cx) + (size_t)nn * ((size_t)(y0 + cy) + (size_t)nn * (size_t)k);
+                du1[I] = a1[m];
+                du2[I] = a2[m];
+                du3[I] = a3[m];
+                dx1[buf][cy * (TX + 1) + cx + 1] = a1[m];
+                dx2[buf][(cy + 1) * TX + cx] = a2[m];
+            }
+            if (hcol || hrow) {
+                T h1, h2, h3;
+                rhs_cell(pm, p0, pp, (hcy + 2) * HX + (hcx + 2), HX, HP, ih, visc, h1, h2, h3);
+                if (hcol) dx1[buf][hcy * (TX + 1)] = h1;
+                else dx2[buf][hcx] = h2;
+            }
+        }
+        if (rr + 1 < ZC) store((q + 2) & 3, nxt);
+        __syncthreads();
+        if (rr >= 0) {
+#pragma unroll
+            for (int m = 0; m < CPT; m++) {
+                const int cy = cy0 + m * (256 / TX);
+                const size_t I = (size_t)(x0 + cx) + (size_t)nn * ((size_t)(y0 + cy) + (size_t)nn * (size_t)k);
+                dout[I] = (a1[m] - dx1[buf][cy * (TX + 1) + cx]) * ih + (a2[m] - dx2[buf][cy * TX + cx]) * ih +
+                          (a3[m] - a3prev[m]) * ih;
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < CPT; m++) a3prev[m] = a3[m];
+    }
+}
+
 template <typename T>
 __global__ void k_divergence(int n, T ih, const T *__restrict__ u1, const T *__restrict__ u2,
                              const T *__restrict__ u3, T *__restrict__ d) {
@@ -423,6 +560,32 @@ extern "C" int ec_ns_rhs(ec_ctx *ctx, int dtype, int n, double l, double visc, v
                                                           (const float *)u[1], (const float *)u[2], (float *)du[0],
                                                           (float *)du[1], (float *)du[2], doadd);
     EC_CHECK_LAUNCH(ctx, "k_rhs");
+    return EC_OK;
+}
+
+extern "C" int ec_ns_rhs_div(ec_ctx *ctx, int dtype, int n, double l, double visc, void *const u[3],
+                             void *const du[3], void *d) {
+    EC_REQUIRE(ctx, ctx && u && du && d, "ec_ns_rhs_div: null argument");
+    EC_CHECK_FIELD(ctx, n);
+    if (!(n % 64 == 0 && (double)n * n * n < 4294967296.0)) {
+        // small grids: the two plain kernels
+        int rc = ec_ns_rhs(ctx, dtype, n, l, visc, u, du, 0);
+        return rc ? rc : ec_ns_divergence(ctx, dtype, n, l, du, d);
+    }
+    const double ih = n / l;
+    const double b = dtype == EC_F64 ? 8 : 4;
+    ec_prof_scope ps(ctx, "ns_rhs_div", 7.0 * ec_cube(n) * b);
+    const int ZC = n % 128 == 0 ? 128 : 64;
+    const unsigned tiles = (unsigned)((n / 64) * (n / 8) * (n / ZC));
+    if (dtype == EC_F64)
+        k_rhs_div_march<double, 64, 8><<<tiles, 256, 0, ctx->stream>>>(
+            n, ZC, ih, visc, (const double *)u[0], (const double *)u[1], (const double *)u[2], (double *)du[0],
+            (double *)du[1], (double *)du[2], (double *)d);
+    else
+        k_rhs_div_march<float, 64, 8><<<tiles, 256, 0, ctx->stream>>>(
+            n, ZC, (float)ih, (float)visc, (const float *)u[0], (const float *)u[1], (const float *)u[2],
+            (float *)du[0], (float *)du[1], (float *)du[2], (float *)d);
+    EC_CHECK_LAUNCH(ctx, "k_rhs_div_march");
     return EC_OK;
 }
 

@@ -200,8 +200,10 @@ class DNSAidedLES:
         n, l = self.g.n, self.g.l
         for st in range(4):
             u = self.u0 if st == 0 else self.us
-            c.call("ec_ns_rhs", self.dt_code, n, l, self.visc, ptrs(u, P3), ptrs(self.du, P3), 0)
-            c.call("ec_ns_pressure_solve", self.ps.h, ptrs(self.du, P3))
+            # du = -div sigma(u) and its divergence in one pass, then the FFT solve
+            c.call("ec_ns_rhs_div", self.dt_code, n, l, self.visc, ptrs(u, P3), ptrs(self.du, P3),
+                   _abi.c_void_p(self.ps.p_ptr))
+            c.call("ec_poisson_solve", self.ps.h)
             self._filter_vel(self.du, self.ps.p_ptr, [L.g_k for L in self.les])
             for L in self.les:
                 gl = L.g

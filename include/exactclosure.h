@@ -89,6 +89,12 @@ int ec_ns_tensordivergence_nonsym(ec_ctx *ctx, int dtype, int n, double l, int s
  * (src/N.jl:310-311, 1306-1307) */
 int ec_ns_rhs(ec_ctx *ctx, int dtype, int n, double l, double visc, void *const u[3],
               void *const du[3], int doadd);
+/* ec_ns_rhs (doadd 0) and divergence!(d, du) fused: du = -div sigma(u) and
+ * d = div_h du, the right-hand side of the pressure Poisson equation of
+ * project!(du) (src/N.jl:256-262).  d is typically ec_poisson_pressure(ps),
+ * followed by ec_poisson_solve. */
+int ec_ns_rhs_div(ec_ctx *ctx, int dtype, int n, double l, double visc, void *const u[3],
+                  void *const du[3], void *d);
 /* divergence! (src/N.jl:222-238) */
 int ec_ns_divergence(ec_ctx *ctx, int dtype, int n, double l, void *const u[3], void *d);
 /* pressuregradient! (src/N.jl:240-254) */
@@ -106,6 +112,9 @@ int ec_ns_project(ec_ctx *ctx, ec_poisson *ps, void *const u[3]);
  * ec_poisson_pressure(ps) and u unchanged (the gradient is then applied by
  * ec_ns_pressuregradient or fused into ec_ns_rk4_update). */
 int ec_ns_pressure_solve(ec_ctx *ctx, ec_poisson *ps, void *const u[3]);
+/* The FFT half of the pressure solve of project! (src/N.jl:263-303): the
+ * pressure buffer ec_poisson_pressure(ps) holds div u on entry and p on exit. */
+int ec_poisson_solve(ec_ctx *ctx, ec_poisson *ps);
 /* plan_rfft * x (unnormalised) and ldiv! (normalised by 1/n^3); spectrum (n/2+1, n, n) */
 int ec_fft_r2c(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out);
 int ec_fft_c2r(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out);

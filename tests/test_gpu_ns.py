@@ -57,6 +57,40 @@ def test_rhs_marching_tiles_fp64(gpu, O, n):
     test_rhs_fp64(gpu, O, n)
 
 
+@pytest.mark.parametrize("n", [16, 64, 128, 192])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_rhs_div(gpu, O, n, prec):
+    """Fused du = -div sigma(u), d = div_h du (marching kernel for n % 64 == 0,
+    the two plain kernels otherwise) against the checker's rhs and divergence;
+    then the FFT solve on d gives the pressure of ec_ns_pressure_solve."""
+    dt = torch.float64 if prec == "f64" else torch.float32
+    tol = 1e-12 if prec == "f64" else 2e-5
+    u = rand_u(O, n, 3 * n + 1)
+    visc, l = 0.011, 1.0
+    ref = O.rhs(u, visc, l)
+    dref = O.divergence(ref, l)
+    ud = [dev(x, dt) for x in u]
+    du = [torch.empty_like(x) for x in ud]
+    d = torch.empty_like(ud[0])
+    ctx = gpu._abi.context()
+    code = 1 if prec == "f64" else 0
+    ctx.call("ec_ns_rhs_div", code, n, l, visc, P(ud, 3), P(du, 3), gpu._abi.c_void_p(d.data_ptr()))
+    for a in range(3):
+        assert rel(host(du[a]), ref[a]) < tol
+    # the divergence cancels: measure it against the size of its terms
+    scale = max(np.abs(r).max() for r in ref) * n
+    assert np.abs(host(d) - dref).max() / scale < (1e-13 if prec == "f64" else 1e-6)
+    NS = gpu.navierstokes
+    g = NS.Grid(l, n)
+    ps1, ps2 = NS.PoissonSolver(g, dt), NS.PoissonSolver(g, dt)
+    ctx.call("ec_ns_pressure_solve", ps1.h, P(du, 3))
+    ctx.call("ec_ns_rhs_div", code, n, l, visc, P(ud, 3), P(du, 3), gpu._abi.c_void_p(ps2.p_ptr))
+    ctx.call("ec_poisson_solve", ps2.h)
+    torch.cuda.synchronize()
+    p1, p2 = ps1.p.data, ps2.p.data
+    assert float((p1 - p2).abs().max()) <= (1e-12 if prec == "f64" else 1e-5) * float(p1.abs().max())
+
+
 @pytest.mark.parametrize("n", [16, 32, 128])
 def test_rhs_fp32(gpu, O, n):
     u = rand_u(O, n, 100 + n)
