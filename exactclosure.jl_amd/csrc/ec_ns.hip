@@ -468,6 +468,16 @@ __global__ void k_rk4_update(int n, T ih, T dt, Ptr3 u0p, Ptr3 usp, Ptr3 dup, co
 
 #define EC_CHECK_FIELD(ctx, n) EC_REQUIRE(ctx, (n) >= 2, "grid size n=%d must be >= 2", (n))
 
+// z-marching kernels: grids of at least 256 cells per side (n % 64 == 0);
+// planes per block chosen so a launch has >= 2048 tiles (chunks of 16..128)
+static bool march_ok(int n) { return n >= 256 && n % 64 == 0 && (double)n * n * n < 4294967296.0; }
+static int march_zc(int n) {
+    int zc = 128;
+    while (zc > 16 && (long)(n / 64) * (n / 8) * (n / zc) < 2048) zc /= 2;
+    while (n % zc) zc /= 2;
+    return zc;
+}
+
 extern "C" int ec_ns_stresstensor(ec_ctx *ctx, int dtype, int n, double l, double visc, void *const u[3],
                                   void *const s[6]) {
     EC_REQUIRE(ctx, ctx && u && s, "ec_ns_stresstensor: null argument");
@@ -536,9 +546,9 @@ extern "C" int ec_ns_rhs(ec_ctx *ctx, int dtype, int n, double l, double visc, v
     const double ih = n / l;
     const double b = dtype == EC_F64 ? 8 : 4;
     ec_prof_scope ps(ctx, "ns_rhs", (doadd ? 9.0 : 6.0) * ec_cube(n) * b);
-    if (n % 64 == 0 && (double)n * n * n < 4294967296.0) {
-        // z-marching tiles of 64 x 8 columns, chunks of up to 128 planes
-        const int ZC = n % 128 == 0 ? 128 : 64;
+    if (march_ok(n)) {
+        // z-marching tiles of 64 x 8 columns
+        const int ZC = march_zc(n);
         const unsigned tiles = (unsigned)((n / 64) * (n / 8) * (n / ZC));
         if (dtype == EC_F64)
             k_rhs_march<double, 64, 8><<<tiles, 256, 0, ctx->stream>>>(
@@ -567,7 +577,7 @@ extern "C" int ec_ns_rhs_div(ec_ctx *ctx, int dtype, int n, double l, double vis
                              void *const du[3], void *d) {
     EC_REQUIRE(ctx, ctx && u && du && d, "ec_ns_rhs_div: null argument");
     EC_CHECK_FIELD(ctx, n);
-    if (!(n % 64 == 0 && (double)n * n * n < 4294967296.0)) {
+    if (!march_ok(n)) {
         // small grids: the two plain kernels
         int rc = ec_ns_rhs(ctx, dtype, n, l, visc, u, du, 0);
         return rc ? rc : ec_ns_divergence(ctx, dtype, n, l, du, d);
@@ -575,7 +585,7 @@ extern "C" int ec_ns_rhs_div(ec_ctx *ctx, int dtype, int n, double l, double vis
     const double ih = n / l;
     const double b = dtype == EC_F64 ? 8 : 4;
     ec_prof_scope ps(ctx, "ns_rhs_div", 7.0 * ec_cube(n) * b);
-    const int ZC = n % 128 == 0 ? 128 : 64;
+    const int ZC = march_zc(n);
     const unsigned tiles = (unsigned)((n / 64) * (n / 8) * (n / ZC));
     if (dtype == EC_F64)
         k_rhs_div_march<double, 64, 8><<<tiles, 256, 0, ctx->stream>>>(

@@ -50,14 +50,15 @@ def test_rhs_fp64(gpu, O, n):
         assert rel(host(du[a]), 2 * ref[a]) < 1e-12
 
 
-@pytest.mark.parametrize("n", [64, 128, 192])
+@pytest.mark.parametrize("n", [64, 256, 320])
 def test_rhs_marching_tiles_fp64(gpu, O, n):
-    """Grids that take the z-marching kernel (64 x 8 column tiles, chunks of
-    64 or 128 planes): every tile and chunk boundary, with and without doadd."""
+    """Grids that take the z-marching kernel (n >= 256: 64 x 8 column tiles,
+    chunks of 16 to 128 planes) and one that does not: every tile and chunk
+    boundary, with and without doadd."""
     test_rhs_fp64(gpu, O, n)
 
 
-@pytest.mark.parametrize("n", [16, 64, 128, 192])
+@pytest.mark.parametrize("n", [16, 256, 320])
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_rhs_div(gpu, O, n, prec):
     """Fused du = -div sigma(u), d = div_h du (marching kernel for n % 64 == 0,
@@ -80,6 +81,8 @@ def test_rhs_div(gpu, O, n, prec):
     # the divergence cancels: measure it against the size of its terms
     scale = max(np.abs(r).max() for r in ref) * n
     assert np.abs(host(d) - dref).max() / scale < (1e-13 if prec == "f64" else 1e-6)
+    if n & (n - 1):
+        return  # the pressure comparison runs on the power-of-two grids
     NS = gpu.navierstokes
     g = NS.Grid(l, n)
     ps1, ps2 = NS.PoissonSolver(g, dt), NS.PoissonSolver(g, dt)
@@ -91,7 +94,7 @@ def test_rhs_div(gpu, O, n, prec):
     assert float((p1 - p2).abs().max()) <= (1e-12 if prec == "f64" else 1e-5) * float(p1.abs().max())
 
 
-@pytest.mark.parametrize("n", [16, 32, 128])
+@pytest.mark.parametrize("n", [16, 32, 256])
 def test_rhs_fp32(gpu, O, n):
     u = rand_u(O, n, 100 + n)
     visc, l = 0.01, 1.0
