@@ -357,9 +357,11 @@ __global__ void __launch_bounds__(256) k_rhs_div_march(int n, int ZC, T ih, T vi
     }
     __syncthreads();
     const int cx = tid % TX, cy0 = tid / TX;
-    // halo cell of this thread: column x0 - 1 (tid < TY) or row y0 - 1
-    const bool hcol = tid < TY, hrow = tid >= TY && tid < TY + TX;
-    const int hcx = hcol ? -1 : tid - TY, hcy = hcol ? tid : -1;
+    // halo cells (column x0 - 1, then row y0 - 1), spread evenly over the waves
+    constexpr int HPW = (TX + TY + 3) / 4;  // per wave (4 waves)
+    const int hidx = (tid & 63) < HPW ? (tid >> 6) * HPW + (tid & 63) : TX + TY;
+    const bool hcol = hidx < TY, hrow = hidx >= TY && hidx < TY + TX;
+    const int hcx = hcol ? -1 : hidx - TY, hcy = hcol ? hidx : -1;
     T a3prev[CPT];
     for (int rr = -1; rr < ZC; ++rr) {
         const int k = z0 + rr, q = rr + 2;  // ring plane index of k
@@ -631,6 +633,91 @@ extern "C" int ec_ns_pressuregradient(ec_ctx *ctx, int dtype, int n, double l, v
     return EC_OK;
 }
 
+// The same stage update with 16-byte accesses: a thread owns V = 16 / sizeof(T)
+// consecutive x cells (n % V == 0), so every field is read and written as one
+// vector per thread; the x + 1 pressure neighbour of the last cell is one
+// extra scalar load.
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+    typedef float4 type;
+    static constexpr int V = 4;
+};
+template <>
+struct Vec16<double> {
+    typedef double2 type;
+    static constexpr int V = 2;
+};
+
+template <typename T, int STAGE>
+__global__ void __launch_bounds__(256) k_rk4_update_vec(int n, T ih, T dt, Ptr3 u0p, Ptr3 usp, Ptr3 dup,
+                                                        const T *__restrict__ p, Ptr3 accp, size_t nvec) {
+    typedef typename Vec16<T>::type VT;
+    constexpr int V = Vec16<T>::V;
+    const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (t >= nvec) return;
+    const unsigned nx = (unsigned)n / V;
+    const unsigned iv = (unsigned)(t % nx);
+    const size_t row = t / nx;  // j + n k
+    const unsigned j = (unsigned)(row % (unsigned)n), k = (unsigned)(row / (unsigned)n);
+    const size_t n1 = (size_t)n, n2 = n1 * n1;
+    const size_t I = (size_t)iv * V + n1 * row;
+    const size_t Iy = (size_t)iv * V + n1 * ((j + 1 == (unsigned)n ? 0 : j + 1) + n1 * k);
+    const size_t Iz = (size_t)iv * V + n1 * (j + n1 * (k + 1 == (unsigned)n ? 0 : k + 1));
+    const unsigned xl = iv * V + V;  // x + 1 of the last cell
+    const T pxl = p[(xl == (unsigned)n ? 0 : xl) + n1 * row];
+    const VT p0 = *(const VT *)(p + I), py = *(const VT *)(p + Iy), pz = *(const VT *)(p + Iz);
+    const T *p0s = (const T *)&p0, *pys = (const T *)&py, *pzs = (const T *)&pz;
+    T gp[3][V];
+#pragma unroll
+    for (int e = 0; e < V; e++) {
+        const T pxp = e + 1 < V ? p0s[e + 1] : pxl;
+        gp[0][e] = (pxp - p0s[e]) * ih;
+        gp[1][e] = (pys[e] - p0s[e]) * ih;
+        gp[2][e] = (pzs[e] - p0s[e]) * ih;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        VT d = *(const VT *)((const T *)dup.p[a] + I);
+        T *ds = (T *)&d;
+        T kk[V];
+#pragma unroll
+        for (int e = 0; e < V; e++) kk[e] = ds[e] - gp[a][e];
+        T *u0 = (T *)u0p.p[a] + I;
+        T *acc = (T *)accp.p[a] + I;
+        VT uv = *(const VT *)u0;
+        const T *us_ = (const T *)&uv;
+        VT o1, o2;
+        T *r1 = (T *)&o1, *r2 = (T *)&o2;
+        if (STAGE == 0) {
+#pragma unroll
+            for (int e = 0; e < V; e++) {
+                r1[e] = kk[e];
+                r2[e] = us_[e] + (dt / 2) * kk[e];
+            }
+            *(VT *)acc = o1;
+            *(VT *)((T *)usp.p[a] + I) = o2;
+        } else if (STAGE == 1 || STAGE == 2) {
+            VT av = *(const VT *)acc;
+            const T *as = (const T *)&av;
+#pragma unroll
+            for (int e = 0; e < V; e++) {
+                r1[e] = as[e] + 2 * kk[e];
+                r2[e] = us_[e] + (STAGE == 1 ? dt / 2 : dt) * kk[e];
+            }
+            *(VT *)acc = o1;
+            *(VT *)((T *)usp.p[a] + I) = o2;
+        } else {
+            VT av = *(const VT *)acc;
+            const T *as = (const T *)&av;
+#pragma unroll
+            for (int e = 0; e < V; e++) r1[e] = us_[e] + (dt / 6) * (as[e] + kk[e]);
+            *(VT *)u0 = o1;
+        }
+    }
+}
+
 template <typename T>
 static int rk4_launch(ec_ctx *ctx, int n, double l, int stage, double dt, void *const u0[3], void *const us[3],
                       void *const du[3], const void *p, void *const acc[3]) {
@@ -638,6 +725,23 @@ static int rk4_launch(ec_ctx *ctx, int n, double l, int stage, double dt, void *
     const T ih = (T)(n / l);
     Ptr3 U0{{u0[0], u0[1], u0[2]}}, US{{us ? us[0] : nullptr, us ? us[1] : nullptr, us ? us[2] : nullptr}},
         DU{{du[0], du[1], du[2]}}, AC{{acc[0], acc[1], acc[2]}};
+    constexpr int V = Vec16<T>::V;
+    bool aligned = n % V == 0 && ((uintptr_t)p % 16) == 0;
+    for (int a = 0; a < 3; a++)
+        aligned = aligned && ((uintptr_t)u0[a] % 16) == 0 && ((uintptr_t)du[a] % 16) == 0 &&
+                  ((uintptr_t)acc[a] % 16) == 0 && (!us || ((uintptr_t)us[a] % 16) == 0);
+    if (aligned) {
+        const size_t nvec = ec_cube(n) / V;
+        const unsigned blocks = (unsigned)((nvec + 255) / 256);
+        switch (stage) {
+            case 0: k_rk4_update_vec<T, 0><<<blocks, 256, 0, ctx->stream>>>(n, ih, (T)dt, U0, US, DU, (const T *)p, AC, nvec); break;
+            case 1: k_rk4_update_vec<T, 1><<<blocks, 256, 0, ctx->stream>>>(n, ih, (T)dt, U0, US, DU, (const T *)p, AC, nvec); break;
+            case 2: k_rk4_update_vec<T, 2><<<blocks, 256, 0, ctx->stream>>>(n, ih, (T)dt, U0, US, DU, (const T *)p, AC, nvec); break;
+            default: k_rk4_update_vec<T, 3><<<blocks, 256, 0, ctx->stream>>>(n, ih, (T)dt, U0, US, DU, (const T *)p, AC, nvec); break;
+        }
+        EC_CHECK_LAUNCH(ctx, "k_rk4_update_vec");
+        return EC_OK;
+    }
     switch (stage) {
         case 0: k_rk4_update<T, 0><<<g.grid, g.block, 0, ctx->stream>>>(n, ih, (T)dt, U0, US, DU, (const T *)p, AC); break;
         case 1: k_rk4_update<T, 1><<<g.grid, g.block, 0, ctx->stream>>>(n, ih, (T)dt, U0, US, DU, (const T *)p, AC); break;
