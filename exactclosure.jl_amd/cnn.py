@@ -144,7 +144,7 @@ def closure_error(pred, exact):
     return (num / den) ** 0.5
 
 
-def generate_pairs(drv, set_index, dt, nsteps, every=1):
+def generate_pairs(drv, set_index, dt, nsteps, every=1, log=None):
     """Run the DNS-aided driver (forcing form) and collect (P_H V u, exact
     closure forcing m) of LES set `set_index` after every `every` steps."""
     X, Y = [], []
@@ -154,6 +154,9 @@ def generate_pairs(drv, set_index, dt, nsteps, every=1):
         if (s + 1) % every == 0:
             X.append([t.clone() for t in L.ubar])
             Y.append([t.clone() for t in L.m])
+        if log and (s + 1) % 64 == 0:
+            torch.cuda.synchronize()
+            log(f"[pairs] step {s + 1}/{nsteps}: {len(X)} pairs")
     return X, Y
 
 
