@@ -522,7 +522,7 @@ __global__ void __launch_bounds__(TK *NTHR) k_fft_cols_ct(cplx<T> *__restrict__ 
     if (MODE == 0) {
         ct_stages<T, N, 1, NTHR, TK, SIGN>(lds + c, t, tw);
     } else {
-        ct_stages<T, N, 1, NTHR, TK, -1, 8>(lds + c, t, tw);
+        ct_stages<T, N, 1, NTHR, TK, -1, (TK * NTHR > 512 ? 8 : 16)>(lds + c, t, tw);
         const int kx = col0 + c;
         const T sxy = (kx < ncols ? eig[kx] : T(0)) + eig[outer];
 #pragma unroll
@@ -536,7 +536,7 @@ __global__ void __launch_bounds__(TK *NTHR) k_fft_cols_ct(cplx<T> *__restrict__ 
             lds[pad8(e) * TK + c] = z;
         }
         __syncthreads();
-        ct_stages<T, N, 1, NTHR, TK, +1, 8>(lds + c, t, tw);
+        ct_stages<T, N, 1, NTHR, TK, +1, (TK * NTHR > 512 ? 8 : 16)>(lds + c, t, tw);
     }
 #pragma unroll
     for (int idx = threadIdx.x; idx < TOTAL; idx += TK * NTHR) {
@@ -578,8 +578,9 @@ int c2r_x_ct(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out, double scal
 template <typename T, int N, int MODE, int SIGN>
 int cols_ct(ec_ctx *ctx, ec_poisson *ps, void *spec, int axis, int pitch) {
     constexpr int NTHR = N / 16;
-    // one 128-byte line of complex values per row
-    constexpr int TK0 = sizeof(T) == 4 ? 16 : 8;
+    // one 128-byte line of complex values per row; the fused Poisson pass (two
+    // transforms) takes 8 columns so that 512-thread blocks can use radix 16
+    constexpr int TK0 = (sizeof(T) == 4 && MODE == 0) ? 16 : 8;
     constexpr int TK = (TK0 * NTHR > 1024) ? 1024 / NTHR : TK0;
     const int nh = ps->nh;
     const long elem_stride = axis == 1 ? (long)pitch : (long)pitch * N;
