@@ -420,67 +420,209 @@ __device__ __forceinline__ void ct_stages(cplx<T> *lds, int t, const cplx<T> *__
     }
 }
 
-template <typename T, int N, int NTHR>
-__global__ void __launch_bounds__(256) k_fft_x_r2c_ct(const T *__restrict__ in, cplx<T> *__restrict__ out, long nlines,
-                                                      int pitch, const cplx<T> *__restrict__ tw) {
+// X passes.  NTHR = N / 8 threads transform a line pair (8 elements each,
+// radix-8 stages); a block of xblock(NTHR) threads (one wave, or the two waves
+// of a 1024-point pair) transforms W = xblock / NTHR pairs at a time and loops
+// over G such groups, loading the next group into registers
+// while the current one is transformed.  Each thread meets the same twiddles
+// in every group, so it keeps, per butterfly and stage, the powers w^(2^i) of
+// its base twiddle w in registers (loaded once) and forms the other powers as
+// products of these (at most three complex products).  No global load is left
+// inside the stages, so the prefetch stays in flight through the transform.
+constexpr int xblock(int nthr) { return nthr > 64 ? nthr : 64; }
+
+template <typename T, int N, int NS, int NTHR, int MAXR, bool MORE = (NS < N)>
+struct RegStages {
+    __device__ __forceinline__ void load(int, const cplx<T> *__restrict__) {}
+    template <int SIGN>
+    __device__ __forceinline__ void run(cplx<T> *, int) {}
+};
+
+template <typename T, int N, int NS, int NTHR, int MAXR>
+struct RegStages<T, N, NS, NTHR, MAXR, true> {
+    static constexpr int R = ct_radix(N / NS, MAXR);
+    static constexpr int NR = N / R;
+    static constexpr int B = (NR + NTHR - 1) / NTHR;
+    static constexpr int TSTEP = N / (NS * R);
+    static constexpr int NB = NS > 1 ? (R == 16 ? 4 : (R == 8 ? 3 : (R == 4 ? 2 : 1))) : 1;
+    cplx<T> w[B][NB];  // w[q][i] = tw[(jm * TSTEP) << i] of butterfly q
+    RegStages<T, N, NS * R, NTHR, MAXR> next;
+
+    __device__ __forceinline__ void load(int t, const cplx<T> *__restrict__ tw) {
+        if constexpr (NS > 1) {
+#pragma unroll
+            for (int q = 0; q < B; q++) {
+                const int b = t + q * NTHR;
+                const int jm = (NR % NTHR == 0 || b < NR) ? (b & (NS - 1)) : 0;
+#pragma unroll
+                for (int i = 0; i < NB; i++) w[q][i] = tw[(jm * TSTEP) << i];
+            }
+        }
+        next.load(t, tw);
+    }
+
+    template <int SIGN>
+    __device__ __forceinline__ void run(cplx<T> *lds, int t) {
+        cplx<T> v[B][R];
+#pragma unroll
+        for (int q = 0; q < B; q++) {
+            const int b = t + q * NTHR;
+            if (NR % NTHR == 0 || b < NR) {
+                cplx<T> pw[R];
+                if constexpr (NS > 1) {
+                    // opaque copies: the products are formed here, not hoisted
+                    // out of the caller's group loop (they would hold registers)
+#pragma unroll
+                    for (int i = 0; i < NB; i++) {
+                        cplx<T> x = w[q][i];
+                        asm volatile("" : "+v"(x.x), "+v"(x.y));
+                        pw[1 << i] = x;
+                    }
+#pragma unroll
+                    for (int r = 1; r < R; r++) {
+                        const int low = r & -r;
+                        if (r != low) pw[r] = cmul(pw[r - low], pw[low]);
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    cplx<T> x = lds[pad8(b + r * NR)];
+                    if (NS > 1 && r > 0) {
+                        cplx<T> ww = pw[r];
+                        if (SIGN > 0) ww.y = -ww.y;
+                        x = cmul(x, ww);
+                    }
+                    v[q][r] = x;
+                }
+                Bfly<T, R>::run(v[q], SIGN);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < B; q++) {
+            const int b = t + q * NTHR;
+            if (NR % NTHR == 0 || b < NR) {
+                const int jm = b & (NS - 1);
+                const int idxD = (b - jm) * R + jm;
+#pragma unroll
+                for (int r = 0; r < R; r++) lds[pad8(idxD + r * NS)] = v[q][r];
+            }
+        }
+        __syncthreads();
+        next.template run<SIGN>(lds, t);
+    }
+};
+
+template <typename T, int N, int NTHR, int G>
+__global__ void __launch_bounds__(xblock(NTHR)) k_fft_x_r2c_ct(const T *__restrict__ in, cplx<T> *__restrict__ out,
+                                                               long nlines, int pitch, const cplx<T> *__restrict__ tw) {
     extern __shared__ char smem_raw[];
     cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
-    constexpr int NH = N / 2 + 1;
+    constexpr int NH = N / 2 + 1, W = xblock(NTHR) / NTHR, PER = N / NTHR;
     const int pair = threadIdx.x / NTHR, t = threadIdx.x % NTHR;
-    const int P = blockDim.x / NTHR;
-    const long L0 = 2 * ((long)blockIdx.x * P + pair);
-    const bool v0 = L0 < nlines, v1 = L0 + 1 < nlines;
     cplx<T> *line = lds + (size_t)pair * padded_len(N);
-    const T *a = in + L0 * N, *b = in + (L0 + 1) * N;
+    const long npairs = (nlines + 1) / 2;
+    RegStages<T, N, 1, NTHR, 8> st;
+    st.load(t, tw);
+    T ra[PER], rb[PER];
+    auto fetch = [&](long pr) {
+        const long L0 = 2 * pr;
+        const bool v0 = pr < npairs, v1 = L0 + 1 < nlines;
+        const T *a = in + L0 * N, *b = a + N;
 #pragma unroll
-    for (int e = t; e < N; e += NTHR) line[pad8(e)] = cplx<T>{v0 ? a[e] : T(0), v1 ? b[e] : T(0)};
-    __syncthreads();
-    ct_stages<T, N, 1, NTHR, 1, -1>(line, t, tw);
-    cplx<T> *oa = out + L0 * pitch, *ob = out + (L0 + 1) * pitch;
-    for (int k = t; k < NH; k += NTHR) {
-        const cplx<T> z = line[pad8(k)];
-        const cplx<T> zc = line[pad8((N - k) & (N - 1))];
-        const cplx<T> A{(z.x + zc.x) / 2, (z.y - zc.y) / 2};
-        const cplx<T> Bv{(z.y + zc.y) / 2, (zc.x - z.x) / 2};
-        if (v0) oa[k] = A;
-        if (v1) ob[k] = Bv;
+        for (int q = 0; q < PER; q++) {
+            const int e = t + q * NTHR;
+            ra[q] = v0 ? a[e] : T(0);
+            rb[q] = v1 ? b[e] : T(0);
+        }
+    };
+    const long g0 = (long)blockIdx.x * G * W;  // first pair of the block
+    fetch(g0 + pair);
+    for (int g = 0; g < G; g++) {
+        if (g0 + (long)g * W >= npairs) break;  // uniform over the block
+        const long pr = g0 + (long)g * W + pair;
+#pragma unroll
+        for (int q = 0; q < PER; q++) line[pad8(t + q * NTHR)] = cplx<T>{ra[q], rb[q]};
+        if (g + 1 < G) fetch(pr + W);
+        __syncthreads();
+        st.template run<-1>(line, t);
+        const long L0 = 2 * pr;
+        const bool v0 = pr < npairs, v1 = L0 + 1 < nlines;
+        cplx<T> *oa = out + L0 * pitch, *ob = oa + pitch;
+#pragma unroll
+        for (int k = t; k < NH; k += NTHR) {
+            const cplx<T> z = line[pad8(k)];
+            const cplx<T> zc = line[pad8((N - k) & (N - 1))];
+            const cplx<T> A{(z.x + zc.x) / 2, (z.y - zc.y) / 2};
+            const cplx<T> Bv{(z.y + zc.y) / 2, (zc.x - z.x) / 2};
+            if (v0) oa[k] = A;
+            if (v1) ob[k] = Bv;
+        }
+        __syncthreads();
     }
 }
 
-template <typename T, int N, int NTHR>
-__global__ void __launch_bounds__(256) k_fft_x_c2r_ct(const cplx<T> *__restrict__ in, T *__restrict__ out, long nlines,
-                                                      int pitch, const cplx<T> *__restrict__ tw, T scale) {
+template <typename T, int N, int NTHR, int G>
+__global__ void __launch_bounds__(xblock(NTHR)) k_fft_x_c2r_ct(const cplx<T> *__restrict__ in, T *__restrict__ out,
+                                                               long nlines, int pitch, const cplx<T> *__restrict__ tw,
+                                                               T scale) {
     extern __shared__ char smem_raw[];
     cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
-    constexpr int NH = N / 2 + 1;
+    constexpr int NH = N / 2 + 1, W = xblock(NTHR) / NTHR, PER = N / NTHR, KQ = (NH + NTHR - 1) / NTHR;
     const int pair = threadIdx.x / NTHR, t = threadIdx.x % NTHR;
-    const int P = blockDim.x / NTHR;
-    const long L0 = 2 * ((long)blockIdx.x * P + pair);
-    const bool v0 = L0 < nlines, v1 = L0 + 1 < nlines;
     cplx<T> *line = lds + (size_t)pair * padded_len(N);
-    const cplx<T> *ia = in + L0 * pitch, *ib = in + (L0 + 1) * pitch;
+    const long npairs = (nlines + 1) / 2;
+    RegStages<T, N, 1, NTHR, 8> st;
+    st.load(t, tw);
+    // only the stored half spectrum is loaded: bin k < NH of both lines
+    cplx<T> ra[KQ], rb[KQ];
+    auto fetch = [&](long pr) {
+        const long L0 = 2 * pr;
+        const bool v0 = pr < npairs, v1 = L0 + 1 < nlines;
+        const cplx<T> *ia = in + L0 * pitch, *ib = ia + pitch;
 #pragma unroll
-    for (int e = t; e < N; e += NTHR) {
-        const int src = e < NH ? e : N - e;
-        cplx<T> A = v0 ? ia[src] : cplx<T>{T(0), T(0)};
-        cplx<T> B = v1 ? ib[src] : cplx<T>{T(0), T(0)};
-        if (e == 0 || e == N / 2) {  // self-conjugate bins: c2r ignores their imaginary part
-            A.y = T(0);
-            B.y = T(0);
-        } else if (e >= NH) {
-            A.y = -A.y;
-            B.y = -B.y;
+        for (int q = 0; q < KQ; q++) {
+            const int k = t + q * NTHR;
+            const bool kin = q < KQ - 1 || k < NH;
+            ra[q] = v0 && kin ? ia[k] : cplx<T>{T(0), T(0)};
+            rb[q] = v1 && kin ? ib[k] : cplx<T>{T(0), T(0)};
         }
-        line[pad8(e)] = cplx<T>{A.x - B.y, A.y + B.x};
-    }
-    __syncthreads();
-    ct_stages<T, N, 1, NTHR, 1, +1>(line, t, tw);
-    T *oa = out + L0 * N, *ob = out + (L0 + 1) * N;
+    };
+    const long g0 = (long)blockIdx.x * G * W;
+    fetch(g0 + pair);
+    for (int g = 0; g < G; g++) {
+        if (g0 + (long)g * W >= npairs) break;  // uniform over the block
+        const long pr = g0 + (long)g * W + pair;
+        // line = A + iB with the Hermitian halves rebuilt: bin k and its
+        // mirror N - k (conj(A) + i conj(B)); the self-conjugate bins 0 and
+        // N/2 drop their imaginary parts
 #pragma unroll
-    for (int e = t; e < N; e += NTHR) {
-        const cplx<T> z = line[pad8(e)];
-        if (v0) oa[e] = z.x * scale;
-        if (v1) ob[e] = z.y * scale;
+        for (int q = 0; q < KQ; q++) {
+            const int k = t + q * NTHR;
+            if (q < KQ - 1 || k < NH) {
+                cplx<T> A = ra[q], B = rb[q];
+                if (k == 0 || k == N / 2) {
+                    A.y = T(0);
+                    B.y = T(0);
+                }
+                line[pad8(k)] = cplx<T>{A.x - B.y, A.y + B.x};
+                if (k > 0 && k < N / 2) line[pad8(N - k)] = cplx<T>{A.x + B.y, B.x - A.y};
+            }
+        }
+        if (g + 1 < G) fetch(pr + W);
+        __syncthreads();
+        st.template run<+1>(line, t);
+        const long L0 = 2 * pr;
+        const bool v0 = pr < npairs, v1 = L0 + 1 < nlines;
+        T *oa = out + L0 * N, *ob = oa + N;
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int e = t + q * NTHR;
+            const cplx<T> z = line[pad8(e)];
+            if (v0) oa[e] = z.x * scale;
+            if (v1) ob[e] = z.y * scale;
+        }
+        __syncthreads();
     }
 }
 
@@ -545,34 +687,59 @@ __global__ void __launch_bounds__(TK *NTHR) k_fft_cols_ct(cplx<T> *__restrict__ 
     }
 }
 
-template <typename T, int N>
-int r2c_x_ct(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out, int pitch) {
-    constexpr int NTHR = N / 16;
-    const int P = 256 / NTHR;
-    const size_t lds = (size_t)P * padded_len(N) * sizeof(cplx<T>);
-    const long nlines = (long)N * N;
-    const int blocks = (int)(((nlines + 1) / 2 + P - 1) / P);
-    EC_HIP(ctx, hipFuncSetAttribute((const void *)k_fft_x_r2c_ct<T, N, NTHR>,
+// groups per block of the x passes: enough blocks to fill the device several times
+inline int x_groups(long npairs, int W) {
+    if (npairs / (16L * W) >= 8192) return 16;
+    if (npairs / (4L * W) >= 2048) return 4;
+    return 1;
+}
+
+template <typename T, int N, int G>
+int r2c_x_launch(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out, int pitch) {
+    constexpr int NTHR = N / 8, W = xblock(NTHR) / NTHR;
+    const size_t lds = (size_t)W * padded_len(N) * sizeof(cplx<T>);
+    const long nlines = (long)N * N, npairs = (nlines + 1) / 2;
+    const long blocks = (npairs + (long)G * W - 1) / ((long)G * W);
+    EC_HIP(ctx, hipFuncSetAttribute((const void *)k_fft_x_r2c_ct<T, N, NTHR, G>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    k_fft_x_r2c_ct<T, N, NTHR><<<blocks, P * NTHR, lds, ctx->stream>>>((const T *)in, (cplx<T> *)out, nlines, pitch,
-                                                                        (const cplx<T> *)ps->twiddle);
+    k_fft_x_r2c_ct<T, N, NTHR, G><<<(unsigned)blocks, xblock(NTHR), lds, ctx->stream>>>((const T *)in, (cplx<T> *)out, nlines,
+                                                                             pitch, (const cplx<T> *)ps->twiddle);
     EC_CHECK_LAUNCH(ctx, "k_fft_x_r2c_ct");
     return EC_OK;
 }
 
 template <typename T, int N>
-int c2r_x_ct(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out, double scale, int pitch) {
-    constexpr int NTHR = N / 16;
-    const int P = 256 / NTHR;
-    const size_t lds = (size_t)P * padded_len(N) * sizeof(cplx<T>);
-    const long nlines = (long)N * N;
-    const int blocks = (int)(((nlines + 1) / 2 + P - 1) / P);
-    EC_HIP(ctx, hipFuncSetAttribute((const void *)k_fft_x_c2r_ct<T, N, NTHR>,
+int r2c_x_ct(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out, int pitch) {
+    constexpr int W = xblock(N / 8) / (N / 8);
+    switch (x_groups(((long)N * N + 1) / 2, W)) {
+        case 16: return r2c_x_launch<T, N, 16>(ctx, ps, in, out, pitch);
+        case 4: return r2c_x_launch<T, N, 4>(ctx, ps, in, out, pitch);
+        default: return r2c_x_launch<T, N, 1>(ctx, ps, in, out, pitch);
+    }
+}
+
+template <typename T, int N, int G>
+int c2r_x_launch(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out, double scale, int pitch) {
+    constexpr int NTHR = N / 8, W = xblock(NTHR) / NTHR;
+    const size_t lds = (size_t)W * padded_len(N) * sizeof(cplx<T>);
+    const long nlines = (long)N * N, npairs = (nlines + 1) / 2;
+    const long blocks = (npairs + (long)G * W - 1) / ((long)G * W);
+    EC_HIP(ctx, hipFuncSetAttribute((const void *)k_fft_x_c2r_ct<T, N, NTHR, G>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    k_fft_x_c2r_ct<T, N, NTHR><<<blocks, P * NTHR, lds, ctx->stream>>>((const cplx<T> *)in, (T *)out, nlines, pitch,
-                                                                        (const cplx<T> *)ps->twiddle, (T)scale);
+    k_fft_x_c2r_ct<T, N, NTHR, G><<<(unsigned)blocks, xblock(NTHR), lds, ctx->stream>>>(
+        (const cplx<T> *)in, (T *)out, nlines, pitch, (const cplx<T> *)ps->twiddle, (T)scale);
     EC_CHECK_LAUNCH(ctx, "k_fft_x_c2r_ct");
     return EC_OK;
+}
+
+template <typename T, int N>
+int c2r_x_ct(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out, double scale, int pitch) {
+    constexpr int W = xblock(N / 8) / (N / 8);
+    switch (x_groups(((long)N * N + 1) / 2, W)) {
+        case 16: return c2r_x_launch<T, N, 16>(ctx, ps, in, out, scale, pitch);
+        case 4: return c2r_x_launch<T, N, 4>(ctx, ps, in, out, scale, pitch);
+        default: return c2r_x_launch<T, N, 1>(ctx, ps, in, out, scale, pitch);
+    }
 }
 
 template <typename T, int N, int MODE, int SIGN>

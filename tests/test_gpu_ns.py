@@ -152,7 +152,7 @@ def test_divergence_gradient(gpu, O, n):
         assert rel(host(ud[a]), ref[a]) < 1e-13
 
 
-@pytest.mark.parametrize("n", SIZES + [48, 64])
+@pytest.mark.parametrize("n", SIZES + [48, 64, 128, 256])
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_fft_roundtrip_and_spectrum(gpu, O, n, prec):
     NS = gpu.navierstokes
@@ -167,6 +167,39 @@ def test_fft_roundtrip_and_spectrum(gpu, O, n, prec):
     assert np.abs(got - ref).max() / np.abs(ref).max() < tol
     back = NS.irfft(X, ps)
     assert rel(host(back), x) < tol
+
+
+@pytest.mark.parametrize("n,prec", [(512, "f64"), (512, "f32"), (1024, "f32")])
+def test_fft_plane_waves_large(gpu, n, prec):
+    """Full-size grids: a sum of plane waves cos(2 pi (a i + b j + c k) / n + phi)
+    has the half spectrum n^3/2 e^{i phi} at (a, b, c) (0 < a < n/2) and zero
+    elsewhere; the inverse returns the field."""
+    NS = gpu.navierstokes
+    dt = torch.float64 if prec == "f64" else torch.float32
+    g = NS.Grid(1.0, n)
+    ps = NS.PoissonSolver(g, dt)
+    modes = [(1, 0, 0, 0.3, 1.0), (5, n - 3, 7, -1.1, 0.5), (n // 2 - 1, n // 2, n - 1, 2.0, 0.25)]
+    ar = torch.arange(n, device="cuda", dtype=torch.float64)
+    x = torch.zeros((n, n, n), device="cuda", dtype=dt)
+    for a, b, c, phi, amp in modes:  # x[k, j, i], i fastest (column-major (i, j, k))
+        ph = (2 * np.pi / n) * ((a * ar).view(1, 1, n) + (b * ar).view(1, n, 1) + (c * ar).view(n, 1, 1))
+        x += (amp * torch.cos(ph + phi)).to(dt)
+        del ph
+    X = NS.rfft(x, ps)
+    half = n ** 3 / 2
+    tol = 1e-12 if prec == "f64" else 3e-6
+    for a, b, c, phi, amp in modes:
+        v = complex(X[c, b, a].item())
+        ref = amp * half * complex(np.cos(phi), np.sin(phi))
+        assert abs(v - ref) < tol * half, (a, b, c, v, ref)
+        X[c, b, a] = 0
+    assert float(X.abs().max()) < tol * half
+    for a, b, c, phi, amp in modes:
+        X[c, b, a] = complex(amp * half * np.cos(phi), amp * half * np.sin(phi))
+    back = NS.irfft(X, ps)
+    assert float(back.sub(x).abs().max()) < 10 * tol * 1.75  # max |x| <= 1.75
+    del X, back, x
+    torch.cuda.empty_cache()
 
 
 @pytest.mark.parametrize("n", SIZES + [64])
