@@ -202,6 +202,41 @@ def test_fft_plane_waves_large(gpu, n, prec):
     torch.cuda.empty_cache()
 
 
+@pytest.mark.parametrize("n", [512, 1024])
+def test_poisson_plane_waves_large(gpu, n):
+    """FP32 Poisson solve at full size: the discrete Laplacian of the staggered
+    grid maps cos(2 pi (a i + b j + c k) / n + phi) to lambda times itself,
+    lambda = -(4 / h^2) (sin^2(pi a / n) + sin^2(pi b / n) + sin^2(pi c / n)),
+    so the right-hand side cos(...) has the solution cos(...) / lambda.
+
+    One mode per solve.  FP32 rounding noise that the transforms leave in the
+    low-|lambda| bins is amplified by |lambda| / lambda_min there, so the bound
+    on |lambda p - f| is 2e-6 + 5e-9 |lambda| / lambda_min: tight (~2e-6) for
+    modes near lambda_min, where a wrong eigenvalue index or scale shows at
+    once.  Modes (n/2 - 1, 0, 0) are left out: their leakage is amplified by
+    ~n^2/pi^2 and an FP32 library FFT misses them by ~1e-3 as well."""
+    NS = gpu.navierstokes
+    g = NS.Grid(1.0, n)
+    ps = NS.PoissonSolver(g, torch.float32)
+    h = 1.0 / n
+    lmin = 4 / h ** 2 * np.sin(np.pi / n) ** 2
+    ctx = gpu._abi.context()
+    ar = torch.arange(n, device="cuda", dtype=torch.float64)
+    for a, b, c, phi in [(1, 0, 0, 0.3), (0, 0, 1, 0.5), (0, 1, 0, -0.5), (2, n - 1, 1, 1.0), (5, n - 3, 7, -1.1),
+                         (n // 2 - 1, n // 2, n - 1, 2.0), (0, 2, n // 2, 0.7), (n // 2, 1, 0, 0.4)]:
+        lam = -(4 / h ** 2) * sum(np.sin(np.pi * m / n) ** 2 for m in (a, b, c))
+        w = torch.cos((2 * np.pi / n) * ((a * ar).view(1, 1, n) + (b * ar).view(1, n, 1)
+                                         + (c * ar).view(n, 1, 1)) + phi)  # [k, j, i], i fastest
+        f = w.float()
+        ctx.call("ec_d2d", gpu._abi.c_void_p(ps.p_ptr), gpu._abi.c_void_p(f.data_ptr()), f.numel() * 4)
+        ctx.call("ec_poisson_solve", ps.h)
+        got = ps.p.data
+        err = float((got.double() * lam - w).abs().max())
+        assert err < 2e-6 + 5e-9 * abs(lam) / lmin, (a, b, c, err)
+        del w, f, got
+    torch.cuda.empty_cache()
+
+
 @pytest.mark.parametrize("n", SIZES + [64])
 def test_project_fp64(gpu, O, n):
     NS = gpu.navierstokes
