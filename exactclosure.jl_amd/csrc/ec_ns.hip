@@ -363,10 +363,12 @@ __global__ void __launch_bounds__(256) k_rhs_div_march(int n, int ZC, T ih, T vi
     const bool hcol = hidx < TY, hrow = hidx >= TY && hidx < TY + TX;
     const int hcx = hcol ? -1 : hidx - TY, hcy = hcol ? hidx : -1;
     T a3prev[CPT];
-    for (int rr = -1; rr < ZC; ++rr) {
-        const int k = z0 + rr, q = rr + 2;  // ring plane index of k
-        T nxt[NLD];
-        if (rr + 1 < ZC) load(k + 2, nxt);
+    // plane k (ring index q = rr + 2): plane k + 2 arrives in `cur` (loaded one
+    // plane earlier) and goes to the ring; plane k + 3 is loaded into `ahead`,
+    // so two planes of loads are in flight while a plane is evaluated
+    auto plane = [&](int rr, T *cur, T *ahead) {
+        const int k = z0 + rr, q = rr + 2;
+        if (rr + 2 < ZC) load(k + 3, ahead);
         const T *pm = ring[(q - 1) & 3], *p0 = ring[q & 3], *pp = ring[(q + 1) & 3];
         T a1[CPT], a2[CPT], a3[CPT];
 #pragma unroll
@@ -393,7 +395,7 @@ __global__ void __launch_bounds__(256) k_rhs_div_march(int n, int ZC, T ih, T vi
                 else dx2[buf][hcx] = h2;
             }
         }
-        if (rr + 1 < ZC) store((q + 2) & 3, nxt);
+        if (rr + 1 < ZC) store((q + 2) & 3, cur);
         __syncthreads();
         if (rr >= 0) {
 #pragma unroll
@@ -406,6 +408,12 @@ __global__ void __launch_bounds__(256) k_rhs_div_march(int n, int ZC, T ih, T vi
         }
 #pragma unroll
         for (int m = 0; m < CPT; m++) a3prev[m] = a3[m];
+    };
+    T ba[NLD], bb[NLD];
+    if (ZC > 0) load(z0 + 1, ba);  // plane k + 2 of the first step (rr = -1)
+    for (int rr = -1; rr < ZC; rr += 2) {
+        plane(rr, ba, bb);
+        if (rr + 1 < ZC) plane(rr + 1, bb, ba);
     }
 }
 
