@@ -169,6 +169,53 @@ __global__ void k_rhs(int n, T ih, T visc, const T *__restrict__ u1, const T *__
     du3[I] = a3;
 }
 
+// -div sigma(u) at one cell of a haloed plane triple (pm, p0, pp: planes k-1,
+// k, k+1 of the three components, each component HP values apart, rows HX
+// apart); o is the cell's offset within a plane.  The stress differences of
+// k_rhs are regrouped so that no stress is formed on its own: with averages
+// taken as sums and the 1/4 applied once,
+//   convective  ((r + l) / 2)^2 differences -> (u+ - u-) (u+ + 2 u0 + u-) / 4,
+//               off-diagonal products (a + a')(b + b') / 4, each formed once;
+//   viscous     the nu (d_a u_b + d_b u_a) differences of one component sum to
+//               ih^2 (2 S_a + S_b + S_c - 8 u0 + two mixed differences),
+//               S_d = sum of the two neighbours along d;
+// so du_a = ih (nu ih V_a - C_a / 4).  Same values as the stress form up to
+// rounding.
+template <typename T>
+__device__ __forceinline__ void rhs_cell(const T *pm, const T *p0, const T *pp, int o, int HX, int HP, T ih, T visc,
+                                         T &a1, T &a2, T &a3) {
+    const T *b0 = p0 + HP, *c0 = p0 + 2 * HP;
+    const T u1_0 = p0[o], u1_xp = p0[o + 1], u1_xm = p0[o - 1], u1_yp = p0[o + HX], u1_ym = p0[o - HX];
+    const T u1_xm_yp = p0[o - 1 + HX], u1_zp = pp[o], u1_zm = pm[o], u1_xm_zp = pp[o - 1];
+    const T u2_0 = b0[o], u2_xp = b0[o + 1], u2_xm = b0[o - 1], u2_yp = b0[o + HX], u2_ym = b0[o - HX];
+    const T u2_xp_ym = b0[o + 1 - HX], u2_zp = pp[HP + o], u2_zm = pm[HP + o], u2_ym_zp = pp[HP + o - HX];
+    const T u3_0 = c0[o], u3_xp = c0[o + 1], u3_xm = c0[o - 1], u3_yp = c0[o + HX], u3_ym = c0[o - HX];
+    const T u3_zp = pp[2 * HP + o], u3_zm = pm[2 * HP + o], u3_xp_zm = pm[2 * HP + o + 1];
+    const T u3_yp_zm = pm[2 * HP + o + HX];
+    // convective, 4x: the nine off-diagonal products and the three diagonal differences
+    const T p12 = (u1_yp + u1_0) * (u2_xp + u2_0), q12y = (u1_0 + u1_ym) * (u2_xp_ym + u2_ym);
+    const T q12x = (u1_xm_yp + u1_xm) * (u2_0 + u2_xm);
+    const T p31 = (u3_xp + u3_0) * (u1_zp + u1_0), q31z = (u3_xp_zm + u3_zm) * (u1_0 + u1_zm);
+    const T q31x = (u3_0 + u3_xm) * (u1_xm_zp + u1_xm);
+    const T p23 = (u2_zp + u2_0) * (u3_yp + u3_0), q23z = (u2_0 + u2_zm) * (u3_yp_zm + u3_zm);
+    const T q23y = (u2_ym_zp + u2_ym) * (u3_0 + u3_ym);
+    const T s1x = u1_xp + u1_xm, s2y = u2_yp + u2_ym, s3z = u3_zp + u3_zm;
+    const T c1 = (u1_xp - u1_xm) * (s1x + 2 * u1_0) + (p12 - q12y) + (p31 - q31z);
+    const T c2 = (p12 - q12x) + (u2_yp - u2_ym) * (s2y + 2 * u2_0) + (p23 - q23z);
+    const T c3 = (p31 - q31x) + (p23 - q23y) + (u3_zp - u3_zm) * (s3z + 2 * u3_0);
+    // viscous, divided by ih^2
+    const T v1 = 2 * s1x + (u1_yp + u1_ym) + (u1_zp + u1_zm) - 8 * u1_0 + ((u2_xp - u2_0) - (u2_xp_ym - u2_ym)) +
+                 ((u3_xp - u3_0) - (u3_xp_zm - u3_zm));
+    const T v2 = (u2_xp + u2_xm) + 2 * s2y + (u2_zp + u2_zm) - 8 * u2_0 + ((u1_yp - u1_0) - (u1_xm_yp - u1_xm)) +
+                 ((u3_yp - u3_0) - (u3_yp_zm - u3_zm));
+    const T v3 = (u3_xp + u3_xm) + (u3_yp + u3_ym) + 2 * s3z - 8 * u3_0 + ((u1_zp - u1_0) - (u1_xm_zp - u1_xm)) +
+                 ((u2_zp - u2_0) - (u2_ym_zp - u2_ym));
+    const T vi = visc * ih;
+    a1 = ih * (vi * v1 - T(0.25) * c1);
+    a2 = ih * (vi * v2 - T(0.25) * c2);
+    a3 = ih * (vi * v3 - T(0.25) * c3);
+}
+
 // Fused RHS for large grids, z-marching: a block owns a TX x TY column tile
 // and marches ZC planes.  Planes k-1 .. k+2 of the three velocity components
 // (with a one-cell x/y halo) rotate through a four-slot LDS ring; the loads of
@@ -232,35 +279,14 @@ __global__ void __launch_bounds__(256) k_rhs_march(int n, int ZC, T ih, T visc, 
         for (int m = 0; m < CPT; m++) {
             const int cy = cy0 + m * (256 / TX);
             const int o = (cy + 1) * HX + (cx + 1);  // the cell in a haloed plane
-            const T *a0 = p0, *b0 = p0 + HP, *c0 = p0 + 2 * HP;
-            const T u1_0 = a0[o], u1_xp = a0[o + 1], u1_xm = a0[o - 1], u1_yp = a0[o + HX], u1_ym = a0[o - HX];
-            const T u1_xm_yp = a0[o - 1 + HX], u1_zp = pp[o], u1_zm = pm[o], u1_xm_zp = pp[o - 1];
-            const T u2_0 = b0[o], u2_xp = b0[o + 1], u2_xm = b0[o - 1], u2_yp = b0[o + HX], u2_ym = b0[o - HX];
-            const T u2_xp_ym = b0[o + 1 - HX], u2_zp = pp[HP + o], u2_zm = pm[HP + o], u2_ym_zp = pp[HP + o - HX];
-            const T u3_0 = c0[o], u3_xp = c0[o + 1], u3_xm = c0[o - 1], u3_yp = c0[o + HX], u3_ym = c0[o - HX];
-            const T u3_zp = pp[2 * HP + o], u3_zm = pm[2 * HP + o], u3_xp_zm = pm[2 * HP + o + 1];
-            const T u3_yp_zm = pm[2 * HP + o + HX];
-
-            const T s11_p = s_diag(u1_xp, u1_0, visc, ih), s11_0 = s_diag(u1_0, u1_xm, visc, ih);
-            const T s22_p = s_diag(u2_yp, u2_0, visc, ih), s22_0 = s_diag(u2_0, u2_ym, visc, ih);
-            const T s33_p = s_diag(u3_zp, u3_0, visc, ih), s33_0 = s_diag(u3_0, u3_zm, visc, ih);
-            const T s12_0 = s_off(u1_yp, u1_0, u2_xp, u2_0, visc, ih);
-            const T s12_ym = s_off(u1_0, u1_ym, u2_xp_ym, u2_ym, visc, ih);
-            const T s12_xm = s_off(u1_xm_yp, u1_xm, u2_0, u2_xm, visc, ih);
-            const T s31_0 = s_off(u3_xp, u3_0, u1_zp, u1_0, visc, ih);
-            const T s31_zm = s_off(u3_xp_zm, u3_zm, u1_0, u1_zm, visc, ih);
-            const T s31_xm = s_off(u3_0, u3_xm, u1_xm_zp, u1_xm, visc, ih);
-            const T s23_0 = s_off(u2_zp, u2_0, u3_yp, u3_0, visc, ih);
-            const T s23_zm = s_off(u2_0, u2_zm, u3_yp_zm, u3_zm, visc, ih);
-            const T s23_ym = s_off(u2_ym_zp, u2_ym, u3_0, u3_ym, visc, ih);
-
+            T a1, a2, a3;
+            rhs_cell(pm, p0, pp, o, HX, HP, ih, visc, a1, a2, a3);
             const size_t I = (size_t)(x0 + cx) + (size_t)nn * ((size_t)(y0 + cy) + (size_t)nn * (size_t)k);
-            T a1 = doadd ? du1[I] : T(0);
-            a1 = a1 - (s11_p - s11_0) * ih - (s12_0 - s12_ym) * ih - (s31_0 - s31_zm) * ih;
-            T a2 = doadd ? du2[I] : T(0);
-            a2 = a2 - (s12_0 - s12_xm) * ih - (s22_p - s22_0) * ih - (s23_0 - s23_zm) * ih;
-            T a3 = doadd ? du3[I] : T(0);
-            a3 = a3 - (s31_0 - s31_xm) * ih - (s23_0 - s23_ym) * ih - (s33_p - s33_0) * ih;
+            if (doadd) {
+                a1 += du1[I];
+                a2 += du2[I];
+                a3 += du3[I];
+            }
             du1[I] = a1;
             du2[I] = a2;
             du3[I] = a3;
@@ -268,37 +294,6 @@ __global__ void __launch_bounds__(256) k_rhs_march(int n, int ZC, T ih, T visc, 
         if (rr + 1 < ZC) store((rr + 3) & 3, nxt);
         __syncthreads();
     }
-}
-
-// -div sigma(u) at one cell of a haloed plane triple (pm, p0, pp: planes k-1,
-// k, k+1 of the three components, each component HP values apart, rows HX
-// apart); o is the cell's offset within a plane.
-template <typename T>
-__device__ __forceinline__ void rhs_cell(const T *pm, const T *p0, const T *pp, int o, int HX, int HP, T ih, T visc,
-                                         T &a1, T &a2, T &a3) {
-    const T *b0 = p0 + HP, *c0 = p0 + 2 * HP;
-    const T u1_0 = p0[o], u1_xp = p0[o + 1], u1_xm = p0[o - 1], u1_yp = p0[o + HX], u1_ym = p0[o - HX];
-    const T u1_xm_yp = p0[o - 1 + HX], u1_zp = pp[o], u1_zm = pm[o], u1_xm_zp = pp[o - 1];
-    const T u2_0 = b0[o], u2_xp = b0[o + 1], u2_xm = b0[o - 1], u2_yp = b0[o + HX], u2_ym = b0[o - HX];
-    const T u2_xp_ym = b0[o + 1 - HX], u2_zp = pp[HP + o], u2_zm = pm[HP + o], u2_ym_zp = pp[HP + o - HX];
-    const T u3_0 = c0[o], u3_xp = c0[o + 1], u3_xm = c0[o - 1], u3_yp = c0[o + HX], u3_ym = c0[o - HX];
-    const T u3_zp = pp[2 * HP + o], u3_zm = pm[2 * HP + o], u3_xp_zm = pm[2 * HP + o + 1];
-    const T u3_yp_zm = pm[2 * HP + o + HX];
-    const T s11_p = s_diag(u1_xp, u1_0, visc, ih), s11_0 = s_diag(u1_0, u1_xm, visc, ih);
-    const T s22_p = s_diag(u2_yp, u2_0, visc, ih), s22_0 = s_diag(u2_0, u2_ym, visc, ih);
-    const T s33_p = s_diag(u3_zp, u3_0, visc, ih), s33_0 = s_diag(u3_0, u3_zm, visc, ih);
-    const T s12_0 = s_off(u1_yp, u1_0, u2_xp, u2_0, visc, ih);
-    const T s12_ym = s_off(u1_0, u1_ym, u2_xp_ym, u2_ym, visc, ih);
-    const T s12_xm = s_off(u1_xm_yp, u1_xm, u2_0, u2_xm, visc, ih);
-    const T s31_0 = s_off(u3_xp, u3_0, u1_zp, u1_0, visc, ih);
-    const T s31_zm = s_off(u3_xp_zm, u3_zm, u1_0, u1_zm, visc, ih);
-    const T s31_xm = s_off(u3_0, u3_xm, u1_xm_zp, u1_xm, visc, ih);
-    const T s23_0 = s_off(u2_zp, u2_0, u3_yp, u3_0, visc, ih);
-    const T s23_zm = s_off(u2_0, u2_zm, u3_yp_zm, u3_zm, visc, ih);
-    const T s23_ym = s_off(u2_ym_zp, u2_ym, u3_0, u3_ym, visc, ih);
-    a1 = -(s11_p - s11_0) * ih - (s12_0 - s12_ym) * ih - (s31_0 - s31_zm) * ih;
-    a2 = -(s12_0 - s12_xm) * ih - (s22_p - s22_0) * ih - (s23_0 - s23_zm) * ih;
-    a3 = -(s31_0 - s31_xm) * ih - (s23_0 - s23_ym) * ih - (s33_p - s33_0) * ih;
 }
 
 // Fused RHS and its divergence, z-marching like k_rhs_march: du = -div sigma(u)
@@ -357,11 +352,16 @@ __global__ void __launch_bounds__(256) k_rhs_div_march(int n, int ZC, T ih, T vi
     }
     __syncthreads();
     const int cx = tid % TX, cy0 = tid / TX;
-    // halo cells (column x0 - 1, then row y0 - 1), spread evenly over the waves
-    constexpr int HPW = (TX + TY + 3) / 4;  // per wave (4 waves)
-    const int hidx = (tid & 63) < HPW ? (tid >> 6) * HPW + (tid & 63) : TX + TY;
-    const bool hcol = hidx < TY, hrow = hidx >= TY && hidx < TY + TX;
-    const int hcx = hcol ? -1 : hidx - TY, hcy = hcol ? hidx : -1;
+    // halo cells: wave 0 evaluates du1 on the column x0 - 1 (TY cells), waves
+    // 1..3 du2 on the row y0 - 1 (TX cells); one component per wave, so each
+    // wave forms only the terms of that component
+    static_assert(TY <= 64 && 3 * ((TX + 2) / 3) <= 192, "halo split");
+    constexpr int RPW = (TX + 2) / 3;  // row cells per wave
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6), ln = tid & 63;
+    const bool hcol = wv == 0 && ln < TY;
+    const int ridx = (wv - 1) * RPW + ln;
+    const bool hrow = wv > 0 && ln < RPW && ridx < TX;
+    const int hcx = wv == 0 ? -1 : ridx, hcy = wv == 0 ? ln : -1;
     T a3prev[CPT];
     // plane k (ring index q = rr + 2): plane k + 2 arrives in `cur` (loaded one
     // plane earlier) and goes to the ring; plane k + 3 is loaded into `ahead`,
@@ -388,11 +388,16 @@ __global__ void __launch_bounds__(256) k_rhs_div_march(int n, int ZC, T ih, T vi
                 dx1[buf][cy * (TX + 1) + cx + 1] = a1[m];
                 dx2[buf][(cy + 1) * TX + cx] = a2[m];
             }
-            if (hcol || hrow) {
+            if (wv == 0) {
+                if (hcol) {
+                    T h1, h2, h3;
+                    rhs_cell(pm, p0, pp, (hcy + 2) * HX + 1, HX, HP, ih, visc, h1, h2, h3);
+                    dx1[buf][hcy * (TX + 1)] = h1;
+                }
+            } else if (hrow) {
                 T h1, h2, h3;
-                rhs_cell(pm, p0, pp, (hcy + 2) * HX + (hcx + 2), HX, HP, ih, visc, h1, h2, h3);
-                if (hcol) dx1[buf][hcy * (TX + 1)] = h1;
-                else dx2[buf][hcx] = h2;
+                rhs_cell(pm, p0, pp, HX + (hcx + 2), HX, HP, ih, visc, h1, h2, h3);
+                dx2[buf][hcx] = h2;
             }
         }
         if (rr + 1 < ZC) store((q + 2) & 3, cur);
