@@ -365,44 +365,65 @@ __device__ __forceinline__ void set_ystage(int r, int ZC, const double *lds, int
     }
 }
 
-// Element offsets are 32-bit: the host requires n^3 < 2^32.
-template <typename T, class G>
-__global__ void __launch_bounds__(G::NT) k_filter_vel(const T *__restrict__ v1, const T *__restrict__ v2,
-                                                     const T *__restrict__ v3, const T *__restrict__ p, int n, int ZC,
-                                                     double ih, FArgs A) {
-    constexpr int TX = G::TX, TY = G::TY, H = G::hmax, NJ = G::NJ, NT = G::NT;
-    extern __shared__ double lds[];
-    // output table in LDS: read only when a coarse plane closes, so it holds
-    // no scalar registers through the plane loop
-    FSet *fsl = reinterpret_cast<FSet *>(lds + G::lds_doubles);
-    if (threadIdx.x < 4) fsl[threadIdx.x] = A.s[threadIdx.x];
-    // 1D grid of (n/TX) x (n/TY) x (n/ZC) tiles, x fastest, in XCD order
-    const int gx = n / TX, gy = n / TY;
-    const int L = (int)ec_xcd_tile(blockIdx.x, gridDim.x);
-    const int x0 = (L % gx) * TX, y0 = ((L / gx) % gy) * TY, z0 = (L / (gx * gy)) * ZC;
-    const int tid = threadIdx.x;
-    const unsigned nn = (unsigned)n, n2 = nn * nn;
+// Component of column slot j of wave w (3: a padding slot).  The component
+// regions are whole waves, so the component is uniform over a wave; every
+// wave's slot components form a code of 2 bits per slot.
+template <class G>
+constexpr int slot_comp(int w, int j) {
+    const int e = w * 64 + j * G::NT;
+    return e < G::P0 ? 0 : (e < G::P0 + G::P1 ? 1 : (e < G::NCC ? 2 : 3));
+}
+template <class G>
+constexpr int wave_code(int w) {
+    int c = 0;
+    for (int j = 0; j < G::NJ; j++) c |= slot_comp<G>(w, j) << (2 * j);
+    return c;
+}
+template <class G>
+constexpr bool first_code(int w) {
+    for (int v = 0; v < w; v++)
+        if (wave_code<G>(v) == wave_code<G>(w)) return false;
+    return true;
+}
 
-    // the thread's columns: component (one per wave), tile-local position,
-    // offsets of the value and of the pressure neighbour along the component
-    int ca[NJ], clx[NJ], cly[NJ];
+struct FvTile {
+    int n, ZC, x0, y0, z0;
+    double ih;
+};
+
+// The plane march of one wave whose slot components are CODE: component,
+// column and window logic are compile-time, so a plane is straight-line code
+// (loads, FP64 z-updates) up to the uniform emit branch.
+// Element offsets are 32-bit: the host requires n^3 < 2^32.
+template <typename T, class G, int CODE>
+__device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__restrict__ v2, const T *__restrict__ v3,
+                                         const T *__restrict__ p, const FvTile &tl, double *lds, const FSet *fsl,
+                                         int tid) {
+    constexpr int TX = G::TX, TY = G::TY, H = G::hmax, NJ = G::NJ, NT = G::NT;
+    const int n = tl.n, ZC = tl.ZC, x0 = tl.x0, y0 = tl.y0, z0 = tl.z0;
+    const unsigned nn = (unsigned)n, n2 = nn * nn;
+    auto comp = [](int j) constexpr { return (CODE >> (2 * j)) & 3; };
+
+    // the thread's columns: tile-local position, offsets of the value and of
+    // the pressure neighbour along the component
+    int clx[NJ], cly[NJ];
     bool ok[NJ];
     unsigned co[NJ], cq[NJ];
 #pragma unroll
     for (int j = 0; j < NJ; j++) {
-        int e = tid + NT * j, a, lx = 0, ly = 0;
-        bool valid;
-        if (e < G::P0) {
-            a = 0, valid = e < G::N0, lx = e % (TX + H), ly = e / (TX + H);
-        } else if (e < G::P0 + G::P1) {
+        const int a = comp(j);
+        int e = tid + NT * j, lx = 0, ly = 0;
+        bool valid = false;
+        if (a == 0) {
+            valid = e < G::N0, lx = e % (TX + H), ly = e / (TX + H);
+        } else if (a == 1) {
             e -= G::P0;
-            a = 1, valid = e < G::N1, lx = e % TX, ly = e / TX;
-        } else {
+            valid = e < G::N1, lx = e % TX, ly = e / TX;
+        } else if (a == 2) {
             e -= G::P0 + G::P1;
-            a = 2, valid = e < G::N2, lx = e % TX, ly = e / TX;
+            valid = e < G::N2, lx = e % TX, ly = e / TX;
         }
         if (!valid) lx = ly = 0;  // padding: reads column (0, 0), writes nothing
-        ca[j] = __builtin_amdgcn_readfirstlane(a);
         ok[j] = valid;
         clx[j] = lx;
         cly[j] = ly;
@@ -429,7 +450,8 @@ __global__ void __launch_bounds__(G::NT) k_filter_vel(const T *__restrict__ v1, 
         const unsigned kk = (unsigned)wrap1(z0 + r, n) * n2, k1 = (unsigned)wrap1(z0 + r + 1, n) * n2;
 #pragma unroll
         for (int j = 0; j < NJ; j++) {
-            const int a = ca[j];
+            const int a = comp(j);
+            if (a == 3) continue;
             const T *vp = a == 0 ? v1 : (a == 1 ? v2 : v3);
             vv[j] = vp[co[j] + kk];
             if (p) {
@@ -445,17 +467,21 @@ __global__ void __launch_bounds__(G::NT) k_filter_vel(const T *__restrict__ v1, 
     for (int r = 0; r <= rend; ++r) {
         T nv[NJ], np[NJ], nq[NJ];
         load(r + 1, nv, np, nq);
-        double src[NJ];
 #pragma unroll
-        for (int j = 0; j < NJ; j++) src[j] = (double)cv[j] - ((double)cqv[j] - (double)cp[j]) * ih;
-        if constexpr (NEST) {
-#pragma unroll
-            for (int j = 0; j < NJ; j++) zupdate_nested<G>(r, ZC, ca[j], ok[j], clx[j], cly[j], src[j], nst[j], lds);
-        } else {
-            set_plane<G, 0>(r, ZC, ca, ok, clx, cly, src, acc, lds);
-            set_plane<G, 1>(r, ZC, ca, ok, clx, cly, src, acc, lds);
-            set_plane<G, 2>(r, ZC, ca, ok, clx, cly, src, acc, lds);
-            set_plane<G, 3>(r, ZC, ca, ok, clx, cly, src, acc, lds);
+        for (int j = 0; j < NJ; j++) {
+            const int a = comp(j);
+            if (a == 3) continue;
+            const double src = (double)cv[j] - ((double)cqv[j] - (double)cp[j]) * tl.ih;
+            if constexpr (NEST) {
+                if (a == 0) nested_box<G, 0>(r, ZC, ok[j], clx[j], cly[j], src, nst[j], lds);
+                else if (a == 1) nested_box<G, 1>(r, ZC, ok[j], clx[j], cly[j], src, nst[j], lds);
+                else nested_face<G>(r, ZC, ok[j], clx[j], cly[j], src, nst[j], lds);
+            } else {
+                if constexpr (G::c(0) > 0) zupdate<G, 0>(r, ZC, a, ok[j], clx[j], cly[j], src, acc[j][0], lds);
+                if constexpr (G::c(1) > 0) zupdate<G, 1>(r, ZC, a, ok[j], clx[j], cly[j], src, acc[j][1], lds);
+                if constexpr (G::c(2) > 0) zupdate<G, 2>(r, ZC, a, ok[j], clx[j], cly[j], src, acc[j][2], lds);
+                if constexpr (G::c(3) > 0) zupdate<G, 3>(r, ZC, a, ok[j], clx[j], cly[j], src, acc[j][3], lds);
+            }
         }
         const bool any = set_emits<G, 0>(r, ZC) || set_emits<G, 1>(r, ZC) || set_emits<G, 2>(r, ZC) ||
                          set_emits<G, 3>(r, ZC);
@@ -482,6 +508,54 @@ __global__ void __launch_bounds__(G::NT) k_filter_vel(const T *__restrict__ v1, 
             cqv[j] = nq[j];
         }
     }
+}
+
+// one instantiation of the march per distinct slot-component code
+template <typename T, class G, int W>
+__device__ __forceinline__ void fv_dispatch(int code, const T *__restrict__ v1, const T *__restrict__ v2,
+                                            const T *__restrict__ v3, const T *__restrict__ p, const FvTile &tl,
+                                            double *lds, const FSet *fsl, int tid) {
+    if constexpr (W < G::NT / 64) {
+        if constexpr (first_code<G>(W)) {
+            if (code == wave_code<G>(W)) {
+                fv_march<T, G, wave_code<G>(W)>(v1, v2, v3, p, tl, lds, fsl, tid);
+                return;
+            }
+        }
+        fv_dispatch<T, G, W + 1>(code, v1, v2, v3, p, tl, lds, fsl, tid);
+    }
+}
+
+template <typename T, class G>
+__global__ void __launch_bounds__(G::NT) k_filter_vel(const T *__restrict__ v1, const T *__restrict__ v2,
+                                                     const T *__restrict__ v3, const T *__restrict__ p, int n, int ZC,
+                                                     double ih, FArgs A) {
+    constexpr int TX = G::TX, TY = G::TY;
+    extern __shared__ double lds[];
+    // output table in LDS: read only when a coarse plane closes, so it holds
+    // no scalar registers through the plane loop
+    FSet *fsl = reinterpret_cast<FSet *>(lds + G::lds_doubles);
+    if (threadIdx.x < 4) fsl[threadIdx.x] = A.s[threadIdx.x];
+    // 1D grid of (n/TX) x (n/TY) x (n/ZC) tiles, x fastest, in XCD order
+    const int gx = n / TX, gy = n / TY;
+    const int L = (int)ec_xcd_tile(blockIdx.x, gridDim.x);
+    FvTile tl;
+    tl.n = n;
+    tl.ZC = ZC;
+    tl.ih = ih;
+    tl.x0 = (L % gx) * TX;
+    tl.y0 = ((L / gx) % gy) * TY;
+    tl.z0 = (L / (gx * gy)) * ZC;
+    const int tid = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    int code = 0;
+#pragma unroll
+    for (int j = 0; j < G::NJ; j++) {
+        const int e = w * 64 + j * G::NT;
+        const int c = e < G::P0 ? 0 : (e < G::P0 + G::P1 ? 1 : (e < G::NCC ? 2 : 3));
+        code |= c << (2 * j);
+    }
+    fv_dispatch<T, G, 0>(code, v1, v2, v3, p, tl, lds, fsl, tid);
 }
 
 int largest_tile(int n, int L, int target) {
