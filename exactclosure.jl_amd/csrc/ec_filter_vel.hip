@@ -251,6 +251,7 @@ constexpr bool nested_list() {
 
 struct NestState {
     double b4, s1, s2, s3;  // components 0, 1: b4, box8, box16; component 2: b4, face4, face8, face16
+    double hd;              // component 2: the last closed compression-4 trapezoid, written two planes later
 };
 
 template <class G, int S, int A>
@@ -292,15 +293,18 @@ __device__ __forceinline__ void nested_face(int r, int ZC, bool ok, int lx, int 
                                             double *lds) {
     const int ph = r & 3, m = r >> 2;
     st.b4 = (ph == 0 ? 0.0 : st.b4) + src;
-    // compression-4 trapezoid: windows start at planes 4K + 1, where (r + 3) % 4 == 0
+    // compression-4 trapezoid: windows start at planes 4K + 1, where (r + 3) % 4 == 0;
+    // the closed sum waits in hd for the next phase-3 plane, so that every
+    // set emits on the phase-3 planes only (one barrier pair per 4 planes)
     if (((r + 3) & 3) == 0) {
-        if (r > 1 && r - 1 <= ZC && ok) zput<G, 2, 2>(lds, lx, ly, (st.s1 + 0.5 * src) * 0.25);
+        st.hd = (st.s1 + 0.5 * src) * 0.25;
         st.s1 = 0.5 * src;
     } else {
         st.s1 += src;
     }
     if (ph == 3) {
         const double B = st.b4, hx = 0.5 * src;
+        if (r > 3 && r - 3 <= ZC && ok) zput<G, 2, 2>(lds, lx, ly, st.hd);  // closed at plane r - 2
         if (r < ZC && ok) zput<G, 3, 2>(lds, lx, ly, src);
         if ((m & 1) == 0) {
             if (m > 0 && m <= ZC / 4 && ok) zput<G, 1, 2>(lds, lx, ly, (st.s2 + B - hx) * 0.125);
@@ -325,11 +329,18 @@ __device__ __forceinline__ void zupdate_nested(int r, int ZC, int a, bool ok, in
     else if (a == 2) nested_face<G>(r, ZC, ok, lx, ly, src, st, lds);
 }
 
+// plane whose face window is emitted at plane r: the compression-4 trapezoid of
+// the nested list is written two planes after it closes (see nested_face)
+template <class G, int S>
+__device__ __forceinline__ int face_plane(int r) {
+    return nested_list<G>() && S == 2 ? r - 2 : r;
+}
+
 template <class G, int S>
 __device__ __forceinline__ bool set_emits(int r, int ZC) {
     if constexpr (G::c(S) > 0) {
-        const ZWin<G::c(S)> w(r, ZC);
-        return w.box_emit || (G::kind(S) == 0 && w.face_emit);
+        const ZWin<G::c(S)> w(r, ZC), wf(face_plane<G, S>(r), ZC);
+        return w.box_emit || (G::kind(S) == 0 && wf.face_emit);
     } else {
         return false;
     }
@@ -338,14 +349,14 @@ __device__ __forceinline__ bool set_emits(int r, int ZC) {
 template <class G, int S>
 __device__ __forceinline__ void set_xstage(int r, int ZC, double *lds, int tid) {
     if constexpr (G::c(S) > 0) {
-        const ZWin<G::c(S)> w(r, ZC);
+        const ZWin<G::c(S)> w(r, ZC), wf(face_plane<G, S>(r), ZC);
         if (w.box_emit) {
             xstage<G, S, 0>(lds, tid);
             xstage<G, S, 1>(lds, tid);
             if constexpr (G::kind(S) == 1) xstage<G, S, 2>(lds, tid);
         }
         if constexpr (G::kind(S) == 0)
-            if (w.face_emit) xstage<G, S, 2>(lds, tid);
+            if (wf.face_emit) xstage<G, S, 2>(lds, tid);
     }
 }
 
@@ -354,14 +365,14 @@ __device__ __forceinline__ void set_ystage(int r, int ZC, const double *lds, int
                                            const FSet *fs) {
     if constexpr (G::c(S) > 0) {
         constexpr int C = G::c(S);
-        const ZWin<C> w(r, ZC);
+        const ZWin<C> w(r, ZC), wf(face_plane<G, S>(r), ZC);
         if (w.box_emit) {
             ystage<G, S, 0>(lds, tid, x0, y0, z0 / C + w.kbox, fs);
             ystage<G, S, 1>(lds, tid, x0, y0, z0 / C + w.kbox, fs);
             if constexpr (G::kind(S) == 1) ystage<G, S, 2>(lds, tid, x0, y0, z0 / C + w.kbox, fs);
         }
         if constexpr (G::kind(S) == 0)
-            if (w.face_emit) ystage<G, S, 2>(lds, tid, x0, y0, z0 / C + w.kface, fs);
+            if (wf.face_emit) ystage<G, S, 2>(lds, tid, x0, y0, z0 / C + wf.kface, fs);
     }
 }
 
@@ -438,7 +449,7 @@ __device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__re
     NestState nst[NJ];
 #pragma unroll
     for (int j = 0; j < NJ; j++) {
-        nst[j] = NestState{0.0, 0.0, 0.0, 0.0};
+        nst[j] = NestState{0.0, 0.0, 0.0, 0.0, 0.0};
         if constexpr (!NEST)
 #pragma unroll
             for (int s = 0; s < 4; s++) acc[j][s] = 0.0;
