@@ -377,8 +377,18 @@ constexpr int ct_radix(int m, int maxr = 16) {
 __device__ __forceinline__ constexpr int pad8(int e) { return e + (e >> EC_FFT_PAD_SHIFT); }
 constexpr int padded_len(int n) { return n + (n >> EC_FFT_PAD_SHIFT); }
 
+// Ordering between the stages of a line: a block barrier, or (WAVE) when one
+// wave owns the line, only the completion of its own LDS operations (a wave's
+// LDS operations execute in order; the clobber keeps the compiler from moving
+// LDS accesses across).
+template <bool WAVE>
+__device__ __forceinline__ void line_sync() {
+    if constexpr (WAVE) asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    else __syncthreads();
+}
+
 // MAXR caps the radix (a 1024-thread block keeps radix 8 to stay within 128 VGPRs)
-template <typename T, int N, int NS, int NTHR, int LS, int SIGN, int MAXR = 16>
+template <typename T, int N, int NS, int NTHR, int LS, int SIGN, int MAXR = 16, bool WAVE = false>
 __device__ __forceinline__ void ct_stages(cplx<T> *lds, int t, const cplx<T> *__restrict__ tw) {
     if constexpr (NS < N) {
         constexpr int R = ct_radix(N / NS, MAXR);
@@ -404,7 +414,7 @@ __device__ __forceinline__ void ct_stages(cplx<T> *lds, int t, const cplx<T> *__
                 Bfly<T, R>::run(v[q], SIGN);
             }
         }
-        __syncthreads();
+        line_sync<WAVE>();
 #pragma unroll
         for (int q = 0; q < B; q++) {
             const int b = t + q * NTHR;
@@ -415,8 +425,8 @@ __device__ __forceinline__ void ct_stages(cplx<T> *lds, int t, const cplx<T> *__
                 for (int r = 0; r < R; r++) lds[pad8(idxD + r * NS) * LS] = v[q][r];
             }
         }
-        __syncthreads();
-        ct_stages<T, N, NS * R, NTHR, LS, SIGN, MAXR>(lds, t, tw);
+        line_sync<WAVE>();
+        ct_stages<T, N, NS * R, NTHR, LS, SIGN, MAXR, WAVE>(lds, t, tw);
     }
 }
 
@@ -687,6 +697,54 @@ __global__ void __launch_bounds__(TK *NTHR) k_fft_cols_ct(cplx<T> *__restrict__ 
     }
 }
 
+// Plain (non-Poisson) column transforms of large FP32 grids with one wave per
+// column: the tile of TK columns is staged column-major (column stride CS,
+// CS = 64 / TK mod 32, so the row-wise transposing writes share banks at most
+// two ways), and the Stockham stages of a column order only through its own
+// wave.  Block barriers remain only after the tile load and before the tile
+// store.  (The fused Poisson pass keeps k_fft_cols_ct: with two transforms per
+// residency this layout needs more than 128 VGPRs and loses a block per CU.)
+constexpr int col_stride(int n, int tk) {
+    const int pl = padded_len(n), want = (64 / tk) % 32;
+    return pl + ((want - pl % 32) % 32 + 32) % 32;
+}
+
+template <typename T, int N, int TK, int SIGN>
+__global__ void __launch_bounds__(TK * 64) k_fft_cols_w(cplx<T> *__restrict__ data, int ncols, int ntiles,
+                                                       long elem_stride, long outer_stride,
+                                                       const cplx<T> *__restrict__ tw) {
+    constexpr int CS = col_stride(N, TK), TOTAL = N * TK, PER = TOTAL / (TK * 64);
+    extern __shared__ char smem_raw[];
+    cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
+    const int L = (int)ec_xcd_tile(blockIdx.x, gridDim.x);
+    const int col0 = (L % ntiles) * TK;
+    const int outer = L / ntiles;
+    const int c = threadIdx.x / 64, t = threadIdx.x % 64;
+    cplx<T> *base = data + col0 + (long)outer * outer_stride;
+    const bool full = col0 + TK <= ncols;
+    {
+        cplx<T> tmp[PER];
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int idx = threadIdx.x + q * TK * 64, cc = idx % TK, e = idx / TK;
+            tmp[q] = (full || col0 + cc < ncols) ? base[cc + e * elem_stride] : cplx<T>{T(0), T(0)};
+        }
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int idx = threadIdx.x + q * TK * 64, cc = idx % TK, e = idx / TK;
+            lds[cc * CS + pad8(e)] = tmp[q];
+        }
+    }
+    __syncthreads();
+    ct_stages<T, N, 1, 64, 1, SIGN, 16, true>(lds + c * CS, t, tw);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int idx = threadIdx.x + q * TK * 64, cc = idx % TK, e = idx / TK;
+        if (full || col0 + cc < ncols) base[cc + e * elem_stride] = lds[cc * CS + pad8(e)];
+    }
+}
+
 // groups per block of the x passes: enough blocks to fill the device several times
 inline int x_groups(long npairs, int W) {
     if (npairs / (16L * W) >= 8192) return 16;
@@ -742,8 +800,25 @@ int c2r_x_ct(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out, double scal
     }
 }
 
+template <typename T, int N, int SIGN>
+int cols_w(ec_ctx *ctx, ec_poisson *ps, void *spec, int axis, int pitch) {
+    constexpr int TK = 16;
+    const int nh = ps->nh;
+    const long elem_stride = axis == 1 ? (long)pitch : (long)pitch * N;
+    const long outer_stride = axis == 1 ? (long)pitch * N : (long)pitch;
+    const int ntiles = (nh + TK - 1) / TK;
+    const size_t lds = (size_t)col_stride(N, TK) * TK * sizeof(cplx<T>);
+    EC_HIP(ctx, hipFuncSetAttribute((const void *)k_fft_cols_w<T, N, TK, SIGN>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    k_fft_cols_w<T, N, TK, SIGN><<<(unsigned)(ntiles * N), TK * 64, lds, ctx->stream>>>(
+        (cplx<T> *)spec, nh, ntiles, elem_stride, outer_stride, (const cplx<T> *)ps->twiddle);
+    EC_CHECK_LAUNCH(ctx, "k_fft_cols_w");
+    return EC_OK;
+}
+
 template <typename T, int N, int MODE, int SIGN>
 int cols_ct(ec_ctx *ctx, ec_poisson *ps, void *spec, int axis, int pitch) {
+    if constexpr (sizeof(T) == 4 && N >= 512 && MODE == 0) return cols_w<T, N, SIGN>(ctx, ps, spec, axis, pitch);
     constexpr int NTHR = N / 16;
     // one 128-byte line of complex values per row; the fused Poisson pass (two
     // transforms) takes 8 columns so that 512-thread blocks can use radix 16
