@@ -320,6 +320,39 @@ def test_rk4_update(gpu, O, stage):
             assert rel(host(T[0][c]), u0[c] + dt / 6 * acc_new) < 1e-13
 
 
+@pytest.mark.parametrize("stage", [0, 1, 2, 3])
+def test_rk4_update_large_fp32(gpu, stage):
+    """The 16-byte vector FP32 update at n = 512 (periodic x/y/z neighbours of p
+    across vector, row and plane ends) against an FP64 torch evaluation of the
+    same stage on the same FP32 inputs: acc += b k, us = u0 + a dt k, k = du - grad p."""
+    n, dt, l = 512, 0.01, 1.0
+    ih = n / l
+    g = torch.Generator(device="cuda").manual_seed(100 + stage)
+    f = lambda: torch.randn((n, n, n), device="cuda", generator=g)  # noqa: E731
+    u0, us, du, acc = [[f() for _ in range(3)] for _ in range(4)]
+    p = f()
+    ref_u0 = [x.double() for x in u0]
+    ref_acc = [x.double() for x in acc]
+    pd = p.double()
+    gp = [(torch.roll(pd, -1, dims=2 - a) - pd) * ih for a in range(3)]  # [k, j, i]: x is dim 2
+    gpu._abi.context().call("ec_ns_rk4_update", 0, n, l, stage, dt, P(u0, 3), P(us, 3), P(du, 3), p.data_ptr(),
+                            P(acc, 3))
+    b = (1, 2, 2, 1)[stage]
+    for c in range(3):
+        k = du[c].double() - gp[c]
+        acc_new = k if stage == 0 else ref_acc[c] + b * k
+        if stage < 3:
+            a = (0.5, 0.5, 1.0)[stage]
+            assert float((acc[c].double() - acc_new).abs().max()) < 1e-6 * float(acc_new.abs().max())
+            want = ref_u0[c] + a * dt * k
+            assert float((us[c].double() - want).abs().max()) < 1e-6 * float(want.abs().max())
+        else:
+            want = ref_u0[c] + dt / 6 * acc_new
+            assert float((u0[c].double() - want).abs().max()) < 1e-6 * float(want.abs().max())
+        del k, acc_new
+    torch.cuda.empty_cache()
+
+
 def test_invalid_arguments_raise(gpu):
     NS = gpu.navierstokes
     ctx = gpu._abi.context()
