@@ -565,8 +565,8 @@ __global__ void __launch_bounds__(xblock(NTHR)) k_fft_x_r2c_ct(const T *__restri
             const cplx<T> zc = line[pad8((N - k) & (N - 1))];
             const cplx<T> A{(z.x + zc.x) / 2, (z.y - zc.y) / 2};
             const cplx<T> Bv{(z.y + zc.y) / 2, (zc.x - z.x) / 2};
-            if (v0) oa[k] = A;
-            if (v1) ob[k] = Bv;
+            if (v0) ec_nt_store(oa + k, A);
+            if (v1) ec_nt_store(ob + k, Bv);
         }
         __syncthreads();
     }
@@ -629,8 +629,8 @@ __global__ void __launch_bounds__(xblock(NTHR)) k_fft_x_c2r_ct(const cplx<T> *__
         for (int q = 0; q < PER; q++) {
             const int e = t + q * NTHR;
             const cplx<T> z = line[pad8(e)];
-            if (v0) oa[e] = z.x * scale;
-            if (v1) ob[e] = z.y * scale;
+            if (v0) ec_nt_store(oa + e, z.x * scale);
+            if (v1) ec_nt_store(ob + e, z.y * scale);
         }
         __syncthreads();
     }
@@ -693,7 +693,7 @@ __global__ void __launch_bounds__(TK *NTHR) k_fft_cols_ct(cplx<T> *__restrict__ 
 #pragma unroll
     for (int idx = threadIdx.x; idx < TOTAL; idx += TK * NTHR) {
         const int cc = idx % TK, e = idx / TK;
-        if (full || col0 + cc < ncols) base[cc + e * elem_stride] = lds[pad8(e) * TK + cc];
+        if (full || col0 + cc < ncols) ec_nt_store(base + cc + e * elem_stride, lds[pad8(e) * TK + cc]);
     }
 }
 
@@ -741,7 +741,7 @@ __global__ void __launch_bounds__(TK * 64) k_fft_cols_w(cplx<T> *__restrict__ da
 #pragma unroll
     for (int q = 0; q < PER; q++) {
         const int idx = threadIdx.x + q * TK * 64, cc = idx % TK, e = idx / TK;
-        if (full || col0 + cc < ncols) base[cc + e * elem_stride] = lds[cc * CS + pad8(e)];
+        if (full || col0 + cc < ncols) ec_nt_store(base + cc + e * elem_stride, lds[cc * CS + pad8(e)]);
     }
 }
 

@@ -78,6 +78,27 @@ struct cplx {
     T x, y;
 };
 
+// Streaming (non-temporal) stores for outputs a kernel writes once and does
+// not read back: they do not displace the lines its loads share in L2.
+template <typename T>
+__device__ __forceinline__ void ec_nt_store(T *p, T v) {
+    __builtin_nontemporal_store(v, p);
+}
+template <typename VT>
+__device__ __forceinline__ void ec_nt_store16(VT *p, const VT &v) {  // a 16-byte vector (float4, double2)
+    static_assert(sizeof(VT) == 16, "16-byte vector");
+    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    u4 w;
+    __builtin_memcpy(&w, &v, 16);
+    __builtin_nontemporal_store(w, reinterpret_cast<u4 *>(p));
+}
+template <typename T>
+__device__ __forceinline__ void ec_nt_store(cplx<T> *p, cplx<T> v) {
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    const v2 w = {v.x, v.y};
+    __builtin_nontemporal_store(w, reinterpret_cast<v2 *>(p));
+}
+
 template <typename T>
 __device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
     return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};

@@ -382,9 +382,11 @@ __global__ void __launch_bounds__(256) k_rhs_div_march(int n, int ZC, T ih, T vi
             for (int m = 0; m < CPT; m++) {
                 const int cy = cy0 + m * (256 / TX);
                 const size_t I = (size_t)(x0 + cx) + (size_t)nn * ((size_t)(y0 + cy) + (size_t)nn * (size_t)k);
-                du1[I] = a1[m];
-                du2[I] = a2[m];
-                du3[I] = a3[m];
+                // streaming stores: the outputs are not re-read here, and keeping
+                // them out of L2 leaves it to the halo lines the neighbouring tiles share
+                __builtin_nontemporal_store(a1[m], du1 + I);
+                __builtin_nontemporal_store(a2[m], du2 + I);
+                __builtin_nontemporal_store(a3[m], du3 + I);
                 dx1[buf][cy * (TX + 1) + cx + 1] = a1[m];
                 dx2[buf][(cy + 1) * TX + cx] = a2[m];
             }
@@ -407,8 +409,9 @@ __global__ void __launch_bounds__(256) k_rhs_div_march(int n, int ZC, T ih, T vi
             for (int m = 0; m < CPT; m++) {
                 const int cy = cy0 + m * (256 / TX);
                 const size_t I = (size_t)(x0 + cx) + (size_t)nn * ((size_t)(y0 + cy) + (size_t)nn * (size_t)k);
-                dout[I] = (a1[m] - dx1[buf][cy * (TX + 1) + cx]) * ih + (a2[m] - dx2[buf][cy * TX + cx]) * ih +
-                          (a3[m] - a3prev[m]) * ih;
+                __builtin_nontemporal_store((a1[m] - dx1[buf][cy * (TX + 1) + cx]) * ih +
+                                                (a2[m] - dx2[buf][cy * TX + cx]) * ih + (a3[m] - a3prev[m]) * ih,
+                                            dout + I);
             }
         }
 #pragma unroll
@@ -690,17 +693,21 @@ __global__ void __launch_bounds__(256) k_rk4_update_vec(int n, T ih, T dt, Ptr3 
         gp[1][e] = (pys[e] - p0s[e]) * ih;
         gp[2][e] = (pzs[e] - p0s[e]) * ih;
     }
+    // all loads first: the three components' fields may alias as far as the
+    // compiler knows, so loads placed after a store would wait for it
+    VT d[3], uv[3], av[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        VT d = *(const VT *)((const T *)dup.p[a] + I);
-        T *ds = (T *)&d;
+        d[a] = *(const VT *)((const T *)dup.p[a] + I);
+        uv[a] = *(const VT *)((const T *)u0p.p[a] + I);
+        if (STAGE != 0) av[a] = *(const VT *)((const T *)accp.p[a] + I);
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const T *ds = (const T *)&d[a], *us_ = (const T *)&uv[a], *as = (const T *)&av[a];
         T kk[V];
 #pragma unroll
         for (int e = 0; e < V; e++) kk[e] = ds[e] - gp[a][e];
-        T *u0 = (T *)u0p.p[a] + I;
-        T *acc = (T *)accp.p[a] + I;
-        VT uv = *(const VT *)u0;
-        const T *us_ = (const T *)&uv;
         VT o1, o2;
         T *r1 = (T *)&o1, *r2 = (T *)&o2;
         if (STAGE == 0) {
@@ -709,24 +716,22 @@ __global__ void __launch_bounds__(256) k_rk4_update_vec(int n, T ih, T dt, Ptr3 
                 r1[e] = kk[e];
                 r2[e] = us_[e] + (dt / 2) * kk[e];
             }
-            *(VT *)acc = o1;
-            *(VT *)((T *)usp.p[a] + I) = o2;
         } else if (STAGE == 1 || STAGE == 2) {
-            VT av = *(const VT *)acc;
-            const T *as = (const T *)&av;
 #pragma unroll
             for (int e = 0; e < V; e++) {
                 r1[e] = as[e] + 2 * kk[e];
                 r2[e] = us_[e] + (STAGE == 1 ? dt / 2 : dt) * kk[e];
             }
-            *(VT *)acc = o1;
-            *(VT *)((T *)usp.p[a] + I) = o2;
         } else {
-            VT av = *(const VT *)acc;
-            const T *as = (const T *)&av;
 #pragma unroll
             for (int e = 0; e < V; e++) r1[e] = us_[e] + (dt / 6) * (as[e] + kk[e]);
-            *(VT *)u0 = o1;
+        }
+        // streaming stores: nothing here reads them back
+        if (STAGE < 3) {
+            ec_nt_store16((VT *)((T *)accp.p[a] + I), o1);
+            ec_nt_store16((VT *)((T *)usp.p[a] + I), o2);
+        } else {
+            ec_nt_store16((VT *)((T *)u0p.p[a] + I), o1);
         }
     }
 }
