@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <numeric>
+#include <type_traits>
 
 namespace {
 
@@ -267,13 +268,13 @@ __device__ __forceinline__ bool zin(int lx, int ly) {
     else return true;
 }
 
-// components 0 and 1: box along z for every set
-template <class G, int A>
+// components 0 and 1: box along z for every set (PH = r % 4, known at compile time)
+template <class G, int A, int PH>
 __device__ __forceinline__ void nested_box(int r, int ZC, bool ok, int lx, int ly, double src, NestState &st,
                                            double *lds) {
-    const int ph = r & 3, m = r >> 2;
-    st.b4 = (ph == 0 ? 0.0 : st.b4) + src;
-    if (ph == 3) {
+    const int m = r >> 2;
+    st.b4 = PH == 0 ? src : st.b4 + src;
+    if constexpr (PH == 3) {
         const double B = st.b4;
         st.s1 = ((m & 1) == 0 ? 0.0 : st.s1) + B;
         st.s2 = ((m & 3) == 0 ? 0.0 : st.s2) + B;
@@ -288,21 +289,21 @@ __device__ __forceinline__ void nested_box(int r, int ZC, bool ok, int lx, int l
 
 // component 2: face average point, compression-4 trapezoid on its own sum,
 // compression-8 and -16 trapezoids from the 4-plane boxes
-template <class G>
+template <class G, int PH>
 __device__ __forceinline__ void nested_face(int r, int ZC, bool ok, int lx, int ly, double src, NestState &st,
                                             double *lds) {
-    const int ph = r & 3, m = r >> 2;
-    st.b4 = (ph == 0 ? 0.0 : st.b4) + src;
-    // compression-4 trapezoid: windows start at planes 4K + 1, where (r + 3) % 4 == 0;
-    // the closed sum waits in hd for the next phase-3 plane, so that every
-    // set emits on the phase-3 planes only (one barrier pair per 4 planes)
-    if (((r + 3) & 3) == 0) {
+    const int m = r >> 2;
+    st.b4 = PH == 0 ? src : st.b4 + src;
+    // compression-4 trapezoid: windows start at planes 4K + 1 (phase 1); the
+    // closed sum waits in hd for the next phase-3 plane, so that every set
+    // emits on the phase-3 planes only (one barrier pair per 4 planes)
+    if constexpr (PH == 1) {
         st.hd = (st.s1 + 0.5 * src) * 0.25;
         st.s1 = 0.5 * src;
     } else {
         st.s1 += src;
     }
-    if (ph == 3) {
+    if constexpr (PH == 3) {
         const double B = st.b4, hx = 0.5 * src;
         if (r > 3 && r - 3 <= ZC && ok) zput<G, 2, 2>(lds, lx, ly, st.hd);  // closed at plane r - 2
         if (r < ZC && ok) zput<G, 3, 2>(lds, lx, ly, src);
@@ -319,14 +320,6 @@ __device__ __forceinline__ void nested_face(int r, int ZC, bool ok, int lx, int 
             st.s3 += B;
         }
     }
-}
-
-template <class G>
-__device__ __forceinline__ void zupdate_nested(int r, int ZC, int a, bool ok, int lx, int ly, double src,
-                                               NestState &st, double *lds) {
-    if (a == 0) nested_box<G, 0>(r, ZC, ok, lx, ly, src, st, lds);
-    else if (a == 1) nested_box<G, 1>(r, ZC, ok, lx, ly, src, st, lds);
-    else if (a == 2) nested_face<G>(r, ZC, ok, lx, ly, src, st, lds);
 }
 
 // plane whose face window is emitted at plane r: the compression-4 trapezoid of
@@ -475,25 +468,8 @@ __device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__re
     };
     load(0, cv, cp, cqv);
     const int rend = ZC - 1 + H;
-    for (int r = 0; r <= rend; ++r) {
-        T nv[NJ], np[NJ], nq[NJ];
-        load(r + 1, nv, np, nq);
-#pragma unroll
-        for (int j = 0; j < NJ; j++) {
-            const int a = comp(j);
-            if (a == 3) continue;
-            const double src = (double)cv[j] - ((double)cqv[j] - (double)cp[j]) * tl.ih;
-            if constexpr (NEST) {
-                if (a == 0) nested_box<G, 0>(r, ZC, ok[j], clx[j], cly[j], src, nst[j], lds);
-                else if (a == 1) nested_box<G, 1>(r, ZC, ok[j], clx[j], cly[j], src, nst[j], lds);
-                else nested_face<G>(r, ZC, ok[j], clx[j], cly[j], src, nst[j], lds);
-            } else {
-                if constexpr (G::c(0) > 0) zupdate<G, 0>(r, ZC, a, ok[j], clx[j], cly[j], src, acc[j][0], lds);
-                if constexpr (G::c(1) > 0) zupdate<G, 1>(r, ZC, a, ok[j], clx[j], cly[j], src, acc[j][1], lds);
-                if constexpr (G::c(2) > 0) zupdate<G, 2>(r, ZC, a, ok[j], clx[j], cly[j], src, acc[j][2], lds);
-                if constexpr (G::c(3) > 0) zupdate<G, 3>(r, ZC, a, ok[j], clx[j], cly[j], src, acc[j][3], lds);
-            }
-        }
+    // emits of every set at plane r (all on phase-3 planes for the nested list)
+    auto emit = [&](int r) {
         const bool any = set_emits<G, 0>(r, ZC) || set_emits<G, 1>(r, ZC) || set_emits<G, 2>(r, ZC) ||
                          set_emits<G, 3>(r, ZC);
         if (any) {
@@ -512,11 +488,59 @@ __device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__re
             set_ystage<G, 2>(r, ZC, lds, t, x0, y0, z0, fsl);
             set_ystage<G, 3>(r, ZC, lds, t, x0, y0, z0, fsl);
         }
+    };
+    auto rotate = [&](const T *nv, const T *np, const T *nq) {
 #pragma unroll
         for (int j = 0; j < NJ; j++) {
             cv[j] = nv[j];
             cp[j] = np[j];
             cqv[j] = nq[j];
+        }
+    };
+    if constexpr (NEST) {
+        // four planes per iteration (ZC + H is a multiple of 4): the window
+        // phase of each is a compile-time constant, and only the phase-3 plane
+        // can emit
+        static_assert(G::hmax % 4 == 0, "nested march: whole 4-plane groups");
+        for (int r0 = 0; r0 <= rend; r0 += 4) {
+            auto plane = [&](auto qc) {
+                constexpr int Q = decltype(qc)::value;
+                const int r = r0 + Q;
+                T nv[NJ], np[NJ], nq[NJ];
+                load(r + 1, nv, np, nq);
+#pragma unroll
+                for (int j = 0; j < NJ; j++) {
+                    const int a = comp(j);
+                    if (a == 3) continue;
+                    const double src = (double)cv[j] - ((double)cqv[j] - (double)cp[j]) * tl.ih;
+                    if (a == 0) nested_box<G, 0, Q>(r, ZC, ok[j], clx[j], cly[j], src, nst[j], lds);
+                    else if (a == 1) nested_box<G, 1, Q>(r, ZC, ok[j], clx[j], cly[j], src, nst[j], lds);
+                    else nested_face<G, Q>(r, ZC, ok[j], clx[j], cly[j], src, nst[j], lds);
+                }
+                if constexpr (Q == 3) emit(r);
+                rotate(nv, np, nq);
+            };
+            plane(std::integral_constant<int, 0>{});
+            plane(std::integral_constant<int, 1>{});
+            plane(std::integral_constant<int, 2>{});
+            plane(std::integral_constant<int, 3>{});
+        }
+    } else {
+        for (int r = 0; r <= rend; ++r) {
+            T nv[NJ], np[NJ], nq[NJ];
+            load(r + 1, nv, np, nq);
+#pragma unroll
+            for (int j = 0; j < NJ; j++) {
+                const int a = comp(j);
+                if (a == 3) continue;
+                const double src = (double)cv[j] - ((double)cqv[j] - (double)cp[j]) * tl.ih;
+                if constexpr (G::c(0) > 0) zupdate<G, 0>(r, ZC, a, ok[j], clx[j], cly[j], src, acc[j][0], lds);
+                if constexpr (G::c(1) > 0) zupdate<G, 1>(r, ZC, a, ok[j], clx[j], cly[j], src, acc[j][1], lds);
+                if constexpr (G::c(2) > 0) zupdate<G, 2>(r, ZC, a, ok[j], clx[j], cly[j], src, acc[j][2], lds);
+                if constexpr (G::c(3) > 0) zupdate<G, 3>(r, ZC, a, ok[j], clx[j], cly[j], src, acc[j][3], lds);
+            }
+            emit(r);
+            rotate(nv, np, nq);
         }
     }
 }
