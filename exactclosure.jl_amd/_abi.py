@@ -64,6 +64,7 @@ _SIGS = {
     "ec_axpy": (c_int, [c_void_p, c_int, c_size_t, c_double, c_void_p, c_void_p]),
     "ec_convert": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_size_t]),
     "ec_axpy2": (c_int, [c_void_p, c_int, c_size_t, c_double, c_int, c_void_p, c_void_p]),
+    "ec_axpby": (c_int, [c_void_p, c_int, c_size_t, c_double, c_void_p, c_double, c_void_p]),
     "ec_profile_begin": (c_int, [c_void_p]),
     "ec_profile_end": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(c_double),
                               ctypes.POINTER(c_double), c_int, ctypes.POINTER(c_int)]),

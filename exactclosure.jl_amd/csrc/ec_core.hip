@@ -134,6 +134,25 @@ extern "C" int ec_axpy(ec_ctx *ctx, int dtype, size_t count, double a, const voi
     return EC_OK;
 }
 
+template <typename T>
+__global__ void k_axpby(size_t count, T a, const T *__restrict__ x, T b, T *__restrict__ y) {
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < count; i += stride) y[i] = a * x[i] + b * y[i];
+}
+
+extern "C" int ec_axpby(ec_ctx *ctx, int dtype, size_t count, double a, const void *x, double b, void *y) {
+    EC_REQUIRE(ctx, ctx && (count == 0 || (x && y)), "ec_axpby: null argument");
+    if (count == 0) return EC_OK;
+    const int blocks = (int)std::min<size_t>((count + 255) / 256, 1 << 20);
+    if (dtype == EC_F64)
+        k_axpby<double><<<blocks, 256, 0, ctx->stream>>>(count, a, (const double *)x, b, (double *)y);
+    else
+        k_axpby<float><<<blocks, 256, 0, ctx->stream>>>(count, (float)a, (const float *)x, (float)b, (float *)y);
+    EC_CHECK_LAUNCH(ctx, "k_axpby");
+    return EC_OK;
+}
+
 template <typename Y, typename X>
 __global__ void k_axpy2(size_t count, double a, const X *__restrict__ x, Y *__restrict__ y) {
     size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;

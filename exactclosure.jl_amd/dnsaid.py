@@ -52,7 +52,6 @@ class LESGrid:
             self.vacc = _vec(nbar, f64)
             self.g_k = _vec(nbar, f64)   # V k of the current stage
             self.ubar = _vec(nbar, f64)  # P_H V u
-            self.fh = _vec(nbar, f64)    # F_H(ubar)
             self.m = _vec(nbar, f64)     # closure forcing V k - F_H(ubar)
             self.zero64 = torch.zeros((nbar,) * 3, dtype=f64, device="cuda")
         self.ps = NS.PoissonSolver(self.g, dtype)
@@ -189,10 +188,10 @@ class DNSAidedLES:
         for a in range(3):
             L.ubar[a].copy_(v[a])
         c.call("ec_ns_project", L.ps64.h, ptrs(L.ubar, P3))
-        c.call("ec_ns_rhs", _abi.EC_F64, gl.n, gl.l, self.visc, ptrs(L.ubar, P3), ptrs(L.fh, P3), 0)
+        # m = V k - F_H(P_H v): F_H into m, then m = g_k - m in one pass
+        c.call("ec_ns_rhs", _abi.EC_F64, gl.n, gl.l, self.visc, ptrs(L.ubar, P3), ptrs(L.m, P3), 0)
         for a in range(3):
-            L.m[a].copy_(L.g_k[a])
-            c.call("ec_axpy", _abi.EC_F64, L.m[a].numel(), -1.0, ptr(L.fh[a]), ptr(L.m[a]))
+            c.call("ec_axpby", _abi.EC_F64, L.m[a].numel(), 1.0, ptr(L.g_k[a]), -1.0, ptr(L.m[a]))
         return L.m
 
     def _step_forcing(self, dt):

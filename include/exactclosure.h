@@ -66,6 +66,9 @@ int ec_zero(ec_ctx *ctx, void *dst, size_t bytes);
 int ec_axpy(ec_ctx *ctx, int dtype, size_t count, double a, const void *x, void *y);
 /* y = a*x + y with x and y of different dtypes (FP64 closure terms into FP32 fields) */
 int ec_axpy2(ec_ctx *ctx, int ydtype, size_t count, double a, int xdtype, const void *x, void *y);
+/* element-wise y = a*x + b*y in place (the closure forcing m = V k - F_H(P_H v)
+ * of src/NavierStokes.jl's DNS-aided step as one pass over x and y) */
+int ec_axpby(ec_ctx *ctx, int dtype, size_t count, double a, const void *x, double b, void *y);
 /* dtype conversion copy (FP64 <-> FP32) */
 int ec_convert(ec_ctx *ctx, int dst_dtype, void *dst, int src_dtype, const void *src, size_t count);
 

@@ -353,6 +353,19 @@ def test_rk4_update_large_fp32(gpu, stage):
     torch.cuda.empty_cache()
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_axpby(gpu, prec):
+    """y = a x + b y in place (the closure forcing m = V k - F_H(P_H v))."""
+    dt = torch.float64 if prec == "f64" else torch.float32
+    g = torch.Generator(device="cuda").manual_seed(7)
+    x = torch.randn(100003, device="cuda", dtype=dt, generator=g)
+    y = torch.randn(100003, device="cuda", dtype=dt, generator=g)
+    want = 1.5 * x.double() - 0.25 * y.double()
+    gpu._abi.context().call("ec_axpby", 1 if prec == "f64" else 0, x.numel(), 1.5, x.data_ptr(), -0.25,
+                            y.data_ptr())
+    assert float((y.double() - want).abs().max()) <= (1e-15 if prec == "f64" else 1e-6) * float(want.abs().max())
+
+
 def test_invalid_arguments_raise(gpu):
     NS = gpu.navierstokes
     ctx = gpu._abi.context()
