@@ -468,6 +468,9 @@ __device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__re
     };
     load(0, cv, cp, cqv);
     const int rend = ZC - 1 + H;
+    // k = v - delta p in the field precision (one rounding at its scale); the
+    // z-sums and everything after them are FP64
+    const T ihT = (T)tl.ih;
     // emits of every set at plane r (all on phase-3 planes for the nested list)
     auto emit = [&](int r) {
         const bool any = set_emits<G, 0>(r, ZC) || set_emits<G, 1>(r, ZC) || set_emits<G, 2>(r, ZC) ||
@@ -512,7 +515,7 @@ __device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__re
                 for (int j = 0; j < NJ; j++) {
                     const int a = comp(j);
                     if (a == 3) continue;
-                    const double src = (double)cv[j] - ((double)cqv[j] - (double)cp[j]) * tl.ih;
+                    const double src = (double)(cv[j] - (cqv[j] - cp[j]) * ihT);
                     if (a == 0) nested_box<G, 0, Q>(r, ZC, ok[j], clx[j], cly[j], src, nst[j], lds);
                     else if (a == 1) nested_box<G, 1, Q>(r, ZC, ok[j], clx[j], cly[j], src, nst[j], lds);
                     else nested_face<G, Q>(r, ZC, ok[j], clx[j], cly[j], src, nst[j], lds);
@@ -533,7 +536,7 @@ __device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__re
             for (int j = 0; j < NJ; j++) {
                 const int a = comp(j);
                 if (a == 3) continue;
-                const double src = (double)cv[j] - ((double)cqv[j] - (double)cp[j]) * tl.ih;
+                const double src = (double)(cv[j] - (cqv[j] - cp[j]) * ihT);
                 if constexpr (G::c(0) > 0) zupdate<G, 0>(r, ZC, a, ok[j], clx[j], cly[j], src, acc[j][0], lds);
                 if constexpr (G::c(1) > 0) zupdate<G, 1>(r, ZC, a, ok[j], clx[j], cly[j], src, acc[j][1], lds);
                 if constexpr (G::c(2) > 0) zupdate<G, 2>(r, ZC, a, ok[j], clx[j], cly[j], src, acc[j][2], lds);
