@@ -76,10 +76,12 @@ _SIGS = {
     "ec_poisson_solve": (c_int, [c_void_p, c_void_p]),
     "ec_ns_divergence": (c_int, [c_void_p, c_int, c_int, c_double, P3, c_void_p]),
     "ec_ns_pressuregradient": (c_int, [c_void_p, c_int, c_int, c_double, P3, c_void_p]),
+    "ec_ns_pressuregradient_into": (c_int, [c_void_p, c_int, c_int, c_double, P3, c_void_p, P3]),
     "ec_poisson_create": (c_int, [c_void_p, c_int, c_int, c_double, ctypes.POINTER(c_void_p)]),
     "ec_poisson_destroy": (c_int, [c_void_p, c_void_p]),
     "ec_poisson_pressure": (c_void_p, [c_void_p]),
     "ec_ns_project": (c_int, [c_void_p, c_void_p, P3]),
+    "ec_ns_project_into": (c_int, [c_void_p, c_void_p, P3, P3]),
     "ec_ns_pressure_solve": (c_int, [c_void_p, c_void_p, P3]),
     "ec_fft_r2c": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "ec_fft_c2r": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),

@@ -1020,6 +1020,17 @@ int project_t(ec_ctx *ctx, ec_poisson *ps, void *const u[3]) {
     return ec_ns_pressuregradient(ctx, ps->dtype, n, ps->l, u, ps->p);
 }
 
+// dst = P src (src unchanged): the divergence of src, the solve, dst = src - grad p
+template <typename T>
+int project_into_t(ec_ctx *ctx, ec_poisson *ps, const void *const src[3], void *const dst[3]) {
+    const int n = ps->n;
+    void *const s3[3] = {const_cast<void *>(src[0]), const_cast<void *>(src[1]), const_cast<void *>(src[2])};
+    int rc = ec_ns_divergence(ctx, ps->dtype, n, ps->l, s3, ps->p);
+    if (rc) return rc;
+    if ((rc = poisson_fft_t<T>(ctx, ps))) return rc;
+    return ec_ns_pressuregradient_into(ctx, ps->dtype, n, ps->l, src, ps->p, dst);
+}
+
 }  // namespace
 
 extern "C" int ec_poisson_create(ec_ctx *ctx, int dtype, int n, double l, ec_poisson **out) {
@@ -1109,6 +1120,11 @@ extern "C" void *ec_poisson_pressure(ec_poisson *ps) { return ps ? ps->p : nullp
 extern "C" int ec_ns_project(ec_ctx *ctx, ec_poisson *ps, void *const u[3]) {
     EC_REQUIRE(ctx, ctx && ps && u, "ec_ns_project: null argument");
     return ps->dtype == EC_F64 ? project_t<double>(ctx, ps, u) : project_t<float>(ctx, ps, u);
+}
+
+extern "C" int ec_ns_project_into(ec_ctx *ctx, ec_poisson *ps, const void *const src[3], void *const dst[3]) {
+    EC_REQUIRE(ctx, ctx && ps && src && dst, "ec_ns_project_into: null argument");
+    return ps->dtype == EC_F64 ? project_into_t<double>(ctx, ps, src, dst) : project_into_t<float>(ctx, ps, src, dst);
 }
 
 extern "C" int ec_ns_pressure_solve(ec_ctx *ctx, ec_poisson *ps, void *const u[3]) {

@@ -447,6 +447,20 @@ __global__ void k_pressuregradient(int n, T ih, T *__restrict__ u1, T *__restric
     u3[I] -= (p[c.at(c.i, c.j, c.kp)] - p0) * ih;
 }
 
+// out-of-place: o = a - grad p (a unchanged)
+template <typename T>
+__global__ void k_pressuregradient_into(int n, T ih, const T *__restrict__ a1, const T *__restrict__ a2,
+                                        const T *__restrict__ a3, T *__restrict__ o1, T *__restrict__ o2,
+                                        T *__restrict__ o3, const T *__restrict__ p) {
+    Cell c;
+    if (!cell_of(n, c)) return;
+    const size_t I = c.at(c.i, c.j, c.k);
+    const T p0 = p[I];
+    o1[I] = a1[I] - (p[c.at(c.ip, c.j, c.k)] - p0) * ih;
+    o2[I] = a2[I] - (p[c.at(c.i, c.jp, c.k)] - p0) * ih;
+    o3[I] = a3[I] - (p[c.at(c.i, c.j, c.kp)] - p0) * ih;
+}
+
 struct Ptr3 {
     void *p[3];
 };
@@ -646,6 +660,25 @@ extern "C" int ec_ns_pressuregradient(ec_ctx *ctx, int dtype, int n, double l, v
         k_pressuregradient<float><<<g.grid, g.block, 0, ctx->stream>>>(n, (float)ih, (float *)u[0], (float *)u[1],
                                                                        (float *)u[2], (const float *)p);
     EC_CHECK_LAUNCH(ctx, "k_pressuregradient");
+    return EC_OK;
+}
+
+extern "C" int ec_ns_pressuregradient_into(ec_ctx *ctx, int dtype, int n, double l, const void *const u[3],
+                                           const void *p, void *const out[3]) {
+    EC_REQUIRE(ctx, ctx && u && p && out, "ec_ns_pressuregradient_into: null argument");
+    EC_CHECK_FIELD(ctx, n);
+    Grid3 g = grid3(n);
+    const double ih = n / l;
+    ec_prof_scope ps(ctx, "pressuregradient", 7.0 * ec_cube(n) * (dtype == EC_F64 ? 8 : 4));
+    if (dtype == EC_F64)
+        k_pressuregradient_into<double><<<g.grid, g.block, 0, ctx->stream>>>(
+            n, ih, (const double *)u[0], (const double *)u[1], (const double *)u[2], (double *)out[0],
+            (double *)out[1], (double *)out[2], (const double *)p);
+    else
+        k_pressuregradient_into<float><<<g.grid, g.block, 0, ctx->stream>>>(
+            n, (float)ih, (const float *)u[0], (const float *)u[1], (const float *)u[2], (float *)out[0],
+            (float *)out[1], (float *)out[2], (const float *)p);
+    EC_CHECK_LAUNCH(ctx, "k_pressuregradient_into");
     return EC_OK;
 }
 

@@ -185,9 +185,7 @@ class DNSAidedLES:
         c = self.ctx
         gl = L.g
         v = L.v0 if st == 0 else L.vs
-        for a in range(3):
-            L.ubar[a].copy_(v[a])
-        c.call("ec_ns_project", L.ps64.h, ptrs(L.ubar, P3))
+        c.call("ec_ns_project_into", L.ps64.h, ptrs(v, P3), ptrs(L.ubar, P3))
         # m = V k - F_H(P_H v): F_H into m, then m = g_k - m in one pass
         c.call("ec_ns_rhs", _abi.EC_F64, gl.n, gl.l, self.visc, ptrs(L.ubar, P3), ptrs(L.m, P3), 0)
         for a in range(3):

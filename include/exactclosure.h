@@ -103,6 +103,9 @@ int ec_ns_divergence(ec_ctx *ctx, int dtype, int n, double l, void *const u[3], 
 /* pressuregradient! (src/N.jl:240-254) */
 int ec_ns_pressuregradient(ec_ctx *ctx, int dtype, int n, double l, void *const u[3],
                            const void *p);
+/* out = u - grad p with u unchanged (the gradient step of ec_ns_project_into) */
+int ec_ns_pressuregradient_into(ec_ctx *ctx, int dtype, int n, double l, const void *const u[3], const void *p,
+                                void *const out[3]);
 
 /* ---- FFT Poisson projection (poissonsolver / project!, src/N.jl:202-220,256-307) ---- */
 int ec_poisson_create(ec_ctx *ctx, int dtype, int n, double l, ec_poisson **ps);
@@ -111,6 +114,9 @@ int ec_poisson_destroy(ec_ctx *ctx, ec_poisson *ps);
 void *ec_poisson_pressure(ec_poisson *ps);
 /* project!(u, poisson): leaves the pressure in ec_poisson_pressure(ps) */
 int ec_ns_project(ec_ctx *ctx, ec_poisson *ps, void *const u[3]);
+/* dst = P src with src unchanged (project! of a copy, without the copy); no
+ * counterpart entry in the reference, which projects in place */
+int ec_ns_project_into(ec_ctx *ctx, ec_poisson *ps, const void *const src[3], void *const dst[3]);
 /* The first half of project!: divergence! and the FFT Poisson solve, leaving p in
  * ec_poisson_pressure(ps) and u unchanged (the gradient is then applied by
  * ec_ns_pressuregradient or fused into ec_ns_rk4_update). */

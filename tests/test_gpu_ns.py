@@ -353,6 +353,27 @@ def test_rk4_update_large_fp32(gpu, stage):
     torch.cuda.empty_cache()
 
 
+@pytest.mark.parametrize("n,prec", [(16, "f64"), (64, "f64"), (256, "f32")])
+def test_project_into(gpu, n, prec):
+    """dst = P src leaves src unchanged and equals copy + in-place projection
+    bit for bit (same divergence, solve and gradient arithmetic)."""
+    NS = gpu.navierstokes
+    dt = torch.float64 if prec == "f64" else torch.float32
+    g = NS.Grid(1.0, n)
+    ps = NS.PoissonSolver(g, dt)
+    gen = torch.Generator(device="cuda").manual_seed(n)
+    src = [torch.randn((n, n, n), device="cuda", dtype=dt, generator=gen) for _ in range(3)]
+    keep = [x.clone() for x in src]
+    dst = [torch.empty_like(x) for x in src]
+    ref = [x.clone() for x in src]
+    ctx = gpu._abi.context()
+    ctx.call("ec_ns_project", ps.h, P(ref, 3))
+    ctx.call("ec_ns_project_into", ps.h, P(src, 3), P(dst, 3))
+    for a in range(3):
+        assert torch.equal(src[a], keep[a])
+        assert torch.equal(dst[a], ref[a])
+
+
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_axpby(gpu, prec):
     """y = a x + b y in place (the closure forcing m = V k - F_H(P_H v))."""
