@@ -97,6 +97,8 @@ _SIGS = {
                               ctypes.POINTER(ClosureSet)]),
     "ec_closure_set_tiles": (c_int, [c_void_p, c_int, c_int, c_int]),
     "ec_ns_filter_vel": (c_int, [c_void_p, c_int, c_int, c_double, P3, c_void_p, c_int, ctypes.POINTER(VelSet)]),
+    "ec_ns_filter_vel_rk4": (c_int, [c_void_p, c_int, c_int, c_double, P3, c_void_p, c_int, ctypes.POINTER(VelSet),
+                                     c_int, c_double, P3, P3, P3]),
     "ec_filter_vel_set_tiles": (c_int, [c_void_p, c_int, c_int]),
     "ec_ns_commutator": (c_int, [c_void_p, c_int, c_double, c_double, P3, P9, P9]),
     "ec_ns_rk4_update": (c_int, [c_void_p, c_int, c_int, c_double, c_int, c_double, P3, P3, P3,

@@ -395,14 +395,45 @@ struct FvTile {
     double ih;
 };
 
+// The DNS RK4 stage fused into the march (ec_ns_filter_vel_rk4): the march
+// already forms k_a = du_a - delta_a p for every column, so the interior cells
+// of the tile also take the stage update of ec_ns_rk4_update (same arithmetic,
+// same FP32 k), and du and p are read once per stage instead of twice.
+struct RkArgs {
+    void *u0[3], *us[3], *acc[3];
+    double dt;
+};
+
+// loads of the stage state of one cell, issued at the top of a plane so that
+// they arrive during the plane's filter work
+template <typename T, int STAGE>
+__device__ __forceinline__ void rk_load(int a, unsigned I, const RkArgs &rk, T &u, T &ac) {
+    u = ((const T *)rk.u0[a])[I];
+    if constexpr (STAGE != 0) ac = ((const T *)rk.acc[a])[I];
+}
+template <typename T, int STAGE>
+__device__ __forceinline__ void rk_store(int a, unsigned I, T k, T u, T ac, const RkArgs &rk) {
+    const T dt = (T)rk.dt;
+    T *acc = (T *)rk.acc[a];
+    if constexpr (STAGE == 0) {
+        ec_nt_store(acc + I, k);
+        ec_nt_store((T *)rk.us[a] + I, u + (dt / 2) * k);
+    } else if constexpr (STAGE == 1 || STAGE == 2) {
+        ec_nt_store(acc + I, ac + 2 * k);
+        ec_nt_store((T *)rk.us[a] + I, u + (STAGE == 1 ? dt / 2 : dt) * k);
+    } else {
+        ec_nt_store((T *)rk.u0[a] + I, u + (dt / 6) * (ac + k));
+    }
+}
+
 // The plane march of one wave whose slot components are CODE: component,
 // column and window logic are compile-time, so a plane is straight-line code
 // (loads, FP64 z-updates) up to the uniform emit branch.
 // Element offsets are 32-bit: the host requires n^3 < 2^32.
-template <typename T, class G, int CODE>
+template <typename T, class G, int CODE, int RK>
 __device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__restrict__ v2, const T *__restrict__ v3,
                                          const T *__restrict__ p, const FvTile &tl, double *lds, const FSet *fsl,
-                                         int tid) {
+                                         int tid, const RkArgs &rk) {
     constexpr int TX = G::TX, TY = G::TY, H = G::hmax, NJ = G::NJ, NT = G::NT;
     const int n = tl.n, ZC = tl.ZC, x0 = tl.x0, y0 = tl.y0, z0 = tl.z0;
     const unsigned nn = (unsigned)n, n2 = nn * nn;
@@ -511,11 +542,21 @@ __device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__re
                 const int r = r0 + Q;
                 T nv[NJ], np[NJ], nq[NJ];
                 load(r + 1, nv, np, nq);
+                T ru[NJ], ra[NJ];
+                if constexpr (RK >= 0)
+#pragma unroll
+                    for (int j = 0; j < NJ; j++)
+                        if (comp(j) != 3 && r < ZC && ok[j] && clx[j] < TX && cly[j] < TY)
+                            rk_load<T, RK>(comp(j), co[j] + (unsigned)(z0 + r) * n2, rk, ru[j], ra[j]);
 #pragma unroll
                 for (int j = 0; j < NJ; j++) {
                     const int a = comp(j);
                     if (a == 3) continue;
-                    const double src = (double)(cv[j] - (cqv[j] - cp[j]) * ihT);
+                    const T kv = cv[j] - (cqv[j] - cp[j]) * ihT;
+                    const double src = (double)kv;
+                    if constexpr (RK >= 0)
+                        if (r < ZC && ok[j] && clx[j] < TX && cly[j] < TY)
+                            rk_store<T, RK>(a, co[j] + (unsigned)(z0 + r) * n2, kv, ru[j], ra[j], rk);
                     if (a == 0) nested_box<G, 0, Q>(r, ZC, ok[j], clx[j], cly[j], src, nst[j], lds);
                     else if (a == 1) nested_box<G, 1, Q>(r, ZC, ok[j], clx[j], cly[j], src, nst[j], lds);
                     else nested_face<G, Q>(r, ZC, ok[j], clx[j], cly[j], src, nst[j], lds);
@@ -532,11 +573,21 @@ __device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__re
         for (int r = 0; r <= rend; ++r) {
             T nv[NJ], np[NJ], nq[NJ];
             load(r + 1, nv, np, nq);
+            T ru[NJ], ra[NJ];
+            if constexpr (RK >= 0)
+#pragma unroll
+                for (int j = 0; j < NJ; j++)
+                    if (comp(j) != 3 && r < ZC && ok[j] && clx[j] < TX && cly[j] < TY)
+                        rk_load<T, RK>(comp(j), co[j] + (unsigned)(z0 + r) * n2, rk, ru[j], ra[j]);
 #pragma unroll
             for (int j = 0; j < NJ; j++) {
                 const int a = comp(j);
                 if (a == 3) continue;
-                const double src = (double)(cv[j] - (cqv[j] - cp[j]) * ihT);
+                const T kv = cv[j] - (cqv[j] - cp[j]) * ihT;
+                const double src = (double)kv;
+                if constexpr (RK >= 0)
+                    if (r < ZC && ok[j] && clx[j] < TX && cly[j] < TY)
+                        rk_store<T, RK>(a, co[j] + (unsigned)(z0 + r) * n2, kv, ru[j], ra[j], rk);
                 if constexpr (G::c(0) > 0) zupdate<G, 0>(r, ZC, a, ok[j], clx[j], cly[j], src, acc[j][0], lds);
                 if constexpr (G::c(1) > 0) zupdate<G, 1>(r, ZC, a, ok[j], clx[j], cly[j], src, acc[j][1], lds);
                 if constexpr (G::c(2) > 0) zupdate<G, 2>(r, ZC, a, ok[j], clx[j], cly[j], src, acc[j][2], lds);
@@ -549,25 +600,26 @@ __device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__re
 }
 
 // one instantiation of the march per distinct slot-component code
-template <typename T, class G, int W>
+template <typename T, class G, int RK, int W>
 __device__ __forceinline__ void fv_dispatch(int code, const T *__restrict__ v1, const T *__restrict__ v2,
                                             const T *__restrict__ v3, const T *__restrict__ p, const FvTile &tl,
-                                            double *lds, const FSet *fsl, int tid) {
+                                            double *lds, const FSet *fsl, int tid, const RkArgs &rk) {
     if constexpr (W < G::NT / 64) {
         if constexpr (first_code<G>(W)) {
             if (code == wave_code<G>(W)) {
-                fv_march<T, G, wave_code<G>(W)>(v1, v2, v3, p, tl, lds, fsl, tid);
+                fv_march<T, G, wave_code<G>(W), RK>(v1, v2, v3, p, tl, lds, fsl, tid, rk);
                 return;
             }
         }
-        fv_dispatch<T, G, W + 1>(code, v1, v2, v3, p, tl, lds, fsl, tid);
+        fv_dispatch<T, G, RK, W + 1>(code, v1, v2, v3, p, tl, lds, fsl, tid, rk);
     }
 }
 
-template <typename T, class G>
+// RK < 0: the filter alone; RK = 0..3: with the fused DNS RK4 stage RK
+template <typename T, class G, int RK>
 __global__ void __launch_bounds__(G::NT) k_filter_vel(const T *__restrict__ v1, const T *__restrict__ v2,
                                                      const T *__restrict__ v3, const T *__restrict__ p, int n, int ZC,
-                                                     double ih, FArgs A) {
+                                                     double ih, FArgs A, RkArgs R) {
     constexpr int TX = G::TX, TY = G::TY;
     extern __shared__ double lds[];
     // output table in LDS: read only when a coarse plane closes, so it holds
@@ -593,7 +645,7 @@ __global__ void __launch_bounds__(G::NT) k_filter_vel(const T *__restrict__ v1, 
         const int c = e < G::P0 ? 0 : (e < G::P0 + G::P1 ? 1 : (e < G::NCC ? 2 : 3));
         code |= c << (2 * j);
     }
-    fv_dispatch<T, G, 0>(code, v1, v2, v3, p, tl, lds, fsl, tid);
+    fv_dispatch<T, G, RK, 0>(code, v1, v2, v3, p, tl, lds, fsl, tid, R);
 }
 
 int largest_tile(int n, int L, int target) {
@@ -606,8 +658,9 @@ int largest_tile(int n, int L, int target) {
     return best;
 }
 
-template <typename T, class G>
-int launch_fv(ec_ctx *ctx, int n, double l, void *const v[3], const void *p, const FArgs &A, int tz_target) {
+template <typename T, class G, int RK = -1>
+int launch_fv(ec_ctx *ctx, int n, double l, void *const v[3], const void *p, const FArgs &A, int tz_target,
+              const RkArgs *rk = nullptr) {
     constexpr int TX = G::TX, TY = G::TY, L = G::lcm();
     static_assert(TX % L == 0 && TY % L == 0, "tile edges must be multiples of every compression");
     if (n % TX || n % TY || n % L) return EC_E_UNSUPPORTED;
@@ -620,11 +673,16 @@ int launch_fv(ec_ctx *ctx, int n, double l, void *const v[3], const void *p, con
     if (lds > 150 * 1024) return EC_E_UNSUPPORTED;
     const long tiles = (long)(n / TX) * (n / TY) * (n / ZC);
     if (tiles > 0x7fffffffL) return EC_E_UNSUPPORTED;
-    ec_prof_scope pr(ctx, "filter_vel", sizeof(T) * (p ? 4.0 : 3.0) * ec_cube(n));
-    EC_HIP(ctx, hipFuncSetAttribute((const void *)k_filter_vel<T, G>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    // fused stage: reads u0 (and acc after stage 0), writes us and acc (u0 at stage 3)
+    constexpr double rk_units = RK < 0 ? 0.0 : (RK == 0 || RK == 3 ? 9.0 : 12.0);
+    ec_prof_scope pr(ctx, RK < 0 ? "filter_vel" : "filter_vel_rk4",
+                     sizeof(T) * ((p ? 4.0 : 3.0) + rk_units) * ec_cube(n));
+    EC_HIP(ctx, hipFuncSetAttribute((const void *)k_filter_vel<T, G, RK>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)lds));
-    k_filter_vel<T, G><<<dim3((unsigned)tiles), G::NT, lds, ctx->stream>>>(
-        (const T *)v[0], (const T *)v[1], (const T *)v[2], (const T *)p, n, ZC, n / l, A);
+    RkArgs R{};
+    if (rk) R = *rk;
+    k_filter_vel<T, G, RK><<<dim3((unsigned)tiles), G::NT, lds, ctx->stream>>>(
+        (const T *)v[0], (const T *)v[1], (const T *)v[2], (const T *)p, n, ZC, n / l, A, R);
     EC_CHECK_LAUNCH(ctx, "k_filter_vel");
     return EC_OK;
 }
@@ -640,6 +698,32 @@ int launch_single(ec_ctx *ctx, int n, double l, void *const v[3], const void *p,
         case 8: return launch_fv<T, Cfg<32, 16, 512, 8, K, 0, 0, 0, 0, 0, 0>>(ctx, n, l, v, p, A, tz);
         case 16: return launch_fv<T, Cfg<32, 16, 512, 16, K, 0, 0, 0, 0, 0, 0>>(ctx, n, l, v, p, A, tz);
         default: return EC_E_UNSUPPORTED;
+    }
+}
+
+// The four-set list with the DNS RK4 stage fused (one launch), or
+// EC_E_UNSUPPORTED when the list or the grid takes another path.
+template <typename T>
+int filter_vel_rk4_t(ec_ctx *ctx, int n, double l, void *const v[3], const void *p, int nsets,
+                     const ec_vel_set *sets, int tz, int stage, const RkArgs &rk) {
+    if (nsets != 4 || !p) return EC_E_UNSUPPORTED;
+    FArgs A;
+    int cs[4], ks[4];
+    for (int s = 0; s < 4; s++) {
+        cs[s] = n / sets[s].nbar;
+        ks[s] = sets[s].kind;
+        for (int a = 0; a < 3; a++) A.s[s].out[a] = (double *)sets[s].out[a];
+        A.s[s].nb = sets[s].nbar;
+    }
+    if (!(cs[0] == 16 && ks[0] == 0 && cs[1] == 8 && ks[1] == 0 && cs[2] == 4 && ks[2] == 0 && cs[3] == 4 &&
+          ks[3] == 1))
+        return EC_E_UNSUPPORTED;
+    typedef Cfg<32, 16, 512, 16, 0, 8, 0, 4, 0, 4, 1> G4;
+    switch (stage) {
+        case 0: return launch_fv<T, G4, 0>(ctx, n, l, v, p, A, tz, &rk);
+        case 1: return launch_fv<T, G4, 1>(ctx, n, l, v, p, A, tz, &rk);
+        case 2: return launch_fv<T, G4, 2>(ctx, n, l, v, p, A, tz, &rk);
+        default: return launch_fv<T, G4, 3>(ctx, n, l, v, p, A, tz, &rk);
     }
 }
 
@@ -736,4 +820,31 @@ extern "C" int ec_ns_filter_vel(ec_ctx *ctx, int dtype, int n, double l, void *c
         if (rc) return rc;
     }
     return EC_OK;
+}
+
+extern "C" int ec_ns_filter_vel_rk4(ec_ctx *ctx, int dtype, int n, double l, void *const du[3], const void *p,
+                                    int nsets, const ec_vel_set *sets, int stage, double dt, void *const u0[3],
+                                    void *const us[3], void *const acc[3]) {
+    EC_REQUIRE(ctx, ctx && du && p && u0 && acc && (nsets == 0 || sets), "ec_ns_filter_vel_rk4: null argument");
+    EC_REQUIRE(ctx, stage >= 0 && stage <= 3, "ec_ns_filter_vel_rk4: stage %d out of range", stage);
+    EC_REQUIRE(ctx, stage == 3 || us, "ec_ns_filter_vel_rk4: stage state required before the last stage");
+    for (int s = 0; s < nsets; s++) {
+        EC_REQUIRE(ctx, sets[s].nbar >= 1 && n % sets[s].nbar == 0, "set %d: n=%d not a multiple of nbar=%d", s, n,
+                   sets[s].nbar);
+        EC_REQUIRE(ctx, sets[s].kind == 0 || sets[s].kind == 1, "set %d: kind %d", s, sets[s].kind);
+        for (int a = 0; a < 3; a++) EC_REQUIRE(ctx, sets[s].out[a], "set %d: null output", s);
+    }
+    RkArgs rk;
+    for (int a = 0; a < 3; a++) {
+        rk.u0[a] = u0[a];
+        rk.us[a] = us ? us[a] : nullptr;
+        rk.acc[a] = acc[a];
+    }
+    rk.dt = dt;
+    int rc = dtype == EC_F64 ? filter_vel_rk4_t<double>(ctx, n, l, du, p, nsets, sets, g_fv_tz, stage, rk)
+                             : filter_vel_rk4_t<float>(ctx, n, l, du, p, nsets, sets, g_fv_tz, stage, rk);
+    if (rc != EC_E_UNSUPPORTED) return rc;
+    // any other set list: the filter, then the stage update
+    if ((rc = ec_ns_filter_vel(ctx, dtype, n, l, du, p, nsets, sets))) return rc;
+    return ec_ns_rk4_update(ctx, dtype, n, l, stage, dt, u0, us, du, p, acc);
 }

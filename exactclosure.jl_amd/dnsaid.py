@@ -201,7 +201,15 @@ class DNSAidedLES:
             c.call("ec_ns_rhs_div", self.dt_code, n, l, self.visc, ptrs(u, P3), ptrs(self.du, P3),
                    _abi.c_void_p(self.ps.p_ptr))
             c.call("ec_poisson_solve", self.ps.h)
-            self._filter_vel(self.du, self.ps.p_ptr, [L.g_k for L in self.les])
+            # V_s(du - grad p) for every LES grid and the DNS RK4 stage in one
+            # pass over du and p (nothing below reads the DNS state)
+            if self.les:
+                c.call("ec_ns_filter_vel_rk4", self.dt_code, n, l, ptrs(self.du, P3), _abi.c_void_p(self.ps.p_ptr),
+                       len(self.les), self._vel_sets([L.g_k for L in self.les]), st, dt, ptrs(self.u0, P3),
+                       ptrs(self.us, P3), ptrs(self.acc, P3))
+            else:
+                c.call("ec_ns_rk4_update", self.dt_code, n, l, st, dt, ptrs(self.u0, P3), ptrs(self.us, P3),
+                       ptrs(self.du, P3), _abi.c_void_p(self.ps.p_ptr), ptrs(self.acc, P3))
             for L in self.les:
                 gl = L.g
                 m = self._les_forcing(L, st)
@@ -217,8 +225,6 @@ class DNSAidedLES:
                        ptrs(L.g_k, P3), ptr(L.zero64), ptrs(L.vacc, P3))
             for h in self.stage_hooks:
                 h.stage(st, dt)
-            c.call("ec_ns_rk4_update", self.dt_code, n, l, st, dt, ptrs(self.u0, P3), ptrs(self.us, P3),
-                   ptrs(self.du, P3), _abi.c_void_p(self.ps.p_ptr), ptrs(self.acc, P3))
 
     def run(self, dt, nsteps):
         for _ in range(nsteps):

@@ -205,6 +205,14 @@ typedef struct {
 } ec_vel_set;
 int ec_ns_filter_vel(ec_ctx *ctx, int dtype, int n, double l, void *const v[3], const void *p,
                      int nsets, const ec_vel_set *sets);
+/* ec_ns_filter_vel of du - grad p followed by ec_ns_rk4_update(stage, dt, u0, us,
+ * du, p, acc) of the same DNS fields, in one pass over du and p where the set
+ * list allows (the volume-16/8/4 + face-4 list); otherwise the two calls.  No
+ * counterpart entry in the reference (its step filters and updates separately,
+ * src/NavierStokes.jl's DNS-aided stepper). */
+int ec_ns_filter_vel_rk4(ec_ctx *ctx, int dtype, int n, double l, void *const du[3], const void *p,
+                         int nsets, const ec_vel_set *sets, int stage, double dt, void *const u0[3],
+                         void *const us[3], void *const acc[3]);
 /* Target block size (x = y, z) of ec_ns_filter_vel; set by the tile autotuner. */
 int ec_filter_vel_set_tiles(ec_ctx *ctx, int tx, int tz);
 /* Target block size (fine cells x, y, z) of the one-launch closure kernel; the

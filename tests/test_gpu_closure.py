@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 import torch
 
-from test_gpu_ns import dev, host, rel
+from test_gpu_ns import P, dev, host, rel
 
 pytestmark = pytest.mark.gpu
 
@@ -163,3 +163,57 @@ def test_fp32_pipeline_closure_error(gpu, O, closure):
         num = sum(float(((a - b) ** 2).sum()) for a, b in zip(t32, t64))
         den = sum(float((b ** 2).sum()) for b in t64)
         assert (num / den) ** 0.5 < 1e-5
+
+
+@pytest.mark.parametrize("n,prec,sets", [(32, "f64", [(16, 0), (8, 0), (4, 0), (4, 1)]),
+                                         (64, "f32", [(16, 0), (8, 0), (4, 0), (4, 1)]),
+                                         (64, "f64", [(16, 0), (8, 0), (4, 0), (4, 1)]),
+                                         (48, "f32", [(3, 0), (3, 1), (16, 1), (2, 0)])])
+@pytest.mark.parametrize("stage", [0, 1, 2, 3])
+def test_filter_vel_rk4_fused(gpu, n, prec, sets, stage):
+    """ec_ns_filter_vel_rk4 equals ec_ns_filter_vel followed by ec_ns_rk4_update
+    on the same fields: the filter outputs bit for bit, the stage update to
+    rounding (fused for the volume-16/8/4 + face-4 list, with 16-plane chunks at
+    n = 64; the other list takes the two calls)."""
+    dt = torch.float64 if prec == "f64" else torch.float32
+    code = 1 if prec == "f64" else 0
+    gen = torch.Generator(device="cuda").manual_seed(10 * n + stage)
+    f = lambda: torch.randn((n, n, n), device="cuda", dtype=dt, generator=gen)  # noqa: E731
+    du, u0, us, acc = [f() for _ in range(3)], [f() for _ in range(3)], [f() for _ in range(3)], [f() for _ in range(3)]
+    p = f()
+    copies = lambda ts: [t.clone() for t in ts]  # noqa: E731
+    state_a = [copies(u0), copies(us), copies(acc)]
+    state_b = [copies(u0), copies(us), copies(acc)]
+
+    def outsets():
+        outs = []
+        arr = (gpu._abi.VelSet * len(sets))()
+        for i, (c, kind) in enumerate(sets):
+            o = [torch.empty((n // c,) * 3, dtype=torch.float64, device="cuda") for _ in range(3)]
+            outs.append(o)
+            arr[i].nbar = n // c
+            arr[i].kind = kind
+            for a in range(3):
+                arr[i].out[a] = o[a].data_ptr()
+        return outs, arr
+
+    oa, arra = outsets()
+    ob, arrb = outsets()
+    ctx = gpu._abi.context()
+    step = 1e-3
+    ctx.call("ec_filter_vel_set_tiles", 16, 16 if n == 64 else 64)
+    try:
+        ctx.call("ec_ns_filter_vel", code, n, 1.0, P(du, 3), p.data_ptr(), len(sets), arra)
+        ctx.call("ec_ns_rk4_update", code, n, 1.0, stage, step, P(state_a[0], 3), P(state_a[1], 3), P(du, 3),
+                 p.data_ptr(), P(state_a[2], 3))
+        ctx.call("ec_ns_filter_vel_rk4", code, n, 1.0, P(du, 3), gpu._abi.c_void_p(p.data_ptr()), len(sets), arrb,
+                 stage, step, P(state_b[0], 3), P(state_b[1], 3), P(state_b[2], 3))
+    finally:
+        ctx.call("ec_filter_vel_set_tiles", 16, 256)
+    for x, y in zip(oa, ob):
+        for a in range(3):
+            assert torch.equal(x[a], y[a])
+    tol = 1e-14 if prec == "f64" else 1e-6
+    for xs, ys in zip(state_a, state_b):
+        for x, y in zip(xs, ys):
+            assert float((x - y).abs().max()) <= tol * float(x.abs().max())
