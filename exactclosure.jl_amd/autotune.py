@@ -5,8 +5,8 @@ Times each candidate shape of a tunable kernel on the device it will run on
 keeps the fastest. Results are cached per (device, kernel, workload key) in
 a JSON file, so a process tunes a given shape once. Tunables:
 
-* ``filter_vel_zchunk``: planes per block of the one-launch velocity filter
-  (``ec_filter_vel_set_tiles``);
+* ``filter_vel_rk4_zchunk``: planes per block of the one-launch velocity
+  filter with the fused DNS RK4 stage (``ec_filter_vel_set_tiles``);
 * ``closure_tiles``: x/y/z block targets of the flux-form closure kernel
   (``ec_closure_set_tiles``);
 * ``burgers_threads``: threads per instance block of the persistent batched
@@ -94,16 +94,23 @@ def _denorm(s, candidates):
 
 
 def tune_filter_vel(tuner, drv, candidates=(64, 128, 256, 512)):
-    """Planes per block of the forcing-form filter, on the driver's own fields."""
+    """Planes per block of the forcing-form filter, on the driver's own fields:
+    the filter with the fused DNS RK4 stage as the step runs it (stage 0, which
+    writes only the stage state us and acc, so the DNS state u0 is untouched)."""
     from . import _abi
+    from ._abi import P3, ptrs
 
     ctx = drv.ctx
     outs = [L.g_k for L in drv.les]
-    n = drv.g.n
+    n, l = drv.g.n, drv.g.l
+
+    def run():
+        ctx.call("ec_ns_filter_vel_rk4", drv.dt_code, n, l, ptrs(drv.du, P3), _abi.c_void_p(drv.ps.p_ptr),
+                 len(drv.les), drv._vel_sets(outs), 0, 0.0, ptrs(drv.u0, P3), ptrs(drv.us, P3), ptrs(drv.acc, P3))
+
     cands = [z for z in candidates if n % z == 0] or [n]
-    return tuner.tune("filter_vel_zchunk", f"n{n}|" + ",".join(f"{L.g.n}{L.kind}" for L in drv.les), cands,
-                      lambda z: ctx.call("ec_filter_vel_set_tiles", 16, int(z)),
-                      lambda: drv._filter_vel(drv.du, drv.ps.p_ptr, outs))
+    return tuner.tune("filter_vel_rk4_zchunk", f"n{n}|" + ",".join(f"{L.g.n}{L.kind}" for L in drv.les), cands,
+                      lambda z: ctx.call("ec_filter_vel_set_tiles", 16, int(z)), run)
 
 
 def tune_burgers_threads(tuner, batch, nsteps=20, candidates=(128, 256, 512, 1024)):
