@@ -161,6 +161,7 @@ class Context:
             raise ECError(rc, f"ec_create(device={device}) failed (no usable GPU?)")
         self.h = h
         self.device = device
+        self.stream = stream  # the HIP stream handle (None: the device's default stream)
 
     def check(self, rc, what=""):
         if rc != EC_OK:

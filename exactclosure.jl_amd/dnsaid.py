@@ -180,9 +180,9 @@ class DNSAidedLES:
             c.call("ec_ns_rk4_update", self.dt_code, n, l, st, dt, ptrs(self.u0, P3), ptrs(self.us, P3),
                    ptrs(self.du, P3), _abi.c_void_p(self.ps.p_ptr), ptrs(self.acc, P3))
 
-    def _les_forcing(self, L, st):
+    def _les_forcing(self, L, st, c=None):
         """m = V k - F_H(P_H v) in FP64 for the stage state of LES grid L."""
-        c = self.ctx
+        c = c or self.ctx
         gl = L.g
         v = L.v0 if st == 0 else L.vs
         c.call("ec_ns_project_into", L.ps64.h, ptrs(v, P3), ptrs(L.ubar, P3))
@@ -207,24 +207,30 @@ class DNSAidedLES:
                 c.call("ec_ns_filter_vel_rk4", self.dt_code, n, l, ptrs(self.du, P3), _abi.c_void_p(self.ps.p_ptr),
                        len(self.les), self._vel_sets([L.g_k for L in self.les]), st, dt, ptrs(self.u0, P3),
                        ptrs(self.us, P3), ptrs(self.acc, P3))
+                self._les_stage(st, dt, c)
             else:
                 c.call("ec_ns_rk4_update", self.dt_code, n, l, st, dt, ptrs(self.u0, P3), ptrs(self.us, P3),
                        ptrs(self.du, P3), _abi.c_void_p(self.ps.p_ptr), ptrs(self.acc, P3))
-            for L in self.les:
-                gl = L.g
-                m = self._les_forcing(L, st)
-                ul = L.u0 if st == 0 else L.us
-                c.call("ec_ns_rhs", self.dt_code, gl.n, gl.l, self.visc, ptrs(ul, P3), ptrs(L.du, P3), 0)
-                for a in range(3):
-                    c.call("ec_axpy2", self.dt_code, L.du[a].numel(), 1.0, _abi.EC_F64, ptr(m[a]), ptr(L.du[a]))
-                c.call("ec_ns_pressure_solve", L.ps.h, ptrs(L.du, P3))
-                c.call("ec_ns_rk4_update", self.dt_code, gl.n, gl.l, st, dt, ptrs(L.u0, P3), ptrs(L.us, P3),
-                       ptrs(L.du, P3), _abi.c_void_p(L.ps.p_ptr), ptrs(L.acc, P3))
-                # v = V u follows the DNS through the same tableau (V k is already projected)
-                c.call("ec_ns_rk4_update", _abi.EC_F64, gl.n, gl.l, st, dt, ptrs(L.v0, P3), ptrs(L.vs, P3),
-                       ptrs(L.g_k, P3), ptr(L.zero64), ptrs(L.vacc, P3))
             for h in self.stage_hooks:
                 h.stage(st, dt)
+
+    def _les_stage(self, st, dt, c):
+        """Stage st of every LES grid (forcing form) on context c: the exact
+        forcing from the stage's filter outputs, the LES RHS with it, the
+        projection and the RK4 updates of the LES state and of v = V u."""
+        for L in self.les:
+            gl = L.g
+            m = self._les_forcing(L, st, c)
+            ul = L.u0 if st == 0 else L.us
+            c.call("ec_ns_rhs", self.dt_code, gl.n, gl.l, self.visc, ptrs(ul, P3), ptrs(L.du, P3), 0)
+            for a in range(3):
+                c.call("ec_axpy2", self.dt_code, L.du[a].numel(), 1.0, _abi.EC_F64, ptr(m[a]), ptr(L.du[a]))
+            c.call("ec_ns_pressure_solve", L.ps.h, ptrs(L.du, P3))
+            c.call("ec_ns_rk4_update", self.dt_code, gl.n, gl.l, st, dt, ptrs(L.u0, P3), ptrs(L.us, P3),
+                   ptrs(L.du, P3), _abi.c_void_p(L.ps.p_ptr), ptrs(L.acc, P3))
+            # v = V u follows the DNS through the same tableau (V k is already projected)
+            c.call("ec_ns_rk4_update", _abi.EC_F64, gl.n, gl.l, st, dt, ptrs(L.v0, P3), ptrs(L.vs, P3),
+                   ptrs(L.g_k, P3), ptr(L.zero64), ptrs(L.vacc, P3))
 
     def run(self, dt, nsteps):
         for _ in range(nsteps):

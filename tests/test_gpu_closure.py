@@ -217,3 +217,4 @@ def test_filter_vel_rk4_fused(gpu, n, prec, sets, stage):
     for xs, ys in zip(state_a, state_b):
         for x, y in zip(xs, ys):
             assert float((x - y).abs().max()) <= tol * float(x.abs().max())
+
