@@ -53,7 +53,7 @@ def test_rhs_fp64(gpu, O, n):
 @pytest.mark.parametrize("n", [64, 256, 320])
 def test_rhs_marching_tiles_fp64(gpu, O, n):
     """Grids that take the z-marching kernel (n >= 256: 64 x 8 column tiles,
-    chunks of 16 to 128 planes) and one that does not: every tile and chunk
+    chunks of 16 to 64 planes) and one that does not: every tile and chunk
     boundary, with and without doadd."""
     test_rhs_fp64(gpu, O, n)
 
