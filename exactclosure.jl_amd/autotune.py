@@ -95,8 +95,11 @@ def _denorm(s, candidates):
 
 def tune_filter_vel(tuner, drv, candidates=(64, 128, 256, 512)):
     """Planes per block of the forcing-form filter, on the driver's own fields:
-    the filter with the fused DNS RK4 stage as the step runs it (stage 0, which
-    writes only the stage state us and acc, so the DNS state u0 is untouched)."""
+    the filter with the fused DNS RK4 stage as the step runs it.  One timed run
+    is stages 0, 1 and 2 with dt = 0 (the stages differ in the stage state they
+    stream, and stage 0 alone did not rank the chunks as the step does); these
+    stages write only the stage state us and acc, which the next step's stage 0
+    rewrites, so call this between steps: the DNS state u0 is untouched."""
     from . import _abi
     from ._abi import P3, ptrs
 
@@ -105,8 +108,10 @@ def tune_filter_vel(tuner, drv, candidates=(64, 128, 256, 512)):
     n, l = drv.g.n, drv.g.l
 
     def run():
-        ctx.call("ec_ns_filter_vel_rk4", drv.dt_code, n, l, ptrs(drv.du, P3), _abi.c_void_p(drv.ps.p_ptr),
-                 len(drv.les), drv._vel_sets(outs), 0, 0.0, ptrs(drv.u0, P3), ptrs(drv.us, P3), ptrs(drv.acc, P3))
+        for st in range(3):
+            ctx.call("ec_ns_filter_vel_rk4", drv.dt_code, n, l, ptrs(drv.du, P3), _abi.c_void_p(drv.ps.p_ptr),
+                     len(drv.les), drv._vel_sets(outs), st, 0.0, ptrs(drv.u0, P3), ptrs(drv.us, P3),
+                     ptrs(drv.acc, P3))
 
     cands = [z for z in candidates if n % z == 0] or [n]
     return tuner.tune("filter_vel_rk4_zchunk", f"n{n}|" + ",".join(f"{L.g.n}{L.kind}" for L in drv.les), cands,

@@ -639,6 +639,15 @@ __global__ void __launch_bounds__(xblock(NTHR)) k_fft_x_c2r_ct(const cplx<T> *__
 // Y / Z passes: TK consecutive kx columns, the whole column staged in LDS
 // (element e of column c at lds[e * TK + c]: consecutive lanes take
 // consecutive columns, so both global rows and LDS rows are contiguous).
+// The fused Poisson pass (MODE 1) also stages the twiddle table and the
+// eigenvalues sin^2(pi k / N) in LDS behind the tile (lds_cols_ct): read from
+// global memory they are small scattered loads, issued between the stages and
+// between the two transforms, whose latency under the streaming traffic of
+// the other blocks left the pass at half the speed of a single transform.
+constexpr size_t lds_cols_ct(int n, int tk, int mode, size_t tsize) {
+    return (size_t)padded_len(n) * tk * 2 * tsize + (mode == 1 ? (size_t)n * 3 * tsize : 0);
+}
+
 template <typename T, int N, int TK, int NTHR, int MODE, int SIGN>
 __global__ void __launch_bounds__(TK *NTHR) k_fft_cols_ct(cplx<T> *__restrict__ data, int ncols, int ntiles,
                                                          long elem_stride, long outer_stride,
@@ -655,14 +664,39 @@ __global__ void __launch_bounds__(TK *NTHR) k_fft_cols_ct(cplx<T> *__restrict__ 
     cplx<T> *base = data + col0 + (long)outer * outer_stride;
     const bool full = col0 + TK <= ncols;
     constexpr int TOTAL = N * TK;
-    // all loads of the tile in flight at once (one block per CU at N = 1024)
+    constexpr int TBQ = (N + TK * NTHR - 1) / (TK * NTHR);  // table entries per thread
+    cplx<T> *twl = lds + (size_t)padded_len(N) * TK;         // MODE 1: N twiddles, then N eigenvalues
+    T *eigl = reinterpret_cast<T *>(twl + N);
+    // all loads of the tile (and of the tables) in flight at once
     {
         constexpr int PER = TOTAL / (TK * NTHR);
         cplx<T> tmp[PER];
+        cplx<T> twr[TBQ];
+        T eir[TBQ];
+        if constexpr (MODE == 1) {
+#pragma unroll
+            for (int q = 0; q < TBQ; q++) {
+                const int i = threadIdx.x + q * TK * NTHR;
+                if (i < N) {
+                    twr[q] = tw[i];
+                    eir[q] = eig[i];
+                }
+            }
+        }
 #pragma unroll
         for (int q = 0; q < PER; q++) {
             const int idx = threadIdx.x + q * TK * NTHR, cc = idx % TK, e = idx / TK;
             tmp[q] = (full || col0 + cc < ncols) ? base[cc + e * elem_stride] : cplx<T>{T(0), T(0)};
+        }
+        if constexpr (MODE == 1) {
+#pragma unroll
+            for (int q = 0; q < TBQ; q++) {
+                const int i = threadIdx.x + q * TK * NTHR;
+                if (i < N) {
+                    twl[i] = twr[q];
+                    eigl[i] = eir[q];
+                }
+            }
         }
 #pragma unroll
         for (int q = 0; q < PER; q++) {
@@ -674,12 +708,12 @@ __global__ void __launch_bounds__(TK *NTHR) k_fft_cols_ct(cplx<T> *__restrict__ 
     if (MODE == 0) {
         ct_stages<T, N, 1, NTHR, TK, SIGN>(lds + c, t, tw);
     } else {
-        ct_stages<T, N, 1, NTHR, TK, -1, (TK * NTHR > 512 ? 8 : 16)>(lds + c, t, tw);
+        ct_stages<T, N, 1, NTHR, TK, -1, (TK * NTHR > 512 ? 8 : 16)>(lds + c, t, twl);
         const int kx = col0 + c;
-        const T sxy = (kx < ncols ? eig[kx] : T(0)) + eig[outer];
+        const T sxy = (kx < ncols ? eigl[kx] : T(0)) + eigl[outer];
 #pragma unroll
         for (int e = t; e < N; e += NTHR) {
-            const T denom = poisson_a * (sxy + eig[e]);
+            const T denom = poisson_a * (sxy + eigl[e]);
             const bool zero_mode = (kx == 0 && outer == 0 && e == 0);
             const T f = zero_mode ? T(0) : scale / denom;
             cplx<T> z = lds[pad8(e) * TK + c];
@@ -688,7 +722,7 @@ __global__ void __launch_bounds__(TK *NTHR) k_fft_cols_ct(cplx<T> *__restrict__ 
             lds[pad8(e) * TK + c] = z;
         }
         __syncthreads();
-        ct_stages<T, N, 1, NTHR, TK, +1, (TK * NTHR > 512 ? 8 : 16)>(lds + c, t, tw);
+        ct_stages<T, N, 1, NTHR, TK, +1, (TK * NTHR > 512 ? 8 : 16)>(lds + c, t, twl);
     }
 #pragma unroll
     for (int idx = threadIdx.x; idx < TOTAL; idx += TK * NTHR) {
@@ -829,7 +863,8 @@ int cols_ct(ec_ctx *ctx, ec_poisson *ps, void *spec, int axis, int pitch) {
     const long outer_stride = axis == 1 ? (long)pitch * N : (long)pitch;
     const int ntiles = (nh + TK - 1) / TK;
     dim3 grid((unsigned)(ntiles * N));
-    const size_t lds = (size_t)padded_len(N) * TK * sizeof(cplx<T>);
+    constexpr size_t lds = lds_cols_ct(N, TK, MODE, sizeof(T));
+    static_assert(lds <= 160 * 1024, "column tile and tables exceed the LDS of a CU");
     const double h = ps->l / N;
     EC_HIP(ctx, hipFuncSetAttribute((const void *)k_fft_cols_ct<T, N, TK, NTHR, MODE, SIGN>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

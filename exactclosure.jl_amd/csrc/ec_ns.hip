@@ -501,15 +501,14 @@ __global__ void k_rk4_update(int n, T ih, T dt, Ptr3 u0p, Ptr3 usp, Ptr3 dup, co
 #define EC_CHECK_FIELD(ctx, n) EC_REQUIRE(ctx, (n) >= 2, "grid size n=%d must be >= 2", (n))
 
 // z-marching kernels: grids of at least 256 cells per side (n % 64 == 0);
-// planes per block chosen so a launch has >= 2048 tiles (chunks of 16..128)
 static bool march_ok(int n) { return n >= 256 && n % 64 == 0 && (double)n * n * n < 4294967296.0; }
-// planes per block of the marching RHS: at most 64 (neighbouring tiles then
-// start and finish nearly together, so their shared halo lines still hit in
-// L2; 128-plane chunks measured 7% slower at 1024^3), fewer where needed for
-// at least 2048 tiles
-static int march_zc(int n) {
+// planes per block of the marching RHS with 64 x ty column tiles: at most 64
+// (neighbouring tiles then start and finish nearly together, so their shared
+// halo lines still hit in L2; 128-plane chunks measured 7% slower at 1024^3),
+// fewer where needed for at least 2048 tiles (chunks of 16..64)
+static int march_zc(int n, int ty = 8) {
     int zc = 64;
-    while (zc > 16 && (long)(n / 64) * (n / 8) * (n / zc) < 2048) zc /= 2;
+    while (zc > 16 && (long)(n / 64) * (n / ty) * (n / zc) < 2048) zc /= 2;
     while (n % zc) zc /= 2;
     return zc;
 }
@@ -621,14 +620,18 @@ extern "C" int ec_ns_rhs_div(ec_ctx *ctx, int dtype, int n, double l, double vis
     const double ih = n / l;
     const double b = dtype == EC_F64 ? 8 : 4;
     ec_prof_scope ps(ctx, "ns_rhs_div", 7.0 * ec_cube(n) * b);
-    const int ZC = march_zc(n);
-    const unsigned tiles = (unsigned)((n / 64) * (n / 8) * (n / ZC));
+    // FP32 takes 64 x 16 tiles (halo rows 19/16 of the tile instead of 11/8;
+    // 78 KB of LDS, two blocks per CU; 3% faster at 1024^3); FP64 keeps 64 x 8,
+    // whose 88 KB already hold it to one block per CU
+    const int TY = dtype == EC_F64 ? 8 : 16;
+    const int ZC = march_zc(n, TY);
+    const unsigned tiles = (unsigned)((n / 64) * (n / TY) * (n / ZC));
     if (dtype == EC_F64)
         k_rhs_div_march<double, 64, 8><<<tiles, 256, 0, ctx->stream>>>(
             n, ZC, ih, visc, (const double *)u[0], (const double *)u[1], (const double *)u[2], (double *)du[0],
             (double *)du[1], (double *)du[2], (double *)d);
     else
-        k_rhs_div_march<float, 64, 8><<<tiles, 256, 0, ctx->stream>>>(
+        k_rhs_div_march<float, 64, 16><<<tiles, 256, 0, ctx->stream>>>(
             n, ZC, (float)ih, (float)visc, (const float *)u[0], (const float *)u[1], (const float *)u[2],
             (float *)du[0], (float *)du[1], (float *)du[2], (float *)d);
     EC_CHECK_LAUNCH(ctx, "k_rhs_div_march");

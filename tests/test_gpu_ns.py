@@ -94,6 +94,36 @@ def test_rhs_div(gpu, O, n, prec):
     assert float((p1 - p2).abs().max()) <= (1e-12 if prec == "f64" else 1e-5) * float(p1.abs().max())
 
 
+@pytest.mark.parametrize("n", [512, 1024])
+def test_rhs_div_large_fp32(gpu, n):
+    """Full-size FP32 grids: the fused march (64 x 16 column tiles) returns the
+    du of ec_ns_rhs (its own 64 x 8 march of the same stress cell) and, as d,
+    the divergence of ec_ns_divergence applied to its own du, on the same
+    seeded field; every tile and chunk edge of the launch is in play."""
+    l, visc = 1.0, 1e-3
+    gen = torch.Generator(device="cuda").manual_seed(n)
+    u = [torch.randn((n, n, n), device="cuda", generator=gen) for _ in range(3)]
+    du = [torch.empty_like(x) for x in u]
+    d = torch.empty_like(u[0])
+    ctx = gpu._abi.context()
+    ctx.call("ec_ns_rhs_div", 0, n, l, visc, P(u, 3), P(du, 3), gpu._abi.c_void_p(d.data_ptr()))
+    ref = [torch.empty_like(x) for x in u]
+    ctx.call("ec_ns_rhs", 0, n, l, visc, P(u, 3), P(ref, 3), 0)
+    torch.cuda.synchronize()
+    for a in range(3):
+        scale = float(ref[a].abs().max())
+        assert float((du[a] - ref[a]).abs().max()) <= 2e-6 * scale, a
+    del u
+    dref = torch.empty_like(d)
+    ctx.call("ec_ns_divergence", 0, n, l, P(du, 3), gpu._abi.c_void_p(dref.data_ptr()))
+    torch.cuda.synchronize()
+    # the divergence cancels: measure it against the size of its terms
+    scale = max(float(r.abs().max()) for r in du) * n / l
+    assert float((d - dref).abs().max()) <= 1e-6 * scale
+    del du, d, ref, dref
+    torch.cuda.empty_cache()
+
+
 @pytest.mark.parametrize("n", [16, 32, 256])
 def test_rhs_fp32(gpu, O, n):
     u = rand_u(O, n, 100 + n)
