@@ -227,7 +227,8 @@ __global__ void __launch_bounds__(256) k_rhs_march(int n, int ZC, T ih, T visc, 
                                                    const T *__restrict__ u2, const T *__restrict__ u3,
                                                    T *__restrict__ du1, T *__restrict__ du2, T *__restrict__ du3,
                                                    int doadd) {
-    constexpr int HX = TX + 2, HY = TY + 2, HP = HX * HY, NLD = (3 * HP + 255) / 256;
+    constexpr int HX = TX + 2, HY = TY + 2, HP = HX * HY;
+    constexpr int NLC = (HP + 255) / 256, NLD = 3 * NLC;  // loads per component and plane, all components
     constexpr int CPT = TX * TY / 256;  // cells per thread and plane
     static_assert(TX * TY % 256 == 0 && 256 % TX == 0, "tile shape");
     __shared__ T ring[4][3 * HP];
@@ -235,29 +236,34 @@ __global__ void __launch_bounds__(256) k_rhs_march(int n, int ZC, T ih, T visc, 
     const int L = (int)ec_xcd_tile(blockIdx.x, gridDim.x);
     const int x0 = (L % gx) * TX, y0 = ((L / gx) % gy) * TY, z0 = (L / (gx * gy)) * ZC;
     const int tid = threadIdx.x;
-    const unsigned nn = (unsigned)n, n2 = nn * nn;
-    // loader: element e = comp * HP + (hy * HX + hx) of a haloed plane
-    unsigned loff[NLD];
-    int lcomp[NLD];
+    const unsigned nn = (unsigned)n;
+    const size_t n2 = (size_t)nn * nn;
+    // loader: load slot jj of every component takes element q = tid + 256 jj of
+    // the haloed plane (threads past the plane load its last element and do not
+    // store it); a load is the uniform plane base plus a loop-invariant byte offset
+    unsigned loff[NLC];
 #pragma unroll
-    for (int j = 0; j < NLD; j++) {
-        const int e = tid + 256 * j;
-        const int comp = e / HP, q = e - comp * HP, hx = q % HX, hy = q / HX;
-        lcomp[j] = e < 3 * HP ? comp : -1;
-        loff[j] = (unsigned)ec_wrap_any(x0 + hx - 1, n) + nn * (unsigned)ec_wrap_any(y0 + hy - 1, n);
+    for (int jj = 0; jj < NLC; jj++) {
+        const int q = min(tid + 256 * jj, HP - 1), hx = q % HX, hy = q / HX;
+        loff[jj] = ((unsigned)ec_wrap_any(x0 + hx - 1, n) + nn * (unsigned)ec_wrap_any(y0 + hy - 1, n)) *
+                   (unsigned)sizeof(T);
     }
     auto load = [&](int k, T *r) {
-        const unsigned kk = (unsigned)ec_wrap_any(k, n) * n2;
+        const size_t kk = (size_t)ec_wrap_any(k, n) * n2;
+        const char *b1 = (const char *)(u1 + kk), *b2 = (const char *)(u2 + kk), *b3 = (const char *)(u3 + kk);
 #pragma unroll
-        for (int j = 0; j < NLD; j++) {
-            const T *sp = lcomp[j] == 1 ? u2 : (lcomp[j] == 2 ? u3 : u1);
-            r[j] = sp[loff[j] + kk];
+        for (int jj = 0; jj < NLC; jj++) {
+            r[jj] = *(const T *)(b1 + loff[jj]);
+            r[NLC + jj] = *(const T *)(b2 + loff[jj]);
+            r[2 * NLC + jj] = *(const T *)(b3 + loff[jj]);
         }
     };
     auto store = [&](int slot, const T *r) {
 #pragma unroll
-        for (int j = 0; j < NLD; j++)
-            if (lcomp[j] >= 0) ring[slot][tid + 256 * j] = r[j];
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int jj = 0; jj < NLC; jj++)
+                if (HP % 256 == 0 || tid + 256 * jj < HP) ring[slot][c * HP + tid + 256 * jj] = r[c * NLC + jj];
     };
     {
         T r[NLD];
@@ -281,7 +287,7 @@ __global__ void __launch_bounds__(256) k_rhs_march(int n, int ZC, T ih, T visc, 
             const int o = (cy + 1) * HX + (cx + 1);  // the cell in a haloed plane
             T a1, a2, a3;
             rhs_cell(pm, p0, pp, o, HX, HP, ih, visc, a1, a2, a3);
-            const size_t I = (size_t)(x0 + cx) + (size_t)nn * ((size_t)(y0 + cy) + (size_t)nn * (size_t)k);
+            const size_t I = (size_t)(x0 + cx) + (size_t)nn * (size_t)(y0 + cy) + n2 * (size_t)k;
             if (doadd) {
                 a1 += du1[I];
                 a2 += du2[I];
@@ -307,7 +313,8 @@ __global__ void __launch_bounds__(256) k_rhs_div_march(int n, int ZC, T ih, T vi
                                                        const T *__restrict__ u2, const T *__restrict__ u3,
                                                        T *__restrict__ du1, T *__restrict__ du2, T *__restrict__ du3,
                                                        T *__restrict__ dout) {
-    constexpr int HX = TX + 3, HY = TY + 3, HP = HX * HY, NLD = (3 * HP + 255) / 256;
+    constexpr int HX = TX + 3, HY = TY + 3, HP = HX * HY;
+    constexpr int NLC = (HP + 255) / 256, NLD = 3 * NLC;  // loads per component and plane, all components
     constexpr int CPT = TX * TY / 256;
     constexpr int D1N = TY * (TX + 1), D2N = (TY + 1) * TX;
     static_assert(TX * TY % 256 == 0 && 256 % TX == 0 && TX + TY <= 256, "tile shape");
@@ -317,29 +324,38 @@ __global__ void __launch_bounds__(256) k_rhs_div_march(int n, int ZC, T ih, T vi
     const int L = (int)ec_xcd_tile(blockIdx.x, gridDim.x);
     const int x0 = (L % gx) * TX, y0 = ((L / gx) % gy) * TY, z0 = (L / (gx * gy)) * ZC;
     const int tid = threadIdx.x;
-    const unsigned nn = (unsigned)n, n2 = nn * nn;
+    const unsigned nn = (unsigned)n;
+    const size_t n2 = (size_t)nn * nn;
 
-    unsigned loff[NLD];
-    int lcomp[NLD];
+    // loader: load slot jj of every component takes element q = tid + 256 jj of
+    // the haloed plane (threads past the plane load its last element and do not
+    // store it).  The byte offset of q within a z-plane is loop invariant and the
+    // plane base is uniform, so a load is a scalar base plus one vector offset:
+    // no per-load address arithmetic or component select.  (Byte offsets stay
+    // below 2^32: n^3 < 2^32 on the marching path.)
+    unsigned loff[NLC];
 #pragma unroll
-    for (int j = 0; j < NLD; j++) {
-        const int e = tid + 256 * j;
-        const int comp = e / HP, q = e - comp * HP, hx = q % HX, hy = q / HX;
-        lcomp[j] = e < 3 * HP ? comp : -1;
-        loff[j] = (unsigned)ec_wrap_any(x0 + hx - 2, n) + nn * (unsigned)ec_wrap_any(y0 + hy - 2, n);
+    for (int jj = 0; jj < NLC; jj++) {
+        const int q = min(tid + 256 * jj, HP - 1), hx = q % HX, hy = q / HX;
+        loff[jj] = ((unsigned)ec_wrap_any(x0 + hx - 2, n) + nn * (unsigned)ec_wrap_any(y0 + hy - 2, n)) *
+                   (unsigned)sizeof(T);
     }
     auto load = [&](int k, T *r) {
-        const unsigned kk = (unsigned)ec_wrap_any(k, n) * n2;
+        const size_t kk = (size_t)ec_wrap_any(k, n) * n2;
+        const char *b1 = (const char *)(u1 + kk), *b2 = (const char *)(u2 + kk), *b3 = (const char *)(u3 + kk);
 #pragma unroll
-        for (int j = 0; j < NLD; j++) {
-            const T *sp = lcomp[j] == 1 ? u2 : (lcomp[j] == 2 ? u3 : u1);
-            r[j] = sp[loff[j] + kk];
+        for (int jj = 0; jj < NLC; jj++) {
+            r[jj] = *(const T *)(b1 + loff[jj]);
+            r[NLC + jj] = *(const T *)(b2 + loff[jj]);
+            r[2 * NLC + jj] = *(const T *)(b3 + loff[jj]);
         }
     };
     auto store = [&](int slot, const T *r) {
 #pragma unroll
-        for (int j = 0; j < NLD; j++)
-            if (lcomp[j] >= 0) ring[slot][tid + 256 * j] = r[j];
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int jj = 0; jj < NLC; jj++)
+                if (HP % 256 == 0 || tid + 256 * jj < HP) ring[slot][c * HP + tid + 256 * jj] = r[c * NLC + jj];
     };
     {
         T r[NLD];
@@ -363,11 +379,17 @@ __global__ void __launch_bounds__(256) k_rhs_div_march(int n, int ZC, T ih, T vi
     const bool hrow = wv > 0 && ln < RPW && ridx < TX;
     const int hcx = wv == 0 ? -1 : ridx, hcy = wv == 0 ? ln : -1;
     T a3prev[CPT];
+    // outputs: byte offset of cell m within the tile's part of a z-plane
+    // (loop invariant); the plane base is uniform
+    unsigned soff[CPT];
+#pragma unroll
+    for (int m = 0; m < CPT; m++) soff[m] = (unsigned)(cx + (int)nn * (cy0 + m * (256 / TX))) * (unsigned)sizeof(T);
     // plane k (ring index q = rr + 2): plane k + 2 arrives in `cur` (loaded one
     // plane earlier) and goes to the ring; plane k + 3 is loaded into `ahead`,
     // so two planes of loads are in flight while a plane is evaluated
     auto plane = [&](int rr, T *cur, T *ahead) {
         const int k = z0 + rr, q = rr + 2;
+        const size_t pk = n2 * (size_t)k + (size_t)x0 + (size_t)nn * (size_t)y0;  // tile corner in plane k
         if (rr + 2 < ZC) load(k + 3, ahead);
         const T *pm = ring[(q - 1) & 3], *p0 = ring[q & 3], *pp = ring[(q + 1) & 3];
         T a1[CPT], a2[CPT], a3[CPT];
@@ -381,12 +403,11 @@ __global__ void __launch_bounds__(256) k_rhs_div_march(int n, int ZC, T ih, T vi
 #pragma unroll
             for (int m = 0; m < CPT; m++) {
                 const int cy = cy0 + m * (256 / TX);
-                const size_t I = (size_t)(x0 + cx) + (size_t)nn * ((size_t)(y0 + cy) + (size_t)nn * (size_t)k);
                 // streaming stores: the outputs are not re-read here, and keeping
                 // them out of L2 leaves it to the halo lines the neighbouring tiles share
-                __builtin_nontemporal_store(a1[m], du1 + I);
-                __builtin_nontemporal_store(a2[m], du2 + I);
-                __builtin_nontemporal_store(a3[m], du3 + I);
+                __builtin_nontemporal_store(a1[m], (T *)((char *)(du1 + pk) + soff[m]));
+                __builtin_nontemporal_store(a2[m], (T *)((char *)(du2 + pk) + soff[m]));
+                __builtin_nontemporal_store(a3[m], (T *)((char *)(du3 + pk) + soff[m]));
                 dx1[buf][cy * (TX + 1) + cx + 1] = a1[m];
                 dx2[buf][(cy + 1) * TX + cx] = a2[m];
             }
@@ -408,10 +429,9 @@ __global__ void __launch_bounds__(256) k_rhs_div_march(int n, int ZC, T ih, T vi
 #pragma unroll
             for (int m = 0; m < CPT; m++) {
                 const int cy = cy0 + m * (256 / TX);
-                const size_t I = (size_t)(x0 + cx) + (size_t)nn * ((size_t)(y0 + cy) + (size_t)nn * (size_t)k);
                 __builtin_nontemporal_store((a1[m] - dx1[buf][cy * (TX + 1) + cx]) * ih +
                                                 (a2[m] - dx2[buf][cy * TX + cx]) * ih + (a3[m] - a3prev[m]) * ih,
-                                            dout + I);
+                                            (T *)((char *)(dout + pk) + soff[m]));
             }
         }
 #pragma unroll
