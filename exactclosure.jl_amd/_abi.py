@@ -72,6 +72,7 @@ _SIGS = {
     "ec_ns_tensordivergence": (c_int, [c_void_p, c_int, c_int, c_double, P6, P3, c_int]),
     "ec_ns_tensordivergence_nonsym": (c_int, [c_void_p, c_int, c_int, c_double, c_int, P9, P3, c_int]),
     "ec_ns_rhs": (c_int, [c_void_p, c_int, c_int, c_double, c_double, P3, P3, c_int]),
+    "ec_ns_rhs_add": (c_int, [c_void_p, c_int, c_int, c_double, c_double, P3, P3, c_double, P3]),
     "ec_ns_rhs_div": (c_int, [c_void_p, c_int, c_int, c_double, c_double, P3, P3, c_void_p]),
     "ec_poisson_solve": (c_int, [c_void_p, c_void_p]),
     "ec_ns_divergence": (c_int, [c_void_p, c_int, c_int, c_double, P3, c_void_p]),

@@ -191,7 +191,6 @@ class LearnedLES:
         c = self.drv.ctx
         gl = self.g
         u = self.u0 if st == 0 else self.us
-        c.call("ec_ns_rhs", self.dt_code, gl.n, gl.l, self.drv.visc, ptrs(u, P3), ptrs(self.du, P3), 0)
         if self.model is not None:
             u64 = [t.double().contiguous() for t in u]
             m = self.model.predict(u64)
@@ -199,9 +198,11 @@ class LearnedLES:
             if self.tol is not None and closure_error(m, self.L.m) > self.tol:
                 m = self.L.m
                 self.fallbacks += 1
-            for a in range(3):
-                c.call("ec_axpy2", self.dt_code, self.du[a].numel(), 1.0, _abi.EC_F64,
-                       _abi.ptr(m[a]), _abi.ptr(self.du[a]))
+            # du = F_H(u) + m in one pass
+            c.call("ec_ns_rhs_add", self.dt_code, gl.n, gl.l, self.drv.visc, ptrs(u, P3), ptrs(self.du, P3), 1.0,
+                   ptrs(m, P3))
+        else:
+            c.call("ec_ns_rhs", self.dt_code, gl.n, gl.l, self.drv.visc, ptrs(u, P3), ptrs(self.du, P3), 0)
         c.call("ec_ns_pressure_solve", self.ps.h, ptrs(self.du, P3))
         c.call("ec_ns_rk4_update", self.dt_code, gl.n, gl.l, st, dt, ptrs(self.u0, P3), ptrs(self.us, P3),
                ptrs(self.du, P3), _abi.c_void_p(self.ps.p_ptr), ptrs(self.acc, P3))

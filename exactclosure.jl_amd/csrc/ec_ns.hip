@@ -121,10 +121,18 @@ __global__ void k_tensordivergence_nonsym(int n, T ih, Ptr9 sp, T *__restrict__ 
 
 // Fused stresstensor + tensordivergence!: 27 neighbour loads, 13 stresses,
 // nothing materialised.
+// Output modes of the RHS kernels (doadd): 0 writes du = F(u); 1 adds, du +=
+// F(u); 2 writes du = s F(u) + add with an FP64 addend, rounded once to T (the
+// arithmetic of an FP64 axpy applied to the written F, in the same pass).
+struct RhsAdd {
+    double s;
+    const double *a1, *a2, *a3;
+};
+
 template <typename T>
 __global__ void k_rhs(int n, T ih, T visc, const T *__restrict__ u1, const T *__restrict__ u2,
                       const T *__restrict__ u3, T *__restrict__ du1, T *__restrict__ du2,
-                      T *__restrict__ du3, int doadd) {
+                      T *__restrict__ du3, int doadd, RhsAdd ad) {
     Cell c;
     if (!cell_of(n, c)) return;
     const int i = c.i, j = c.j, k = c.k;
@@ -158,12 +166,17 @@ __global__ void k_rhs(int n, T ih, T visc, const T *__restrict__ u1, const T *__
     const T s23_zm = s_off(u2_0, u2_zm, u3_yp_zm, u3_zm, visc, ih);
     const T s23_ym = s_off(u2_ym_zp, u2_ym, u3_0, u3_ym, visc, ih);
 
-    T a1 = doadd ? du1[I] : T(0);
+    T a1 = doadd == 1 ? du1[I] : T(0);
     a1 = a1 - (s11_p - s11_0) * ih - (s12_0 - s12_ym) * ih - (s31_0 - s31_zm) * ih;
-    T a2 = doadd ? du2[I] : T(0);
+    T a2 = doadd == 1 ? du2[I] : T(0);
     a2 = a2 - (s12_0 - s12_xm) * ih - (s22_p - s22_0) * ih - (s23_0 - s23_zm) * ih;
-    T a3 = doadd ? du3[I] : T(0);
+    T a3 = doadd == 1 ? du3[I] : T(0);
     a3 = a3 - (s31_0 - s31_xm) * ih - (s23_0 - s23_ym) * ih - (s33_p - s33_0) * ih;
+    if (doadd == 2) {
+        a1 = (T)(ad.s * (double)a1 + ad.a1[I]);
+        a2 = (T)(ad.s * (double)a2 + ad.a2[I]);
+        a3 = (T)(ad.s * (double)a3 + ad.a3[I]);
+    }
     du1[I] = a1;
     du2[I] = a2;
     du3[I] = a3;
@@ -226,7 +239,7 @@ template <typename T, int TX, int TY>
 __global__ void __launch_bounds__(256) k_rhs_march(int n, int ZC, T ih, T visc, const T *__restrict__ u1,
                                                    const T *__restrict__ u2, const T *__restrict__ u3,
                                                    T *__restrict__ du1, T *__restrict__ du2, T *__restrict__ du3,
-                                                   int doadd) {
+                                                   int doadd, RhsAdd ad) {
     constexpr int HX = TX + 2, HY = TY + 2, HP = HX * HY;
     constexpr int NLC = (HP + 255) / 256, NLD = 3 * NLC;  // loads per component and plane, all components
     constexpr int CPT = TX * TY / 256;  // cells per thread and plane
@@ -288,10 +301,14 @@ __global__ void __launch_bounds__(256) k_rhs_march(int n, int ZC, T ih, T visc, 
             T a1, a2, a3;
             rhs_cell(pm, p0, pp, o, HX, HP, ih, visc, a1, a2, a3);
             const size_t I = (size_t)(x0 + cx) + (size_t)nn * (size_t)(y0 + cy) + n2 * (size_t)k;
-            if (doadd) {
+            if (doadd == 1) {
                 a1 += du1[I];
                 a2 += du2[I];
                 a3 += du3[I];
+            } else if (doadd == 2) {
+                a1 = (T)(ad.s * (double)a1 + ad.a1[I]);
+                a2 = (T)(ad.s * (double)a2 + ad.a2[I]);
+                a3 = (T)(ad.s * (double)a3 + ad.a3[I]);
             }
             du1[I] = a1;
             du2[I] = a2;
@@ -593,14 +610,12 @@ extern "C" int ec_ns_tensordivergence_nonsym(ec_ctx *ctx, int dtype, int n, doub
     return EC_OK;
 }
 
-extern "C" int ec_ns_rhs(ec_ctx *ctx, int dtype, int n, double l, double visc, void *const u[3],
-                         void *const du[3], int doadd) {
-    EC_REQUIRE(ctx, ctx && u && du, "ec_ns_rhs: null argument");
-    EC_CHECK_FIELD(ctx, n);
+static int rhs_launch(ec_ctx *ctx, int dtype, int n, double l, double visc, void *const u[3], void *const du[3],
+                      int doadd, const RhsAdd &ad) {
     Grid3 g = grid3(n);
     const double ih = n / l;
     const double b = dtype == EC_F64 ? 8 : 4;
-    ec_prof_scope ps(ctx, "ns_rhs", (doadd ? 9.0 : 6.0) * ec_cube(n) * b);
+    ec_prof_scope ps(ctx, "ns_rhs", (doadd == 1 ? 9.0 * b : (doadd == 2 ? 6.0 * b + 24.0 : 6.0 * b)) * ec_cube(n));
     if (march_ok(n)) {
         // z-marching tiles of 64 x 8 columns
         const int ZC = march_zc(n);
@@ -608,24 +623,39 @@ extern "C" int ec_ns_rhs(ec_ctx *ctx, int dtype, int n, double l, double visc, v
         if (dtype == EC_F64)
             k_rhs_march<double, 64, 8><<<tiles, 256, 0, ctx->stream>>>(
                 n, ZC, ih, visc, (const double *)u[0], (const double *)u[1], (const double *)u[2], (double *)du[0],
-                (double *)du[1], (double *)du[2], doadd);
+                (double *)du[1], (double *)du[2], doadd, ad);
         else
             k_rhs_march<float, 64, 8><<<tiles, 256, 0, ctx->stream>>>(
                 n, ZC, (float)ih, (float)visc, (const float *)u[0], (const float *)u[1], (const float *)u[2],
-                (float *)du[0], (float *)du[1], (float *)du[2], doadd);
+                (float *)du[0], (float *)du[1], (float *)du[2], doadd, ad);
         EC_CHECK_LAUNCH(ctx, "k_rhs_march");
         return EC_OK;
     }
     if (dtype == EC_F64)
         k_rhs<double><<<g.grid, g.block, 0, ctx->stream>>>(n, ih, visc, (const double *)u[0], (const double *)u[1],
                                                            (const double *)u[2], (double *)du[0], (double *)du[1],
-                                                           (double *)du[2], doadd);
+                                                           (double *)du[2], doadd, ad);
     else
         k_rhs<float><<<g.grid, g.block, 0, ctx->stream>>>(n, (float)ih, (float)visc, (const float *)u[0],
                                                           (const float *)u[1], (const float *)u[2], (float *)du[0],
-                                                          (float *)du[1], (float *)du[2], doadd);
+                                                          (float *)du[1], (float *)du[2], doadd, ad);
     EC_CHECK_LAUNCH(ctx, "k_rhs");
     return EC_OK;
+}
+
+extern "C" int ec_ns_rhs(ec_ctx *ctx, int dtype, int n, double l, double visc, void *const u[3],
+                         void *const du[3], int doadd) {
+    EC_REQUIRE(ctx, ctx && u && du, "ec_ns_rhs: null argument");
+    EC_CHECK_FIELD(ctx, n);
+    return rhs_launch(ctx, dtype, n, l, visc, u, du, doadd ? 1 : 0, RhsAdd{1.0, nullptr, nullptr, nullptr});
+}
+
+extern "C" int ec_ns_rhs_add(ec_ctx *ctx, int dtype, int n, double l, double visc, void *const u[3],
+                             void *const du[3], double s, const void *const add[3]) {
+    EC_REQUIRE(ctx, ctx && u && du && add && add[0] && add[1] && add[2], "ec_ns_rhs_add: null argument");
+    EC_CHECK_FIELD(ctx, n);
+    return rhs_launch(ctx, dtype, n, l, visc, u, du, 2,
+                      RhsAdd{s, (const double *)add[0], (const double *)add[1], (const double *)add[2]});
 }
 
 extern "C" int ec_ns_rhs_div(ec_ctx *ctx, int dtype, int n, double l, double visc, void *const u[3],

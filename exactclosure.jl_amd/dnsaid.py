@@ -186,10 +186,9 @@ class DNSAidedLES:
         gl = L.g
         v = L.v0 if st == 0 else L.vs
         c.call("ec_ns_project_into", L.ps64.h, ptrs(v, P3), ptrs(L.ubar, P3))
-        # m = V k - F_H(P_H v): F_H into m, then m = g_k - m in one pass
-        c.call("ec_ns_rhs", _abi.EC_F64, gl.n, gl.l, self.visc, ptrs(L.ubar, P3), ptrs(L.m, P3), 0)
-        for a in range(3):
-            c.call("ec_axpby", _abi.EC_F64, L.m[a].numel(), 1.0, ptr(L.g_k[a]), -1.0, ptr(L.m[a]))
+        # m = V k - F_H(P_H v) = -F_H(ubar) + g_k, formed in the RHS pass
+        c.call("ec_ns_rhs_add", _abi.EC_F64, gl.n, gl.l, self.visc, ptrs(L.ubar, P3), ptrs(L.m, P3), -1.0,
+               ptrs(L.g_k, P3))
         return L.m
 
     def _step_forcing(self, dt):
@@ -222,9 +221,9 @@ class DNSAidedLES:
             gl = L.g
             m = self._les_forcing(L, st, c)
             ul = L.u0 if st == 0 else L.us
-            c.call("ec_ns_rhs", self.dt_code, gl.n, gl.l, self.visc, ptrs(ul, P3), ptrs(L.du, P3), 0)
-            for a in range(3):
-                c.call("ec_axpy2", self.dt_code, L.du[a].numel(), 1.0, _abi.EC_F64, ptr(m[a]), ptr(L.du[a]))
+            # du = F_H(u) + m in one pass (the sum in FP64, rounded once)
+            c.call("ec_ns_rhs_add", self.dt_code, gl.n, gl.l, self.visc, ptrs(ul, P3), ptrs(L.du, P3), 1.0,
+                   ptrs(m, P3))
             c.call("ec_ns_pressure_solve", L.ps.h, ptrs(L.du, P3))
             c.call("ec_ns_rk4_update", self.dt_code, gl.n, gl.l, st, dt, ptrs(L.u0, P3), ptrs(L.us, P3),
                    ptrs(L.du, P3), _abi.c_void_p(L.ps.p_ptr), ptrs(L.acc, P3))

@@ -92,6 +92,13 @@ int ec_ns_tensordivergence_nonsym(ec_ctx *ctx, int dtype, int n, double l, int s
  * (src/N.jl:310-311, 1306-1307) */
 int ec_ns_rhs(ec_ctx *ctx, int dtype, int n, double l, double visc, void *const u[3],
               void *const du[3], int doadd);
+/* No counterpart in the reference (it forms the RHS and then adds a forcing in
+ * a second broadcast): du = s * (-div sigma(u)) + add in one pass, add three
+ * FP64 fields, the sum formed in FP64 and rounded once to dtype.  Used for the
+ * exact forcing m = V k - F_H(P_H v) (s = -1, add = V k) and for the LES
+ * right-hand side F_H(u) + m (s = 1, add = m). */
+int ec_ns_rhs_add(ec_ctx *ctx, int dtype, int n, double l, double visc, void *const u[3],
+                  void *const du[3], double s, const void *const add[3]);
 /* ec_ns_rhs (doadd 0) and divergence!(d, du) fused: du = -div sigma(u) and
  * d = div_h du, the right-hand side of the pressure Poisson equation of
  * project!(du) (src/N.jl:256-262).  d is typically ec_poisson_pressure(ps),

@@ -124,6 +124,43 @@ def test_rhs_div_large_fp32(gpu, n):
     torch.cuda.empty_cache()
 
 
+@pytest.mark.parametrize("n", [16, 256])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("s", [1.0, -1.0])
+def test_rhs_add(gpu, O, n, prec, s):
+    """du = s F(u) + add (FP64 addend, one rounding) on the plain kernel (16)
+    and the marching kernel (256): against the checker's s rhs + add, and
+    bitwise against the two-pass form it replaces (the RHS written, then an
+    FP64 axpy of the addend: ec_axpby for FP64 fields, ec_axpy2 for FP32)."""
+    dt = torch.float64 if prec == "f64" else torch.float32
+    code = 1 if prec == "f64" else 0
+    u = rand_u(O, n, 5 * n + 3)
+    add = rand_u(O, n, 5 * n + 4)
+    visc, l = 0.012, 1.0
+    ref = O.rhs(u, visc, l)
+    ud = [dev(x, dt) for x in u]
+    ad = [dev(x) for x in add]
+    du = [torch.empty_like(x) for x in ud]
+    ctx = gpu._abi.context()
+    ctx.call("ec_ns_rhs_add", code, n, l, visc, P(ud, 3), P(du, 3), s, P(ad, 3))
+    tol = 1e-12 if prec == "f64" else 2e-5
+    for a in range(3):
+        want = s * ref[a] + add[a]
+        assert rel(host(du[a]), want) < tol, a
+    two = [torch.empty_like(x) for x in ud]
+    ctx.call("ec_ns_rhs", code, n, l, visc, P(ud, 3), P(two, 3), 0)
+    for a in range(3):
+        if prec == "f64":
+            ctx.call("ec_axpby", 1, two[a].numel(), 1.0, ad[a].data_ptr(), s, two[a].data_ptr())
+        else:
+            if s != 1.0:
+                two[a].neg_()  # exact
+            ctx.call("ec_axpy2", 0, two[a].numel(), 1.0, 1, ad[a].data_ptr(), two[a].data_ptr())
+    torch.cuda.synchronize()
+    for a in range(3):
+        assert torch.equal(du[a], two[a]), a
+
+
 @pytest.mark.parametrize("n", [16, 32, 256])
 def test_rhs_fp32(gpu, O, n):
     u = rand_u(O, n, 100 + n)
