@@ -209,6 +209,22 @@ def main():
     closure_err = None
     les_err = drv.les_errors()
     div = drv.divergence_norm()
+    # energy spectrum of the final DNS state (spectral_stuff + spectrum,
+    # src/N.jl:547-590): three forward FFTs and the FP64 shell sums; outside
+    # the timed steps (the entry point synchronises to return the sums)
+    stuff = NS.spectral_stuff(g)
+    uf = tuple(NS.Field(g, x) for x in drv.u0)
+    spec = NS.spectrum(uf, stuff, drv.ps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        spec = NS.spectrum(uf, stuff, drv.ps)
+    t_spec = (time.perf_counter() - t0) / 3
+    bsz = 4 if a.dtype == "f32" else 8
+    spec_bytes = 3 * (bsz * n ** 3 + 6 * 2 * bsz * (n // 2 + 1) * n * n)
+    spectrum_diag = {"ms": 1e3 * t_spec, "gbs": spec_bytes / t_spec / 1e9, "shells": len(stuff["shells"]),
+                     "energy_shells_1_to_kcut": sum(spec["s"])}
+    del uf
     if not a.no_closure_check:
         taus32 = drv.closure_terms()
         u_state = [x.double() for x in drv.u0]
@@ -251,6 +267,7 @@ def main():
         "closure_relerr_vs_fp64": closure_err,
         "divergence": div,
         "les_error_vs_filtered_dns": les_err,
+        "spectrum": spectrum_diag,
         "roofline": {"bound": "hbm", "kernel": dname, "achieved": d_achieved, "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": d_achieved / PEAK_HBM_GBS, "launches": dcount,
                      "avg_ms": d_avg_ms, "bytes_per_launch": dbytes / dcount,

@@ -1269,3 +1269,44 @@ extern "C" int ec_ns_randomfield(ec_ctx *ctx, ec_poisson *ps, int profile, doubl
     return ps->dtype == EC_F64 ? randomfield_t<double>(ctx, ps, profile, param, totalenergy, seed, u)
                                : randomfield_t<float>(ctx, ps, profile, param, totalenergy, seed, u);
 }
+
+// Kinetic-energy spectrum over integer shells (spectrum, src/N.jl:576-590, with
+// the shells of getshells, src/N.jl:478-545): out[s] = sum over half-spectrum
+// bins with s^2 <= |k|^2 < (s+1)^2 of w(kx) |u^_a(k) / n^3|^2 / 2, summed over
+// the three components; w = 2 for 1 <= kx <= n/2 - 1 (their conjugate bins are
+// not stored), 1 otherwise.  The same shell histogram as the initial-field
+// rescale, FP64 sums; the transforms use and overwrite the plan's spectrum.
+template <typename T>
+static int spectrum_t(ec_ctx *ctx, ec_poisson *ps, void *const u[3], int nshell, double *out) {
+    const int n = ps->n, nh = ps->nh, kmax = n / 2;
+    const double inv_n3 = 1.0 / ((double)n * n * n);
+    const size_t nspec = (size_t)nh * n * n;
+    const int blocks = (int)std::min<size_t>((nspec + 255) / 256, 4096);
+    double *dhist = nullptr;
+    EC_HIP(ctx, hipMallocAsync((void **)&dhist, sizeof(double) * nshell, ctx->stream));
+    int rc = EC_OK;
+    if (hipMemsetAsync(dhist, 0, sizeof(double) * nshell, ctx->stream) != hipSuccess)
+        rc = ec_fail(ctx, EC_E_HIP, "ec_ns_spectrum: clearing the shell sums failed");
+    for (int c = 0; c < 3 && rc == EC_OK; c++) {
+        rc = ec_fft_r2c(ctx, ps, u[c], ps->spec);
+        if (rc == EC_OK) {
+            k_shell_hist<T><<<blocks, 256, sizeof(double) * nshell, ctx->stream>>>(
+                (const cplx<T> *)ps->spec, n, nh, ps->pitch, kmax, nshell, inv_n3, dhist);
+            if (hipGetLastError() != hipSuccess) rc = ec_fail(ctx, EC_E_HIP, "ec_ns_spectrum: k_shell_hist launch");
+        }
+    }
+    if (rc == EC_OK &&
+        (hipMemcpyAsync(out, dhist, sizeof(double) * nshell, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+         hipStreamSynchronize(ctx->stream) != hipSuccess))
+        rc = ec_fail(ctx, EC_E_HIP, "ec_ns_spectrum: reading the shell sums failed");
+    const hipError_t fe = hipFreeAsync(dhist, ctx->stream);
+    if (rc == EC_OK && fe != hipSuccess) rc = ec_fail(ctx, EC_E_HIP, "ec_ns_spectrum: free failed");
+    return rc;
+}
+
+extern "C" int ec_ns_spectrum(ec_ctx *ctx, ec_poisson *ps, void *const u[3], int nshell, double *out) {
+    EC_REQUIRE(ctx, ctx && ps && u && u[0] && u[1] && u[2] && out, "ec_ns_spectrum: null argument");
+    EC_REQUIRE(ctx, nshell >= 1 && nshell <= 8192, "ec_ns_spectrum: nshell %d outside [1, 8192]", nshell);
+    return ps->dtype == EC_F64 ? spectrum_t<double>(ctx, ps, u, nshell, out)
+                               : spectrum_t<float>(ctx, ps, u, nshell, out);
+}

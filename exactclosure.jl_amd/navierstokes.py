@@ -12,6 +12,7 @@ Layout: a ``Field`` holds a contiguous tensor of shape (n, n, n) indexed
 """
 from __future__ import annotations
 
+import ctypes
 import math
 from dataclasses import dataclass
 
@@ -275,6 +276,34 @@ def randomfield(grid, poisson, profile="peak", param=5.0, totalenergy=0.5, seed=
     _ctx().call("ec_ns_randomfield", poisson.h, pid, param, totalenergy, seed,
                 ptrs([x.data for x in u], P3))
     return u
+
+
+def spectral_stuff(grid, npoint=None):
+    """spectral_stuff (src/N.jl:547-574): the output shells 1..kcut, kcut =
+    div(2 kmax, 3), kmax = div(n, 2) (or npoint log-spaced integer shells of
+    that range), and their wavenumbers 2 pi / l * shell.  The shells
+    themselves (getshells, src/N.jl:478-545) are formed on the device by
+    ec_ns_spectrum."""
+    kmax = grid.n // 2
+    kcut = (2 * kmax) // 3
+    if npoint is None:
+        kuse = list(range(1, kcut + 1))
+    else:
+        kuse = sorted(set(int(round(x)) for x in np.geomspace(1.0, float(kcut), npoint)))
+    return {"shells": kuse, "k": [2 * np.pi / grid.l * s for s in kuse]}
+
+
+def spectrum(u, stuff, poisson):
+    """spectrum(u, stuff, poisson) (src/N.jl:576-590): (k, s) with s[i] the
+    kinetic energy of shell stuff["shells"][i], i.e. the sum of |u^/n^3|^2 / 2
+    over the wavenumbers with shell^2 <= |k|^2 < (shell + 1)^2, conjugate
+    half included, summed over the components (FP64).  Overwrites the
+    solver's spectrum buffer, as the reference overwrites poisson.phat."""
+    shells = stuff["shells"]
+    nshell = max(shells) + 1
+    out = (ctypes.c_double * nshell)()
+    _ctx().call("ec_ns_spectrum", poisson.h, ptrs([x.data for x in u], P3), nshell, out)
+    return {"k": list(stuff["k"]), "s": [out[i] for i in shells]}
 
 
 def step_forwardeuler_(u, du, poisson, visc, dt):

@@ -147,6 +147,13 @@ int ec_reduce_diffsq(ec_ctx *ctx, int dtype, size_t count, const void *x, const 
  * profile 1: (k > 0) k^param (src/N.jl:1245).  Counter-based normal variates. */
 int ec_ns_randomfield(ec_ctx *ctx, ec_poisson *ps, int profile, double param,
                       double totalenergy, uint64_t seed, void *const u[3]);
+/* spectrum(u, stuff, poisson) (src/N.jl:576-590) over the shells of getshells
+ * (src/N.jl:478-545): out[s] = sum over s^2 <= |k|^2 < (s+1)^2 of
+ * |u^(k) / n^3|^2 / 2 summed over the components (the conjugate half of the
+ * rfft counted), s in [0, nshell), 1 <= nshell <= 8192.  The host mirror picks
+ * the shells 1..n/3 of spectral_stuff.  Uses (overwrites) the plan's spectrum;
+ * synchronises the context's stream. */
+int ec_ns_spectrum(ec_ctx *ctx, ec_poisson *ps, void *const u[3], int nshell, double *out);
 int ec_randn(ec_ctx *ctx, int dtype, size_t count, uint64_t seed, uint64_t offset, void *out);
 
 /* ---- separable filters and closure (src/N.jl:592-698, 999-1135) ----

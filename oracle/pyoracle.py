@@ -160,6 +160,33 @@ def irfft3(X):
     return out
 
 
+def spectrum_shells(u, nshell):
+    """Energy per integer shell, FP64: getshells (src/N.jl:478-545) and
+    spectrum (src/N.jl:576-590).  Shell s holds the half-spectrum bins with
+    s^2 <= k1^2 + k2^2 + k3^2 < (s+1)^2 (k1 = x index 0..n/2, k2, k3 from
+    wavenumber_int, src/N.jl:32); each bin contributes |u^ / n^3|^2 / 2 per
+    component, and the bins 1 <= k1 <= n/2 - 1 count twice (the "conjinds" of
+    getshells: their conjugates are not stored by the rfft).  Returns shells
+    0..nshell-1."""
+    n = u[0].shape[0]
+    kmax = n // 2
+    k1 = np.arange(kmax + 1)
+    kw = np.array([i if i <= (n - 1) // 2 else i - n for i in range(n)])
+    kk = k1[:, None, None] ** 2 + kw[None, :, None] ** 2 + kw[None, None, :] ** 2
+    shell = np.floor(np.sqrt(kk.astype(np.float64))).astype(np.int64)
+    # integer correction of the float square root
+    shell[(shell + 1) ** 2 <= kk] += 1
+    shell[shell ** 2 > kk] -= 1
+    w = np.where((k1 >= 1) & (k1 <= kmax - 1), 2.0, 1.0)[:, None, None]
+    e = np.zeros(kk.shape)
+    for x in u:
+        X = rfft3(x) / float(n) ** 3
+        e += np.abs(X) ** 2 / 2
+    e *= w
+    keep = shell < nshell
+    return np.bincount(shell[keep].ravel(), weights=e[keep].ravel(), minlength=nshell)[:nshell]
+
+
 def fft1d(x, sign=-1):
     x = np.array(x, dtype=np.complex128)
     lib().or_fft1d(len(x), sign, _p(x))

@@ -454,6 +454,70 @@ def test_axpby(gpu, prec):
     assert float((y.double() - want).abs().max()) <= (1e-15 if prec == "f64" else 1e-6) * float(want.abs().max())
 
 
+@pytest.mark.parametrize("n,prec", [(16, "f64"), (32, "f64"), (24, "f64"), (64, "f32")])
+def test_spectrum(gpu, O, n, prec):
+    """spectrum(u, spectral_stuff(grid), poisson) (src/N.jl:547-590) against the
+    checker's shell sums on the same field: the shells 1..n/3 and their
+    wavenumbers 2 pi / l * shell; all shells through the C entry point."""
+    NS = gpu.navierstokes
+    dt = torch.float64 if prec == "f64" else torch.float32
+    l = 2.0
+    g = NS.Grid(l, n)
+    ps = NS.PoissonSolver(g, dt)
+    u = rand_u(O, n, 9 * n)
+    uf = tuple(NS.Field(g, dev(x, dt)) for x in u)
+    stuff = NS.spectral_stuff(g)
+    assert stuff["shells"] == list(range(1, (2 * (n // 2)) // 3 + 1))
+    sp = NS.spectrum(uf, stuff, ps)
+    nshell = int(np.floor(np.sqrt(3) * (n // 2))) + 1
+    ref = O.spectrum_shells(u, nshell)
+    tol = 1e-12 if prec == "f64" else 1e-5
+    for k, e, sh in zip(sp["k"], sp["s"], stuff["shells"]):
+        assert abs(k - 2 * np.pi / l * sh) < 1e-12
+        assert abs(e - ref[sh]) <= tol * ref.max(), (sh, e, ref[sh])
+    import ctypes
+    out = (ctypes.c_double * nshell)()
+    gpu._abi.context().call("ec_ns_spectrum", ps.h, P([x.data for x in uf], 3), nshell, out)
+    got = np.array(out[:])
+    assert np.abs(got - ref).max() <= tol * ref.max()
+    logs = NS.spectral_stuff(g, npoint=5)["shells"]
+    assert logs[0] == 1 and logs[-1] == (2 * (n // 2)) // 3 and logs == sorted(set(logs))
+
+
+@pytest.mark.parametrize("n", [512, 1024])
+def test_spectrum_large_fp32(gpu, n):
+    """Full-size FP32: all shells together hold the kinetic energy mean(|u|^2)/2
+    (Parseval, FP64 sums on both sides, 1e-5); a single mode cos(2 pi (a i +
+    b j + c k)/n) of amplitude A put in one component lands whole in shell
+    floor(|k|) with energy A^2/4."""
+    NS = gpu.navierstokes
+    g = NS.Grid(1.0, n)
+    ps = NS.PoissonSolver(g, torch.float32)
+    gen = torch.Generator(device="cuda").manual_seed(n + 5)
+    u = tuple(NS.Field(g, torch.randn((n, n, n), device="cuda", generator=gen)) for _ in range(3))
+    import ctypes
+    nshell = int(np.floor(np.sqrt(3) * (n // 2))) + 1
+    out = (ctypes.c_double * nshell)()
+    ctx = gpu._abi.context()
+    ctx.call("ec_ns_spectrum", ps.h, P([x.data for x in u], 3), nshell, out)
+    energy = sum(float((x.data.double() ** 2).sum()) for x in u) / 2 / n ** 3
+    assert abs(sum(out[:]) - energy) <= 1e-5 * energy
+    ar = torch.arange(n, device="cuda", dtype=torch.float64)
+    a, b, c, amp = 7, n - 5, 3, 1.5  # signed wavenumbers (7, -5, 3)
+    w = amp * torch.cos((2 * np.pi / n) * ((a * ar).view(1, 1, n) + (b * ar).view(1, n, 1) + (c * ar).view(n, 1, 1))
+                        + 0.4)  # [k, j, i], i fastest
+    z = torch.zeros((n, n, n), device="cuda")
+    one = (NS.Field(g, z), NS.Field(g, w.float()), NS.Field(g, z))
+    del w
+    ctx.call("ec_ns_spectrum", ps.h, P([x.data for x in one], 3), nshell, out)
+    sh = int(np.floor(np.sqrt(7 ** 2 + 5 ** 2 + 3 ** 2)))
+    got = np.array(out[:])
+    assert abs(got[sh] - amp ** 2 / 4) <= 1e-5 * amp ** 2 / 4
+    assert np.abs(np.delete(got, sh)).max() <= 1e-9
+    del u, one, z
+    torch.cuda.empty_cache()
+
+
 def test_invalid_arguments_raise(gpu):
     NS = gpu.navierstokes
     ctx = gpu._abi.context()
@@ -462,3 +526,9 @@ def test_invalid_arguments_raise(gpu):
     with pytest.raises(gpu._abi.ECError):
         P3 = gpu._abi.P3
         ctx.call("ec_ns_rk4_update", 1, 8, 1.0, 5, 0.1, P3(), P3(), P3(), None, P3())
+    with pytest.raises(gpu._abi.ECError):
+        import ctypes
+        g8 = NS.Grid(1.0, 8)
+        ps8 = NS.PoissonSolver(g8, torch.float64)
+        u8 = NS.vectorfield(g8)
+        ctx.call("ec_ns_spectrum", ps8.h, P([x.data for x in u8], 3), 0, (ctypes.c_double * 1)())

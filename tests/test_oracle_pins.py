@@ -246,6 +246,58 @@ def test_randomfield_spectrum(O):
         assert abs(spec[k] - target[k]) < 0.02 * target[k] + 1e-6, (k, spec[k], target[k])
 
 
+# ---------------------------------------------------------------- spectrum
+def _isqrt(kk):
+    s = np.floor(np.sqrt(kk.astype(np.float64))).astype(np.int64)
+    s[(s + 1) ** 2 <= kk] += 1
+    s[s ** 2 > kk] -= 1
+    return s
+
+
+@pytest.mark.parametrize("n", [4, 8, 12])
+def test_spectrum_full_fft_and_parseval(O, n):
+    """Shells of the half spectrum with the conjugate bins counted twice
+    (getshells, src/N.jl:478-545) equal the shells of the full complex FFT
+    (numpy.fft.fftn over all signed wavenumbers), and they sum to the kinetic
+    energy mean(|u|^2)/2 (Parseval) -- the identity the getshells comment
+    states for even n (src/N.jl:516-523).  A missing or misplaced conjugate
+    weight, a wrong normalisation or a shell off by one fails here."""
+    u = rnd(n, 40 + n)
+    nshell = int(np.floor(np.sqrt(3) * (n // 2))) + 1
+    s = O.spectrum_shells(u, nshell)
+    kw = np.round(np.fft.fftfreq(n) * n).astype(np.int64)
+    sh = _isqrt(kw[:, None, None] ** 2 + kw[None, :, None] ** 2 + kw[None, None, :] ** 2)
+    full = np.zeros(nshell)
+    for x in u:
+        F = np.fft.fftn(x) / n ** 3
+        np.add.at(full, sh.ravel(), (np.abs(F) ** 2 / 2).ravel())
+    assert np.abs(s - full).max() < 1e-14 * full.max()
+    energy = sum((x ** 2).sum() for x in u) / 2 / n ** 3
+    assert abs(s.sum() - energy) < 1e-13 * energy
+
+
+def test_spectrum_single_modes(O):
+    """amp cos(2 pi (a i + b j + c k)/n + phi) in one component has energy
+    amp^2/4, all of it in shell floor(sqrt(a^2 + b^2 + c^2)); modes with
+    kx = 0 (stored twice in the half spectrum, weight 1) and kx > 0 (stored
+    once, weight 2) must both give it."""
+    n = 16
+    nshell = 14
+    i = np.arange(n)
+    for a, b, c, amp in [(3, 0, 0, 1.0), (0, 2, 5, 0.5), (2, -3, 1, 2.0), (0, 0, 4, 1.5), (5, 5, 5, 1.0)]:
+        ph = 2 * np.pi * (a * i[:, None, None] + b * i[None, :, None] + c * i[None, None, :]) / n + 0.3
+        x = np.asfortranarray(amp * np.cos(ph))
+        z = np.zeros_like(x, order="F")
+        for comp in range(3):
+            u = [z, z, z]
+            u[comp] = x
+            s = O.spectrum_shells(u, nshell)
+            k = int(math.isqrt(a * a + b * b + c * c))
+            assert abs(s[k] - amp ** 2 / 4) < 1e-13, (a, b, c, comp, s[k])
+            rest = np.delete(s, k)
+            assert np.abs(rest).max() < 1e-13
+
+
 def test_randn_moments(O):
     x = O.randn(400000, 5)
     assert abs(x.mean()) < 5e-3 and abs(x.std() - 1) < 5e-3
