@@ -3,7 +3,8 @@
 Modules mirror the reference's call surface:
   NavierStokes  (src/N.jl)        -> navierstokes
   Burgers       (src/Burgers.jl)  -> burgers
-plus dnsaid (the RK4 DNS-aided LES driver) and cnn (the learned closure).  All compute runs in
+plus dnsaid (the RK4 DNS-aided LES driver), cnn (the learned closure) and snapshot
+(velocity fields on disk in the layout of the reference's saved array).  All compute runs in
 libexactclosure.so (hand-written gfx950 HIP kernels behind a C ABI).
 """
 from . import _abi  # noqa: F401
@@ -14,5 +15,6 @@ from . import burgers as Burgers  # noqa: F401
 from . import dnsaid  # noqa: F401
 from . import cnn  # noqa: F401
 from . import autotune  # noqa: F401
+from . import snapshot  # noqa: F401
 
 lib = _abi.lib
