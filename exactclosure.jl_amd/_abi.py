@@ -114,6 +114,9 @@ _SIGS = {
     "ec_burgers_batch_get": (c_int, [c_void_p, c_void_p, c_int, c_int, ctypes.POINTER(c_double), c_int]),
     "ec_burgers_batch_dt": (c_int, [c_void_p, c_int, ctypes.POINTER(c_double)]),
     "ec_burgers_rhs": (c_int, [c_void_p, c_int, c_int, c_double, c_double, c_void_p, c_void_p]),
+    "ec_burgers_fractions": (c_int, [c_void_p, c_int, c_int, c_double, c_double, c_int, c_void_p, c_int,
+                                     ctypes.POINTER(c_double), c_int, ctypes.POINTER(c_double),
+                                     ctypes.POINTER(c_double)]),
     "ec_cnn_create": (c_int, [c_void_p, c_u64, ctypes.POINTER(c_void_p)]),
     "ec_cnn_destroy": (c_int, [c_void_p, c_void_p]),
     "ec_cnn_num_params": (c_int, [c_void_p]),

@@ -46,6 +46,44 @@ def build_doublekernel(F, G):
     return I + J, np.convolve(f, g)
 
 
+def les_kernels(nh, nH, L, delta_ratio, lesfiltertype="tophat"):
+    """The LES kernel and the double kernel of compute_fractions
+    (src/Burgers.jl:1030-1048): Delta = H * delta_ratio; the finite-volume
+    top-hat over comp = nh / nH fine cells; the LES top-hat (width rounded to
+    the nearest odd count of fine cells) or Gaussian; their convolution."""
+    h, H = L / nh, L / nH
+    delta = H * delta_ratio
+    fvm = tophat_weights(nh // nH)
+    if lesfiltertype == "tophat":
+        R = int(np.rint(delta / h)) // 2
+        les = tophat_weights(2 * R + 1)
+    elif lesfiltertype == "gaussian":
+        les = gaussian_weights(h, delta)
+    else:
+        raise ValueError(f"lesfiltertype {lesfiltertype!r}")
+    return les, build_doublekernel(fvm, les)
+
+
+def compute_fractions(u, nH, L=2 * math.pi, visc=5e-4, delta_ratio=1.0, lesfiltertype="tophat"):
+    """compute_fractions for one (nH, delta_ratio) (src/Burgers.jl:1022-1099):
+    u holds DNS snapshots as a (nsamples, nh) FP64 CUDA tensor (one sample per
+    row, the columns of the reference's data matrix); returns the mean
+    fractions {classic, flux, div} of the closure parts, computed on the GPU
+    in FP64."""
+    import torch
+
+    assert u.dtype == torch.float64 and u.is_cuda and u.dim() == 2 and u.is_contiguous()
+    ns, nh = u.shape
+    (Rl, wl), (K, wd) = les_kernels(nh, nH, L, delta_ratio, lesfiltertype)
+    wl = np.ascontiguousarray(wl, dtype=np.float64)
+    wd = np.ascontiguousarray(wd, dtype=np.float64)
+    out = (ctypes.c_double * 3)()
+    dp = ctypes.POINTER(ctypes.c_double)
+    _abi.context().call("ec_burgers_fractions", nh, nH, L, visc, ns, _abi.c_void_p(u.data_ptr()), Rl,
+                        wl.ctypes.data_as(dp), K, wd.ctypes.data_as(dp), out)
+    return {"classic": out[0], "flux": out[1], "div": out[2]}
+
+
 class BurgersBatch:
     """count instances: n[i] points, compression c[i], RK4 with the exact
     closure and a no-model LES, FP32 or FP64 state, FP64 commutator."""

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <map>
+#include <vector>
 
 namespace {
 
@@ -489,5 +490,111 @@ extern "C" int ec_burgers_rhs(ec_ctx *ctx, int dtype, int n, double L, double vi
         k_burgers_rhs<float><<<(n + 255) / 256, 256, 0, ctx->stream>>>(n, (float)ih, (float)visc, (const float *)u,
                                                                        (float *)du);
     EC_CHECK_LAUNCH(ctx, "k_burgers_rhs");
+    return EC_OK;
+}
+
+// ---- closure fractions (compute_fractions, src/Burgers.jl:1022-1099) ----
+// For DNS snapshots u (n points, samples stored one after another) with the
+// LES kernel F = (Rl, wl) and the double kernel D = (K, wd) (the finite-volume
+// top-hat convolved with F, build_doublekernel), per coarse point J:
+//   fsu = C_c(F * s), vsu = C_c(D * s), vu = C_f(D * u), svu = s_H(vu),
+//   sVU = C_c(s_h(D * u)),  s = s_h(u) the DNS stress,
+//   classic = vsu - sVU, flux = sVU - svu, div = fsu - vsu,
+//   f_x = |x| / (|classic| + |flux| + |div| + eps),
+// with C_c sampling fine centre (J+1) comp - comp/2 - 1 and C_f fine face
+// (J+1) comp - 1 (0-based forms of lazycoarsegrain, src/Burgers.jl:174-180).
+// All FP64.  The fine pass forms s and D * u; the coarse pass one block per
+// sample, the three fraction sums reduced in a fixed order.
+__device__ __forceinline__ int bwrap(int i, int n) { return ((i % n) + n) % n; }
+
+__device__ __forceinline__ double bstress(double ua, double ub, double ih, double visc) {
+    return -((ua + ub) * (ua + ub)) / 8 + visc * (ub - ua) * ih;
+}
+
+__global__ void k_bfrac_fine(int n, int ns, double ih, double visc, const double *__restrict__ u, int K,
+                             const double *__restrict__ wd, double *__restrict__ s, double *__restrict__ vu_f) {
+    const long total = (long)n * ns;
+    for (long I = blockIdx.x * (long)blockDim.x + threadIdx.x; I < total; I += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(I % n);
+        const double *us = u + (I - i);
+        s[I] = bstress(us[bwrap(i - 1, n)], us[i], ih, visc);
+        double acc = 0.0;
+        for (int k = -K; k <= K; k++) acc += wd[k + K] * us[bwrap(i + k, n)];
+        vu_f[I] = acc;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_bfrac_coarse(int n, int nH, double ih, double iH, double visc,
+                                                      const double *__restrict__ s, const double *__restrict__ vu_f,
+                                                      int Rl, const double *__restrict__ wl, int K,
+                                                      const double *__restrict__ wd, double *__restrict__ sums) {
+    const int comp = n / nH, a = comp / 2;
+    const double *ss = s + (long)blockIdx.x * n, *vf = vu_f + (long)blockIdx.x * n;
+    double fc = 0.0, ff = 0.0, fd = 0.0;
+    for (int J = threadIdx.x; J < nH; J += blockDim.x) {
+        const int ic = (J + 1) * comp - a - 1;  // fine centre of coarse cell J
+        double fsu = 0.0, vsu = 0.0;
+        for (int r = -Rl; r <= Rl; r++) fsu += wl[r + Rl] * ss[bwrap(ic + r, n)];
+        for (int k = -K; k <= K; k++) vsu += wd[k + K] * ss[bwrap(ic + k, n)];
+        const double vb = vf[(J + 1) * comp - 1], va = vf[bwrap(J * comp - 1, n)];  // coarse faces J, J-1
+        const double svu = -((va + vb) * (va + vb)) / 8 + visc * (vb - va) * iH;
+        const double sVU = bstress(vf[bwrap(ic - 1, n)], vf[ic], ih, visc);
+        const double c = vsu - sVU, f = sVU - svu, d = fsu - vsu;
+        const double tot = fabs(c) + fabs(f) + fabs(d) + 2.220446049250313e-16;
+        fc += fabs(c) / tot;
+        ff += fabs(f) / tot;
+        fd += fabs(d) / tot;
+    }
+    __shared__ double red[3][256];
+    red[0][threadIdx.x] = fc;
+    red[1][threadIdx.x] = ff;
+    red[2][threadIdx.x] = fd;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int x = 0; x < 3; x++) red[x][threadIdx.x] += red[x][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int x = 0; x < 3; x++) sums[3 * blockIdx.x + x] = red[x][0];
+}
+
+extern "C" int ec_burgers_fractions(ec_ctx *ctx, int n, int nH, double L, double visc, int nsamples,
+                                    const void *u, int Rl, const double *wl, int K, const double *wd,
+                                    double out[3]) {
+    EC_REQUIRE(ctx, ctx && u && wl && wd && out, "ec_burgers_fractions: null argument");
+    EC_REQUIRE(ctx, n >= 2 && nH >= 1 && n % nH == 0 && nsamples >= 1 && L > 0,
+               "ec_burgers_fractions: need n %% nH == 0 (n=%d, nH=%d), nsamples=%d >= 1", n, nH, nsamples);
+    EC_REQUIRE(ctx, Rl >= 0 && K >= 0 && 2 * Rl + 1 <= n && 2 * K + 1 <= n,
+               "ec_burgers_fractions: kernel radii Rl=%d, K=%d must fit the grid n=%d", Rl, K, n);
+    const long total = (long)n * nsamples;
+    double *buf = nullptr;
+    const size_t bytes = sizeof(double) * ((size_t)2 * total + (2 * Rl + 1) + (2 * K + 1) + 3 * (size_t)nsamples);
+    EC_HIP(ctx, hipMallocAsync((void **)&buf, bytes, ctx->stream));
+    double *s = buf, *vf = buf + total, *dwl = vf + total, *dwd = dwl + (2 * Rl + 1), *sums = dwd + (2 * K + 1);
+    std::vector<double> hs(nsamples * 3);
+    int rc = EC_OK;
+    if (hipMemcpyAsync(dwl, wl, sizeof(double) * (2 * Rl + 1), hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(dwd, wd, sizeof(double) * (2 * K + 1), hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+        rc = ec_fail(ctx, EC_E_HIP, "ec_burgers_fractions: kernel upload failed");
+    if (rc == EC_OK) {
+        const int blocks = (int)std::min<long>((total + 255) / 256, 65536);
+        k_bfrac_fine<<<blocks, 256, 0, ctx->stream>>>(n, nsamples, n / L, visc, (const double *)u, K, dwd, s, vf);
+        k_bfrac_coarse<<<nsamples, 256, 0, ctx->stream>>>(n, nH, n / L, nH / L, visc, s, vf, Rl, dwl, K, dwd, sums);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(hs.data(), sums, sizeof(double) * 3 * nsamples, hipMemcpyDeviceToHost, ctx->stream) !=
+                hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess)
+            rc = ec_fail(ctx, EC_E_HIP, "ec_burgers_fractions: kernels failed");
+    }
+    const hipError_t fe = hipFreeAsync(buf, ctx->stream);
+    if (rc == EC_OK && fe != hipSuccess) rc = ec_fail(ctx, EC_E_HIP, "ec_burgers_fractions: free failed");
+    if (rc != EC_OK) return rc;
+    // per sample: sum / nH; then the mean over samples (src/Burgers.jl:1090-1096)
+    for (int x = 0; x < 3; x++) {
+        double acc = 0.0;
+        for (int q = 0; q < nsamples; q++) acc += hs[3 * q + x] / nH;
+        out[x] = acc / nsamples;
+    }
     return EC_OK;
 }

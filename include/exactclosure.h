@@ -262,6 +262,16 @@ int ec_burgers_batch_get(ec_ctx *ctx, ec_burgers_batch *batch, int instance, int
 int ec_burgers_batch_dt(ec_burgers_batch *batch, int instance, double *dt);
 /* Single-instance operators for parity tests (src/Burgers.jl:89-142). */
 int ec_burgers_rhs(ec_ctx *ctx, int dtype, int n, double L, double visc, const void *u, void *du);
+/* compute_fractions (src/Burgers.jl:1022-1099) over nsamples FP64 DNS
+ * snapshots u (n points each, stored one after another, device memory): the
+ * mean over coarse points and samples of |x| / (|classic| + |flux| + |div| +
+ * eps) for x = classic (vsu - sVU), flux (sVU - svu), div (fsu - vsu), with the
+ * LES kernel (Rl, wl[2 Rl + 1]) and the double kernel (K, wd[2 K + 1]) given
+ * as host arrays (tophat/gaussian_weights and build_doublekernel); nH divides
+ * n.  out = {classic, flux, div}.  FP64; synchronises the context's stream. */
+int ec_burgers_fractions(ec_ctx *ctx, int n, int nH, double L, double visc, int nsamples,
+                         const void *u, int Rl, const double *wl, int K, const double *wd,
+                         double out[3]);
 
 /* ---- learned closure: 3D CNN 3 -> 32 -> 32 -> 32 -> 3, 3x3x3 kernels, periodic
  * padding, GELU; BF16 weights and activations, FP32 accumulation (MFMA), Adam.

@@ -187,6 +187,50 @@ def spectrum_shells(u, nshell):
     return np.bincount(shell[keep].ravel(), weights=e[keep].ravel(), minlength=nshell)[:nshell]
 
 
+def burgers_fractions(U, nH, L, visc, les, dbl, parts=False):
+    """compute_fractions (src/Burgers.jl:1022-1099) for the DNS samples U
+    (nsamples, n), FP64: the LES kernel les = (R, w) and the double kernel
+    dbl = (K, w).  Restated with periodic shifts (np.roll): lazyfilter
+    y[i] = sum_r w[r + R] x[i + r] (src/Burgers.jl:164-172), stress at centre
+    i from faces i - 1 and i (src/Burgers.jl:97-102), lazycoarsegrain at the
+    1-based fine index J comp - !stag (comp / 2) (src/Burgers.jl:174-180).
+    Returns the mean fractions (classic, flux, div), or with parts=True the
+    per-point closure parts (classic, flux, div) of shape (nsamples, nH)."""
+    U = np.asarray(U, dtype=np.float64)
+    ns, n = U.shape
+    comp = n // nH
+    a = comp // 2
+    h, H = L / n, L / nH
+    Rl, wl = les
+    K, wd = dbl
+
+    def stress(u, sp):
+        ua = np.roll(u, 1, axis=-1)  # u[i - 1]
+        return -((ua + u) ** 2) / 8 + visc * (u - ua) / sp
+
+    def filt(x, R, w):
+        out = np.zeros_like(x)
+        for r in range(-R, R + 1):
+            out += w[r + R] * np.roll(x, -r, axis=-1)  # x[i + r]
+        return out
+
+    J1 = np.arange(1, nH + 1)
+    centre = J1 * comp - a - 1  # 0-based forms of the 1-based coarse-grain indices
+    face = J1 * comp - 1
+    s = stress(U, h)
+    VU = filt(U, K, wd)
+    fsu = filt(s, Rl, wl)[:, centre]
+    vsu = filt(s, K, wd)[:, centre]
+    svu = stress(VU[:, face], H)
+    sVU = stress(VU, h)[:, centre]
+    cl, fl, dv = vsu - sVU, sVU - svu, fsu - vsu
+    if parts:
+        return cl, fl, dv
+    tot = np.abs(cl) + np.abs(fl) + np.abs(dv) + np.finfo(np.float64).eps
+    per = [np.sum(np.abs(x) / tot, axis=1) / nH for x in (cl, fl, dv)]
+    return np.array([float(np.mean(x)) for x in per])
+
+
 def fft1d(x, sign=-1):
     x = np.array(x, dtype=np.complex128)
     lib().or_fft1d(len(x), sign, _p(x))

@@ -85,3 +85,27 @@ def test_invalid_batch_rejected(gpu):
         B.BurgersBatch([100], [3])  # 3 does not divide 100
     with pytest.raises(gpu._abi.ECError):
         B.BurgersBatch([32768], [4], dtype="f64")  # exceeds LDS
+
+
+@pytest.mark.parametrize("n,nH,kind,dr,ns", [(225, 15, "tophat", 1.0, 4), (225, 15, "gaussian", 2.0, 4),
+                                             (135, 45, "tophat", 3.0, 3), (64, 64, "tophat", 3.0, 2),
+                                             (13500, 300, "gaussian", 2.0, 6)])
+def test_burgers_fractions(gpu, O, n, nH, kind, dr, ns):
+    """compute_fractions (src/Burgers.jl:1022-1099) on the GPU against the
+    checker on the same snapshots, FP64: the mean fractions to 1e-12 (the last
+    case is the reference's own setup, nh = 13500, nH = 300, Delta = 2H,
+    Gaussian, on a few samples); the kernels come from the host mirror of
+    tophat/gaussian_weights and build_doublekernel."""
+    import math
+    B = gpu.burgers
+    L, visc = 2 * math.pi, 5e-4
+    rng = np.random.default_rng(n + nH)
+    x = np.arange(n) * L / n
+    U = np.array([sum(rng.standard_normal() / k * np.cos(k * x + rng.uniform(0, 2 * np.pi)) for k in range(1, 11))
+                  for _ in range(ns)])
+    got = B.compute_fractions(torch.from_numpy(U).cuda(), nH, L, visc, dr, kind)
+    les, dbl = B.les_kernels(n, nH, L, dr, kind)
+    want = O.burgers_fractions(U, nH, L, visc, les, dbl)
+    for key, w in zip(("classic", "flux", "div"), want):
+        assert abs(got[key] - w) < 1e-12, (key, got[key], w)
+    assert abs(sum(got.values()) - 1) < 1e-9

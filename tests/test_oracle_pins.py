@@ -22,7 +22,10 @@ so every oracle function is pinned by checks chosen so a plausible mistake
   randomfield  divergence-free, total energy, shell spectrum follows the
                profile (src/N.jl:441-449)
   Burgers      RK4 4th-order convergence; swap-filter exactness; the
-               identity classic + flux + div = swap (src/Burgers.jl:1087-1089)
+               identity classic + flux + div = swap (src/Burgers.jl:1087-1089);
+               closure fractions: mirror symmetry of every part under the
+               staggered reflection, the comp = 1 and constant-field limits
+  spectrum     full-FFT shells, Parseval, single modes
 """
 import math
 
@@ -296,6 +299,71 @@ def test_spectrum_single_modes(O):
             assert abs(s[k] - amp ** 2 / 4) < 1e-13, (a, b, c, comp, s[k])
             rest = np.delete(s, k)
             assert np.abs(rest).max() < 1e-13
+
+
+# ---------------------------------------------------------------- Burgers closure fractions
+def _bkernels(n, nH, L, dr, kind):
+    from conftest import ROOT  # noqa: F401
+    h, H = L / n, L / nH
+    comp = n // nH
+    fvm = (comp // 2, np.full(comp, 1.0 / comp))
+    if kind == "tophat":
+        R = int(np.rint(H * dr / h)) // 2
+        les = (R, np.full(2 * R + 1, 1.0 / (2 * R + 1)))
+    else:
+        s2 = (H * dr / h) ** 2 / 12
+        R = int(np.rint(3 * math.sqrt(s2)))
+        w = np.exp(-np.arange(-R, R + 1) ** 2 / (2 * s2))
+        les = (R, w / w.sum())
+    return les, (fvm[0] + les[0], np.convolve(fvm[1], les[1]))
+
+
+def _bsamples(n, ns, seed):
+    rng = np.random.default_rng(seed)
+    x = np.arange(n) * 2 * np.pi / n
+    return np.array([sum(rng.standard_normal() / k * np.cos(k * x + rng.uniform(0, 2 * np.pi))
+                         for k in range(1, 11)) for _ in range(ns)])
+
+
+@pytest.mark.parametrize("kind,dr", [("tophat", 1.0), ("tophat", 2.0), ("gaussian", 2.0)])
+def test_burgers_fractions_reflection(O, kind, dr):
+    """The staggered reflection u~[i] = -u[n - i] (faces i = 1..n, u[0] = u[n])
+    maps the Burgers stress to itself at the mirrored centre, so with odd comp
+    and symmetric kernels every closure part at coarse centre J equals the
+    part at nH + 1 - J of the original field; a centre or face sampled one
+    cell off, or a filter applied in the wrong direction, breaks this."""
+    n, nH, L, visc = 225, 15, 2 * math.pi, 5e-3
+    les, dbl = _bkernels(n, nH, L, dr, kind)
+    U = _bsamples(n, 3, 7)
+    idx = (n - np.arange(n) - 2) % n  # 0-based form of u~[i] = -u[n - i]
+    Ur = -U[:, idx]
+    a = O.burgers_fractions(U, nH, L, visc, les, dbl, parts=True)
+    b = O.burgers_fractions(Ur, nH, L, visc, les, dbl, parts=True)
+    for x, y in zip(a, b):
+        assert np.abs(y - x[:, ::-1]).max() < 1e-12 * np.abs(x).max()
+    fa = O.burgers_fractions(U, nH, L, visc, les, dbl)
+    fb = O.burgers_fractions(Ur, nH, L, visc, les, dbl)
+    assert np.abs(fa - fb).max() < 1e-13
+    assert abs(fa.sum() - 1.0) < 1e-12  # fractions of each point sum to one
+
+
+def test_burgers_fractions_degenerate_cases(O):
+    """comp = 1 (the LES grid is the DNS grid): the double kernel is the LES
+    kernel and the two stresses of the flux part coincide, so flux and div
+    vanish exactly and the classic part carries everything; a constant field
+    has no closure at all (all parts zero to rounding)."""
+    n, L, visc = 64, 2 * math.pi, 1e-2
+    les, dbl = _bkernels(n, n, L, 3.0, "tophat")
+    assert les[0] == 1 and dbl[0] == 1
+    U = _bsamples(n, 2, 3)
+    cl, fl, dv = O.burgers_fractions(U, n, L, visc, les, dbl, parts=True)
+    assert np.abs(fl).max() == 0 and np.abs(dv).max() == 0 and np.abs(cl).max() > 1e-3
+    f = O.burgers_fractions(U, n, L, visc, les, dbl)
+    assert abs(f[0] - 1) < 1e-9 and f[1] == 0 and f[2] == 0
+    les, dbl = _bkernels(135, 9, L, 2.0, "gaussian")
+    C = np.full((2, 135), 0.7)
+    # (the fractions of rounding-level parts are not meaningful; eps only keeps them finite)
+    assert all(np.abs(x).max() < 1e-15 for x in O.burgers_fractions(C, 9, L, visc, les, dbl, parts=True))
 
 
 def test_randn_moments(O):
