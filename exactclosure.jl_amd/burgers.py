@@ -84,6 +84,50 @@ def compute_fractions(u, nH, L=2 * math.pi, visc=5e-4, delta_ratio=1.0, lesfilte
     return {"classic": out[0], "flux": out[1], "div": out[2]}
 
 
+def smagorinsky_coefficients(U, nH, L=2 * math.pi, visc=5e-4, delta_ratio=2.0, lesfiltertype="gaussian"):
+    """theta_classic and theta_discret (smagorinsky.jl:24-36 over the fields
+    of smagorinsky_fields, src/Burgers.jl:547-680): least-squares Smagorinsky
+    coefficients theta = sqrt(-<S,T>/<S,S>) against the classic sub-filter
+    stress on the DNS grid and the discretisation-consistent one on the LES
+    stencil.  U: DNS snapshots as a (nsamples, nh) FP64 CUDA tensor.  A
+    negative theta^2 raises, as Julia's sqrt does."""
+    import torch
+
+    assert U.dtype == torch.float64 and U.is_cuda and U.dim() == 2 and U.is_contiguous()
+    ns, nh = U.shape
+    (Rl, wl), (K, wd) = les_kernels(nh, nH, L, delta_ratio, lesfiltertype)
+    wl = np.ascontiguousarray(wl, dtype=np.float64)
+    wd = np.ascontiguousarray(wd, dtype=np.float64)
+    out = (ctypes.c_double * 4)()
+    dp = ctypes.POINTER(ctypes.c_double)
+    _abi.context().call("ec_burgers_smag_fit", nh, nH, L, visc, L / nH * delta_ratio, ns,
+                        _abi.c_void_p(U.data_ptr()), Rl, wl.ctypes.data_as(dp), K, wd.ctypes.data_as(dp), out)
+    sums = list(out)
+    return {"classic": math.sqrt(-sums[0] / sums[1]), "discret": math.sqrt(-sums[2] / sums[3]), "sums": sums}
+
+
+def solve_smagorinsky(ustart, nH, theta, L=2 * math.pi, visc=5e-4, delta_ratio=2.0, lesfiltertype="gaussian",
+                      tstop=0.1, cfl=0.3):
+    """solve_smagorinsky for one (Delta_ratio, nH) (src/Burgers.jl:682-772):
+    every sample of ustart ((nsamples, nh) FP64 CUDA tensor) filtered to the
+    LES grid and advanced to tstop with the Smagorinsky closure
+    C = theta^2 (Delta^2 + H^2), all samples in one launch; returns the
+    (nsamples, nH) FP64 LES states."""
+    import torch
+
+    assert ustart.dtype == torch.float64 and ustart.is_cuda and ustart.dim() == 2 and ustart.is_contiguous()
+    ns, nh = ustart.shape
+    _, (K, wd) = les_kernels(nh, nH, L, delta_ratio, lesfiltertype)
+    wd = np.ascontiguousarray(wd, dtype=np.float64)
+    H = L / nH
+    C = theta ** 2 * ((H * delta_ratio) ** 2 + H ** 2)
+    out = torch.empty((ns, nH), dtype=torch.float64, device=ustart.device)
+    _abi.context().call("ec_burgers_smag_solve", nh, nH, L, visc, C, tstop, cfl, ns,
+                        _abi.c_void_p(ustart.data_ptr()), K, wd.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                        _abi.c_void_p(out.data_ptr()))
+    return out
+
+
 class BurgersBatch:
     """count instances: n[i] points, compression c[i], RK4 with the exact
     closure and a no-model LES, FP32 or FP64 state, FP64 commutator."""

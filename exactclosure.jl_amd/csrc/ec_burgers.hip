@@ -598,3 +598,202 @@ extern "C" int ec_burgers_fractions(ec_ctx *ctx, int n, int nH, double L, double
     }
     return EC_OK;
 }
+
+// ---- Smagorinsky study (smagorinsky_fields, src/Burgers.jl:547-680; the
+// least-squares coefficients of smagorinsky.jl:24-36; solve_smagorinsky,
+// src/Burgers.jl:682-772) ----
+// Fit: per fine point i and sample, with U = D * u and s = s_h(u),
+//   classic   T = (D * s)[i] - s_h(U)[i],  S = -c |dU| dU,  dU = (U[i] - U[i-1]) / h
+//   discrete  T = (F * s)[i] - s_H(U[i-a-1], U[i+a]),  S likewise with dU = (U[i+a] - U[i-a-1]) / H
+// (c = Delta^2 + H^2, a = comp / 2), summed into <S,T> and <S,S> of each kind;
+// theta^2 = -<S,T> / <S,S>.  Blocks write partial sums, reduced on the host in
+// a fixed order.
+__global__ void __launch_bounds__(256) k_bsmag_fit(int n, int ns, int a, double ih, double iH, double visc,
+                                                   double cs, const double *__restrict__ s,
+                                                   const double *__restrict__ U, int Rl,
+                                                   const double *__restrict__ wl, int K,
+                                                   const double *__restrict__ wd, double *__restrict__ part) {
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const long total = (long)n * ns;
+    for (long I = blockIdx.x * (long)blockDim.x + threadIdx.x; I < total; I += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(I % n);
+        const double *ss = s + (I - i), *UU = U + (I - i);
+        double ds = 0.0, fs = 0.0;
+        for (int k = -K; k <= K; k++) ds += wd[k + K] * ss[bwrap(i + k, n)];
+        for (int r = -Rl; r <= Rl; r++) fs += wl[r + Rl] * ss[bwrap(i + r, n)];
+        const double um = UU[bwrap(i - 1, n)], u0 = UU[i];
+        const double th = ds - bstress(um, u0, ih, visc);
+        const double dh = (u0 - um) * ih, sh = -cs * fabs(dh) * dh;
+        const double ul = UU[bwrap(i - a - 1, n)], ur = UU[bwrap(i + a, n)];
+        const double tH = fs - bstress(ul, ur, iH, visc);
+        const double dH = (ur - ul) * iH, sH = -cs * fabs(dH) * dH;
+        acc[0] += sh * th;
+        acc[1] += sh * sh;
+        acc[2] += sH * tH;
+        acc[3] += sH * sH;
+    }
+    __shared__ double red[4][256];
+    for (int x = 0; x < 4; x++) red[x][threadIdx.x] = acc[x];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int x = 0; x < 4; x++) red[x][threadIdx.x] += red[x][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int x = 0; x < 4; x++) part[4 * blockIdx.x + x] = red[x][0];
+}
+
+extern "C" int ec_burgers_smag_fit(ec_ctx *ctx, int n, int nH, double L, double visc, double delta, int nsamples,
+                                   const void *u, int Rl, const double *wl, int K, const double *wd,
+                                   double out[4]) {
+    EC_REQUIRE(ctx, ctx && u && wl && wd && out, "ec_burgers_smag_fit: null argument");
+    EC_REQUIRE(ctx, n >= 2 && nH >= 1 && n % nH == 0 && nsamples >= 1 && L > 0 && delta >= 0,
+               "ec_burgers_smag_fit: need n %% nH == 0 (n=%d, nH=%d), nsamples=%d >= 1", n, nH, nsamples);
+    EC_REQUIRE(ctx, Rl >= 0 && K >= 0 && 2 * Rl + 1 <= n && 2 * K + 1 <= n,
+               "ec_burgers_smag_fit: kernel radii Rl=%d, K=%d must fit the grid n=%d", Rl, K, n);
+    const long total = (long)n * nsamples;
+    const int blocks = (int)std::min<long>((total + 255) / 256, 4096);
+    double *buf = nullptr;
+    const size_t bytes = sizeof(double) * ((size_t)2 * total + (2 * Rl + 1) + (2 * K + 1) + 4 * (size_t)blocks);
+    EC_HIP(ctx, hipMallocAsync((void **)&buf, bytes, ctx->stream));
+    double *s = buf, *U = buf + total, *dwl = U + total, *dwd = dwl + (2 * Rl + 1), *part = dwd + (2 * K + 1);
+    std::vector<double> hp(4 * (size_t)blocks);
+    const double H = L / nH;
+    int rc = EC_OK;
+    if (hipMemcpyAsync(dwl, wl, sizeof(double) * (2 * Rl + 1), hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(dwd, wd, sizeof(double) * (2 * K + 1), hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+        rc = ec_fail(ctx, EC_E_HIP, "ec_burgers_smag_fit: kernel upload failed");
+    if (rc == EC_OK) {
+        const int fblocks = (int)std::min<long>((total + 255) / 256, 65536);
+        k_bfrac_fine<<<fblocks, 256, 0, ctx->stream>>>(n, nsamples, n / L, visc, (const double *)u, K, dwd, s, U);
+        k_bsmag_fit<<<blocks, 256, 0, ctx->stream>>>(n, nsamples, (n / nH) / 2, n / L, 1.0 / H, visc,
+                                                     delta * delta + H * H, s, U, Rl, dwl, K, dwd, part);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(hp.data(), part, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, ctx->stream) !=
+                hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess)
+            rc = ec_fail(ctx, EC_E_HIP, "ec_burgers_smag_fit: kernels failed");
+    }
+    const hipError_t fe = hipFreeAsync(buf, ctx->stream);
+    if (rc == EC_OK && fe != hipSuccess) rc = ec_fail(ctx, EC_E_HIP, "ec_burgers_smag_fit: free failed");
+    if (rc != EC_OK) return rc;
+    for (int x = 0; x < 4; x++) {
+        double acc = 0.0;
+        for (int b = 0; b < blocks; b++) acc += hp[4 * b + x];
+        out[x] = acc;
+    }
+    return EC_OK;
+}
+
+// Solve: one block per sample; ubar = C_f(D * u) (coarse faces), then RK4 of
+// du = delta_coll(s_H(ubar) + C |delta_stag ubar| delta_stag ubar) to tstop
+// with dt = min(cfl min(H / max|ubar|, H^2 / visc), tstop - t).  The stages
+// run as the reference writes them: each axpy! lands on the running state
+// (ubar += dt/2 k1; ubar += dt/2 k2; ubar += dt k3), so stage 3 sees
+// ubar0 + dt/2 (k1 + k2); the step ends at ubar0 + dt/6 (k1 + 2 k2 + 2 k3 + k4).
+// State, stage sums and the stress in LDS (5 nH doubles).
+constexpr long SMAG_MAX_STEPS = 1000000;  // exit bound for pathological inputs
+
+__global__ void __launch_bounds__(256) k_bsmag_solve(int n, int nH, double iH, double H, double visc, double C,
+                                                     double tstop, double cflf, const double *__restrict__ u,
+                                                     int K, const double *__restrict__ wd,
+                                                     double *__restrict__ ubar_out) {
+    extern __shared__ double sm[];
+    double *ub = sm, *ub0 = sm + nH, *kk = sm + 2 * nH, *acc = sm + 3 * nH, *sg = sm + 4 * nH;
+    __shared__ double red[256];
+    __shared__ double s_dt;
+    const int comp = n / nH, tid = threadIdx.x;
+    const double *us = u + (long)blockIdx.x * n;
+    for (int J = tid; J < nH; J += blockDim.x) {
+        const int f = (J + 1) * comp - 1;  // fine face of coarse face J
+        double v = 0.0;
+        for (int k = -K; k <= K; k++) v += wd[k + K] * us[bwrap(f + k, n)];
+        ub[J] = v;
+    }
+    __syncthreads();
+    auto rhs = [&]() {  // kk = delta_coll(sigma), sigma from ub
+        for (int J = tid; J < nH; J += blockDim.x) {
+            const double ua = ub[J == 0 ? nH - 1 : J - 1], ubb = ub[J];
+            const double d = (ubb - ua) * iH;
+            sg[J] = -((ua + ubb) * (ua + ubb)) / 8 + visc * d + C * fabs(d) * d;
+        }
+        __syncthreads();
+        for (int J = tid; J < nH; J += blockDim.x) kk[J] = (sg[J == nH - 1 ? 0 : J + 1] - sg[J]) * iH;
+        __syncthreads();
+    };
+    double t = 0.0;
+    for (long step = 0; step < SMAG_MAX_STEPS && t < tstop; step++) {
+        double m = 0.0;
+        for (int J = tid; J < nH; J += blockDim.x) m = fmax(m, fabs(ub[J]));
+        red[tid] = m;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (tid < w) red[tid] = fmax(red[tid], red[tid + w]);
+            __syncthreads();
+        }
+        if (tid == 0) {
+            const double cfl = fmin(H / red[0], H * H / visc);
+            s_dt = fmin(cflf * cfl, tstop - t);
+        }
+        __syncthreads();
+        const double dt = s_dt;
+        if (!(dt > 0)) break;  // max|ubar| infinite: no step can advance t
+        t += dt;
+        for (int J = tid; J < nH; J += blockDim.x) ub0[J] = ub[J];
+        __syncthreads();
+        rhs();  // k1
+        for (int J = tid; J < nH; J += blockDim.x) {
+            acc[J] = kk[J];
+            ub[J] += dt / 2 * kk[J];
+        }
+        __syncthreads();
+        rhs();  // k2
+        for (int J = tid; J < nH; J += blockDim.x) {
+            acc[J] += 2 * kk[J];
+            ub[J] += dt / 2 * kk[J];
+        }
+        __syncthreads();
+        rhs();  // k3
+        for (int J = tid; J < nH; J += blockDim.x) {
+            acc[J] += 2 * kk[J];
+            ub[J] += dt * kk[J];
+        }
+        __syncthreads();
+        rhs();  // k4
+        for (int J = tid; J < nH; J += blockDim.x) ub[J] = ub0[J] + dt / 6 * (acc[J] + kk[J]);
+        __syncthreads();
+    }
+    for (int J = tid; J < nH; J += blockDim.x) ubar_out[(long)blockIdx.x * nH + J] = ub[J];
+}
+
+extern "C" int ec_burgers_smag_solve(ec_ctx *ctx, int n, int nH, double L, double visc, double C, double tstop,
+                                     double cfl, int nsamples, const void *ustart, int K, const double *wd,
+                                     void *ubar_out) {
+    EC_REQUIRE(ctx, ctx && ustart && wd && ubar_out, "ec_burgers_smag_solve: null argument");
+    EC_REQUIRE(ctx, n >= 2 && nH >= 2 && n % nH == 0 && nsamples >= 1 && L > 0 && visc > 0 && cfl > 0 &&
+                        tstop >= 0 && C >= 0,
+               "ec_burgers_smag_solve: bad argument (n=%d, nH=%d, nsamples=%d, visc=%g, cfl=%g, C=%g)", n, nH,
+               nsamples, visc, cfl, C);
+    const size_t lds = sizeof(double) * 5 * (size_t)nH;
+    EC_REQUIRE(ctx, lds + sizeof(double) * 257 <= 160 * 1024, "ec_burgers_smag_solve: nH=%d exceeds the LDS state",
+               nH);
+    EC_REQUIRE(ctx, K >= 0 && 2 * K + 1 <= n, "ec_burgers_smag_solve: kernel radius K=%d must fit n=%d", K, n);
+    double *dwd = nullptr;
+    EC_HIP(ctx, hipMallocAsync((void **)&dwd, sizeof(double) * (2 * K + 1), ctx->stream));
+    int rc = EC_OK;
+    if (hipMemcpyAsync(dwd, wd, sizeof(double) * (2 * K + 1), hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+        rc = ec_fail(ctx, EC_E_HIP, "ec_burgers_smag_solve: kernel upload failed");
+    if (rc == EC_OK &&
+        hipFuncSetAttribute((const void *)k_bsmag_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+            hipSuccess)
+        rc = ec_fail(ctx, EC_E_HIP, "ec_burgers_smag_solve: LDS attribute");
+    if (rc == EC_OK) {
+        k_bsmag_solve<<<nsamples, 256, lds, ctx->stream>>>(n, nH, nH / L, L / nH, visc, C, tstop, cfl,
+                                                           (const double *)ustart, K, dwd, (double *)ubar_out);
+        if (hipGetLastError() != hipSuccess) rc = ec_fail(ctx, EC_E_HIP, "ec_burgers_smag_solve: launch failed");
+    }
+    const hipError_t fe = hipFreeAsync(dwd, ctx->stream);
+    if (rc == EC_OK && fe != hipSuccess) rc = ec_fail(ctx, EC_E_HIP, "ec_burgers_smag_solve: free failed");
+    return rc;
+}

@@ -272,6 +272,23 @@ int ec_burgers_rhs(ec_ctx *ctx, int dtype, int n, double L, double visc, const v
 int ec_burgers_fractions(ec_ctx *ctx, int n, int nH, double L, double visc, int nsamples,
                          const void *u, int Rl, const double *wl, int K, const double *wd,
                          double out[3]);
+/* Smagorinsky coefficient fit (smagorinsky_fields, src/Burgers.jl:547-680, and
+ * smagorinsky.jl:24-36) over nsamples FP64 DNS snapshots u (device): out =
+ * {<S_h,T_h>, <S_h,S_h>, <S_H,T_H>, <S_H,S_H>} summed over points and samples
+ * for the classic (fine-grid) and discretisation-consistent (coarse-stencil)
+ * sub-filter stress T and Smagorinsky stress S = -(delta^2 + H^2)|dU| dU;
+ * theta^2 = -<S,T>/<S,S>.  Kernels as for ec_burgers_fractions. */
+int ec_burgers_smag_fit(ec_ctx *ctx, int n, int nH, double L, double visc, double delta, int nsamples,
+                        const void *u, int Rl, const double *wl, int K, const double *wd, double out[4]);
+/* solve_smagorinsky (src/Burgers.jl:682-772): per sample, ubar = C_f(D * u)
+ * on the coarse grid advanced to tstop by the reference's RK4 (stage states
+ * accumulate on the running state) with dt = min(cfl min(H/max|ubar|,
+ * H^2/visc), tstop - t), closure coefficient C = theta^2 (delta^2 + H^2);
+ * ubar_out (device, nsamples x nH, FP64).  Asynchronous; 5 nH doubles of LDS
+ * per sample (nH up to 4000). */
+int ec_burgers_smag_solve(ec_ctx *ctx, int n, int nH, double L, double visc, double C, double tstop,
+                          double cfl, int nsamples, const void *ustart, int K, const double *wd,
+                          void *ubar_out);
 
 /* ---- learned closure: 3D CNN 3 -> 32 -> 32 -> 32 -> 3, 3x3x3 kernels, periodic
  * padding, GELU; BF16 weights and activations, FP32 accumulation (MFMA), Adam.

@@ -231,6 +231,76 @@ def burgers_fractions(U, nH, L, visc, les, dbl, parts=False):
     return np.array([float(np.mean(x)) for x in per])
 
 
+def _bstress(ua, ub, sp, visc):
+    return -((ua + ub) ** 2) / 8 + visc * (ub - ua) / sp
+
+
+def _bfilt(x, R, w):
+    out = np.zeros_like(x)
+    for r in range(-R, R + 1):
+        out += w[r + R] * np.roll(x, -r, axis=-1)  # x[i + r]
+    return out
+
+
+def smagorinsky_sums(U, nH, L, visc, delta, les, dbl):
+    """The four sums behind theta_classic and theta_discret (smagorinsky.jl:
+    24-36) over the fields of smagorinsky_fields (src/Burgers.jl:547-680), FP64:
+    <S_h,T_h>, <S_h,S_h>, <S_H,T_H>, <S_H,S_H> over all points and samples."""
+    U = np.asarray(U, dtype=np.float64)
+    ns, n = U.shape
+    comp = n // nH
+    a = comp // 2
+    h, H = L / n, L / nH
+    cs = delta ** 2 + H ** 2
+    s = _bstress(np.roll(U, 1, axis=-1), U, h, visc)
+    Ud = _bfilt(U, *dbl)
+    Um = np.roll(Ud, 1, axis=-1)  # Ud[i - 1]
+    Th = _bfilt(s, *dbl) - _bstress(Um, Ud, h, visc)
+    dh = (Ud - Um) / h
+    Sh = -cs * np.abs(dh) * dh
+    Ul, Ur = np.roll(Ud, a + 1, axis=-1), np.roll(Ud, -a, axis=-1)  # Ud[i - a - 1], Ud[i + a]
+    TH = _bfilt(s, *les) - _bstress(Ul, Ur, H, visc)
+    dH = (Ur - Ul) / H
+    SH = -cs * np.abs(dH) * dH
+    return np.array([np.sum(Sh * Th), np.sum(Sh * Sh), np.sum(SH * TH), np.sum(SH * SH)])
+
+
+def smagorinsky_solve(Ustart, nH, L, visc, C, tstop, cfl, dbl):
+    """solve_smagorinsky (src/Burgers.jl:682-772) for each sample: the coarse
+    face samples of D * u, then the reference's RK4 loop (its axpy! stages
+    accumulate on the running state) with dt = min(cfl min(H / max|u|,
+    H^2 / visc), tstop - t); FP64."""
+    Ustart = np.asarray(Ustart, dtype=np.float64)
+    ns, n = Ustart.shape
+    comp = n // nH
+    H = L / nH
+    out = np.empty((ns, nH))
+
+    def rhs(u):
+        um = np.roll(u, 1)
+        d = (u - um) / H
+        sig = _bstress(um, u, H, visc) + C * np.abs(d) * d  # s - (-C |du| du)
+        return (np.roll(sig, -1) - sig) / H
+
+    for q in range(ns):
+        ub = _bfilt(Ustart[q], *dbl)[comp - 1::comp].copy()  # faces J comp (1-based)
+        t = 0.0
+        while t < tstop:
+            dt = min(cfl * min(H / np.abs(ub).max(), H * H / visc), tstop - t)
+            t += dt
+            u0 = ub.copy()
+            k1 = rhs(ub)
+            ub = ub + dt / 2 * k1
+            k2 = rhs(ub)
+            ub = ub + dt / 2 * k2
+            k3 = rhs(ub)
+            ub = ub + dt * k3
+            k4 = rhs(ub)
+            ub = u0 + dt / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+        out[q] = ub
+    return out
+
+
 def fft1d(x, sign=-1):
     x = np.array(x, dtype=np.complex128)
     lib().or_fft1d(len(x), sign, _p(x))

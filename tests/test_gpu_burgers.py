@@ -109,3 +109,55 @@ def test_burgers_fractions(gpu, O, n, nH, kind, dr, ns):
     for key, w in zip(("classic", "flux", "div"), want):
         assert abs(got[key] - w) < 1e-12, (key, got[key], w)
     assert abs(sum(got.values()) - 1) < 1e-9
+
+
+def _smag_samples(n, ns, seed, L):
+    rng = np.random.default_rng(seed)
+    x = np.arange(n) * L / n
+    return np.array([sum(rng.standard_normal() / k * np.cos(k * x + rng.uniform(0, 2 * np.pi)) for k in range(1, 11))
+                     for _ in range(ns)])
+
+
+@pytest.mark.parametrize("n,nH,kind,dr", [(225, 15, "gaussian", 2.0), (225, 45, "tophat", 1.0),
+                                          (13500, 300, "gaussian", 2.0), (13500, 900, "gaussian", 0.0)])
+def test_smagorinsky_fit(gpu, O, n, nH, kind, dr):
+    """The four sums of the Smagorinsky fit (smagorinsky.jl:24-36) against the
+    checker on the same snapshots (1e-10 relative; FP64 sums in another
+    order); theta from them; the reference's setup sizes included."""
+    import math
+    B = gpu.burgers
+    L, visc = 2 * math.pi, 5e-4
+    U = _smag_samples(n, 3, n + nH, L)
+    les, dbl = B.les_kernels(n, nH, L, dr, kind)
+    want = O.smagorinsky_sums(U, nH, L, visc, dr * L / nH, les, dbl)
+    Ud = torch.from_numpy(U).cuda()
+    import ctypes
+    out = (ctypes.c_double * 4)()
+    dp = ctypes.POINTER(ctypes.c_double)
+    wl, wd = np.ascontiguousarray(les[1]), np.ascontiguousarray(dbl[1])
+    gpu._abi.context().call("ec_burgers_smag_fit", n, nH, L, visc, dr * L / nH, 3, gpu._abi.c_void_p(Ud.data_ptr()),
+                            les[0], wl.ctypes.data_as(dp), dbl[0], wd.ctypes.data_as(dp), out)
+    got = np.array(out[:])
+    assert np.abs(got - want).max() <= 1e-10 * np.abs(want).max(), (got, want)
+    if want[0] < 0 and want[2] < 0:
+        th = B.smagorinsky_coefficients(Ud, nH, L, visc, dr, kind)
+        assert abs(th["classic"] - math.sqrt(-want[0] / want[1])) < 1e-9
+        assert abs(th["discret"] - math.sqrt(-want[2] / want[3])) < 1e-9
+
+
+@pytest.mark.parametrize("n,nH,kind,dr,C", [(225, 15, "gaussian", 2.0, 2e-3), (192, 64, "tophat", 1.0, 0.0),
+                                            (13500, 300, "gaussian", 2.0, 1e-3)])
+def test_smagorinsky_solve(gpu, O, n, nH, kind, dr, C):
+    """solve_smagorinsky (src/Burgers.jl:682-772) on the GPU, every sample in
+    one launch, against the checker's per-sample loop (same CFL steps, the
+    reference's accumulating RK4 stages), FP64 to 1e-11."""
+    import math
+    B = gpu.burgers
+    L, visc, tstop = 2 * math.pi, 5e-4, 0.1
+    U = _smag_samples(n, 4, 3 * n + nH, L)
+    H = L / nH
+    theta = math.sqrt(C / ((dr * H) ** 2 + H ** 2))
+    got = B.solve_smagorinsky(torch.from_numpy(U).cuda(), nH, theta, L, visc, dr, kind, tstop).cpu().numpy()
+    _, dbl = B.les_kernels(n, nH, L, dr, kind)
+    want = O.smagorinsky_solve(U, nH, L, visc, C, tstop, 0.3, dbl)
+    assert np.abs(got - want).max() <= 1e-11 * np.abs(want).max()

@@ -310,6 +310,8 @@ def _bkernels(n, nH, L, dr, kind):
     if kind == "tophat":
         R = int(np.rint(H * dr / h)) // 2
         les = (R, np.full(2 * R + 1, 1.0 / (2 * R + 1)))
+    elif H * dr < 1e-15:
+        les = (0, np.ones(1))  # gaussian_weights at Delta = 0 (src/Burgers.jl:146-156)
     else:
         s2 = (H * dr / h) ** 2 / 12
         R = int(np.rint(3 * math.sqrt(s2)))
@@ -364,6 +366,57 @@ def test_burgers_fractions_degenerate_cases(O):
     C = np.full((2, 135), 0.7)
     # (the fractions of rounding-level parts are not meaningful; eps only keeps them finite)
     assert all(np.abs(x).max() < 1e-15 for x in O.burgers_fractions(C, 9, L, visc, les, dbl, parts=True))
+
+
+@pytest.mark.parametrize("kind,dr", [("tophat", 2.0), ("gaussian", 2.0), ("gaussian", 0.0)])
+def test_smagorinsky_sums_reflection(O, kind, dr):
+    """<S,T> and <S,S> of both fits are unchanged by the staggered reflection
+    u~[i] = -u[n - i]: S and T are centre quantities, even under it, and the
+    coarse stencil U[i - a - 1], U[i + a] is centred on i; a stencil one cell
+    off or a filter in the wrong direction breaks the invariance."""
+    n, nH, L, visc = 225, 15, 2 * math.pi, 5e-3
+    les, dbl = _bkernels(n, nH, L, dr, kind)
+    U = _bsamples(n, 3, 11)
+    Ur = -U[:, (n - np.arange(n) - 2) % n]
+    a = O.smagorinsky_sums(U, nH, L, visc, dr * L / nH, les, dbl)
+    b = O.smagorinsky_sums(Ur, nH, L, visc, dr * L / nH, les, dbl)
+    assert np.abs(a - b).max() < 1e-12 * np.abs(a).max()
+    assert a[1] > 0 and a[3] > 0
+
+
+def test_smagorinsky_sums_identity_limit(O):
+    """comp = 1 and Delta = 0: every filter is the identity, both sub-filter
+    stresses vanish and the two Smagorinsky stresses coincide."""
+    n, L, visc = 64, 2 * math.pi, 1e-2
+    les, dbl = _bkernels(n, n, L, 0.0, "gaussian")
+    assert les[0] == 0 and len(les[1]) == 1 and dbl[0] == 0
+    U = _bsamples(n, 2, 5)
+    st = O.smagorinsky_sums(U, n, L, visc, 0.0, les, dbl)
+    assert st[0] == 0 and st[2] == 0 and st[1] > 0 and abs(st[1] - st[3]) < 1e-14 * st[1]
+
+
+def test_smagorinsky_solve_first_order_and_reflection(O):
+    """The reference's RK4 loop lands each axpy! on the running state
+    (src/Burgers.jl:748-755), so its stages sit at u0 + dt/2 k1, u0 + dt/2 (k1
+    + k2), ...: a consistent but first-order scheme.  With C = 0 the solve is
+    the coarse Burgers flow: against the oracle's pinned classical RK4 at a
+    tiny step its error halves with the CFL factor (order 1; the textbook
+    tableau would drop it 16-fold).  With C > 0 the solve commutes with the
+    staggered reflection (the Smagorinsky stencil and the coarse sampling)."""
+    n, nH, L, visc, tstop = 192, 64, 2 * math.pi, 1e-2, 0.05
+    les, dbl = _bkernels(n, nH, L, 1.0, "tophat")
+    U = _bsamples(n, 1, 13)
+    ub0 = O.smagorinsky_solve(U, nH, L, visc, 0.0, 0.0, 0.3, dbl)[0]  # tstop 0: the filtered start
+    ref = ub0.copy()
+    O.lib().or_burgers_rk4(nH, L, visc, tstop / 4000, 4000, O._p(ref))
+    e = [np.abs(O.smagorinsky_solve(U, nH, L, visc, 0.0, tstop, c, dbl)[0] - ref).max() for c in (0.1, 0.05, 0.025)]
+    assert 1.7 < e[0] / e[1] < 2.3 and 1.7 < e[1] / e[2] < 2.3, e
+    Ur = -U[:, (n - np.arange(n) - 2) % n]
+    C = 1e-3  # theta^2 (Delta^2 + H^2) for theta ~ 0.2, Delta = H ~ 0.1
+    a = O.smagorinsky_solve(U, nH, L, visc, C, tstop, 0.3, dbl)[0]
+    b = O.smagorinsky_solve(Ur, nH, L, visc, C, tstop, 0.3, dbl)[0]
+    assert np.abs(b - (-a[(nH - np.arange(nH) - 2) % nH])).max() < 1e-12 * np.abs(a).max()
+    assert np.abs(a - O.smagorinsky_solve(U, nH, L, visc, 0.0, tstop, 0.3, dbl)[0]).max() > 1e-6
 
 
 def test_randn_moments(O):
