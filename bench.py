@@ -48,6 +48,9 @@ def parse():
     ap.add_argument("--burgers", action="store_true",
                     help="time the batched Burgers workload (4096 instances, 1000 RK4 steps) instead")
     ap.add_argument("--burgers-steps", type=int, default=1000)
+    ap.add_argument("--burgers-study", action="store_true",
+                    help="time the Burgers closure study (closure fractions, Smagorinsky fit and solve) "
+                         "at nh = 13500, nH in {300, 900, 2700}, Delta = 2H Gaussian, 1000 samples")
     return ap.parse_args()
 
 
@@ -83,6 +86,70 @@ def burgers_main(a):
     print(json.dumps(out), flush=True)
 
 
+def burgers_study_main(a):
+    """compute_fractions (src/Burgers.jl:1022-1099), the Smagorinsky fits
+    (smagorinsky.jl:24-36) and solve_smagorinsky (src/Burgers.jl:682-772) on
+    the reference setup (src/Burgers.jl:236-246), all samples on the GPU in
+    FP64; snapshots are synthetic (modes k <= 10, amplitude N(0,1)/k, uniform
+    phase).  The oracle's numpy restatement on 10 samples is the CPU leg."""
+    sys.path.insert(0, ROOT)
+    import math
+
+    import torch
+    import __graft_entry__ as ge
+
+    ec = ge.load_package()
+    B = ec.burgers
+    L, visc, nh, ns, dr, tstop = 2 * math.pi, 5e-4, 13500, 1000, 2.0, 0.1
+    g = torch.Generator(device="cuda").manual_seed(123)
+    x = torch.arange(nh, device="cuda", dtype=torch.float64) * L / nh
+    U = torch.zeros((ns, nh), device="cuda", dtype=torch.float64)
+    for k in range(1, 11):
+        amp = torch.randn((ns, 1), device="cuda", dtype=torch.float64, generator=g) / k
+        ph = torch.rand((ns, 1), device="cuda", dtype=torch.float64, generator=g) * 2 * math.pi
+        U += amp * torch.cos(k * x[None, :] + ph)
+    per = {}
+    total = 0.0
+    for nH in (300, 900, 2700):
+        B.compute_fractions(U, nH, L, visc, dr, "gaussian")  # warm
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fr = B.compute_fractions(U, nH, L, visc, dr, "gaussian")
+        t1 = time.perf_counter()
+        th = B.smagorinsky_coefficients(U, nH, L, visc, dr, "gaussian")
+        t2 = time.perf_counter()
+        B.solve_smagorinsky(U, nH, th["classic"], L, visc, dr, "gaussian", tstop)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        per[str(nH)] = {"fractions_ms": 1e3 * (t1 - t0), "fit_ms": 1e3 * (t2 - t1), "solve_ms": 1e3 * (t3 - t2),
+                        "fractions": fr, "theta_classic": th["classic"], "theta_discret": th["discret"]}
+        total += t3 - t0
+    cpu = None
+    if not a.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as O
+
+        les, dbl = B.les_kernels(nh, 300, L, dr, "gaussian")
+        Uh = U[:10].cpu().numpy()
+        t0 = time.perf_counter()
+        O.burgers_fractions(Uh, 300, L, visc, les, dbl)
+        O.smagorinsky_sums(Uh, 300, L, visc, dr * L / 300, les, dbl)
+        H = L / 300
+        O.smagorinsky_solve(Uh, 300, L, visc, per["300"]["theta_classic"] ** 2 * ((dr * H) ** 2 + H ** 2), tstop,
+                            0.3, dbl)
+        tc = time.perf_counter() - t0
+        cpu = {"value": 1e3 * tc * ns / 10, "unit": "ms per 1000 samples (nH = 300 only)", "cores": 1,
+               "kind": "port", "sample": f"numpy oracle on 10 samples at nH = 300: {tc:.2f} s, scaled by 100"}
+    out = {"metric": "Burgers closure study: fractions + Smagorinsky fit + solve, ms per 1000 samples",
+           "value": 1e3 * total, "unit": "ms", "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": 1e3 * total,
+           "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "fp64",
+           "data": "synthetic snapshots: modes k<=10, amplitude N(0,1)/k, uniform phase, seed 123",
+           "config": {"workload": "nh=13500, nH in {300, 900, 2700}, Delta=2H Gaussian, 1000 samples, "
+                                  "Smagorinsky LES to t=0.1 (CFL 0.3)", "per_nH": per},
+           "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -110,6 +177,8 @@ def main():
     a = parse()
     if a.burgers and a.gpus == 1:
         return burgers_main(a)
+    if a.burgers_study and a.gpus == 1:
+        return burgers_study_main(a)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); the multi-GPU sharding is "
               "described in DESIGN.md and not part of this build", file=sys.stderr)

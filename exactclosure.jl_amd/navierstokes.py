@@ -300,6 +300,8 @@ def spectrum(u, stuff, poisson):
     half included, summed over the components (FP64).  Overwrites the
     solver's spectrum buffer, as the reference overwrites poisson.phat."""
     shells = stuff["shells"]
+    if not shells:  # n < 3: spectral_stuff selects no shell
+        return {"k": [], "s": []}
     nshell = max(shells) + 1
     out = (ctypes.c_double * nshell)()
     _ctx().call("ec_ns_spectrum", poisson.h, ptrs([x.data for x in u], P3), nshell, out)
