@@ -180,8 +180,8 @@ def main():
     if a.burgers_study and a.gpus == 1:
         return burgers_study_main(a)
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); the multi-GPU sharding is "
-              "described in DESIGN.md and not part of this build", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); sharding the step over several "
+              "GPUs is not part of this build", file=sys.stderr)
         sys.exit(2)
     import torch
 

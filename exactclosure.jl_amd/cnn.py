@@ -6,7 +6,7 @@ workload: 4 layers of 3x3x3 convolutions with periodic padding, 3 -> 32 ->
 32 -> 32 -> 3 channels, GELU between layers, BF16 weights and activations with
 FP32 accumulation, trained with Adam. Forward, input-gradient and
 weight-gradient kernels are hand-written MFMA implicit GEMMs
-(`csrc/ec_cnn.hip`). The exact closure stays the judge: `closure_error`
+(`csrc/ec_cnn.hip`). The exact closure stays the arbiter: `closure_error`
 compares a prediction with it, and `LearnedLES` falls back to it when the
 prediction misses by more than a tolerance.
 """
@@ -165,7 +165,7 @@ class LearnedLES:
     advanced through the same RK4 tableau as the driver's LES:
     du = F_H(u) + m_pred(u).  With `fallback_tol` set, a stage whose prediction
     misses the driver's exact closure by more than that relative L2 error uses
-    the exact closure instead (the exact closure is the judge and fallback).
+    the exact closure instead (the exact closure is the arbiter and fallback).
     `model=None` runs the no-model LES (m = 0)."""
 
     def __init__(self, drv, set_index, model=None, fallback_tol=None):

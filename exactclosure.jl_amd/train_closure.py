@@ -8,7 +8,7 @@
    (P_H V u, exact closure forcing m) of that grid is kept on the GPU.
 2. The CNN (3 -> 32 -> 32 -> 32 -> 3, 3x3x3, GELU, periodic) is trained with
    Adam on 7/8 of the pairs (the earliest), in BF16 with FP32 accumulation.
-3. The closure error of the held-out (latest) pairs is judged against the exact
+3. The closure error of the held-out (latest) pairs is measured against the exact
    closure (relative L2; the LES without a model scores 1).
 4. A posteriori: from the filtered DNS state, three LES run `post-steps` RK4
    steps beside the DNS: no model, the network, and the network with the
