@@ -161,3 +161,27 @@ def test_smagorinsky_solve(gpu, O, n, nH, kind, dr, C):
     _, dbl = B.les_kernels(n, nH, L, dr, kind)
     want = O.smagorinsky_solve(U, nH, L, visc, C, tstop, 0.3, dbl)
     assert np.abs(got - want).max() <= 1e-11 * np.abs(want).max()
+
+
+def test_study_entry_points_reject_bad_arguments(gpu):
+    """The closure-study entry points refuse inconsistent sizes before any
+    launch: nH not dividing n, kernels wider than the grid, an LES state
+    beyond the LDS of one block."""
+    import ctypes
+    ctx = gpu._abi.context()
+    dp = ctypes.POINTER(ctypes.c_double)
+    w = np.ones(3) / 3
+    u = torch.zeros((2, 90), dtype=torch.float64, device="cuda")
+    out = (ctypes.c_double * 4)()
+    with pytest.raises(gpu._abi.ECError):
+        ctx.call("ec_burgers_fractions", 90, 7, 1.0, 1e-3, 2, gpu._abi.c_void_p(u.data_ptr()), 1,
+                 w.ctypes.data_as(dp), 1, w.ctypes.data_as(dp), out)
+    big = np.ones(201) / 201
+    with pytest.raises(gpu._abi.ECError):
+        ctx.call("ec_burgers_smag_fit", 90, 9, 1.0, 1e-3, 0.1, 2, gpu._abi.c_void_p(u.data_ptr()), 100,
+                 big.ctypes.data_as(dp), 1, w.ctypes.data_as(dp), out)
+    ub = torch.empty((2, 9000), dtype=torch.float64, device="cuda")
+    u2 = torch.zeros((2, 18000), dtype=torch.float64, device="cuda")
+    with pytest.raises(gpu._abi.ECError):
+        ctx.call("ec_burgers_smag_solve", 18000, 9000, 1.0, 1e-3, 0.0, 0.1, 0.3, 2, gpu._abi.c_void_p(u2.data_ptr()),
+                 1, w.ctypes.data_as(dp), gpu._abi.c_void_p(ub.data_ptr()))
