@@ -267,6 +267,14 @@ __global__ void k_fft_x_c2r(const cplx<T> *__restrict__ in, T *__restrict__ out,
 // ---- Y / Z passes over tiles of TK consecutive kx columns ----
 // MODE 0: one transform with `sign`.  MODE 1 (Z pass only): forward, Poisson
 // scale, zero mode, inverse.
+// The Poisson scale 1 / (-4/h^2 sum sin^2) of a spectral bin: FP32 takes the
+// hardware reciprocal (1 ulp; the FP32 transforms round at ~1e-7 of the
+// field anyway), FP64 the correctly rounded division.
+__device__ __forceinline__ float poisson_factor(float scale, float denom) {
+    return scale * __builtin_amdgcn_rcpf(denom);
+}
+__device__ __forceinline__ double poisson_factor(double scale, double denom) { return scale / denom; }
+
 template <typename T, int MODE>
 __global__ void k_fft_cols(cplx<T> *__restrict__ data, int n, int ncols, long elem_stride, long outer_stride,
                            int TK, RadixPlan rp, const cplx<T> *__restrict__ tw, int sign,
@@ -294,7 +302,7 @@ __global__ void k_fft_cols(cplx<T> *__restrict__ data, int n, int ncols, long el
         for (int e = t; e < n; e += nthr) {
             const T denom = poisson_a * (sxy + eig[e]);
             const bool zero_mode = (kx == 0 && outer == 0 && e == 0);
-            const T f = zero_mode ? T(0) : scale / denom;
+            const T f = zero_mode ? T(0) : poisson_factor(scale, denom);
             cplx<T> z = lds[e * TK + c];
             z.x *= f;
             z.y *= f;
@@ -715,7 +723,7 @@ __global__ void __launch_bounds__(TK *NTHR) k_fft_cols_ct(cplx<T> *__restrict__ 
         for (int e = t; e < N; e += NTHR) {
             const T denom = poisson_a * (sxy + eigl[e]);
             const bool zero_mode = (kx == 0 && outer == 0 && e == 0);
-            const T f = zero_mode ? T(0) : scale / denom;
+            const T f = zero_mode ? T(0) : poisson_factor(scale, denom);
             cplx<T> z = lds[pad8(e) * TK + c];
             z.x *= f;
             z.y *= f;
