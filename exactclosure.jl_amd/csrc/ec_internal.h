@@ -92,6 +92,21 @@ __device__ __forceinline__ void ec_nt_store16(VT *p, const VT &v) {  // a 16-byt
     __builtin_memcpy(&w, &v, 16);
     __builtin_nontemporal_store(w, reinterpret_cast<u4 *>(p));
 }
+// Non-temporal stores through a raw buffer descriptor of a uniform base: the
+// address is the descriptor's base plus a 32-bit byte offset, so a kernel that
+// writes many fields at the same offsets (a plane of a tile) forms no per-lane
+// 64-bit addresses.  Word 3 is the gfx9 raw-buffer format (32-bit data);
+// num_records only bounds the offsets, which stay below 2^31.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t ec_rsrc(void *base) {
+    return __builtin_amdgcn_make_buffer_rsrc(base, (short)0, 0x7fffffff, 0x00020000);
+}
+__device__ __forceinline__ void ec_nt_store_at(__amdgpu_buffer_rsrc_t r, unsigned byte_off, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, byte_off, 0, 2 /* nt */);
+}
+__device__ __forceinline__ void ec_nt_store_at(__amdgpu_buffer_rsrc_t r, unsigned byte_off, double v) {
+    typedef unsigned u2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, byte_off, 0, 2 /* nt */);
+}
 template <typename T>
 __device__ __forceinline__ void ec_nt_store(cplx<T> *p, cplx<T> v) {
     typedef T v2 __attribute__((ext_vector_type(2)));

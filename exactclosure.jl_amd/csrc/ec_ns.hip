@@ -408,6 +408,7 @@ __global__ void __launch_bounds__(256) k_rhs_div_march(int n, int ZC, T ih, T vi
         const int k = z0 + rr, q = rr + 2;
         const size_t pk = n2 * (size_t)k + (size_t)x0 + (size_t)nn * (size_t)y0;  // tile corner in plane k
         if (rr + 2 < ZC) load(k + 3, ahead);
+        const auto r1 = ec_rsrc(du1 + pk), r2 = ec_rsrc(du2 + pk), r3 = ec_rsrc(du3 + pk), rd = ec_rsrc(dout + pk);
         const T *pm = ring[(q - 1) & 3], *p0 = ring[q & 3], *pp = ring[(q + 1) & 3];
         T a1[CPT], a2[CPT], a3[CPT];
 #pragma unroll
@@ -422,9 +423,9 @@ __global__ void __launch_bounds__(256) k_rhs_div_march(int n, int ZC, T ih, T vi
                 const int cy = cy0 + m * (256 / TX);
                 // streaming stores: the outputs are not re-read here, and keeping
                 // them out of L2 leaves it to the halo lines the neighbouring tiles share
-                __builtin_nontemporal_store(a1[m], (T *)((char *)(du1 + pk) + soff[m]));
-                __builtin_nontemporal_store(a2[m], (T *)((char *)(du2 + pk) + soff[m]));
-                __builtin_nontemporal_store(a3[m], (T *)((char *)(du3 + pk) + soff[m]));
+                ec_nt_store_at(r1, soff[m], a1[m]);
+                ec_nt_store_at(r2, soff[m], a2[m]);
+                ec_nt_store_at(r3, soff[m], a3[m]);
                 dx1[buf][cy * (TX + 1) + cx + 1] = a1[m];
                 dx2[buf][(cy + 1) * TX + cx] = a2[m];
             }
@@ -446,9 +447,9 @@ __global__ void __launch_bounds__(256) k_rhs_div_march(int n, int ZC, T ih, T vi
 #pragma unroll
             for (int m = 0; m < CPT; m++) {
                 const int cy = cy0 + m * (256 / TX);
-                __builtin_nontemporal_store((a1[m] - dx1[buf][cy * (TX + 1) + cx]) * ih +
-                                                (a2[m] - dx2[buf][cy * TX + cx]) * ih + (a3[m] - a3prev[m]) * ih,
-                                            (T *)((char *)(dout + pk) + soff[m]));
+                ec_nt_store_at(rd, soff[m],
+                               (a1[m] - dx1[buf][cy * (TX + 1) + cx]) * ih + (a2[m] - dx2[buf][cy * TX + cx]) * ih +
+                                   (a3[m] - a3prev[m]) * ih);
             }
         }
 #pragma unroll
