@@ -218,3 +218,52 @@ def test_filter_vel_rk4_fused(gpu, n, prec, sets, stage):
         for x, y in zip(xs, ys):
             assert float((x - y).abs().max()) <= tol * float(x.abs().max())
 
+
+
+@pytest.mark.parametrize("n", [512, 1024])
+@pytest.mark.parametrize("stage", [1, 3])
+def test_filter_vel_rk4_fused_large(gpu, n, stage):
+    """Full-size FP32 grids with the benchmark's set list and its default
+    256-plane chunks (every tile edge, chunk edge and emit of the launch): the
+    fused filter + RK4 stage against ec_ns_filter_vel followed by
+    ec_ns_rk4_update on the same fields, the filter outputs bit for bit and
+    the stage state to FP32 rounding (stage 1 reads and writes acc, stage 3
+    writes u0)."""
+    sets = [(16, 0), (8, 0), (4, 0), (4, 1)]
+    gen = torch.Generator(device="cuda").manual_seed(n + stage)
+    f = lambda: torch.randn((n, n, n), device="cuda", generator=gen)  # noqa: E731
+    du = [f() for _ in range(3)]
+    p = f()
+    state_a = [[f() for _ in range(3)] for _ in range(3)]  # u0, us, acc
+    state_b = [[t.clone() for t in xs] for xs in state_a]
+
+    def outsets():
+        outs = []
+        arr = (gpu._abi.VelSet * len(sets))()
+        for i, (c, kind) in enumerate(sets):
+            o = [torch.empty((n // c,) * 3, dtype=torch.float64, device="cuda") for _ in range(3)]
+            outs.append(o)
+            arr[i].nbar = n // c
+            arr[i].kind = kind
+            for a in range(3):
+                arr[i].out[a] = o[a].data_ptr()
+        return outs, arr
+
+    oa, arra = outsets()
+    ob, arrb = outsets()
+    ctx = gpu._abi.context()
+    step = 1e-4
+    ctx.call("ec_ns_filter_vel", 0, n, 1.0, P(du, 3), p.data_ptr(), len(sets), arra)
+    ctx.call("ec_ns_rk4_update", 0, n, 1.0, stage, step, P(state_a[0], 3), P(state_a[1], 3), P(du, 3),
+             p.data_ptr(), P(state_a[2], 3))
+    ctx.call("ec_ns_filter_vel_rk4", 0, n, 1.0, P(du, 3), gpu._abi.c_void_p(p.data_ptr()), len(sets), arrb,
+             stage, step, P(state_b[0], 3), P(state_b[1], 3), P(state_b[2], 3))
+    torch.cuda.synchronize()
+    for x, y in zip(oa, ob):
+        for a in range(3):
+            assert torch.equal(x[a], y[a])
+    for xs, ys in zip(state_a, state_b):
+        for x, y in zip(xs, ys):
+            assert float((x - y).abs().max()) <= 1e-6 * float(x.abs().max())
+    del du, p, state_a, state_b, oa, ob
+    torch.cuda.empty_cache()
