@@ -230,7 +230,7 @@ def main():
     tuned = {}
     if not a.no_autotune:
         tuner = ec.autotune.Autotuner()
-        tuned["filter_vel_rk4_zchunk"] = ec.autotune.tune_filter_vel(tuner, drv)
+        tuned["filter_vel_rk4_zchunk"] = ec.autotune.tune_filter_vel(tuner, drv, dt)
         torch.cuda.synchronize()
         log(f"[bench] autotune: {tuner.log}")
 

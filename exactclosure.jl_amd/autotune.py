@@ -93,13 +93,15 @@ def _denorm(s, candidates):
     raise KeyError(s)
 
 
-def tune_filter_vel(tuner, drv, candidates=(64, 128, 256, 512)):
-    """Planes per block of the forcing-form filter, on the driver's own fields:
-    the filter with the fused DNS RK4 stage as the step runs it.  One timed run
-    is stages 0, 1 and 2 with dt = 0 (the stages differ in the stage state they
-    stream, and stage 0 alone did not rank the chunks as the step does); these
-    stages write only the stage state us and acc, which the next step's stage 0
-    rewrites, so call this between steps: the DNS state u0 is untouched."""
+def tune_filter_vel(tuner, drv, dt=None, candidates=(64, 128, 256, 512)):
+    """Planes per block of the forcing-form filter with the fused DNS RK4
+    stage, on the driver's own fields.  With dt, one timed run is a whole
+    DNS-aided step (it advances the driver; call it between steps): the chunk
+    changes how the filter's halo lines meet the other kernels' data in L2,
+    and isolated launches ranked 128 above 256 on a box where 256 made the
+    step 3% faster.  Without dt, one run is stages 0, 1 and 2 with dt = 0,
+    which write only the stage state us and acc (the next step's stage 0
+    rewrites them) and leave the DNS state u0 untouched."""
     from . import _abi
     from ._abi import P3, ptrs
 
@@ -107,15 +109,17 @@ def tune_filter_vel(tuner, drv, candidates=(64, 128, 256, 512)):
     outs = [L.g_k for L in drv.les]
     n, l = drv.g.n, drv.g.l
 
-    def run():
+    def run_stages():
         for st in range(3):
             ctx.call("ec_ns_filter_vel_rk4", drv.dt_code, n, l, ptrs(drv.du, P3), _abi.c_void_p(drv.ps.p_ptr),
                      len(drv.les), drv._vel_sets(outs), st, 0.0, ptrs(drv.u0, P3), ptrs(drv.us, P3),
                      ptrs(drv.acc, P3))
 
+    run = run_stages if dt is None else (lambda: drv.step(dt))
     cands = [z for z in candidates if n % z == 0] or [n]
-    return tuner.tune("filter_vel_rk4_zchunk", f"n{n}|" + ",".join(f"{L.g.n}{L.kind}" for L in drv.les), cands,
-                      lambda z: ctx.call("ec_filter_vel_set_tiles", 16, int(z)), run)
+    key = f"n{n}|" + ",".join(f"{L.g.n}{L.kind}" for L in drv.les) + ("|step" if dt is not None else "|stages")
+    return tuner.tune("filter_vel_rk4_zchunk", key, cands, lambda z: ctx.call("ec_filter_vel_set_tiles", 16, int(z)),
+                      run)
 
 
 def tune_burgers_threads(tuner, batch, nsteps=20, candidates=(128, 256, 512, 1024)):

@@ -496,14 +496,12 @@ int tiled_launch(ec_ctx *ctx, int n, double l, double visc, void *const u[3], co
 
 }  // namespace
 
-static int g_tile_x = 16, g_tile_y = 16, g_tile_z = 64;
-
 extern "C" int ec_closure_set_tiles(ec_ctx *ctx, int tx, int ty, int tz) {
     EC_REQUIRE(ctx, ctx, "null context");
     EC_REQUIRE(ctx, tx >= 1 && ty >= 1 && tz >= 1, "tile targets must be positive");
-    g_tile_x = tx;
-    g_tile_y = ty;
-    g_tile_z = tz;
+    ctx->cl_tx = tx;
+    ctx->cl_ty = ty;
+    ctx->cl_tz = tz;
     return EC_OK;
 }
 
@@ -520,6 +518,6 @@ int ec_closure_tiled(ec_ctx *ctx, int dtype, int n, double l, double visc, void 
         for (int m = 0; m < 9; m++)
             if (!sets[s].flux[m]) return EC_E_UNSUPPORTED;
     }
-    return dtype == EC_F64 ? tiled_launch<double>(ctx, n, l, visc, u, p, nsets, sets, g_tile_x, g_tile_y, g_tile_z)
-                           : tiled_launch<float>(ctx, n, l, visc, u, p, nsets, sets, g_tile_x, g_tile_y, g_tile_z);
+    return dtype == EC_F64 ? tiled_launch<double>(ctx, n, l, visc, u, p, nsets, sets, ctx->cl_tx, ctx->cl_ty, ctx->cl_tz)
+                           : tiled_launch<float>(ctx, n, l, visc, u, p, nsets, sets, ctx->cl_tx, ctx->cl_ty, ctx->cl_tz);
 }

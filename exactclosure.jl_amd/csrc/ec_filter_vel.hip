@@ -761,12 +761,10 @@ int filter_vel_t(ec_ctx *ctx, int n, double l, void *const v[3], const void *p, 
 
 }  // namespace
 
-static int g_fv_tz = 256;  // z planes per block (tile chunk); 256 is fastest at 1024^3
-
 // tx is kept for the interface; the column tile edge is fixed per set list
 extern "C" int ec_filter_vel_set_tiles(ec_ctx *ctx, int tx, int tz) {
     EC_REQUIRE(ctx, ctx && tx >= 1 && tz >= 1, "ec_filter_vel_set_tiles: bad argument");
-    g_fv_tz = tz;
+    ctx->fv_tz = tz;
     return EC_OK;
 }
 
@@ -779,8 +777,8 @@ extern "C" int ec_ns_filter_vel(ec_ctx *ctx, int dtype, int n, double l, void *c
         EC_REQUIRE(ctx, sets[s].kind == 0 || sets[s].kind == 1, "set %d: kind %d", s, sets[s].kind);
         for (int a = 0; a < 3; a++) EC_REQUIRE(ctx, sets[s].out[a], "set %d: null output", s);
     }
-    int rc = dtype == EC_F64 ? filter_vel_t<double>(ctx, n, l, v, p, nsets, sets, g_fv_tz)
-                             : filter_vel_t<float>(ctx, n, l, v, p, nsets, sets, g_fv_tz);
+    int rc = dtype == EC_F64 ? filter_vel_t<double>(ctx, n, l, v, p, nsets, sets, ctx->fv_tz)
+                             : filter_vel_t<float>(ctx, n, l, v, p, nsets, sets, ctx->fv_tz);
     if (rc != EC_E_UNSUPPORTED) return rc;
     // any other compression: the per-quantity separable filter
     static const int src_k[3] = {EC_SRC_K1, EC_SRC_K2, EC_SRC_K3};
@@ -841,8 +839,8 @@ extern "C" int ec_ns_filter_vel_rk4(ec_ctx *ctx, int dtype, int n, double l, voi
         rk.acc[a] = acc[a];
     }
     rk.dt = dt;
-    int rc = dtype == EC_F64 ? filter_vel_rk4_t<double>(ctx, n, l, du, p, nsets, sets, g_fv_tz, stage, rk)
-                             : filter_vel_rk4_t<float>(ctx, n, l, du, p, nsets, sets, g_fv_tz, stage, rk);
+    int rc = dtype == EC_F64 ? filter_vel_rk4_t<double>(ctx, n, l, du, p, nsets, sets, ctx->fv_tz, stage, rk)
+                             : filter_vel_rk4_t<float>(ctx, n, l, du, p, nsets, sets, ctx->fv_tz, stage, rk);
     if (rc != EC_E_UNSUPPORTED) return rc;
     // any other set list: the filter, then the stage update
     if ((rc = ec_ns_filter_vel(ctx, dtype, n, l, du, p, nsets, sets))) return rc;

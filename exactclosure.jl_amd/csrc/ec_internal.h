@@ -32,6 +32,11 @@ struct ec_ctx {
     std::vector<ec_prof_rec> prof_recs;
     std::vector<hipEvent_t> event_pool;
     size_t event_next = 0;
+    // Launch-shape choices of this context (set by the autotuner or by hand):
+    // z planes per block of the velocity filter (256 is fastest at 1024^3) and
+    // the x/y/z block targets of the flux-form closure kernel.
+    int fv_tz = 256;
+    int cl_tx = 16, cl_ty = 16, cl_tz = 64;
 };
 
 // ---- error handling: no C++ exception crosses the C ABI ----

@@ -227,11 +227,13 @@ int ec_ns_filter_vel(ec_ctx *ctx, int dtype, int n, double l, void *const v[3], 
 int ec_ns_filter_vel_rk4(ec_ctx *ctx, int dtype, int n, double l, void *const du[3], const void *p,
                          int nsets, const ec_vel_set *sets, int stage, double dt, void *const u0[3],
                          void *const us[3], void *const acc[3]);
-/* Target block size (x = y, z) of ec_ns_filter_vel; set by the tile autotuner. */
+/* Target block size (x = y, z) of ec_ns_filter_vel and ec_ns_filter_vel_rk4 on
+ * this context (default z 256); set by the tile autotuner. */
 int ec_filter_vel_set_tiles(ec_ctx *ctx, int tx, int tz);
-/* Target block size (fine cells x, y, z) of the one-launch closure kernel; the
- * library picks the largest multiples of every compression not above them.
- * Set by the tile autotuner; no counterpart in the reference. */
+/* Target block size (fine cells x, y, z) of the one-launch closure kernel on
+ * this context (default 16, 16, 64); the library picks the largest multiples
+ * of every compression not above them.  Set by the tile autotuner; no
+ * counterpart in the reference. */
 int ec_closure_set_tiles(ec_ctx *ctx, int tx, int ty, int tz);
 /* tau = flux - sigma_H(ubar) in FP64 on the LES grid (the closure commutator) */
 int ec_ns_commutator(ec_ctx *ctx, int nbar, double l, double visc, void *const ubar[3],
