@@ -98,8 +98,9 @@ def tune_filter_vel(tuner, drv, dt=None, candidates=(64, 128, 256, 512)):
     stage, on the driver's own fields.  With dt, one timed run is a whole
     DNS-aided step (it advances the driver; call it between steps): the chunk
     changes how the filter's halo lines meet the other kernels' data in L2,
-    and isolated launches ranked 128 above 256 on a box where 256 made the
-    step 3% faster.  Without dt, one run is stages 0, 1 and 2 with dt = 0,
+    and isolated launches have ranked the chunks differently from whole steps
+    (stage 0 alone once chose 512 planes, 2 ms per step slower in the step
+    than 256).  Without dt, one run is stages 0, 1 and 2 with dt = 0,
     which write only the stage state us and acc (the next step's stage 0
     rewrites them) and leave the DNS state u0 untouched."""
     from . import _abi
