@@ -1235,38 +1235,48 @@ static int randomfield_t(ec_ctx *ctx, ec_poisson *ps, int profile, double param,
     const int nshell = kdiag + 1;
     double *dhist = nullptr;
     EC_HIP(ctx, hipMallocAsync((void **)&dhist, sizeof(double) * 2 * nshell, ctx->stream));
-    EC_HIP(ctx, hipMemsetAsync(dhist, 0, sizeof(double) * 2 * nshell, ctx->stream));
-    const double inv_n3 = 1.0 / ((double)n * n * n);
-    const size_t nspec = (size_t)nh * n * n;
-    const int blocks = (int)std::min<size_t>((nspec + 255) / 256, 4096);
-    for (int c = 0; c < 3; c++) {
-        if ((rc = ec_fft_r2c(ctx, ps, u[c], ps->spec))) return rc;
-        k_shell_hist<T><<<blocks, 256, sizeof(double) * nshell, ctx->stream>>>((const cplx<T> *)ps->spec, n, nh,
-                                                                                ps->pitch, kmax, nshell, inv_n3, dhist);
-        EC_CHECK_LAUNCH(ctx, "k_shell_hist");
-    }
-    std::vector<double> hist(nshell), factor(nshell);
-    EC_HIP(ctx, hipMemcpyAsync(hist.data(), dhist, sizeof(double) * nshell, hipMemcpyDeviceToHost, ctx->stream));
-    EC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    double totalprofile = 0.0;
-    for (int k = 0; k <= kdiag; k++) totalprofile += profile_eval(profile, param, k);
-    for (int k = 0; k <= kdiag; k++) {
-        const double E0 = totalenergy * profile_eval(profile, param, k) / totalprofile;
-        factor[k] = sqrt(E0 / hist[k]);
-    }
-    EC_HIP(ctx, hipMemcpyAsync(dhist + nshell, factor.data(), sizeof(double) * nshell, hipMemcpyHostToDevice, ctx->stream));
-    for (int c = 0; c < 3; c++) {
-        if ((rc = ec_fft_r2c(ctx, ps, u[c], ps->spec))) return rc;
-        // rfft / n^3, times the shell factor; the inverse then runs unnormalised
-        k_shell_scale<T><<<blocks, 256, 0, ctx->stream>>>((cplx<T> *)ps->spec, n, nh, ps->pitch, nshell,
-                                                          dhist + nshell, inv_n3);
-        EC_CHECK_LAUNCH(ctx, "k_shell_scale");
-        if ((rc = cols<T>(ctx, ps, ps->spec, 2, +1, 0, ps->pitch))) return rc;
-        if ((rc = cols<T>(ctx, ps, ps->spec, 1, +1, 0, ps->pitch))) return rc;
-        if ((rc = c2r_x<T>(ctx, ps, ps->spec, u[c], 1.0, ps->pitch))) return rc;
-    }
-    EC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    EC_HIP(ctx, hipFreeAsync(dhist, ctx->stream));
+    // the shell sums and factors live until the end of the body; every exit frees them
+    auto body = [&]() -> int {
+        EC_HIP(ctx, hipMemsetAsync(dhist, 0, sizeof(double) * 2 * nshell, ctx->stream));
+        const double inv_n3 = 1.0 / ((double)n * n * n);
+        const size_t nspec = (size_t)nh * n * n;
+        const int blocks = (int)std::min<size_t>((nspec + 255) / 256, 4096);
+        int rc2;
+        for (int c = 0; c < 3; c++) {
+            if ((rc2 = ec_fft_r2c(ctx, ps, u[c], ps->spec))) return rc2;
+            k_shell_hist<T><<<blocks, 256, sizeof(double) * nshell, ctx->stream>>>(
+                (const cplx<T> *)ps->spec, n, nh, ps->pitch, kmax, nshell, inv_n3, dhist);
+            EC_CHECK_LAUNCH(ctx, "k_shell_hist");
+        }
+        std::vector<double> hist(nshell), factor(nshell);
+        EC_HIP(ctx, hipMemcpyAsync(hist.data(), dhist, sizeof(double) * nshell, hipMemcpyDeviceToHost, ctx->stream));
+        EC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        double totalprofile = 0.0;
+        for (int k = 0; k <= kdiag; k++) totalprofile += profile_eval(profile, param, k);
+        for (int k = 0; k <= kdiag; k++) {
+            const double E0 = totalenergy * profile_eval(profile, param, k) / totalprofile;
+            factor[k] = sqrt(E0 / hist[k]);
+        }
+        EC_HIP(ctx, hipMemcpyAsync(dhist + nshell, factor.data(), sizeof(double) * nshell, hipMemcpyHostToDevice,
+                                   ctx->stream));
+        for (int c = 0; c < 3; c++) {
+            if ((rc2 = ec_fft_r2c(ctx, ps, u[c], ps->spec))) return rc2;
+            // rfft / n^3, times the shell factor; the inverse then runs unnormalised
+            k_shell_scale<T><<<blocks, 256, 0, ctx->stream>>>((cplx<T> *)ps->spec, n, nh, ps->pitch, nshell,
+                                                              dhist + nshell, inv_n3);
+            EC_CHECK_LAUNCH(ctx, "k_shell_scale");
+            if ((rc2 = cols<T>(ctx, ps, ps->spec, 2, +1, 0, ps->pitch))) return rc2;
+            if ((rc2 = cols<T>(ctx, ps, ps->spec, 1, +1, 0, ps->pitch))) return rc2;
+            if ((rc2 = c2r_x<T>(ctx, ps, ps->spec, u[c], 1.0, ps->pitch))) return rc2;
+        }
+        // the host factor vector must outlive the copy above
+        EC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        return EC_OK;
+    };
+    rc = body();
+    const hipError_t fe = hipFreeAsync(dhist, ctx->stream);
+    if (rc != EC_OK) return rc;
+    if (fe != hipSuccess) return ec_fail(ctx, EC_E_HIP, "ec_ns_randomfield: free failed");
     return ec_ns_project(ctx, ps, u);
 }
 
