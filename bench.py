@@ -87,8 +87,9 @@ def burgers_main(a):
 
 
 def burgers_study_main(a):
-    """compute_fractions (src/Burgers.jl:1022-1099), the Smagorinsky fits
-    (smagorinsky.jl:24-36) and solve_smagorinsky (src/Burgers.jl:682-772) on
+    """compute_fractions (src/Burgers.jl:1022-1099), compute_dissipation
+    (src/Burgers.jl:447-516), the Smagorinsky fits (smagorinsky.jl:24-36) and
+    solve_smagorinsky (src/Burgers.jl:682-772) on
     the reference setup (src/Burgers.jl:236-246), all samples on the GPU in
     FP64; snapshots are synthetic (modes k <= 10, amplitude N(0,1)/k, uniform
     phase).  The oracle's numpy restatement on 10 samples is the CPU leg."""
@@ -116,13 +117,18 @@ def burgers_study_main(a):
         t0 = time.perf_counter()
         fr = B.compute_fractions(U, nH, L, visc, dr, "gaussian")
         t1 = time.perf_counter()
+        di = B.compute_dissipation(U, nH, L, visc, dr, "gaussian")
+        t1d = time.perf_counter()
         th = B.smagorinsky_coefficients(U, nH, L, visc, dr, "gaussian")
         t2 = time.perf_counter()
         B.solve_smagorinsky(U, nH, th["classic"], L, visc, dr, "gaussian", tstop)
         torch.cuda.synchronize()
         t3 = time.perf_counter()
-        per[str(nH)] = {"fractions_ms": 1e3 * (t1 - t0), "fit_ms": 1e3 * (t2 - t1), "solve_ms": 1e3 * (t3 - t2),
-                        "fractions": fr, "theta_classic": th["classic"], "theta_discret": th["discret"]}
+        per[str(nH)] = {"fractions_ms": 1e3 * (t1 - t0), "dissipation_ms": 1e3 * (t1d - t1),
+                        "fit_ms": 1e3 * (t2 - t1d), "solve_ms": 1e3 * (t3 - t2), "fractions": fr,
+                        "mean_dissipation": {k: float(di[k].mean()) for k in ("class_m", "class_p", "swapfil")},
+                        "theta_classic": th["classic"], "theta_discret": th["discret"]}
+        del di
         total += t3 - t0
     cpu = None
     if not a.no_cpu_baseline:
@@ -133,6 +139,7 @@ def burgers_study_main(a):
         Uh = U[:10].cpu().numpy()
         t0 = time.perf_counter()
         O.burgers_fractions(Uh, 300, L, visc, les, dbl)
+        O.burgers_dissipation(Uh, 300, L, visc, les, dbl)
         O.smagorinsky_sums(Uh, 300, L, visc, dr * L / 300, les, dbl)
         H = L / 300
         O.smagorinsky_solve(Uh, 300, L, visc, per["300"]["theta_classic"] ** 2 * ((dr * H) ** 2 + H ** 2), tstop,
@@ -140,7 +147,7 @@ def burgers_study_main(a):
         tc = time.perf_counter() - t0
         cpu = {"value": 1e3 * tc * ns / 10, "unit": "ms per 1000 samples (nH = 300 only)", "cores": 1,
                "kind": "port", "sample": f"numpy oracle on 10 samples at nH = 300: {tc:.2f} s, scaled by 100"}
-    out = {"metric": "Burgers closure study: fractions + Smagorinsky fit + solve, ms per 1000 samples",
+    out = {"metric": "Burgers closure study: fractions + dissipation + Smagorinsky fit + solve, ms per 1000 samples",
            "value": 1e3 * total, "unit": "ms", "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": 1e3 * total,
            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "fp64",
            "data": "synthetic snapshots: modes k<=10, amplitude N(0,1)/k, uniform phase, seed 123",

@@ -117,6 +117,8 @@ _SIGS = {
     "ec_burgers_fractions": (c_int, [c_void_p, c_int, c_int, c_double, c_double, c_int, c_void_p, c_int,
                                      ctypes.POINTER(c_double), c_int, ctypes.POINTER(c_double),
                                      ctypes.POINTER(c_double)]),
+    "ec_burgers_dissipation": (c_int, [c_void_p, c_int, c_int, c_double, c_double, c_int, c_void_p, c_int,
+                                       ctypes.POINTER(c_double), c_int, ctypes.POINTER(c_double), c_void_p]),
     "ec_burgers_smag_fit": (c_int, [c_void_p, c_int, c_int, c_double, c_double, c_double, c_int, c_void_p, c_int,
                                     ctypes.POINTER(c_double), c_int, ctypes.POINTER(c_double),
                                     ctypes.POINTER(c_double)]),

@@ -84,6 +84,29 @@ def compute_fractions(u, nH, L=2 * math.pi, visc=5e-4, delta_ratio=1.0, lesfilte
     return {"classic": out[0], "flux": out[1], "div": out[2]}
 
 
+def compute_dissipation(u, nH, L=2 * math.pi, visc=5e-4, delta_ratio=1.0, lesfiltertype="tophat"):
+    """compute_dissipation for one (nH, delta_ratio) (src/Burgers.jl:447-516):
+    the sub-filter dissipation coefficients sigma * delta_stag(vu) at every
+    coarse point of every sample for the stresses class_m (vsu - s_h(D*u)),
+    class_p (vsu - s_H(vu)) and swapfil (fsu - s_H(vu)), FP64 on the GPU.
+    u: (nsamples, nh) FP64 CUDA tensor.  Returns the reference's named tuple:
+    nomodel (one zero) and one (nsamples * nH,) CUDA tensor per model, coarse
+    point fastest, then sample (vec(d[:, m, :]))."""
+    import torch
+
+    assert u.dtype == torch.float64 and u.is_cuda and u.dim() == 2 and u.is_contiguous()
+    ns, nh = u.shape
+    (Rl, wl), (K, wd) = les_kernels(nh, nH, L, delta_ratio, lesfiltertype)
+    wl = np.ascontiguousarray(wl, dtype=np.float64)
+    wd = np.ascontiguousarray(wd, dtype=np.float64)
+    out = torch.empty((3, ns * nH), dtype=torch.float64, device=u.device)
+    dp = ctypes.POINTER(ctypes.c_double)
+    _abi.context().call("ec_burgers_dissipation", nh, nH, L, visc, ns, _abi.c_void_p(u.data_ptr()), Rl,
+                        wl.ctypes.data_as(dp), K, wd.ctypes.data_as(dp), _abi.c_void_p(out.data_ptr()))
+    return {"nomodel": torch.zeros(1, dtype=torch.float64, device=u.device), "class_m": out[0],
+            "class_p": out[1], "swapfil": out[2]}
+
+
 def smagorinsky_coefficients(U, nH, L=2 * math.pi, visc=5e-4, delta_ratio=2.0, lesfiltertype="gaussian"):
     """theta_classic and theta_discret (smagorinsky.jl:24-36 over the fields
     of smagorinsky_fields, src/Burgers.jl:547-680): least-squares Smagorinsky

@@ -599,6 +599,71 @@ extern "C" int ec_burgers_fractions(ec_ctx *ctx, int n, int nH, double L, double
     return EC_OK;
 }
 
+// ---- SFS dissipation coefficients (compute_dissipation, src/Burgers.jl:447-516) ----
+// Per coarse point J of sample q, from the filtered quantities of the
+// fractions pass: sigma_class_m = vsu - sVU, sigma_class_p = vsu - svu and
+// sigma_swapfil = fsu - svu, each times delta_stag(vu)_J = (vu_J - vu_{J-1}) / H
+// (dissipation, src/Burgers.jl:204-208).  compute_dissipation calls
+// three-argument forms of stress, delta_stag and the coarse-grain convolutions
+// that src/Burgers.jl does not define; they are read as the two-argument stress
+// and delta_stag (src/Burgers.jl:86-94) and the coarse-graining of
+// compute_fractions, whose quantities they name.  One thread per coarse point;
+// out[(m * ns + q) * nH + J] with m = 0 class_m, 1 class_p, 2 swapfil.
+__global__ void __launch_bounds__(256) k_bdiss_coarse(int n, int nH, int ns, double ih, double iH, double visc,
+                                                      const double *__restrict__ s, const double *__restrict__ vu_f,
+                                                      int Rl, const double *__restrict__ wl, int K,
+                                                      const double *__restrict__ wd, double *__restrict__ out) {
+    const int comp = n / nH, a = comp / 2;
+    const long total = (long)nH * ns;
+    for (long I = blockIdx.x * (long)blockDim.x + threadIdx.x; I < total; I += (long)gridDim.x * blockDim.x) {
+        const int J = (int)(I % nH);
+        const long q = I / nH;
+        const double *ss = s + q * n, *vf = vu_f + q * n;
+        const int ic = (J + 1) * comp - a - 1;  // fine centre of coarse cell J
+        double fsu = 0.0, vsu = 0.0;
+        for (int r = -Rl; r <= Rl; r++) fsu += wl[r + Rl] * ss[bwrap(ic + r, n)];
+        for (int k = -K; k <= K; k++) vsu += wd[k + K] * ss[bwrap(ic + k, n)];
+        const double vb = vf[(J + 1) * comp - 1], va = vf[bwrap(J * comp - 1, n)];  // coarse faces J, J-1
+        const double svu = -((va + vb) * (va + vb)) / 8 + visc * (vb - va) * iH;
+        const double sVU = bstress(vf[bwrap(ic - 1, n)], vf[ic], ih, visc);
+        const double dv = (vb - va) * iH;
+        out[I] = (vsu - sVU) * dv;
+        out[total + I] = (vsu - svu) * dv;
+        out[2 * total + I] = (fsu - svu) * dv;
+    }
+}
+
+extern "C" int ec_burgers_dissipation(ec_ctx *ctx, int n, int nH, double L, double visc, int nsamples,
+                                      const void *u, int Rl, const double *wl, int K, const double *wd, void *out) {
+    EC_REQUIRE(ctx, ctx && u && wl && wd && out, "ec_burgers_dissipation: null argument");
+    EC_REQUIRE(ctx, n >= 2 && nH >= 1 && n % nH == 0 && nsamples >= 1 && L > 0,
+               "ec_burgers_dissipation: need n %% nH == 0 (n=%d, nH=%d), nsamples=%d >= 1", n, nH, nsamples);
+    EC_REQUIRE(ctx, Rl >= 0 && K >= 0 && 2 * Rl + 1 <= n && 2 * K + 1 <= n,
+               "ec_burgers_dissipation: kernel radii Rl=%d, K=%d must fit the grid n=%d", Rl, K, n);
+    const long total = (long)n * nsamples, ctotal = (long)nH * nsamples;
+    double *buf = nullptr;
+    const size_t bytes = sizeof(double) * ((size_t)2 * total + (2 * Rl + 1) + (2 * K + 1));
+    EC_HIP(ctx, hipMallocAsync((void **)&buf, bytes, ctx->stream));
+    double *s = buf, *vf = buf + total, *dwl = vf + total, *dwd = dwl + (2 * Rl + 1);
+    int rc = EC_OK;
+    if (hipMemcpyAsync(dwl, wl, sizeof(double) * (2 * Rl + 1), hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(dwd, wd, sizeof(double) * (2 * K + 1), hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+        rc = ec_fail(ctx, EC_E_HIP, "ec_burgers_dissipation: kernel upload failed");
+    if (rc == EC_OK) {
+        const int fblocks = (int)std::min<long>((total + 255) / 256, 65536);
+        const int cblocks = (int)std::min<long>((ctotal + 255) / 256, 65536);
+        k_bfrac_fine<<<fblocks, 256, 0, ctx->stream>>>(n, nsamples, n / L, visc, (const double *)u, K, dwd, s, vf);
+        k_bdiss_coarse<<<cblocks, 256, 0, ctx->stream>>>(n, nH, nsamples, n / L, nH / L, visc, s, vf, Rl, dwl, K,
+                                                         dwd, (double *)out);
+        // the weights were copied from host memory the caller may free on return
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess)
+            rc = ec_fail(ctx, EC_E_HIP, "ec_burgers_dissipation: kernels failed");
+    }
+    const hipError_t fe = hipFreeAsync(buf, ctx->stream);
+    if (rc == EC_OK && fe != hipSuccess) rc = ec_fail(ctx, EC_E_HIP, "ec_burgers_dissipation: free failed");
+    return rc;
+}
+
 // ---- Smagorinsky study (smagorinsky_fields, src/Burgers.jl:547-680; the
 // least-squares coefficients of smagorinsky.jl:24-36; solve_smagorinsky,
 // src/Burgers.jl:682-772) ----

@@ -274,6 +274,16 @@ int ec_burgers_rhs(ec_ctx *ctx, int dtype, int n, double L, double visc, const v
 int ec_burgers_fractions(ec_ctx *ctx, int n, int nH, double L, double visc, int nsamples,
                          const void *u, int Rl, const double *wl, int K, const double *wd,
                          double out[3]);
+/* SFS dissipation coefficients (compute_dissipation, src/Burgers.jl:447-516):
+ * per coarse point and sample, sigma * (vu_J - vu_{J-1}) / H for the three
+ * sub-filter stresses class_m = vsu - s_h(D*u), class_p = vsu - s_H(vu) and
+ * swapfil = fsu - s_H(vu), FP64.  out (device, 3 nH nsamples doubles) holds
+ * out[(m nsamples + q) nH + J], m = 0 class_m, 1 class_p, 2 swapfil: each
+ * model's block is the reference's vec(d[:, m, :]).  Kernels as for
+ * ec_burgers_fractions; returns when out is written. */
+int ec_burgers_dissipation(ec_ctx *ctx, int n, int nH, double L, double visc, int nsamples,
+                           const void *u, int Rl, const double *wl, int K, const double *wd,
+                           void *out);
 /* Smagorinsky coefficient fit (smagorinsky_fields, src/Burgers.jl:547-680, and
  * smagorinsky.jl:24-36) over nsamples FP64 DNS snapshots u (device): out =
  * {<S_h,T_h>, <S_h,S_h>, <S_H,T_H>, <S_H,S_H>} summed over points and samples

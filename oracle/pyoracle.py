@@ -231,6 +231,39 @@ def burgers_fractions(U, nH, L, visc, les, dbl, parts=False):
     return np.array([float(np.mean(x)) for x in per])
 
 
+def burgers_dissipation(U, nH, L, visc, les, dbl):
+    """compute_dissipation (src/Burgers.jl:447-516) for the DNS samples U
+    (nsamples, n), FP64: d = sigma * delta_stag(vu) (dissipation,
+    src/Burgers.jl:204-208) for sigma_class_m = vsu - sVU, sigma_class_p =
+    vsu - svu and sigma_swapfil = fsu - svu, with the filtered quantities of
+    compute_fractions (same restatement as burgers_fractions above) and
+    delta_stag(vu)_J = (vu_J - vu_{J-1}) / H (src/Burgers.jl:86).  The
+    three-argument stress / delta_stag / coarse-grain convolutions that
+    compute_dissipation calls are not defined in src/Burgers.jl; they are read
+    as these two-argument forms.  Returns (class_m, class_p, swapfil), each of
+    shape (nsamples, nH)."""
+    U = np.asarray(U, dtype=np.float64)
+    ns, n = U.shape
+    comp = n // nH
+    a = comp // 2
+    h, H = L / n, L / nH
+    Rl, wl = les
+    K, wd = dbl
+    J1 = np.arange(1, nH + 1)
+    centre = J1 * comp - a - 1
+    face = J1 * comp - 1
+    s = _bstress(np.roll(U, 1, axis=-1), U, h, visc)
+    VU = _bfilt(U, K, wd)
+    fsu = _bfilt(s, Rl, wl)[:, centre]
+    vsu = _bfilt(s, K, wd)[:, centre]
+    vu = VU[:, face]
+    vum = np.roll(vu, 1, axis=-1)  # vu[J - 1]
+    svu = _bstress(vum, vu, H, visc)
+    sVU = _bstress(np.roll(VU, 1, axis=-1), VU, h, visc)[:, centre]
+    dv = (vu - vum) / H
+    return (vsu - sVU) * dv, (vsu - svu) * dv, (fsu - svu) * dv
+
+
 def _bstress(ua, ub, sp, visc):
     return -((ua + ub) ** 2) / 8 + visc * (ub - ua) / sp
 

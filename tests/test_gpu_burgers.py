@@ -111,6 +111,32 @@ def test_burgers_fractions(gpu, O, n, nH, kind, dr, ns):
     assert abs(sum(got.values()) - 1) < 1e-9
 
 
+@pytest.mark.parametrize("n,nH,kind,dr,ns", [(225, 15, "tophat", 1.0, 4), (225, 45, "gaussian", 2.0, 3),
+                                             (64, 64, "tophat", 3.0, 2), (13500, 300, "gaussian", 2.0, 5),
+                                             (13500, 2700, "tophat", 4.0, 3)])
+def test_burgers_dissipation(gpu, O, n, nH, kind, dr, ns):
+    """compute_dissipation (src/Burgers.jl:447-516) on the GPU against the
+    checker on the same snapshots, FP64, every coefficient of every model to
+    1e-10 of the model's largest (the parts are differences of O(1) stresses
+    filtered by sums of up to 2K + 1 terms that the GPU contracts into FMAs, so
+    they carry about 1e-15 absolute rounding against values near 1e-3); the
+    reference's setup sizes are included; the layout is the reference's
+    vec(d[:, m, :]), coarse point fastest."""
+    import math
+    B = gpu.burgers
+    L, visc = 2 * math.pi, 5e-4
+    U = _smag_samples(n, ns, n + 3 * nH, L)
+    got = B.compute_dissipation(torch.from_numpy(U).cuda(), nH, L, visc, dr, kind)
+    les, dbl = B.les_kernels(n, nH, L, dr, kind)
+    want = O.burgers_dissipation(U, nH, L, visc, les, dbl)
+    assert got["nomodel"].cpu().tolist() == [0.0]
+    for key, w in zip(("class_m", "class_p", "swapfil"), want):
+        g = got[key].cpu().numpy()
+        assert g.shape == (ns * nH,)
+        scale = np.abs(w).max()
+        assert np.abs(g - w.ravel()).max() <= 1e-10 * scale, (key, np.abs(g - w.ravel()).max(), scale)
+
+
 def _smag_samples(n, ns, seed, L):
     rng = np.random.default_rng(seed)
     x = np.arange(n) * L / n
@@ -176,7 +202,14 @@ def test_study_entry_points_reject_bad_arguments(gpu):
     with pytest.raises(gpu._abi.ECError):
         ctx.call("ec_burgers_fractions", 90, 7, 1.0, 1e-3, 2, gpu._abi.c_void_p(u.data_ptr()), 1,
                  w.ctypes.data_as(dp), 1, w.ctypes.data_as(dp), out)
+    d = torch.empty((3, 2 * 7), dtype=torch.float64, device="cuda")
+    with pytest.raises(gpu._abi.ECError):
+        ctx.call("ec_burgers_dissipation", 90, 7, 1.0, 1e-3, 2, gpu._abi.c_void_p(u.data_ptr()), 1,
+                 w.ctypes.data_as(dp), 1, w.ctypes.data_as(dp), gpu._abi.c_void_p(d.data_ptr()))
     big = np.ones(201) / 201
+    with pytest.raises(gpu._abi.ECError):
+        ctx.call("ec_burgers_dissipation", 90, 9, 1.0, 1e-3, 2, gpu._abi.c_void_p(u.data_ptr()), 100,
+                 big.ctypes.data_as(dp), 1, w.ctypes.data_as(dp), gpu._abi.c_void_p(d.data_ptr()))
     with pytest.raises(gpu._abi.ECError):
         ctx.call("ec_burgers_smag_fit", 90, 9, 1.0, 1e-3, 0.1, 2, gpu._abi.c_void_p(u.data_ptr()), 100,
                  big.ctypes.data_as(dp), 1, w.ctypes.data_as(dp), out)

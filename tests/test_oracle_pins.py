@@ -368,6 +368,54 @@ def test_burgers_fractions_degenerate_cases(O):
     assert all(np.abs(x).max() < 1e-15 for x in O.burgers_fractions(C, 9, L, visc, les, dbl, parts=True))
 
 
+@pytest.mark.parametrize("n,nH,kind,dr", [(225, 15, "tophat", 1.0), (225, 45, "gaussian", 2.0),
+                                          (135, 9, "tophat", 2.0)])
+def test_burgers_dissipation_closed_form_gradient(O, n, nH, kind, dr):
+    """compute_dissipation's coefficients are the closure parts times the
+    coarse velocity gradient: class_m = classic g, class_p = (classic + flux) g,
+    swapfil = (classic + flux + div) g, with the parts of the pinned fractions
+    restatement and g_J = (vu_J - vu_{J-1}) / H written from the field's
+    Fourier modes: u[i] = sum_k c_k cos(k t_i + p_k), t_i = 2 pi i / n, and a
+    symmetric double kernel give vu_J = sum_k c_k T_k cos(k t_f + p_k) at the
+    coarse face f = (J + 1) comp - 1 with T_k = sum_j w_j cos(2 pi k j / n).  A
+    gradient of the wrong sign, one cell off or at the wrong face fails."""
+    L, visc = 2 * math.pi, 5e-3
+    les, dbl = _bkernels(n, nH, L, dr, kind)
+    rng = np.random.default_rng(n + nH)
+    t = 2 * np.pi * np.arange(n) / n
+    modes = [[(rng.standard_normal() / k, k, rng.uniform(0, 2 * np.pi)) for k in range(1, 9)] for _ in range(3)]
+    U = np.array([sum(c * np.cos(k * t + p) for c, k, p in m) for m in modes])
+    K, wd = dbl
+    comp = n // nH
+    tf = 2 * np.pi * ((np.arange(nH) + 1) * comp - 1) / n
+    g = []
+    for m in modes:
+        vu = sum(c * sum(wd[j + K] * math.cos(2 * math.pi * k * j / n) for j in range(-K, K + 1)) * np.cos(k * tf + p)
+                 for c, k, p in m)
+        g.append((vu - np.roll(vu, 1)) / (L / nH))
+    g = np.array(g)
+    cl, fl, dv = O.burgers_fractions(U, nH, L, visc, les, dbl, parts=True)
+    d = O.burgers_dissipation(U, nH, L, visc, les, dbl)
+    for got, part in zip(d, (cl, cl + fl, cl + fl + dv)):
+        want = part * g
+        assert np.abs(got - want).max() < 1e-11 * np.abs(want).max()
+
+
+def test_burgers_dissipation_reflection(O):
+    """Under the staggered reflection u~[i] = -u[n - i] both the closure parts
+    and the coarse velocity gradient are even, so every dissipation
+    coefficient at coarse centre J equals the one at nH + 1 - J of the
+    original field (a forward difference instead of the backward one breaks
+    this)."""
+    n, nH, L, visc = 225, 15, 2 * math.pi, 5e-3
+    les, dbl = _bkernels(n, nH, L, 2.0, "gaussian")
+    U = _bsamples(n, 3, 11)
+    Ur = -U[:, (n - np.arange(n) - 2) % n]
+    for x, y in zip(O.burgers_dissipation(U, nH, L, visc, les, dbl),
+                    O.burgers_dissipation(Ur, nH, L, visc, les, dbl)):
+        assert np.abs(y - x[:, ::-1]).max() < 1e-12 * np.abs(x).max()
+
+
 @pytest.mark.parametrize("kind,dr", [("tophat", 2.0), ("gaussian", 2.0), ("gaussian", 0.0)])
 def test_smagorinsky_sums_reflection(O, kind, dr):
     """<S,T> and <S,S> of both fits are unchanged by the staggered reflection
