@@ -24,7 +24,9 @@ so every oracle function is pinned by checks chosen so a plausible mistake
   Burgers      RK4 4th-order convergence; swap-filter exactness; the
                identity classic + flux + div = swap (src/Burgers.jl:1087-1089);
                closure fractions: mirror symmetry of every part under the
-               staggered reflection, the comp = 1 and constant-field limits
+               staggered reflection, the comp = 1 and constant-field limits;
+               dissipation coefficients: the pinned parts times a closed-form
+               coarse velocity gradient, mirror symmetry
   spectrum     full-FFT shells, Parseval, single modes
 """
 import math
