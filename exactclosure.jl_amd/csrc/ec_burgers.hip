@@ -757,16 +757,21 @@ extern "C" int ec_burgers_smag_fit(ec_ctx *ctx, int n, int nH, double L, double 
 // run as the reference writes them: each axpy! lands on the running state
 // (ubar += dt/2 k1; ubar += dt/2 k2; ubar += dt k3), so stage 3 sees
 // ubar0 + dt/2 (k1 + k2); the step ends at ubar0 + dt/6 (k1 + 2 k2 + 2 k3 + k4).
-// State, stage sums and the stress in LDS (5 nH doubles).
+// State, stage sums and the stress in LDS (5 nH doubles).  NT threads per
+// block: from nH of about 2000 on the LDS state allows one block per CU, so
+// such grids take 1024 threads to keep four waves per SIMD; every point is
+// updated by the same arithmetic whatever thread owns it and the only
+// cross-thread result is a max, so NT does not change a bit of the output.
 constexpr long SMAG_MAX_STEPS = 1000000;  // exit bound for pathological inputs
 
-__global__ void __launch_bounds__(256) k_bsmag_solve(int n, int nH, double iH, double H, double visc, double C,
-                                                     double tstop, double cflf, const double *__restrict__ u,
-                                                     int K, const double *__restrict__ wd,
-                                                     double *__restrict__ ubar_out) {
+template <int NT>
+__global__ void __launch_bounds__(NT) k_bsmag_solve(int n, int nH, double iH, double H, double visc, double C,
+                                                    double tstop, double cflf, const double *__restrict__ u,
+                                                    int K, const double *__restrict__ wd,
+                                                    double *__restrict__ ubar_out) {
     extern __shared__ double sm[];
     double *ub = sm, *ub0 = sm + nH, *kk = sm + 2 * nH, *acc = sm + 3 * nH, *sg = sm + 4 * nH;
-    __shared__ double red[256];
+    __shared__ double red[NT];
     __shared__ double s_dt;
     const int comp = n / nH, tid = threadIdx.x;
     const double *us = u + (long)blockIdx.x * n;
@@ -793,7 +798,7 @@ __global__ void __launch_bounds__(256) k_bsmag_solve(int n, int nH, double iH, d
         for (int J = tid; J < nH; J += blockDim.x) m = fmax(m, fabs(ub[J]));
         red[tid] = m;
         __syncthreads();
-        for (int w = 128; w > 0; w >>= 1) {
+        for (int w = NT / 2; w > 0; w >>= 1) {
             if (tid < w) red[tid] = fmax(red[tid], red[tid + w]);
             __syncthreads();
         }
@@ -843,6 +848,10 @@ extern "C" int ec_burgers_smag_solve(ec_ctx *ctx, int n, int nH, double L, doubl
     const size_t lds = sizeof(double) * 5 * (size_t)nH;
     EC_REQUIRE(ctx, lds + sizeof(double) * 257 <= 160 * 1024, "ec_burgers_smag_solve: nH=%d exceeds the LDS state",
                nH);
+    // 1024 threads where a 256-thread block would already hold the CU's LDS alone
+    // and the wider reduction array still fits
+    const bool wide = 2 * (lds + sizeof(double) * 257) > 160 * 1024 && lds + sizeof(double) * 1025 <= 160 * 1024;
+    const void *kern = wide ? (const void *)k_bsmag_solve<1024> : (const void *)k_bsmag_solve<256>;
     EC_REQUIRE(ctx, K >= 0 && 2 * K + 1 <= n, "ec_burgers_smag_solve: kernel radius K=%d must fit n=%d", K, n);
     double *dwd = nullptr;
     EC_HIP(ctx, hipMallocAsync((void **)&dwd, sizeof(double) * (2 * K + 1), ctx->stream));
@@ -850,12 +859,18 @@ extern "C" int ec_burgers_smag_solve(ec_ctx *ctx, int n, int nH, double L, doubl
     if (hipMemcpyAsync(dwd, wd, sizeof(double) * (2 * K + 1), hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
         rc = ec_fail(ctx, EC_E_HIP, "ec_burgers_smag_solve: kernel upload failed");
     if (rc == EC_OK &&
-        hipFuncSetAttribute((const void *)k_bsmag_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+        hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
             hipSuccess)
         rc = ec_fail(ctx, EC_E_HIP, "ec_burgers_smag_solve: LDS attribute");
     if (rc == EC_OK) {
-        k_bsmag_solve<<<nsamples, 256, lds, ctx->stream>>>(n, nH, nH / L, L / nH, visc, C, tstop, cfl,
-                                                           (const double *)ustart, K, dwd, (double *)ubar_out);
+        if (wide)
+            k_bsmag_solve<1024><<<nsamples, 1024, lds, ctx->stream>>>(n, nH, nH / L, L / nH, visc, C, tstop, cfl,
+                                                                      (const double *)ustart, K, dwd,
+                                                                      (double *)ubar_out);
+        else
+            k_bsmag_solve<256><<<nsamples, 256, lds, ctx->stream>>>(n, nH, nH / L, L / nH, visc, C, tstop, cfl,
+                                                                    (const double *)ustart, K, dwd,
+                                                                    (double *)ubar_out);
         if (hipGetLastError() != hipSuccess) rc = ec_fail(ctx, EC_E_HIP, "ec_burgers_smag_solve: launch failed");
     }
     const hipError_t fe = hipFreeAsync(dwd, ctx->stream);

@@ -172,11 +172,14 @@ def test_smagorinsky_fit(gpu, O, n, nH, kind, dr):
 
 
 @pytest.mark.parametrize("n,nH,kind,dr,C", [(225, 15, "gaussian", 2.0, 2e-3), (192, 64, "tophat", 1.0, 0.0),
-                                            (13500, 300, "gaussian", 2.0, 1e-3)])
+                                            (13500, 300, "gaussian", 2.0, 1e-3), (13500, 2700, "gaussian", 2.0, 4e-8)])
 def test_smagorinsky_solve(gpu, O, n, nH, kind, dr, C):
     """solve_smagorinsky (src/Burgers.jl:682-772) on the GPU, every sample in
     one launch, against the checker's per-sample loop (same CFL steps, the
-    reference's accumulating RK4 stages), FP64 to 1e-11."""
+    reference's accumulating RK4 stages), FP64 to 1e-11; nH = 2700 runs the
+    1024-thread blocks, with C near the fitted theta^2 (Delta^2 + H^2) of that
+    grid (the CFL step ignores the eddy viscosity, so a C far above it blows
+    the explicit scheme up in both codes)."""
     import math
     B = gpu.burgers
     L, visc, tstop = 2 * math.pi, 5e-4, 0.1
@@ -186,6 +189,7 @@ def test_smagorinsky_solve(gpu, O, n, nH, kind, dr, C):
     got = B.solve_smagorinsky(torch.from_numpy(U).cuda(), nH, theta, L, visc, dr, kind, tstop).cpu().numpy()
     _, dbl = B.les_kernels(n, nH, L, dr, kind)
     want = O.smagorinsky_solve(U, nH, L, visc, C, tstop, 0.3, dbl)
+    assert np.isfinite(want).all()
     assert np.abs(got - want).max() <= 1e-11 * np.abs(want).max()
 
 
