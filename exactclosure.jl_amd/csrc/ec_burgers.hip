@@ -524,6 +524,49 @@ __global__ void k_bfrac_fine(int n, int ns, double ih, double visc, const double
     }
 }
 
+// The same fine pass with the block's stretch of the sample row staged in LDS:
+// block (x, q) writes points [x TILE, x TILE + TILE) of sample q from LDS copies
+// of u[i - K - 1 .. i + K]; each output sums its taps in the same order as
+// k_bfrac_fine, so the results are the same, with one global load per staged
+// value instead of one per tap.
+constexpr int BF_TILE = 1024;
+constexpr int BF_LDS = 8192;  // doubles of staged row per block (64 KB)
+
+__global__ void __launch_bounds__(256) k_bfrac_fine_t(int n, double ih, double visc, const double *__restrict__ u,
+                                                      int K, const double *__restrict__ wd, double *__restrict__ s,
+                                                      double *__restrict__ vu_f) {
+    extern __shared__ double xs[];
+    const int i0 = blockIdx.x * BF_TILE, cnt = min(BF_TILE, n - i0);
+    const long row = (long)blockIdx.y * n;
+    const double *us = u + row;
+    const int len = cnt + 2 * K + 1;  // xs[j] = u[i0 - K - 1 + j]
+    for (int j = threadIdx.x; j < len; j += 256) xs[j] = us[bwrap(i0 - K - 1 + j, n)];
+    __syncthreads();
+    for (int t = threadIdx.x; t < cnt; t += 256) {
+        const int o = t + K + 1;  // xs index of point i0 + t
+        s[row + i0 + t] = bstress(xs[o - 1], xs[o], ih, visc);
+        double acc = 0.0;
+        for (int k = -K; k <= K; k++) acc += wd[k + K] * xs[o + k];
+        vu_f[row + i0 + t] = acc;
+    }
+}
+
+// s = s_h(u) and vu_f = D * u for every fine point of every sample
+static void bfrac_fine(ec_ctx *ctx, int n, int ns, double ih, double visc, const double *u, int K,
+                       const double *wd, double *s, double *vf) {
+    if (BF_TILE + 2 * K + 1 <= BF_LDS && ns <= 65535) {
+        const dim3 grid((unsigned)((n + BF_TILE - 1) / BF_TILE), (unsigned)ns);
+        const size_t lds = sizeof(double) * (size_t)(std::min(BF_TILE, n) + 2 * K + 1);
+        // a failed attribute call shows up as a failed launch, which the callers check
+        (void)hipFuncSetAttribute((const void *)k_bfrac_fine_t, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        k_bfrac_fine_t<<<grid, 256, lds, ctx->stream>>>(n, ih, visc, u, K, wd, s, vf);
+    } else {
+        const long total = (long)n * ns;
+        const int blocks = (int)std::min<long>((total + 255) / 256, 65536);
+        k_bfrac_fine<<<blocks, 256, 0, ctx->stream>>>(n, ns, ih, visc, u, K, wd, s, vf);
+    }
+}
+
 __global__ void __launch_bounds__(256) k_bfrac_coarse(int n, int nH, double ih, double iH, double visc,
                                                       const double *__restrict__ s, const double *__restrict__ vu_f,
                                                       int Rl, const double *__restrict__ wl, int K,
@@ -578,8 +621,7 @@ extern "C" int ec_burgers_fractions(ec_ctx *ctx, int n, int nH, double L, double
         hipMemcpyAsync(dwd, wd, sizeof(double) * (2 * K + 1), hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
         rc = ec_fail(ctx, EC_E_HIP, "ec_burgers_fractions: kernel upload failed");
     if (rc == EC_OK) {
-        const int blocks = (int)std::min<long>((total + 255) / 256, 65536);
-        k_bfrac_fine<<<blocks, 256, 0, ctx->stream>>>(n, nsamples, n / L, visc, (const double *)u, K, dwd, s, vf);
+        bfrac_fine(ctx, n, nsamples, n / L, visc, (const double *)u, K, dwd, s, vf);
         k_bfrac_coarse<<<nsamples, 256, 0, ctx->stream>>>(n, nH, n / L, nH / L, visc, s, vf, Rl, dwl, K, dwd, sums);
         if (hipGetLastError() != hipSuccess ||
             hipMemcpyAsync(hs.data(), sums, sizeof(double) * 3 * nsamples, hipMemcpyDeviceToHost, ctx->stream) !=
@@ -650,9 +692,8 @@ extern "C" int ec_burgers_dissipation(ec_ctx *ctx, int n, int nH, double L, doub
         hipMemcpyAsync(dwd, wd, sizeof(double) * (2 * K + 1), hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
         rc = ec_fail(ctx, EC_E_HIP, "ec_burgers_dissipation: kernel upload failed");
     if (rc == EC_OK) {
-        const int fblocks = (int)std::min<long>((total + 255) / 256, 65536);
         const int cblocks = (int)std::min<long>((ctotal + 255) / 256, 65536);
-        k_bfrac_fine<<<fblocks, 256, 0, ctx->stream>>>(n, nsamples, n / L, visc, (const double *)u, K, dwd, s, vf);
+        bfrac_fine(ctx, n, nsamples, n / L, visc, (const double *)u, K, dwd, s, vf);
         k_bdiss_coarse<<<cblocks, 256, 0, ctx->stream>>>(n, nH, nsamples, n / L, nH / L, visc, s, vf, Rl, dwl, K,
                                                          dwd, (double *)out);
         // the weights were copied from host memory the caller may free on return
@@ -709,6 +750,56 @@ __global__ void __launch_bounds__(256) k_bsmag_fit(int n, int ns, int a, double 
         for (int x = 0; x < 4; x++) part[4 * blockIdx.x + x] = red[x][0];
 }
 
+// The fit with the block's stretch of s and U staged in LDS (rows
+// [i0 - K - 1, i0 + cnt + K] of one sample; a <= K and Rl <= K, as the double
+// kernel is the LES kernel widened by the volume filter).  Per-point sums in
+// the order of k_bsmag_fit; one partial per block.
+__global__ void __launch_bounds__(256) k_bsmag_fit_t(int n, int a, double ih, double iH, double visc, double cs,
+                                                     const double *__restrict__ s, const double *__restrict__ U,
+                                                     int Rl, const double *__restrict__ wl, int K,
+                                                     const double *__restrict__ wd, double *__restrict__ part) {
+    extern __shared__ double xs[];
+    const int i0 = blockIdx.x * BF_TILE, cnt = min(BF_TILE, n - i0);
+    const long row = (long)blockIdx.y * n;
+    const int len = cnt + 2 * K + 2;  // xs[j] = s[i0 - K - 1 + j], Us[j] = U[i0 - K - 1 + j]
+    double *Us = xs + len;
+    for (int j = threadIdx.x; j < len; j += 256) {
+        const long g = row + bwrap(i0 - K - 1 + j, n);
+        xs[j] = s[g];
+        Us[j] = U[g];
+    }
+    __syncthreads();
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int t = threadIdx.x; t < cnt; t += 256) {
+        const int o = t + K + 1;
+        double ds = 0.0, fs = 0.0;
+        for (int k = -K; k <= K; k++) ds += wd[k + K] * xs[o + k];
+        for (int r = -Rl; r <= Rl; r++) fs += wl[r + Rl] * xs[o + r];
+        const double um = Us[o - 1], u0 = Us[o];
+        const double th = ds - bstress(um, u0, ih, visc);
+        const double dh = (u0 - um) * ih, sh = -cs * fabs(dh) * dh;
+        const double ul = Us[o - a - 1], ur = Us[o + a];
+        const double tH = fs - bstress(ul, ur, iH, visc);
+        const double dH = (ur - ul) * iH, sH = -cs * fabs(dH) * dH;
+        acc[0] += sh * th;
+        acc[1] += sh * sh;
+        acc[2] += sH * tH;
+        acc[3] += sH * sH;
+    }
+    __shared__ double red[4][256];
+    for (int x = 0; x < 4; x++) red[x][threadIdx.x] = acc[x];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int x = 0; x < 4; x++) red[x][threadIdx.x] += red[x][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const long b = (long)blockIdx.y * gridDim.x + blockIdx.x;
+        for (int x = 0; x < 4; x++) part[4 * b + x] = red[x][0];
+    }
+}
+
 extern "C" int ec_burgers_smag_fit(ec_ctx *ctx, int n, int nH, double L, double visc, double delta, int nsamples,
                                    const void *u, int Rl, const double *wl, int K, const double *wd,
                                    double out[4]) {
@@ -718,7 +809,12 @@ extern "C" int ec_burgers_smag_fit(ec_ctx *ctx, int n, int nH, double L, double 
     EC_REQUIRE(ctx, Rl >= 0 && K >= 0 && 2 * Rl + 1 <= n && 2 * K + 1 <= n,
                "ec_burgers_smag_fit: kernel radii Rl=%d, K=%d must fit the grid n=%d", Rl, K, n);
     const long total = (long)n * nsamples;
-    const int blocks = (int)std::min<long>((total + 255) / 256, 4096);
+    const int a = (n / nH) / 2;
+    // LDS-staged pass while the staged rows fit (a <= K and Rl <= K hold for the
+    // reference's kernels; others take the direct pass)
+    const bool tiled = 2 * (BF_TILE + 2 * K + 2) <= BF_LDS && a <= K && Rl <= K && nsamples <= 65535;
+    const int ntile = (n + BF_TILE - 1) / BF_TILE;
+    const long blocks = tiled ? (long)ntile * nsamples : std::min<long>((total + 255) / 256, 4096);
     double *buf = nullptr;
     const size_t bytes = sizeof(double) * ((size_t)2 * total + (2 * Rl + 1) + (2 * K + 1) + 4 * (size_t)blocks);
     EC_HIP(ctx, hipMallocAsync((void **)&buf, bytes, ctx->stream));
@@ -730,14 +826,25 @@ extern "C" int ec_burgers_smag_fit(ec_ctx *ctx, int n, int nH, double L, double 
         hipMemcpyAsync(dwd, wd, sizeof(double) * (2 * K + 1), hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
         rc = ec_fail(ctx, EC_E_HIP, "ec_burgers_smag_fit: kernel upload failed");
     if (rc == EC_OK) {
-        const int fblocks = (int)std::min<long>((total + 255) / 256, 65536);
-        k_bfrac_fine<<<fblocks, 256, 0, ctx->stream>>>(n, nsamples, n / L, visc, (const double *)u, K, dwd, s, U);
-        k_bsmag_fit<<<blocks, 256, 0, ctx->stream>>>(n, nsamples, (n / nH) / 2, n / L, 1.0 / H, visc,
-                                                     delta * delta + H * H, s, U, Rl, dwl, K, dwd, part);
-        if (hipGetLastError() != hipSuccess ||
-            hipMemcpyAsync(hp.data(), part, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, ctx->stream) !=
-                hipSuccess ||
-            hipStreamSynchronize(ctx->stream) != hipSuccess)
+        bfrac_fine(ctx, n, nsamples, n / L, visc, (const double *)u, K, dwd, s, U);
+        if (tiled) {
+            const size_t lds = sizeof(double) * 2 * (size_t)(std::min(BF_TILE, n) + 2 * K + 2);
+            if (hipFuncSetAttribute((const void *)k_bsmag_fit_t, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds) != hipSuccess)
+                rc = ec_fail(ctx, EC_E_HIP, "ec_burgers_smag_fit: LDS attribute");
+            else
+                k_bsmag_fit_t<<<dim3((unsigned)ntile, (unsigned)nsamples), 256, lds, ctx->stream>>>(
+                    n, a, n / L, 1.0 / H, visc, delta * delta + H * H, s, U, Rl, dwl, K, dwd, part);
+        } else {
+            k_bsmag_fit<<<(unsigned)blocks, 256, 0, ctx->stream>>>(n, nsamples, a, n / L, 1.0 / H, visc,
+                                                                   delta * delta + H * H, s, U, Rl, dwl, K, dwd,
+                                                                   part);
+        }
+        if (rc == EC_OK &&
+            (hipGetLastError() != hipSuccess ||
+             hipMemcpyAsync(hp.data(), part, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, ctx->stream) !=
+                 hipSuccess ||
+             hipStreamSynchronize(ctx->stream) != hipSuccess))
             rc = ec_fail(ctx, EC_E_HIP, "ec_burgers_smag_fit: kernels failed");
     }
     const hipError_t fe = hipFreeAsync(buf, ctx->stream);
@@ -745,7 +852,7 @@ extern "C" int ec_burgers_smag_fit(ec_ctx *ctx, int n, int nH, double L, double 
     if (rc != EC_OK) return rc;
     for (int x = 0; x < 4; x++) {
         double acc = 0.0;
-        for (int b = 0; b < blocks; b++) acc += hp[4 * b + x];
+        for (long b = 0; b < blocks; b++) acc += hp[4 * b + x];
         out[x] = acc;
     }
     return EC_OK;

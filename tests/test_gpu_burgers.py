@@ -89,13 +89,15 @@ def test_invalid_batch_rejected(gpu):
 
 @pytest.mark.parametrize("n,nH,kind,dr,ns", [(225, 15, "tophat", 1.0, 4), (225, 15, "gaussian", 2.0, 4),
                                              (135, 45, "tophat", 3.0, 3), (64, 64, "tophat", 3.0, 2),
-                                             (13500, 300, "gaussian", 2.0, 6)])
+                                             (13500, 300, "gaussian", 2.0, 6), (9000, 10, "tophat", 8.0, 2)])
 def test_burgers_fractions(gpu, O, n, nH, kind, dr, ns):
     """compute_fractions (src/Burgers.jl:1022-1099) on the GPU against the
     checker on the same snapshots, FP64: the mean fractions to 1e-12 (the last
     case is the reference's own setup, nh = 13500, nH = 300, Delta = 2H,
     Gaussian, on a few samples); the kernels come from the host mirror of
-    tophat/gaussian_weights and build_doublekernel."""
+    tophat/gaussian_weights and build_doublekernel.  The fine pass stages each
+    block's stretch of the row in LDS; the 9000 / 10 case (double kernel of
+    8101 taps) is wider than that stage and takes the direct pass."""
     import math
     B = gpu.burgers
     L, visc = 2 * math.pi, 5e-4
@@ -145,11 +147,14 @@ def _smag_samples(n, ns, seed, L):
 
 
 @pytest.mark.parametrize("n,nH,kind,dr", [(225, 15, "gaussian", 2.0), (225, 45, "tophat", 1.0),
-                                          (13500, 300, "gaussian", 2.0), (13500, 900, "gaussian", 0.0)])
+                                          (13500, 300, "gaussian", 2.0), (13500, 900, "gaussian", 0.0),
+                                          (4000, 8, "tophat", 6.0), (9000, 10, "tophat", 8.0)])
 def test_smagorinsky_fit(gpu, O, n, nH, kind, dr):
     """The four sums of the Smagorinsky fit (smagorinsky.jl:24-36) against the
     checker on the same snapshots (1e-10 relative; FP64 sums in another
-    order); theta from them; the reference's setup sizes included."""
+    order); theta from them; the reference's setup sizes included.  The last
+    two cases have kernels wider than the LDS-staged passes hold (K = 1750:
+    direct fit pass; K = 4050: direct fine and fit passes)."""
     import math
     B = gpu.burgers
     L, visc = 2 * math.pi, 5e-4
