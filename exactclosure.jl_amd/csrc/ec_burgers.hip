@@ -567,26 +567,68 @@ static void bfrac_fine(ec_ctx *ctx, int n, int ns, double ih, double visc, const
     }
 }
 
-__global__ void __launch_bounds__(256) k_bfrac_coarse(int n, int nH, double ih, double iH, double visc,
+// Coarse samples of the fine fields: G lanes (a power of two dividing 64) share
+// one coarse point, each summing every G-th tap of the two filters, so the
+// lanes of a group read neighbouring fine points (the points of one wave are
+// comp fine cells apart); the group's partial sums meet in a fixed xor-shuffle
+// order.  Every lane of a group ends with the full sums.
+__device__ __forceinline__ int bwrap1(int m, int n) {  // m in (-n, 2n)
+    m += m < 0 ? n : 0;
+    return m >= n ? m - n : m;
+}
+
+__device__ __forceinline__ double grp_filter(const double *__restrict__ x, int n, int c, int R,
+                                             const double *__restrict__ w, int g, int G) {
+    double acc = 0.0;
+    for (int k = -R + g; k <= R; k += G) acc += w[k + R] * x[bwrap1(c + k, n)];
+    for (int o = G >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    return acc;
+}
+
+// The closure parts at coarse point J of one sample row (fractions and
+// dissipation): fsu = (F * s)(centre), vsu = (D * s)(centre), svu = s_H(vu),
+// sVU = s_h(D * u)(centre), vb - va = vu_J - vu_{J-1}.
+struct BParts {
+    double fsu, vsu, svu, sVU, dvu;
+};
+
+__device__ __forceinline__ BParts bparts(const double *__restrict__ ss, const double *__restrict__ vf, int n, int comp,
+                                         int a, int J, double ih, double iH, double visc, int Rl,
+                                         const double *__restrict__ wl, int K, const double *__restrict__ wd, int g,
+                                         int G) {
+    const int ic = (J + 1) * comp - a - 1;  // fine centre of coarse cell J
+    BParts p;
+    p.fsu = grp_filter(ss, n, ic, Rl, wl, g, G);
+    p.vsu = grp_filter(ss, n, ic, K, wd, g, G);
+    const double vb = vf[(J + 1) * comp - 1], va = vf[bwrap1(J * comp - 1, n)];  // coarse faces J, J-1
+    p.svu = -((va + vb) * (va + vb)) / 8 + visc * (vb - va) * iH;
+    p.sVU = bstress(vf[bwrap1(ic - 1, n)], vf[ic], ih, visc);
+    p.dvu = vb - va;
+    return p;
+}
+
+// Fractions: one block per sample, groups of G lanes over the coarse points,
+// per-thread sums over the points a group's leader owns, then a fixed-order
+// block reduction.
+__global__ void __launch_bounds__(256) k_bfrac_coarse(int n, int nH, int G, double ih, double iH, double visc,
                                                       const double *__restrict__ s, const double *__restrict__ vu_f,
                                                       int Rl, const double *__restrict__ wl, int K,
                                                       const double *__restrict__ wd, double *__restrict__ sums) {
     const int comp = n / nH, a = comp / 2;
     const double *ss = s + (long)blockIdx.x * n, *vf = vu_f + (long)blockIdx.x * n;
+    const int lane = threadIdx.x & 63, g = lane % G, per_wave = 64 / G;
+    const int J0 = (threadIdx.x >> 6) * per_wave + lane / G;  // wave-uniform loop over point batches
     double fc = 0.0, ff = 0.0, fd = 0.0;
-    for (int J = threadIdx.x; J < nH; J += blockDim.x) {
-        const int ic = (J + 1) * comp - a - 1;  // fine centre of coarse cell J
-        double fsu = 0.0, vsu = 0.0;
-        for (int r = -Rl; r <= Rl; r++) fsu += wl[r + Rl] * ss[bwrap(ic + r, n)];
-        for (int k = -K; k <= K; k++) vsu += wd[k + K] * ss[bwrap(ic + k, n)];
-        const double vb = vf[(J + 1) * comp - 1], va = vf[bwrap(J * comp - 1, n)];  // coarse faces J, J-1
-        const double svu = -((va + vb) * (va + vb)) / 8 + visc * (vb - va) * iH;
-        const double sVU = bstress(vf[bwrap(ic - 1, n)], vf[ic], ih, visc);
-        const double c = vsu - sVU, f = sVU - svu, d = fsu - vsu;
-        const double tot = fabs(c) + fabs(f) + fabs(d) + 2.220446049250313e-16;
-        fc += fabs(c) / tot;
-        ff += fabs(f) / tot;
-        fd += fabs(d) / tot;
+    for (int Jb = J0 - lane / G; Jb < nH; Jb += 4 * per_wave) {
+        const int J = Jb + lane / G;
+        const BParts p = bparts(ss, vf, n, comp, a, min(J, nH - 1), ih, iH, visc, Rl, wl, K, wd, g, G);
+        if (J < nH && g == 0) {
+            const double c = p.vsu - p.sVU, f = p.sVU - p.svu, d = p.fsu - p.vsu;
+            const double tot = fabs(c) + fabs(f) + fabs(d) + 2.220446049250313e-16;
+            fc += fabs(c) / tot;
+            ff += fabs(f) / tot;
+            fd += fabs(d) / tot;
+        }
     }
     __shared__ double red[3][256];
     red[0][threadIdx.x] = fc;
@@ -600,6 +642,14 @@ __global__ void __launch_bounds__(256) k_bfrac_coarse(int n, int nH, double ih, 
     }
     if (threadIdx.x == 0)
         for (int x = 0; x < 3; x++) sums[3 * blockIdx.x + x] = red[x][0];
+}
+
+// lanes per coarse point: about a quarter of the wider kernel's taps, at most 64
+static int coarse_group(int Rl, int K) {
+    const int taps = 2 * std::max(Rl, K) + 1;
+    int G = 1;
+    while (G < 64 && 4 * G <= taps) G *= 2;
+    return G;
 }
 
 extern "C" int ec_burgers_fractions(ec_ctx *ctx, int n, int nH, double L, double visc, int nsamples,
@@ -622,7 +672,8 @@ extern "C" int ec_burgers_fractions(ec_ctx *ctx, int n, int nH, double L, double
         rc = ec_fail(ctx, EC_E_HIP, "ec_burgers_fractions: kernel upload failed");
     if (rc == EC_OK) {
         bfrac_fine(ctx, n, nsamples, n / L, visc, (const double *)u, K, dwd, s, vf);
-        k_bfrac_coarse<<<nsamples, 256, 0, ctx->stream>>>(n, nH, n / L, nH / L, visc, s, vf, Rl, dwl, K, dwd, sums);
+        k_bfrac_coarse<<<nsamples, 256, 0, ctx->stream>>>(n, nH, coarse_group(Rl, K), n / L, nH / L, visc, s, vf,
+                                                          Rl, dwl, K, dwd, sums);
         if (hipGetLastError() != hipSuccess ||
             hipMemcpyAsync(hs.data(), sums, sizeof(double) * 3 * nsamples, hipMemcpyDeviceToHost, ctx->stream) !=
                 hipSuccess ||
@@ -649,29 +700,31 @@ extern "C" int ec_burgers_fractions(ec_ctx *ctx, int n, int nH, double L, double
 // three-argument forms of stress, delta_stag and the coarse-grain convolutions
 // that src/Burgers.jl does not define; they are read as the two-argument stress
 // and delta_stag (src/Burgers.jl:86-94) and the coarse-graining of
-// compute_fractions, whose quantities they name.  One thread per coarse point;
+// compute_fractions, whose quantities they name.  G lanes per coarse point
+// (bparts), grid-stride over all points of all samples;
 // out[(m * ns + q) * nH + J] with m = 0 class_m, 1 class_p, 2 swapfil.
-__global__ void __launch_bounds__(256) k_bdiss_coarse(int n, int nH, int ns, double ih, double iH, double visc,
-                                                      const double *__restrict__ s, const double *__restrict__ vu_f,
-                                                      int Rl, const double *__restrict__ wl, int K,
+__global__ void __launch_bounds__(256) k_bdiss_coarse(int n, int nH, int ns, int G, double ih, double iH,
+                                                      double visc, const double *__restrict__ s,
+                                                      const double *__restrict__ vu_f, int Rl,
+                                                      const double *__restrict__ wl, int K,
                                                       const double *__restrict__ wd, double *__restrict__ out) {
     const int comp = n / nH, a = comp / 2;
     const long total = (long)nH * ns;
-    for (long I = blockIdx.x * (long)blockDim.x + threadIdx.x; I < total; I += (long)gridDim.x * blockDim.x) {
+    const int lane = threadIdx.x & 63, g = lane % G, sub = lane / G;
+    // point batches of 64 / G per wave; the loop bound is the same for every lane of a wave
+    const long wave0 = ((long)blockIdx.x * 256 + (threadIdx.x & ~63)) / G;
+    const long stride = (long)gridDim.x * 256 / G;
+    for (long P0 = wave0; P0 < total; P0 += stride) {
+        const long I = min(P0 + sub, total - 1);
         const int J = (int)(I % nH);
         const long q = I / nH;
-        const double *ss = s + q * n, *vf = vu_f + q * n;
-        const int ic = (J + 1) * comp - a - 1;  // fine centre of coarse cell J
-        double fsu = 0.0, vsu = 0.0;
-        for (int r = -Rl; r <= Rl; r++) fsu += wl[r + Rl] * ss[bwrap(ic + r, n)];
-        for (int k = -K; k <= K; k++) vsu += wd[k + K] * ss[bwrap(ic + k, n)];
-        const double vb = vf[(J + 1) * comp - 1], va = vf[bwrap(J * comp - 1, n)];  // coarse faces J, J-1
-        const double svu = -((va + vb) * (va + vb)) / 8 + visc * (vb - va) * iH;
-        const double sVU = bstress(vf[bwrap(ic - 1, n)], vf[ic], ih, visc);
-        const double dv = (vb - va) * iH;
-        out[I] = (vsu - sVU) * dv;
-        out[total + I] = (vsu - svu) * dv;
-        out[2 * total + I] = (fsu - svu) * dv;
+        const BParts p = bparts(s + q * n, vu_f + q * n, n, comp, a, J, ih, iH, visc, Rl, wl, K, wd, g, G);
+        if (P0 + sub < total && g == 0) {
+            const double dv = p.dvu * iH;
+            out[I] = (p.vsu - p.sVU) * dv;
+            out[total + I] = (p.vsu - p.svu) * dv;
+            out[2 * total + I] = (p.fsu - p.svu) * dv;
+        }
     }
 }
 
@@ -692,10 +745,11 @@ extern "C" int ec_burgers_dissipation(ec_ctx *ctx, int n, int nH, double L, doub
         hipMemcpyAsync(dwd, wd, sizeof(double) * (2 * K + 1), hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
         rc = ec_fail(ctx, EC_E_HIP, "ec_burgers_dissipation: kernel upload failed");
     if (rc == EC_OK) {
-        const int cblocks = (int)std::min<long>((ctotal + 255) / 256, 65536);
+        const int G = coarse_group(Rl, K);
+        const int cblocks = (int)std::min<long>((ctotal * G + 255) / 256, 65536);
         bfrac_fine(ctx, n, nsamples, n / L, visc, (const double *)u, K, dwd, s, vf);
-        k_bdiss_coarse<<<cblocks, 256, 0, ctx->stream>>>(n, nH, nsamples, n / L, nH / L, visc, s, vf, Rl, dwl, K,
-                                                         dwd, (double *)out);
+        k_bdiss_coarse<<<cblocks, 256, 0, ctx->stream>>>(n, nH, nsamples, G, n / L, nH / L, visc, s, vf, Rl, dwl,
+                                                         K, dwd, (double *)out);
         // the weights were copied from host memory the caller may free on return
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess)
             rc = ec_fail(ctx, EC_E_HIP, "ec_burgers_dissipation: kernels failed");
