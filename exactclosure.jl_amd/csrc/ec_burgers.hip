@@ -918,84 +918,117 @@ extern "C" int ec_burgers_smag_fit(ec_ctx *ctx, int n, int nH, double L, double 
 // run as the reference writes them: each axpy! lands on the running state
 // (ubar += dt/2 k1; ubar += dt/2 k2; ubar += dt k3), so stage 3 sees
 // ubar0 + dt/2 (k1 + k2); the step ends at ubar0 + dt/6 (k1 + 2 k2 + 2 k3 + k4).
-// State, stage sums and the stress in LDS (5 nH doubles).  NT threads per
-// block: from nH of about 2000 on the LDS state allows one block per CU, so
-// such grids take 1024 threads to keep four waves per SIMD; every point is
-// updated by the same arithmetic whatever thread owns it and the only
-// cross-thread result is a max, so NT does not change a bit of the output.
+// Only ubar lives in LDS (nH doubles).  Thread t owns the points t + m 256,
+// m < PT, and keeps their step-start state, stage sum and slope in registers;
+// a slope is the difference of the two face stresses, each formed from ubar
+// where it is needed, so a stage takes two block barriers (slopes from the
+// old state, then the update).  max|ubar| comes from wave shuffles and one
+// value per wave in LDS, written in the step's last update.
 constexpr long SMAG_MAX_STEPS = 1000000;  // exit bound for pathological inputs
+constexpr int SMAG_NT = 256;
 
-template <int NT>
-__global__ void __launch_bounds__(NT) k_bsmag_solve(int n, int nH, double iH, double H, double visc, double C,
-                                                    double tstop, double cflf, const double *__restrict__ u,
-                                                    int K, const double *__restrict__ wd,
-                                                    double *__restrict__ ubar_out) {
-    extern __shared__ double sm[];
-    double *ub = sm, *ub0 = sm + nH, *kk = sm + 2 * nH, *acc = sm + 3 * nH, *sg = sm + 4 * nH;
-    __shared__ double red[NT];
-    __shared__ double s_dt;
+template <int PT>
+__global__ void __launch_bounds__(SMAG_NT) k_bsmag_solve(int n, int nH, double iH, double H, double visc, double C,
+                                                         double tstop, double cflf, const double *__restrict__ u,
+                                                         int K, const double *__restrict__ wd,
+                                                         double *__restrict__ ubar_out) {
+    extern __shared__ double ub[];
+    __shared__ double red[2][SMAG_NT / 64];
     const int comp = n / nH, tid = threadIdx.x;
     const double *us = u + (long)blockIdx.x * n;
-    for (int J = tid; J < nH; J += blockDim.x) {
+    for (int J = tid; J < nH; J += SMAG_NT) {
         const int f = (J + 1) * comp - 1;  // fine face of coarse face J
         double v = 0.0;
         for (int k = -K; k <= K; k++) v += wd[k + K] * us[bwrap(f + k, n)];
         ub[J] = v;
     }
     __syncthreads();
-    auto rhs = [&]() {  // kk = delta_coll(sigma), sigma from ub
-        for (int J = tid; J < nH; J += blockDim.x) {
-            const double ua = ub[J == 0 ? nH - 1 : J - 1], ubb = ub[J];
-            const double d = (ubb - ua) * iH;
-            sg[J] = -((ua + ubb) * (ua + ubb)) / 8 + visc * d + C * fabs(d) * d;
-        }
-        __syncthreads();
-        for (int J = tid; J < nH; J += blockDim.x) kk[J] = (sg[J == nH - 1 ? 0 : J + 1] - sg[J]) * iH;
-        __syncthreads();
+    auto wave_max_to = [&](double m, int slot) {
+        for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+        if ((tid & 63) == 0) red[slot][tid >> 6] = m;
     };
+    auto sigma = [&](double ua, double ubb) {
+        const double d = (ubb - ua) * iH;
+        return -((ua + ubb) * (ua + ubb)) / 8 + visc * d + C * fabs(d) * d;
+    };
+    auto slopes = [&](double (&kk)[PT]) {  // kk = delta_coll(sigma) at the owned points
+#pragma unroll
+        for (int m = 0; m < PT; m++) {
+            const int J = tid + m * SMAG_NT;
+            if (J < nH) {
+                const double um = ub[J == 0 ? nH - 1 : J - 1], u0 = ub[J], up = ub[J == nH - 1 ? 0 : J + 1];
+                kk[m] = (sigma(u0, up) - sigma(um, u0)) * iH;
+            }
+        }
+    };
+    double m0 = 0.0;
+    for (int J = tid; J < nH; J += SMAG_NT) m0 = fmax(m0, fabs(ub[J]));
+    wave_max_to(m0, 0);
+    __syncthreads();
+    double u0r[PT], acc[PT], kk[PT];
     double t = 0.0;
+    int slot = 0;
     for (long step = 0; step < SMAG_MAX_STEPS && t < tstop; step++) {
-        double m = 0.0;
-        for (int J = tid; J < nH; J += blockDim.x) m = fmax(m, fabs(ub[J]));
-        red[tid] = m;
-        __syncthreads();
-        for (int w = NT / 2; w > 0; w >>= 1) {
-            if (tid < w) red[tid] = fmax(red[tid], red[tid + w]);
-            __syncthreads();
-        }
-        if (tid == 0) {
-            const double cfl = fmin(H / red[0], H * H / visc);
-            s_dt = fmin(cflf * cfl, tstop - t);
-        }
-        __syncthreads();
-        const double dt = s_dt;
+        double mx = red[slot][0];
+        for (int w = 1; w < SMAG_NT / 64; w++) mx = fmax(mx, red[slot][w]);
+        const double dt = fmin(cflf * fmin(H / mx, H * H / visc), tstop - t);
         if (!(dt > 0)) break;  // max|ubar| infinite: no step can advance t
         t += dt;
-        for (int J = tid; J < nH; J += blockDim.x) ub0[J] = ub[J];
+#pragma unroll
+        for (int m = 0; m < PT; m++) {
+            const int J = tid + m * SMAG_NT;
+            if (J < nH) u0r[m] = ub[J];
+        }
+        slopes(kk);  // k1
         __syncthreads();
-        rhs();  // k1
-        for (int J = tid; J < nH; J += blockDim.x) {
-            acc[J] = kk[J];
-            ub[J] += dt / 2 * kk[J];
+#pragma unroll
+        for (int m = 0; m < PT; m++) {
+            const int J = tid + m * SMAG_NT;
+            if (J < nH) {
+                acc[m] = kk[m];
+                ub[J] += dt / 2 * kk[m];
+            }
         }
         __syncthreads();
-        rhs();  // k2
-        for (int J = tid; J < nH; J += blockDim.x) {
-            acc[J] += 2 * kk[J];
-            ub[J] += dt / 2 * kk[J];
+        slopes(kk);  // k2
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < PT; m++) {
+            const int J = tid + m * SMAG_NT;
+            if (J < nH) {
+                acc[m] += 2 * kk[m];
+                ub[J] += dt / 2 * kk[m];
+            }
         }
         __syncthreads();
-        rhs();  // k3
-        for (int J = tid; J < nH; J += blockDim.x) {
-            acc[J] += 2 * kk[J];
-            ub[J] += dt * kk[J];
+        slopes(kk);  // k3
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < PT; m++) {
+            const int J = tid + m * SMAG_NT;
+            if (J < nH) {
+                acc[m] += 2 * kk[m];
+                ub[J] += dt * kk[m];
+            }
         }
         __syncthreads();
-        rhs();  // k4
-        for (int J = tid; J < nH; J += blockDim.x) ub[J] = ub0[J] + dt / 6 * (acc[J] + kk[J]);
+        slopes(kk);  // k4
+        __syncthreads();
+        double mnew = 0.0;
+#pragma unroll
+        for (int m = 0; m < PT; m++) {
+            const int J = tid + m * SMAG_NT;
+            if (J < nH) {
+                const double v = u0r[m] + dt / 6 * (acc[m] + kk[m]);
+                ub[J] = v;
+                mnew = fmax(mnew, fabs(v));
+            }
+        }
+        slot ^= 1;
+        wave_max_to(mnew, slot);
         __syncthreads();
     }
-    for (int J = tid; J < nH; J += blockDim.x) ubar_out[(long)blockIdx.x * nH + J] = ub[J];
+    for (int J = tid; J < nH; J += SMAG_NT) ubar_out[(long)blockIdx.x * nH + J] = ub[J];
 }
 
 extern "C" int ec_burgers_smag_solve(ec_ctx *ctx, int n, int nH, double L, double visc, double C, double tstop,
@@ -1006,13 +1039,14 @@ extern "C" int ec_burgers_smag_solve(ec_ctx *ctx, int n, int nH, double L, doubl
                         tstop >= 0 && C >= 0,
                "ec_burgers_smag_solve: bad argument (n=%d, nH=%d, nsamples=%d, visc=%g, cfl=%g, C=%g)", n, nH,
                nsamples, visc, cfl, C);
-    const size_t lds = sizeof(double) * 5 * (size_t)nH;
-    EC_REQUIRE(ctx, lds + sizeof(double) * 257 <= 160 * 1024, "ec_burgers_smag_solve: nH=%d exceeds the LDS state",
-               nH);
-    // 1024 threads where a 256-thread block would already hold the CU's LDS alone
-    // and the wider reduction array still fits
-    const bool wide = 2 * (lds + sizeof(double) * 257) > 160 * 1024 && lds + sizeof(double) * 1025 <= 160 * 1024;
-    const void *kern = wide ? (const void *)k_bsmag_solve<1024> : (const void *)k_bsmag_solve<256>;
+    EC_REQUIRE(ctx, nH <= 16 * SMAG_NT, "ec_burgers_smag_solve: nH=%d above %d (points per thread)", nH,
+               16 * SMAG_NT);
+    const size_t lds = sizeof(double) * (size_t)nH;
+    const int pt = nH <= 4 * SMAG_NT ? 4 : nH <= 8 * SMAG_NT ? 8 : nH <= 12 * SMAG_NT ? 12 : 16;
+    const void *kern = pt == 4    ? (const void *)k_bsmag_solve<4>
+                       : pt == 8  ? (const void *)k_bsmag_solve<8>
+                       : pt == 12 ? (const void *)k_bsmag_solve<12>
+                                  : (const void *)k_bsmag_solve<16>;
     EC_REQUIRE(ctx, K >= 0 && 2 * K + 1 <= n, "ec_burgers_smag_solve: kernel radius K=%d must fit n=%d", K, n);
     double *dwd = nullptr;
     EC_HIP(ctx, hipMallocAsync((void **)&dwd, sizeof(double) * (2 * K + 1), ctx->stream));
@@ -1024,14 +1058,18 @@ extern "C" int ec_burgers_smag_solve(ec_ctx *ctx, int n, int nH, double L, doubl
             hipSuccess)
         rc = ec_fail(ctx, EC_E_HIP, "ec_burgers_smag_solve: LDS attribute");
     if (rc == EC_OK) {
-        if (wide)
-            k_bsmag_solve<1024><<<nsamples, 1024, lds, ctx->stream>>>(n, nH, nH / L, L / nH, visc, C, tstop, cfl,
-                                                                      (const double *)ustart, K, dwd,
-                                                                      (double *)ubar_out);
+#define EC_SMAG_LAUNCH(P)                                                                                  \
+    k_bsmag_solve<P><<<nsamples, SMAG_NT, lds, ctx->stream>>>(n, nH, nH / L, L / nH, visc, C, tstop, cfl,     \
+                                                              (const double *)ustart, K, dwd, (double *)ubar_out)
+        if (pt == 4)
+            EC_SMAG_LAUNCH(4);
+        else if (pt == 8)
+            EC_SMAG_LAUNCH(8);
+        else if (pt == 12)
+            EC_SMAG_LAUNCH(12);
         else
-            k_bsmag_solve<256><<<nsamples, 256, lds, ctx->stream>>>(n, nH, nH / L, L / nH, visc, C, tstop, cfl,
-                                                                    (const double *)ustart, K, dwd,
-                                                                    (double *)ubar_out);
+            EC_SMAG_LAUNCH(16);
+#undef EC_SMAG_LAUNCH
         if (hipGetLastError() != hipSuccess) rc = ec_fail(ctx, EC_E_HIP, "ec_burgers_smag_solve: launch failed");
     }
     const hipError_t fe = hipFreeAsync(dwd, ctx->stream);

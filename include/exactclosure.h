@@ -296,8 +296,8 @@ int ec_burgers_smag_fit(ec_ctx *ctx, int n, int nH, double L, double visc, doubl
  * on the coarse grid advanced to tstop by the reference's RK4 (stage states
  * accumulate on the running state) with dt = min(cfl min(H/max|ubar|,
  * H^2/visc), tstop - t), closure coefficient C = theta^2 (delta^2 + H^2);
- * ubar_out (device, nsamples x nH, FP64).  Asynchronous; 5 nH doubles of LDS
- * per sample (nH up to 4000). */
+ * ubar_out (device, nsamples x nH, FP64).  Asynchronous; nH doubles of LDS
+ * per sample (nH up to 4096). */
 int ec_burgers_smag_solve(ec_ctx *ctx, int n, int nH, double L, double visc, double C, double tstop,
                           double cfl, int nsamples, const void *ustart, int K, const double *wd,
                           void *ubar_out);

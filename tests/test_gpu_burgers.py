@@ -181,8 +181,8 @@ def test_smagorinsky_fit(gpu, O, n, nH, kind, dr):
 def test_smagorinsky_solve(gpu, O, n, nH, kind, dr, C):
     """solve_smagorinsky (src/Burgers.jl:682-772) on the GPU, every sample in
     one launch, against the checker's per-sample loop (same CFL steps, the
-    reference's accumulating RK4 stages), FP64 to 1e-11; nH = 2700 runs the
-    1024-thread blocks, with C near the fitted theta^2 (Delta^2 + H^2) of that
+    reference's accumulating RK4 stages), FP64 to 1e-11; nH = 2700 (eleven
+    points per thread) with C near the fitted theta^2 (Delta^2 + H^2) of that
     grid (the CFL step ignores the eddy viscosity, so a C far above it blows
     the explicit scheme up in both codes)."""
     import math
@@ -201,7 +201,7 @@ def test_smagorinsky_solve(gpu, O, n, nH, kind, dr, C):
 def test_study_entry_points_reject_bad_arguments(gpu):
     """The closure-study entry points refuse inconsistent sizes before any
     launch: nH not dividing n, kernels wider than the grid, an LES state
-    beyond the LDS of one block."""
+    beyond the 4096 points one block holds."""
     import ctypes
     ctx = gpu._abi.context()
     dp = ctypes.POINTER(ctypes.c_double)
