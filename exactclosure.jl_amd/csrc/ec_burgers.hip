@@ -117,10 +117,11 @@ __global__ void __launch_bounds__(NT) k_burgers_persistent(const BInst *__restri
         const int id = order[w];
         const BInst I = inst[id];
         const int n = I.n, c = I.c, nH = n / c;
-        T *us = reinterpret_cast<T *>(smem_raw);
+        // coarse face averages V u (FP64) first, so every array stays aligned
+        double *vbs = reinterpret_cast<double *>(smem_raw);
+        T *us = reinterpret_cast<T *>(vbs + nH);
         T *ul = us + n;
         T *um = ul + nH;
-        T *tau = um + nH;
         const double h = L / n, H = L / nH;
         const T ih = (T)(1.0 / h), iH = (T)(1.0 / H), tv = (T)visc;
 
@@ -164,22 +165,9 @@ __global__ void __launch_bounds__(NT) k_burgers_persistent(const BInst *__restri
         for (int step = 0; step < nsteps; step++) {
 #pragma unroll
             for (int st = 0; st < 4; st++) {
-                // --- closure commutator in FP64: tau = Pc(s) - s_H(V u)
-                for (int J = tid; J < nH; J += nthr) {
-                    const double vb = vavg(us, n, c, J);
-                    const double va = vavg(us, n, c, J == 0 ? nH - 1 : J - 1);
-                    const int f = c * (J + 1) - 1;
-                    const int m0i = f - c / 2;
-                    auto sfine = [&](int m) {
-                        const int mm = m < 0 ? m + n : (m >= n ? m - n : m);
-                        const int ml = mm == 0 ? n - 1 : mm - 1;
-                        const double a = (double)us[ml], b = (double)us[mm];
-                        return -((a + b) * (a + b)) / 8 + visc * (b - a) / h;
-                    };
-                    const double fsu = (c & 1) ? sfine(m0i) : 0.5 * (sfine(m0i) + sfine(m0i + 1));
-                    const double svu = -((va + vb) * (va + vb)) / 8 + visc * (vb - va) / H;
-                    tau[J] = (T)(fsu - svu);
-                }
+                // --- coarse face averages of the stage state, once per face (the
+                // commutator of coarse point J reads faces J - 1 and J)
+                for (int J = tid; J < nH; J += nthr) vbs[J] = vavg(us, n, c, J);
                 // --- DNS right-hand side (rhs!, src/Burgers.jl:103-109)
                 T kd[PT];
 #pragma unroll
@@ -191,6 +179,20 @@ __global__ void __launch_bounds__(NT) k_burgers_persistent(const BInst *__restri
                     }
                 }
                 __syncthreads();
+                // --- closure commutator in FP64 at coarse point Jx: tau = Pc(s) - s_H(V u)
+                auto tau_at = [&](int Jx) {
+                    const double vb = vbs[Jx], va = vbs[Jx == 0 ? nH - 1 : Jx - 1];
+                    const int m0i = c * (Jx + 1) - 1 - c / 2;
+                    auto sfine = [&](int m) {
+                        const int mm = m < 0 ? m + n : (m >= n ? m - n : m);
+                        const int ml = mm == 0 ? n - 1 : mm - 1;
+                        const double a = (double)us[ml], b = (double)us[mm];
+                        return -((a + b) * (a + b)) / 8 + visc * (b - a) / h;
+                    };
+                    const double fsu = (c & 1) ? sfine(m0i) : 0.5 * (sfine(m0i) + sfine(m0i + 1));
+                    const double svu = -((va + vb) * (va + vb)) / 8 + visc * (vb - va) / H;
+                    return (T)(fsu - svu);
+                };
                 // --- LES right-hand sides: exact closure and no model
                 T kl[PTL], km[PTL];
 #pragma unroll
@@ -198,8 +200,8 @@ __global__ void __launch_bounds__(NT) k_burgers_persistent(const BInst *__restri
                     const int J = tid + q * nthr;
                     if (J < nH) {
                         const int Jm = J == 0 ? nH - 1 : J - 1, Jp = J + 1 == nH ? 0 : J + 1;
-                        const T s0 = bstress(ul[Jm], ul[J], tv, iH) + tau[J];
-                        const T s1 = bstress(ul[J], ul[Jp], tv, iH) + tau[Jp];
+                        const T s0 = bstress(ul[Jm], ul[J], tv, iH) + tau_at(J);
+                        const T s1 = bstress(ul[J], ul[Jp], tv, iH) + tau_at(Jp);
                         kl[q] = (s1 - s0) * iH;
                         km[q] = (bstress(um[J], um[Jp], tv, iH) - bstress(um[Jm], um[J], tv, iH)) * iH;
                     }
@@ -292,7 +294,8 @@ static int pt_for(int nmax, int threads) {
 
 template <typename T>
 static size_t batch_lds(const ec_burgers_batch *b) {
-    return sizeof(T) * ((size_t)b->nmax + 3 * (size_t)b->nHmax);
+    // V u at the coarse faces (FP64), then the DNS and the two LES stage states
+    return sizeof(double) * (size_t)b->nHmax + sizeof(T) * ((size_t)b->nmax + 2 * (size_t)b->nHmax);
 }
 
 template <typename T, int PT, int NT>
