@@ -124,7 +124,17 @@ def tune_filter_vel(tuner, drv, dt=None, candidates=(64, 128, 256, 512)):
 
 
 def tune_burgers_threads(tuner, batch, nsteps=20, candidates=(128, 256, 512, 1024)):
-    """Threads per instance of the persistent Burgers kernel (advances the batch)."""
-    key = f"count{batch.count}|n{int(batch.n.max())}"
-    return tuner.tune("burgers_threads", key, list(candidates), batch.set_threads,
-                      lambda: batch.run(nsteps, use_graph=True))
+    """Threads per instance of the persistent Burgers kernel (advances the batch).
+    Block sizes whose threads cannot cover the largest instance with the
+    kernel's largest points-per-thread (min(64, 32768 / threads)) are skipped."""
+    nmax = int(batch.n.max())
+
+    def fits(t):
+        pt = 1
+        while pt * t < nmax:
+            pt *= 2
+        return pt <= min(64, 32768 // t)
+
+    cands = [t for t in candidates if fits(t)]
+    key = f"count{batch.count}|n{nmax}"
+    return tuner.tune("burgers_threads", key, cands, batch.set_threads, lambda: batch.run(nsteps, use_graph=True))

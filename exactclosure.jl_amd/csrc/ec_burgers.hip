@@ -97,7 +97,9 @@ __global__ void k_reset_counter(int *counter) {
     if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] = 0;
 }
 
-template <typename T, int PT, int NT>
+// PTL: LES points per thread, PT / 2 (any compression c >= 2) or, where the
+// batch's largest LES grid allows, PT / 4 (c >= 4), which frees registers.
+template <typename T, int PT, int NT, int PTL>
 __global__ void __launch_bounds__(NT) k_burgers_persistent(const BInst *__restrict__ inst, const int *__restrict__ order,
                                                             int count, int *__restrict__ counter, T *__restrict__ U,
                                                             T *__restrict__ UL, T *__restrict__ UM,
@@ -107,7 +109,6 @@ __global__ void __launch_bounds__(NT) k_burgers_persistent(const BInst *__restri
     __shared__ int s_idx;
     __shared__ double s_red[16];
     const int tid = threadIdx.x, nthr = blockDim.x;
-    constexpr int PTL = PT > 1 ? PT / 2 : 1;
     for (;;) {
         if (tid == 0) s_idx = atomicAdd(counter, 1);
         __syncthreads();
@@ -292,21 +293,31 @@ static int pt_for(int nmax, int threads) {
     return pt;
 }
 
+// the largest points-per-thread the persistent kernel is built with for a block size
+static int pt_max(int threads) { return std::min(64, 32768 / threads); }
+
 template <typename T>
 static size_t batch_lds(const ec_burgers_batch *b) {
     // V u at the coarse faces (FP64), then the DNS and the two LES stage states
     return sizeof(double) * (size_t)b->nHmax + sizeof(T) * ((size_t)b->nmax + 2 * (size_t)b->nHmax);
 }
 
-template <typename T, int PT, int NT>
-static int launch_pt(ec_ctx *ctx, ec_burgers_batch *b, int nsteps, int blocks, size_t lds, hipStream_t st) {
-    EC_HIP(ctx, hipFuncSetAttribute((const void *)k_burgers_persistent<T, PT, NT>,
+template <typename T, int PT, int NT, int PTL>
+static int launch_ptl(ec_ctx *ctx, ec_burgers_batch *b, int nsteps, int blocks, size_t lds, hipStream_t st) {
+    EC_HIP(ctx, hipFuncSetAttribute((const void *)k_burgers_persistent<T, PT, NT, PTL>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    k_burgers_persistent<T, PT, NT><<<blocks, NT, lds, st>>>(
+    k_burgers_persistent<T, PT, NT, PTL><<<blocks, NT, lds, st>>>(
         b->inst, b->order, b->count, b->counter, (T *)b->U, (T *)b->UL, (T *)b->UM, b->dts, b->L, b->visc, b->cfl,
         nsteps);
     EC_CHECK_LAUNCH(ctx, "k_burgers_persistent");
     return EC_OK;
+}
+
+template <typename T, int PT, int NT>
+static int launch_pt(ec_ctx *ctx, ec_burgers_batch *b, int nsteps, int blocks, size_t lds, hipStream_t st) {
+    constexpr int WIDE = PT > 1 ? PT / 2 : 1, NARROW = PT > 3 ? PT / 4 : WIDE;
+    if (NARROW < WIDE && b->nHmax <= NARROW * NT) return launch_ptl<T, PT, NT, NARROW>(ctx, b, nsteps, blocks, lds, st);
+    return launch_ptl<T, PT, NT, WIDE>(ctx, b, nsteps, blocks, lds, st);
 }
 
 // Points per thread PT (a power of two with PT * NT >= the largest instance);
@@ -314,7 +325,7 @@ static int launch_pt(ec_ctx *ctx, ec_burgers_batch *b, int nsteps, int blocks, s
 template <typename T, int NT>
 static int dispatch_pt(ec_ctx *ctx, ec_burgers_batch *b, int nsteps, int blocks, size_t lds, hipStream_t st) {
     const int pt = pt_for(b->nmax, NT);
-    if (pt * NT > 32768) return ec_fail(ctx, EC_E_UNSUPPORTED, "instance size %d too large for %d threads", b->nmax, NT);
+    if (pt > pt_max(NT)) return ec_fail(ctx, EC_E_UNSUPPORTED, "instance size %d too large for %d threads", b->nmax, NT);
     switch (pt) {
         case 1: return launch_pt<T, 1, NT>(ctx, b, nsteps, blocks, lds, st);
         case 2: return launch_pt<T, 2, NT>(ctx, b, nsteps, blocks, lds, st);
@@ -424,7 +435,8 @@ extern "C" int ec_burgers_batch_destroy(ec_ctx *ctx, ec_burgers_batch *b) {
 extern "C" int ec_burgers_batch_set_threads(ec_ctx *ctx, ec_burgers_batch *b, int threads) {
     EC_REQUIRE(ctx, ctx && b, "null argument");
     EC_REQUIRE(ctx, threads == 128 || threads == 256 || threads == 512 || threads == 1024, "threads=%d", threads);
-    EC_REQUIRE(ctx, pt_for(b->nmax, threads) * threads <= 32768, "threads=%d: n=%d needs more points per thread than supported", threads, b->nmax);
+    EC_REQUIRE(ctx, pt_for(b->nmax, threads) <= pt_max(threads),
+               "threads=%d: n=%d needs more than %d points per thread", threads, b->nmax, pt_max(threads));
     b->threads = threads;
     return EC_OK;
 }

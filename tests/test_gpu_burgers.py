@@ -85,6 +85,12 @@ def test_invalid_batch_rejected(gpu):
         B.BurgersBatch([100], [3])  # 3 does not divide 100
     with pytest.raises(gpu._abi.ECError):
         B.BurgersBatch([32768], [4], dtype="f64")  # exceeds LDS
+    # 128 threads reach 64 points each: 8192 points, not 16384
+    b = B.BurgersBatch([16384, 2048], [4, 8], dtype="f32")
+    with pytest.raises(gpu._abi.ECError):
+        b.set_threads(128)
+    b.set_threads(256)
+    b.close()
 
 
 @pytest.mark.parametrize("n,nH,kind,dr,ns", [(225, 15, "tophat", 1.0, 4), (225, 15, "gaussian", 2.0, 4),
