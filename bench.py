@@ -39,6 +39,10 @@ def parse():
     ap.add_argument("--visc", type=float, default=1.0 / 8000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-closure-check", action="store_true")
+    ap.add_argument("--fp64-steps", type=int, default=3,
+                    help="timed steps of the plain FP64 form of the same step after the FP32 leg (0: skip)")
+    ap.add_argument("--precision-study", action="store_true",
+                    help="run the 512^3 FP32-vs-FP64 study (nu = 1/4000, 50 steps, 64^3 and 128^3 closures) instead")
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain 1 GiB read + 1 GiB write kernel first (HBM counter calibration)")
@@ -161,9 +165,21 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(n=256, nsteps=5):
     """The FP64 C oracle (OpenMP, all host threads) on the 256^3 DNS-aided RK4
-    step with the 32^3 volume-average closure; steps/s scaled to 1024^3 by N."""
+    step with the 32^3 volume-average closure (the reference's CPU
+    configuration); steps/s scaled to 1024^3 by the cell count, since the
+    work per cell of every pass is independent of n (FFT: log n)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
 
@@ -175,9 +191,52 @@ def cpu_baseline(n=256, nsteps=5):
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     sps = nsteps / t
     return {"value": sps * (n / 1024) ** 3, "unit": "steps/s (1024^3-equivalent)", "cores": threads,
-            "kind": "port",
+            "kind": "port", "cpu_model": cpu_model(), "nproc": os.cpu_count(),
             "sample": f"{nsteps} RK4 DNS-aided steps at {n}^3 FP64 with the 32^3 closure: {t:.2f} s "
                       f"({sps:.4f} steps/s at {n}^3), scaled by (n/1024)^3"}
+
+
+def fp64_leg(ec, n, visc, sets, dt, steps):
+    """The plain FP64 form of the same DNS-aided step (the same kernels with
+    FP64 fields), on the same seeded initial field: mean ms per step over
+    `steps` timed steps after one warm-up step."""
+    import torch
+
+    NS, D = ec.navierstokes, ec.dnsaid
+    g = NS.Grid(1.0, n)
+    ps64 = NS.PoissonSolver(g, torch.float64)
+    u64 = NS.randomfield(g, ps64, "peak", 5.0, 0.5, seed=123)
+    del ps64
+    drv = D.DNSAidedLES(g, visc, sets, dtype=torch.float64)
+    drv.set_state([x.data for x in u64])
+    del u64
+    drv.step(dt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        drv.step(dt)
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    del drv
+    torch.cuda.empty_cache()
+    return ms
+
+
+def precision_main(a):
+    """The 512^3 FP32 pipeline against the plain FP64 form (velocity,
+    divergence and closure errors after 50 steps from the same seed)."""
+    sys.path.insert(0, ROOT)
+    import __graft_entry__ as ge
+
+    ec = ge.load_package()
+    r = ec.dnsaid.precision_study(512, 1.0 / 4000, [(64, "va"), (128, "va")], 50, timing=True)
+    out = {"metric": "FP32 pipeline vs FP64 form at 512^3 after 50 RK4 steps: velocity rel-L2 error",
+           "value": r["velocity_relerr"], "unit": "relative L2", "n_gpus": 1, "steps": 50, "warmup": 0,
+           "ms_per_step": r["ms_per_step_f32"], "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+           "dtype": "fp32 fields + fp64 closure vs fp64", "data": "synthetic: spectrum k^4 exp(-2(k/5)^2), E=1/2, seed 123",
+           "config": {"workload": "NS 512^3, nu=1/4000, RK4 50 steps, exact closure at 64^3 and 128^3 (volume average)"},
+           "study": r}
+    print(json.dumps(out), flush=True)
 
 
 def main():
@@ -186,6 +245,8 @@ def main():
         return burgers_main(a)
     if a.burgers_study and a.gpus == 1:
         return burgers_study_main(a)
+    if a.precision_study and a.gpus == 1:
+        return precision_main(a)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); sharding the step over several "
               "GPUs is not part of this build", file=sys.stderr)
@@ -314,7 +375,16 @@ def main():
                 num += float(((x - y) ** 2).sum())
                 den += float((y ** 2).sum())
         closure_err = math.sqrt(num / den)
-        del d64
+        del d64, taus64, taus32, u_state
+    else:
+        del drv
+    torch.cuda.empty_cache()
+
+    fp64_ms = None
+    if a.fp64_steps > 0 and a.dtype == "f32":
+        log(f"[bench] FP64 form: {a.fp64_steps} timed steps ...")
+        fp64_ms = fp64_leg(ec, n, a.visc, sets, dt, a.fp64_steps)
+        log(f"[bench] FP64 form {fp64_ms:.1f} ms/step against {ms_step:.1f} ms/step in FP32")
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -343,6 +413,8 @@ def main():
         "closure_relerr_vs_fp64": closure_err,
         "divergence": div,
         "les_error_vs_filtered_dns": les_err,
+        "fp64_ms_per_step": fp64_ms,
+        "speedup_vs_fp64": (fp64_ms / ms_step) if fp64_ms else None,
         "spectrum": spectrum_diag,
         "roofline": {"bound": "hbm", "kernel": dname, "achieved": d_achieved, "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": d_achieved / PEAK_HBM_GBS, "launches": dcount,

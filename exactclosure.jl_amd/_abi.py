@@ -73,7 +73,7 @@ _SIGS = {
     "ec_ns_tensordivergence_nonsym": (c_int, [c_void_p, c_int, c_int, c_double, c_int, P9, P3, c_int]),
     "ec_ns_rhs": (c_int, [c_void_p, c_int, c_int, c_double, c_double, P3, P3, c_int]),
     "ec_ns_rhs_add": (c_int, [c_void_p, c_int, c_int, c_double, c_double, P3, P3, c_double, P3]),
-    "ec_ns_rhs_div": (c_int, [c_void_p, c_int, c_int, c_double, c_double, P3, P3, c_void_p]),
+    "ec_ns_rhs_div": (c_int, [c_void_p, c_int, c_int, c_double, c_double, P3, P3, c_void_p, c_double]),
     "ec_poisson_solve": (c_int, [c_void_p, c_void_p]),
     "ec_ns_divergence": (c_int, [c_void_p, c_int, c_int, c_double, P3, c_void_p]),
     "ec_ns_pressuregradient": (c_int, [c_void_p, c_int, c_int, c_double, P3, c_void_p]),
@@ -112,7 +112,7 @@ _SIGS = {
     "ec_burgers_batch_set_threads": (c_int, [c_void_p, c_void_p, c_int]),
     "ec_burgers_batch_run": (c_int, [c_void_p, c_void_p, c_int, c_int]),
     "ec_burgers_batch_get": (c_int, [c_void_p, c_void_p, c_int, c_int, ctypes.POINTER(c_double), c_int]),
-    "ec_burgers_batch_dt": (c_int, [c_void_p, c_int, ctypes.POINTER(c_double)]),
+    "ec_burgers_batch_dt": (c_int, [c_void_p, c_void_p, c_int, ctypes.POINTER(c_double)]),
     "ec_burgers_rhs": (c_int, [c_void_p, c_int, c_int, c_double, c_double, c_void_p, c_void_p]),
     "ec_burgers_fractions": (c_int, [c_void_p, c_int, c_int, c_double, c_double, c_int, c_void_p, c_int,
                                      ctypes.POINTER(c_double), c_int, ctypes.POINTER(c_double),

@@ -18,6 +18,7 @@ import json
 import os
 import statistics
 
+import numpy as np
 import torch
 
 DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_tune_cache.json")
@@ -125,16 +126,22 @@ def tune_filter_vel(tuner, drv, dt=None, candidates=(64, 128, 256, 512)):
 
 def tune_burgers_threads(tuner, batch, nsteps=20, candidates=(128, 256, 512, 1024)):
     """Threads per instance of the persistent Burgers kernel (advances the batch).
-    Block sizes whose threads cannot cover the largest instance with the
-    kernel's largest points-per-thread (min(64, 32768 / threads)) are skipped."""
+    A block size the library refuses for this batch (threads that cannot
+    cover the largest instance with its largest points per thread) is
+    skipped; the library's own check decides, so the rule lives in one place.
+    The cache key holds the state precision and the compression mix, which
+    set the LDS per block and with it the occupancy."""
+    from ._abi import ECError
+
     nmax = int(batch.n.max())
-
-    def fits(t):
-        pt = 1
-        while pt * t < nmax:
-            pt *= 2
-        return pt <= min(64, 32768 // t)
-
-    cands = [t for t in candidates if fits(t)]
-    key = f"count{batch.count}|n{nmax}"
+    cands = []
+    for t in candidates:
+        try:
+            batch.set_threads(t)
+        except ECError:
+            continue
+        cands.append(t)
+    cs = np.asarray(batch.c)
+    mix = "-".join(f"{int(c)}x{int((cs == c).sum())}" for c in np.unique(cs))
+    key = f"count{batch.count}|n{nmax}|{getattr(batch, 'dtype', 'f32')}|c{mix}"
     return tuner.tune("burgers_threads", key, cands, batch.set_threads, lambda: batch.run(nsteps, use_graph=True))

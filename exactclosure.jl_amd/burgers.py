@@ -163,6 +163,7 @@ class BurgersBatch:
         self.L = L
         self.visc = visc
         code = _abi.EC_F64 if dtype in ("f64", "float64") else _abi.EC_F32
+        self.dtype = "f64" if code == _abi.EC_F64 else "f32"
         h = _abi.c_void_p()
         nn = (ctypes.c_int * self.count)(*self.n.tolist())
         cc = (ctypes.c_int * self.count)(*self.c.tolist())
@@ -185,8 +186,7 @@ class BurgersBatch:
 
     def dt(self, i):
         d = ctypes.c_double()
-        rc = self.ctx.lib.ec_burgers_batch_dt(self.h, i, ctypes.byref(d))
-        self.ctx.check(rc, "ec_burgers_batch_dt")
+        self.ctx.call("ec_burgers_batch_dt", self.h, i, ctypes.byref(d))
         return d.value
 
     def close(self):

@@ -460,8 +460,11 @@ extern "C" int ec_burgers_batch_run(ec_ctx *ctx, ec_burgers_batch *b, int nsteps
         EC_HIP(ctx, hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
         int rc = run(b->cap_stream);
         hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
-        if (rc) return rc;
-        if (e != hipSuccess) return ec_fail(ctx, EC_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
+        if (e != hipSuccess) return rc ? rc : ec_fail(ctx, EC_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
+        if (rc) {
+            hipGraphDestroy(g);
+            return rc;
+        }
         hipGraphExec_t ge;
         e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
         hipGraphDestroy(g);
@@ -490,9 +493,12 @@ extern "C" int ec_burgers_batch_get(ec_ctx *ctx, ec_burgers_batch *b, int instan
     return EC_OK;
 }
 
-extern "C" int ec_burgers_batch_dt(ec_burgers_batch *b, int instance, double *dt) {
-    if (!b || !dt || instance < 0 || instance >= b->count) return EC_E_INVALID;
-    if (hipMemcpy(dt, b->dts + instance, sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) return EC_E_HIP;
+extern "C" int ec_burgers_batch_dt(ec_ctx *ctx, ec_burgers_batch *b, int instance, double *dt) {
+    EC_REQUIRE(ctx, ctx && b && dt, "ec_burgers_batch_dt: null argument");
+    EC_REQUIRE(ctx, instance >= 0 && instance < b->count, "instance %d out of range", instance);
+    // ordered after the persistent kernel that writes dts on the context stream
+    EC_HIP(ctx, hipMemcpyAsync(dt, b->dts + instance, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    EC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return EC_OK;
 }
 

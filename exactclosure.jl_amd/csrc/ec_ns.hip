@@ -196,7 +196,7 @@ __global__ void k_rhs(int n, T ih, T visc, const T *__restrict__ u1, const T *__
 // rounding.
 template <typename T>
 __device__ __forceinline__ void rhs_cell(const T *pm, const T *p0, const T *pp, int o, int HX, int HP, T ih, T visc,
-                                         T &a1, T &a2, T &a3) {
+                                         T &a1, T &a2, T &a3, T &divu) {
     const T *b0 = p0 + HP, *c0 = p0 + 2 * HP;
     const T u1_0 = p0[o], u1_xp = p0[o + 1], u1_xm = p0[o - 1], u1_yp = p0[o + HX], u1_ym = p0[o - HX];
     const T u1_xm_yp = p0[o - 1 + HX], u1_zp = pp[o], u1_zm = pm[o], u1_xm_zp = pp[o - 1];
@@ -227,6 +227,15 @@ __device__ __forceinline__ void rhs_cell(const T *pm, const T *p0, const T *pp, 
     a1 = ih * (vi * v1 - T(0.25) * c1);
     a2 = ih * (vi * v2 - T(0.25) * c2);
     a3 = ih * (vi * v3 - T(0.25) * c3);
+    // h div_h u at the cell centre (divergence!, src/N.jl:222-238), from values already loaded
+    divu = (u1_0 - u1_xm) + (u2_0 - u2_ym) + (u3_0 - u3_zm);
+}
+
+template <typename T>
+__device__ __forceinline__ void rhs_cell(const T *pm, const T *p0, const T *pp, int o, int HX, int HP, T ih, T visc,
+                                         T &a1, T &a2, T &a3) {
+    T divu;
+    rhs_cell(pm, p0, pp, o, HX, HP, ih, visc, a1, a2, a3, divu);
 }
 
 // Fused RHS for large grids, z-marching: a block owns a TX x TY column tile
@@ -325,11 +334,13 @@ __global__ void __launch_bounds__(256) k_rhs_march(int n, int ZC, T ih, T visc, 
 // evaluates du1 on the column x0 - 1 and du2 on the row y0 - 1; du3 of the
 // previous plane stays in registers (one extra plane z0 - 1 starts the march).
 // du1 / du2 go through a double-buffered LDS exchange, one barrier per plane.
+// With rdh = r / h, d = div_h du + r div_h u: the divergence of the stage
+// state u is folded into the Poisson right-hand side (see ec_ns_rhs_div).
 template <typename T, int TX, int TY>
 __global__ void __launch_bounds__(256) k_rhs_div_march(int n, int ZC, T ih, T visc, const T *__restrict__ u1,
                                                        const T *__restrict__ u2, const T *__restrict__ u3,
                                                        T *__restrict__ du1, T *__restrict__ du2, T *__restrict__ du3,
-                                                       T *__restrict__ dout) {
+                                                       T *__restrict__ dout, T rdh) {
     constexpr int HX = TX + 3, HY = TY + 3, HP = HX * HY;
     constexpr int NLC = (HP + 255) / 256, NLD = 3 * NLC;  // loads per component and plane, all components
     constexpr int CPT = TX * TY / 256;
@@ -410,11 +421,11 @@ __global__ void __launch_bounds__(256) k_rhs_div_march(int n, int ZC, T ih, T vi
         if (rr + 2 < ZC) load(k + 3, ahead);
         const auto r1 = ec_rsrc(du1 + pk), r2 = ec_rsrc(du2 + pk), r3 = ec_rsrc(du3 + pk), rd = ec_rsrc(dout + pk);
         const T *pm = ring[(q - 1) & 3], *p0 = ring[q & 3], *pp = ring[(q + 1) & 3];
-        T a1[CPT], a2[CPT], a3[CPT];
+        T a1[CPT], a2[CPT], a3[CPT], dv[CPT];
 #pragma unroll
         for (int m = 0; m < CPT; m++) {
             const int cy = cy0 + m * (256 / TX);
-            rhs_cell(pm, p0, pp, (cy + 2) * HX + (cx + 2), HX, HP, ih, visc, a1[m], a2[m], a3[m]);
+            rhs_cell(pm, p0, pp, (cy + 2) * HX + (cx + 2), HX, HP, ih, visc, a1[m], a2[m], a3[m], dv[m]);
         }
         const int buf = (rr + 2) & 1;
         if (rr >= 0) {
@@ -449,7 +460,7 @@ __global__ void __launch_bounds__(256) k_rhs_div_march(int n, int ZC, T ih, T vi
                 const int cy = cy0 + m * (256 / TX);
                 ec_nt_store_at(rd, soff[m],
                                (a1[m] - dx1[buf][cy * (TX + 1) + cx]) * ih + (a2[m] - dx2[buf][cy * TX + cx]) * ih +
-                                   (a3[m] - a3prev[m]) * ih);
+                                   (a3[m] - a3prev[m]) * ih + rdh * dv[m]);
             }
         }
 #pragma unroll
@@ -471,6 +482,17 @@ __global__ void k_divergence(int n, T ih, const T *__restrict__ u1, const T *__r
     const size_t I = c.at(c.i, c.j, c.k);
     d[I] = (u1[I] - u1[c.at(c.im, c.j, c.k)]) * ih + (u2[I] - u2[c.at(c.i, c.jm, c.k)]) * ih +
            (u3[I] - u3[c.at(c.i, c.j, c.km)]) * ih;
+}
+
+// d += s div_h u (the divergence term of ec_ns_rhs_div on the plain path)
+template <typename T>
+__global__ void k_divergence_add(int n, T sih, const T *__restrict__ u1, const T *__restrict__ u2,
+                                 const T *__restrict__ u3, T *__restrict__ d) {
+    Cell c;
+    if (!cell_of(n, c)) return;
+    const size_t I = c.at(c.i, c.j, c.k);
+    d[I] += sih * ((u1[I] - u1[c.at(c.im, c.j, c.k)]) + (u2[I] - u2[c.at(c.i, c.jm, c.k)]) +
+                   (u3[I] - u3[c.at(c.i, c.j, c.km)]));
 }
 
 template <typename T>
@@ -660,13 +682,25 @@ extern "C" int ec_ns_rhs_add(ec_ctx *ctx, int dtype, int n, double l, double vis
 }
 
 extern "C" int ec_ns_rhs_div(ec_ctx *ctx, int dtype, int n, double l, double visc, void *const u[3],
-                             void *const du[3], void *d) {
+                             void *const du[3], void *d, double rdiv) {
     EC_REQUIRE(ctx, ctx && u && du && d, "ec_ns_rhs_div: null argument");
     EC_CHECK_FIELD(ctx, n);
     if (!march_ok(n)) {
-        // small grids: the two plain kernels
+        // small grids: the plain kernels
         int rc = ec_ns_rhs(ctx, dtype, n, l, visc, u, du, 0);
-        return rc ? rc : ec_ns_divergence(ctx, dtype, n, l, du, d);
+        if (!rc) rc = ec_ns_divergence(ctx, dtype, n, l, du, d);
+        if (rc || rdiv == 0.0) return rc;
+        Grid3 g = grid3(n);
+        const double sih = rdiv * n / l;
+        ec_prof_scope ps(ctx, "divergence", 5.0 * ec_cube(n) * (dtype == EC_F64 ? 8 : 4));
+        if (dtype == EC_F64)
+            k_divergence_add<double><<<g.grid, g.block, 0, ctx->stream>>>(
+                n, sih, (const double *)u[0], (const double *)u[1], (const double *)u[2], (double *)d);
+        else
+            k_divergence_add<float><<<g.grid, g.block, 0, ctx->stream>>>(
+                n, (float)sih, (const float *)u[0], (const float *)u[1], (const float *)u[2], (float *)d);
+        EC_CHECK_LAUNCH(ctx, "k_divergence_add");
+        return EC_OK;
     }
     const double ih = n / l;
     const double b = dtype == EC_F64 ? 8 : 4;
@@ -680,11 +714,11 @@ extern "C" int ec_ns_rhs_div(ec_ctx *ctx, int dtype, int n, double l, double vis
     if (dtype == EC_F64)
         k_rhs_div_march<double, 64, 8><<<tiles, 256, 0, ctx->stream>>>(
             n, ZC, ih, visc, (const double *)u[0], (const double *)u[1], (const double *)u[2], (double *)du[0],
-            (double *)du[1], (double *)du[2], (double *)d);
+            (double *)du[1], (double *)du[2], (double *)d, rdiv * ih);
     else
         k_rhs_div_march<float, 64, 16><<<tiles, 256, 0, ctx->stream>>>(
             n, ZC, (float)ih, (float)visc, (const float *)u[0], (const float *)u[1], (const float *)u[2],
-            (float *)du[0], (float *)du[1], (float *)du[2], (float *)d);
+            (float *)du[0], (float *)du[1], (float *)du[2], (float *)d, (float)(rdiv * ih));
     EC_CHECK_LAUNCH(ctx, "k_rhs_div_march");
     return EC_OK;
 }

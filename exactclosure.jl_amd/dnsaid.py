@@ -69,9 +69,14 @@ class DNSAidedLES:
     (face average).  Memory at n = 1024 FP32: 12 DNS fields + p + half
     spectrum ~ 60 GB."""
 
-    def __init__(self, g_dns, visc, sets, dtype=torch.float32, closure="forcing"):
+    def __init__(self, g_dns, visc, sets, dtype=torch.float32, closure="forcing", clean_divergence=True):
         assert closure in ("forcing", "flux")
         self.closure = closure
+        # fold 6/dt div_h u_3 into the last stage's Poisson right-hand side, so
+        # that the step removes the divergence the state carried in (FP32
+        # rounding of the projected stage tendencies would otherwise add up
+        # step after step); in FP64 the term is at rounding level
+        self.clean_divergence = clean_divergence
         self.g = g_dns
         self.visc = float(visc)
         self.dtype = dtype
@@ -197,8 +202,12 @@ class DNSAidedLES:
         for st in range(4):
             u = self.u0 if st == 0 else self.us
             # du = -div sigma(u) and its divergence in one pass, then the FFT solve
+            # at the last stage u_new = u0 + dt (acc + k / 6): with d = div du +
+            # (6/dt) div u_3, div k = -(6/dt) div u_3 cancels the divergence
+            # that u0 (= u_3 up to projected terms) brought into the step
+            rdiv = 6.0 / dt if (st == 3 and self.clean_divergence) else 0.0
             c.call("ec_ns_rhs_div", self.dt_code, n, l, self.visc, ptrs(u, P3), ptrs(self.du, P3),
-                   _abi.c_void_p(self.ps.p_ptr))
+                   _abi.c_void_p(self.ps.p_ptr), rdiv)
             c.call("ec_poisson_solve", self.ps.h)
             # V_s(du - grad p) for every LES grid and the DNS RK4 stage in one
             # pass over du and p (nothing below reads the DNS state)
@@ -287,3 +296,60 @@ class DNSAidedLES:
                 s += float(((la - r[a]) ** 2).sum() / (r[a] ** 2).sum())
             out.append((s / 3) ** 0.5)
         return out
+
+
+def precision_study(n, visc, sets, nsteps, seed=123, cfl=0.4, l=1.0, dt=None, timing=False):
+    """The FP32 pipeline (FP32 fields, FP64 closure commutator and reductions)
+    against the plain FP64 form, both from the same seeded initial field
+    (randomfield, src/N.jl:394-476, spectrum k^4 exp(-2 (k/5)^2), energy 1/2)
+    and advanced `nsteps` RK4 steps with dt from propose_timestep (src/N.jl:
+    348-356) at t = 0 unless given.  Returns a dict with
+      velocity_relerr   compute_errors (src/N.jl:1334-1349) of the FP32 DNS
+                        velocity against the FP64 one after the steps;
+      divergence_f32/f64 h ||div_h u||_2 / ||u||_2 of each final state;
+      closure_relerr    ||m_32 - m_64|| / ||m_64|| over every LES grid and
+                        component, the FP64 closure evaluated from the FP32
+                        final state promoted to FP64 (the same input state);
+      les_relerr        per LES grid, compute_errors of the FP32 LES against
+                        the FP64 LES after the steps;
+      exactness_f32/f64 per LES grid, compute_errors of the LES against the
+                        filtered DNS of its own run.
+    With timing, also the mean ms per step of each form."""
+    import time
+
+    f32, f64 = torch.float32, torch.float64
+    g = NS.Grid(l, n)
+    ps64 = NS.PoissonSolver(g, f64)
+    u64 = NS.randomfield(g, ps64, "peak", 5.0, 0.5, seed=seed)
+    if dt is None:
+        dt = NS.propose_timestep(u64, visc, cfl)
+    d64 = DNSAidedLES(g, visc, sets, dtype=f64)
+    d64.set_state([x.data for x in u64])
+    d32 = DNSAidedLES(g, visc, sets, dtype=f32)
+    d32.set_state([x.data.to(f32) for x in u64])
+    del u64, ps64
+    out = {"n": n, "visc": visc, "dt": dt, "steps": nsteps, "sets": [f"{nb}:{k}" for nb, k in sets]}
+    for name, drv in (("f64", d64), ("f32", d32)):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        drv.run(dt, nsteps)
+        torch.cuda.synchronize()
+        if timing:
+            out[f"ms_per_step_{name}"] = 1e3 * (time.perf_counter() - t0) / max(nsteps, 1)
+    u32d = [x.to(f64) for x in d32.u0]
+    out["velocity_relerr"] = NS.compute_errors(u32d, d64.u0)
+    out["divergence_f32"] = d32.divergence_norm()
+    out["divergence_f64"] = d64.divergence_norm()
+    out["les_relerr"] = [NS.compute_errors([x.to(f64) for x in L32.u0], L64.u0)
+                         for L32, L64 in zip(d32.les, d64.les)]
+    out["exactness_f32"] = d32.les_errors()
+    out["exactness_f64"] = d64.les_errors()
+    m32 = d32.closure_terms()
+    m64 = d64.closure_terms(u32d)
+    num = den = 0.0
+    for s32, s64 in zip(m32, m64):
+        for x, y in zip(s32, s64):
+            num += float(((x - y) ** 2).sum())
+            den += float((y ** 2).sum())
+    out["closure_relerr"] = (num / den) ** 0.5
+    return out

@@ -100,11 +100,15 @@ int ec_ns_rhs(ec_ctx *ctx, int dtype, int n, double l, double visc, void *const 
 int ec_ns_rhs_add(ec_ctx *ctx, int dtype, int n, double l, double visc, void *const u[3],
                   void *const du[3], double s, const void *const add[3]);
 /* ec_ns_rhs (doadd 0) and divergence!(d, du) fused: du = -div sigma(u) and
- * d = div_h du, the right-hand side of the pressure Poisson equation of
- * project!(du) (src/N.jl:256-262).  d is typically ec_poisson_pressure(ps),
- * followed by ec_poisson_solve. */
+ * d = div_h du + rdiv * div_h u, the right-hand side of the pressure Poisson
+ * equation of project!(du) (src/N.jl:256-262).  d is typically
+ * ec_poisson_pressure(ps), followed by ec_poisson_solve.  With rdiv = 0 this
+ * is the reference's projection of du; rdiv = 1/tau makes k = du - grad p
+ * also remove the divergence of the stage state u at rate 1/tau (the
+ * divergence that FP32 rounding of the projected k leaves behind and that
+ * would otherwise accumulate step after step). */
 int ec_ns_rhs_div(ec_ctx *ctx, int dtype, int n, double l, double visc, void *const u[3],
-                  void *const du[3], void *d);
+                  void *const du[3], void *d, double rdiv);
 /* divergence! (src/N.jl:222-238) */
 int ec_ns_divergence(ec_ctx *ctx, int dtype, int n, double l, void *const u[3], void *d);
 /* pressuregradient! (src/N.jl:240-254) */
@@ -261,7 +265,9 @@ int ec_burgers_batch_run(ec_ctx *ctx, ec_burgers_batch *batch, int nsteps, int u
 /* which: 0 DNS, 1 exact-closure LES, 2 no-model LES; FP64 host output */
 int ec_burgers_batch_get(ec_ctx *ctx, ec_burgers_batch *batch, int instance, int which,
                          double *out, int cap);
-int ec_burgers_batch_dt(ec_burgers_batch *batch, int instance, double *dt);
+/* The time step of an instance (cfl, src/Burgers.jl:103-109, at its initial
+ * state); read on the context stream after the work queued before it. */
+int ec_burgers_batch_dt(ec_ctx *ctx, ec_burgers_batch *batch, int instance, double *dt);
 /* Single-instance operators for parity tests (src/Burgers.jl:89-142). */
 int ec_burgers_rhs(ec_ctx *ctx, int dtype, int n, double L, double visc, const void *u, void *du);
 /* compute_fractions (src/Burgers.jl:1022-1099) over nsamples FP64 DNS

@@ -407,6 +407,74 @@ def composekernel(F, G):
     return out
 
 
+def tophat_weights(comp):
+    """tophat_weights (src/Burgers.jl:158-162): (R, w), R = comp div 2, w = 1/comp
+    over comp points (odd comp; the reference has no even form)."""
+    assert comp % 2 == 1
+    return comp // 2, [1.0 / comp] * comp
+
+
+def trapezoid_weights(comp):
+    """The even-compression finite-volume kernel used where the reference's
+    tophat_weights would index past its array (c + 1 faces, ends 1/(2c)):
+    the box of c cells sampled with the trapezoid rule."""
+    assert comp % 2 == 0
+    w = [1.0 / comp] * (comp + 1)
+    w[0] = w[comp] = 0.5 / comp
+    return comp // 2, w
+
+
+def gaussian_weights(h, delta, nsigma=3):
+    """gaussian_weights (src/Burgers.jl:146-156): sigma^2 = (delta/h)^2 / 12 in
+    cells, R = round(nsigma sigma) with Julia's ties-to-even, weights
+    exp(-r^2 / (2 sigma^2)) normalised; delta < 1e-15 gives [1]."""
+    s2 = (delta / h) ** 2 / 12
+    R = round(nsigma * math.sqrt(s2))  # Python round: ties to even, as Julia's
+    if delta < 1e-15:
+        return R, [1.0]
+    w = [math.exp(-r * r / (2 * s2)) for r in range(-R, R + 1)]
+    t = sum(w)
+    return R, [x / t for x in w]
+
+
+def build_doublekernel(F, G):
+    """build_doublekernel (src/Burgers.jl:217-233): the full convolution of the
+    two kernels, radius I + J, by the definition sum_j f[k - j] g[j]."""
+    I, f = F
+    J, g = G
+    K = I + J
+    out = []
+    for k in range(-K, K + 1):
+        s = 0.0
+        for j in range(-J, J + 1):
+            if abs(k - j) <= I:
+                s += f[I + k - j] * g[J + j]
+        out.append(s)
+    return K, out
+
+
+def burgers_les_kernels(nh, nH, L, delta_ratio, lesfiltertype="tophat"):
+    """The LES kernel and the double kernel of compute_fractions
+    (src/Burgers.jl:1030-1048): Delta = H delta_ratio; the finite-volume kernel
+    over comp = nh div nH fine cells (tophat_weights; the trapezoid for even
+    comp); the LES top-hat of 2 (round(Delta/h) div 2) + 1 cells, or the
+    Gaussian of gaussian_weights; and their double kernel.  Returns numpy
+    arrays."""
+    h, H = L / nh, L / nH
+    delta = H * delta_ratio
+    comp = nh // nH
+    fvm = tophat_weights(comp) if comp % 2 else trapezoid_weights(comp)
+    if lesfiltertype == "tophat":
+        R = round(delta / h) // 2
+        les = tophat_weights(2 * R + 1)
+    elif lesfiltertype == "gaussian":
+        les = gaussian_weights(h, delta)
+    else:
+        raise ValueError(lesfiltertype)
+    dbl = build_doublekernel(fvm, les)
+    return (les[0], np.array(les[1])), (dbl[0], np.array(dbl[1]))
+
+
 def applyfilter(u, w1, w2, w3):
     """applyfilter!(ubar, u, kernelproduct(w1, w2, w3)) (src/N.jl:639-653)."""
     u = _f(u)

@@ -44,9 +44,37 @@ def test_batch_fp64_matches_checker(gpu, O, threads):
         assert rel(b.get(i, "swapfil"), us) < 1e-11
         assert rel(b.get(i, "nomodel"), um) < 1e-11
         # exactness: the swap-filter LES is the filtered DNS
-        _, w = B.tophat_weights(c[i])
+        _, w = O.tophat_weights(c[i]) if c[i] % 2 else O.trapezoid_weights(c[i])
         vu = O.burgers_filter_coarse(u, n[i] // c[i], w, -(c[i] // 2))
         assert rel(b.get(i, "swapfil"), vu) < 1e-12
+
+
+@pytest.mark.parametrize("prec,threads,n,c", [
+    # 8 points per thread; LES arrays of PT/4 = 2 points (largest LES grid 512 <= 2 x 256)
+    ("f64", 256, [2048, 1024, 2048], [8, 4, 16]),
+    # 8 points per thread; LES arrays of PT/2 = 4 points (LES grid 1024 > 2 x 256)
+    ("f64", 256, [2048, 1024], [2, 4]),
+    # the benchmark's build: 1024 threads, 16 points per thread, LES arrays of 4
+    ("f32", 1024, [16384, 8192, 2048], [4, 8, 64]),
+    ("f64", 1024, [16384, 4096], [16, 4]),
+])
+def test_batch_points_per_thread_builds(gpu, O, prec, threads, n, c):
+    """Builds with several points per thread (the register arrays of the DNS
+    state and of the LES states in their PT/2 and PT/4 widths) against the
+    checker; FP64 to 1e-12, FP32 to 2e-5 after 10 steps."""
+    B = gpu.burgers
+    visc, L, seed, steps = 5e-4, 2 * np.pi, 21, 10
+    b = B.BurgersBatch(n, c, visc=visc, L=L, cfl=0.3, seed=seed, dtype=prec)
+    b.set_threads(threads)
+    b.run(steps, use_graph=False)
+    tol = 1e-12 if prec == "f64" else 2e-5
+    for i in range(len(n)):
+        u0 = O.burgers_ic(n[i], seed + i)
+        u, us, um = O.burgers_dnsaid_rk4(u0, c[i], visc, b.dt(i), steps, L)
+        assert rel(b.get(i, "dns"), u) < tol, i
+        assert rel(b.get(i, "swapfil"), us) < 10 * tol, i
+        assert rel(b.get(i, "nomodel"), um) < 10 * tol, i
+    b.close()
 
 
 def test_batch_fp32_and_graph_replay(gpu, O):
@@ -100,8 +128,9 @@ def test_burgers_fractions(gpu, O, n, nH, kind, dr, ns):
     """compute_fractions (src/Burgers.jl:1022-1099) on the GPU against the
     checker on the same snapshots, FP64: the mean fractions to 1e-12 (the last
     case is the reference's own setup, nh = 13500, nH = 300, Delta = 2H,
-    Gaussian, on a few samples); the kernels come from the host mirror of
-    tophat/gaussian_weights and build_doublekernel.  The fine pass stages each
+    Gaussian, on a few samples); the GPU path builds its kernels with the host
+    mirror of tophat/gaussian_weights and build_doublekernel, the checker with
+    its own restatement of them.  The fine pass stages each
     block's stretch of the row in LDS; the 9000 / 10 case (double kernel of
     8101 taps) is wider than that stage and takes the direct pass."""
     import math
@@ -112,7 +141,7 @@ def test_burgers_fractions(gpu, O, n, nH, kind, dr, ns):
     U = np.array([sum(rng.standard_normal() / k * np.cos(k * x + rng.uniform(0, 2 * np.pi)) for k in range(1, 11))
                   for _ in range(ns)])
     got = B.compute_fractions(torch.from_numpy(U).cuda(), nH, L, visc, dr, kind)
-    les, dbl = B.les_kernels(n, nH, L, dr, kind)
+    les, dbl = O.burgers_les_kernels(n, nH, L, dr, kind)
     want = O.burgers_fractions(U, nH, L, visc, les, dbl)
     for key, w in zip(("classic", "flux", "div"), want):
         assert abs(got[key] - w) < 1e-12, (key, got[key], w)
@@ -135,7 +164,7 @@ def test_burgers_dissipation(gpu, O, n, nH, kind, dr, ns):
     L, visc = 2 * math.pi, 5e-4
     U = _smag_samples(n, ns, n + 3 * nH, L)
     got = B.compute_dissipation(torch.from_numpy(U).cuda(), nH, L, visc, dr, kind)
-    les, dbl = B.les_kernels(n, nH, L, dr, kind)
+    les, dbl = O.burgers_les_kernels(n, nH, L, dr, kind)
     want = O.burgers_dissipation(U, nH, L, visc, les, dbl)
     assert got["nomodel"].cpu().tolist() == [0.0]
     for key, w in zip(("class_m", "class_p", "swapfil"), want):
@@ -166,7 +195,8 @@ def test_smagorinsky_fit(gpu, O, n, nH, kind, dr):
     L, visc = 2 * math.pi, 5e-4
     U = _smag_samples(n, 3, n + nH, L)
     les, dbl = B.les_kernels(n, nH, L, dr, kind)
-    want = O.smagorinsky_sums(U, nH, L, visc, dr * L / nH, les, dbl)
+    ol, od = O.burgers_les_kernels(n, nH, L, dr, kind)
+    want = O.smagorinsky_sums(U, nH, L, visc, dr * L / nH, ol, od)
     Ud = torch.from_numpy(U).cuda()
     import ctypes
     out = (ctypes.c_double * 4)()
@@ -198,7 +228,7 @@ def test_smagorinsky_solve(gpu, O, n, nH, kind, dr, C):
     H = L / nH
     theta = math.sqrt(C / ((dr * H) ** 2 + H ** 2))
     got = B.solve_smagorinsky(torch.from_numpy(U).cuda(), nH, theta, L, visc, dr, kind, tstop).cpu().numpy()
-    _, dbl = B.les_kernels(n, nH, L, dr, kind)
+    _, dbl = O.burgers_les_kernels(n, nH, L, dr, kind)
     want = O.smagorinsky_solve(U, nH, L, visc, C, tstop, 0.3, dbl)
     assert np.isfinite(want).all()
     assert np.abs(got - want).max() <= 1e-11 * np.abs(want).max()

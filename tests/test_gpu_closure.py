@@ -94,7 +94,9 @@ def test_filter_coarse_gaussian(gpu, O):
     FD = NS.gaussian(g, 2 * NS.spacing(gb), 2)
     FDH = NS.composekernel(NS.tophat(g, c), FD)
     u = O.randomfield(n, seed=3)
-    ref = O.coarsegrain(O.applyfilter(u[0], FDH, FDH, FDH), n // c, (True, False, False))
+    # the checker builds its kernel with its own gaussian / composekernel / tophat
+    OFDH = O.composekernel(O.tophat(c), O.gaussian(1.0 / n, 2.0 * c / n, 2))
+    ref = O.coarsegrain(O.applyfilter(u[0], OFDH, OFDH, OFDH), n // c, (True, False, False))
     ub = NS.Field(gb)
     NS.applyfilter_coarsegrain(ub, NS.Field(g, dev(u[0])), NS.KernelProduct([FDH] * 3), (True, False, False))
     assert rel(ub.numpy(), ref) < 1e-12

@@ -1,0 +1,178 @@
+"""The workload configurations at their stated sizes on the GPU.
+
+* 256^3 FP64 DNS-aided RK4 (nu = 1/2000, dt = 1e-3, 20 steps, 32^3
+  volume-average exact closure): the DNS and the LES against the FP64 CPU
+  checker's RK4 DNS-aided step run on the same initial field;
+* 4096 batched Burgers instances (N in {2048, 4096, 8192, 16384}, 40/30/20/10
+  %, c in {4, 8, 16, 32, 64}, nu = 5e-4), 1000 RK4 steps in one
+  graph-replayed persistent launch at the 1024-thread build: every size
+  class and compression checked against the checker;
+* 512^3 FP32 pipeline against the plain FP64 form (nu = 1/4000, 50 steps,
+  64^3 and 128^3 exact closures): velocity, divergence and closure errors;
+* the FP32 divergence over many steps (the state's divergence is removed
+  each step at the last stage's projection).
+"""
+import json
+
+import numpy as np
+import pytest
+import torch
+
+from test_gpu_ns import dev, host
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def test_config_256_fp64_rk4_20_steps_vs_checker(gpu, O):
+    """256^3 FP64, 20 RK4 steps of dt = 1e-3 with the 32^3 volume-average
+    closure at every stage, from the checker's random field (seed 123): the
+    DNS state and the LES state agree with the checker's dnsaid_rk4 to 1e-10
+    (max-norm relative; the FFTs differ in their rounding), and the LES equals
+    the filtered DNS to rounding (the exactness invariant)."""
+    NS, D = gpu.navierstokes, gpu.dnsaid
+    n, c, visc, dt, steps = 256, 8, 1.0 / 2000, 1e-3, 20
+    u = O.randomfield(n, seed=123)
+    drv = D.DNSAidedLES(NS.Grid(1.0, n), visc, [(n // c, "va")], dtype=torch.float64)
+    drv.set_state([dev(x) for x in u])
+    drv.run(dt, steps)
+    torch.cuda.synchronize()
+    u_gpu = [host(x) for x in drv.u0]
+    les_gpu = [host(x) for x in drv.les[0].u0]
+    exact = max(drv.les_errors())
+    del drv
+    torch.cuda.empty_cache()
+    u_ref, les_ref, _ = O.dnsaid_rk4([x.copy(order="F") for x in u], [(c, "va")], visc, dt, steps,
+                                     closure="forcing")
+    err = max(rel(a, b) for a, b in zip(u_gpu, u_ref))
+    err_les = max(rel(a, b) for a, b in zip(les_gpu, les_ref[0]))
+    print(f"\n[config 256^3 FP64] DNS rel err {err:.3e}, LES rel err {err_les:.3e}, exactness {exact:.3e}")
+    assert err < 1e-10 and err_les < 1e-10
+    assert exact < 1e-12
+
+
+def _class_sample(n, c):
+    """One instance per (size class, compression) present in the batch."""
+    pick = {}
+    for i, (ni, ci) in enumerate(zip(n.tolist(), c.tolist())):
+        pick.setdefault((ni, ci), i)
+    return sorted(pick.values())
+
+
+def test_config_burgers_4096_instances_1000_steps(gpu, O):
+    """The batched workload at its stated size: 4096 instances, 1000 RK4
+    steps, FP32 state with the FP64 commutator, one graph-replayed launch at
+    1024 threads (16 points per thread for the 16384-point instances, LES
+    arrays of 4 points per thread).  For one instance of every (size,
+    compression) pair the DNS, exact-closure LES and no-model LES are checked
+    against the checker's FP64 RK4 from the same initial field and dt
+    (FP32 rounding over 1000 steps, measured 1.4e-5 of the field's maximum:
+    bound 1e-4), and the exact-closure LES against the filter of the FP32
+    DNS (exactness, measured 1.0e-5: bound 1e-4)."""
+    B = gpu.burgers
+    n, c = B.baseline_batch(4096)
+    visc, steps, seed, L = 5e-4, 1000, 0, 2 * np.pi
+    b = B.BurgersBatch(n, c, visc=visc, seed=seed, dtype="f32")
+    b.set_threads(1024)
+    b.run(steps, use_graph=True)
+    torch.cuda.synchronize()
+    worst = {"dns": 0.0, "swapfil": 0.0, "nomodel": 0.0, "exact": 0.0}
+    for i in _class_sample(n, c):
+        u0 = O.burgers_ic(int(n[i]), seed + i)
+        dt = b.dt(i)
+        u, us, um = O.burgers_dnsaid_rk4(u0, int(c[i]), visc, dt, steps, L)
+        gd, gs, gm = b.get(i, "dns"), b.get(i, "swapfil"), b.get(i, "nomodel")
+        assert np.isfinite(gd).all() and np.isfinite(gs).all() and np.isfinite(gm).all()
+        worst["dns"] = max(worst["dns"], rel(gd, u))
+        worst["swapfil"] = max(worst["swapfil"], rel(gs, us))
+        worst["nomodel"] = max(worst["nomodel"], rel(gm, um))
+        ci = int(c[i])
+        _, w = O.tophat_weights(ci) if ci % 2 else O.trapezoid_weights(ci)
+        vu = O.burgers_filter_coarse(gd, int(n[i]) // int(c[i]), w, -(int(c[i]) // 2))
+        worst["exact"] = max(worst["exact"], rel(gs, vu))
+    print("\n[config Burgers 4096 x 1000 FP32]", json.dumps(worst))
+    assert worst["dns"] < 1e-4 and worst["swapfil"] < 1e-4 and worst["nomodel"] < 1e-4
+    assert worst["exact"] < 1e-4
+    b.close()
+
+
+def test_config_burgers_1000_steps_fp64_vs_checker(gpu, O):
+    """The same 1000-step graph-replayed launch in FP64, checked to 1e-9
+    against the checker: the algorithm, not only the FP32 rounding, at the
+    full length.  FP64 state takes twice the LDS, so the classes run as two
+    batches whose per-block state fits (8 nmax + 24 nHmax bytes <= 160 KiB):
+    every class up to 8192 points, then the 16384-point classes with LES
+    grids of at most 1024 points (c >= 16)."""
+    B = gpu.burgers
+    n_all, c_all = B.baseline_batch(4096)
+    pick = _class_sample(n_all, c_all)
+    groups = [[i for i in pick if n_all[i] <= 8192], [i for i in pick if n_all[i] == 16384 and c_all[i] >= 16]]
+    visc, steps, seed, L = 5e-4, 1000, 0, 2 * np.pi
+    worst, count = 0.0, 0
+    for idx in groups:
+        n, c = n_all[idx], c_all[idx]
+        b = B.BurgersBatch(n, c, visc=visc, seed=seed, dtype="f64")
+        b.set_threads(1024)
+        b.run(steps, use_graph=True)
+        torch.cuda.synchronize()
+        for i in range(len(n)):
+            u0 = O.burgers_ic(int(n[i]), seed + i)
+            u, us, um = O.burgers_dnsaid_rk4(u0, int(c[i]), visc, b.dt(i), steps, L)
+            worst = max(worst, rel(b.get(i, "dns"), u), rel(b.get(i, "swapfil"), us),
+                        rel(b.get(i, "nomodel"), um))
+        count += len(n)
+        b.close()
+    print(f"\n[config Burgers 1000 steps FP64] worst rel err {worst:.3e} over {count} instances")
+    assert worst < 1e-9
+
+
+def test_config_512_fp32_against_fp64_50_steps(gpu):
+    """512^3, nu = 1/4000, 50 RK4 steps (dt from propose_timestep at CFL 0.4),
+    exact closures at 64^3 and 128^3 (volume average), FP32 fields + FP64
+    commutator against the plain FP64 form from the same seed: closure error
+    (same input state) <= 1e-5, FP32 divergence <= 1e-6, and the FP32
+    trajectory within 1e-6 of the FP64 one after 50 steps (measured 6.5e-8)."""
+    r = gpu.dnsaid.precision_study(512, 1.0 / 4000, [(64, "va"), (128, "va")], 50)
+    print("\n[config 512^3 FP32 vs FP64]", json.dumps(r))
+    assert r["closure_relerr"] <= 1e-5
+    assert r["divergence_f32"] <= 1e-6
+    assert r["divergence_f64"] <= 1e-13
+    assert r["velocity_relerr"] <= 1e-6
+    assert max(r["les_relerr"]) <= 1e-5
+    assert max(r["exactness_f64"]) <= 1e-11
+    assert max(r["exactness_f32"]) <= 1e-5
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("n", [128, 256])
+def test_fp32_divergence_bounded_over_many_steps(gpu, n):
+    """FP32 DNS-aided RK4 over 240 steps (n = 256 takes the fused marching
+    kernels, n = 128 the plain ones): with the state's divergence folded into
+    the last stage's Poisson right-hand side, h ||div_h u|| / ||u|| stays at
+    the FP32 rounding floor, below 1e-6 and no larger after 240 steps than
+    about twice its value after 24; without it the divergence of the
+    projected stage tendencies adds up step after step."""
+    NS, D = gpu.navierstokes, gpu.dnsaid
+    g = NS.Grid(1.0, n)
+    visc = 1.0 / 2000
+    ps64 = NS.PoissonSolver(g, torch.float64)
+    u64 = NS.randomfield(g, ps64, "peak", 5.0, 0.5, seed=7)
+    dt = NS.propose_timestep(u64, visc, 0.4)
+    res = {}
+    for clean in (True, False):
+        drv = D.DNSAidedLES(g, visc, [(n // 8, "va")], dtype=torch.float32, clean_divergence=clean)
+        drv.set_state([x.data.float() for x in u64])
+        drv.run(dt, 24)
+        d24 = drv.divergence_norm()
+        drv.run(dt, 216)
+        res[clean] = (d24, drv.divergence_norm(), max(drv.les_errors()))
+        del drv
+    print(f"\n[divergence n={n}] cleaned {res[True]}, uncleaned {res[False]}")
+    d24, d240, exact = res[True]
+    assert d240 <= 1e-6
+    assert d240 <= 2.5 * d24
+    assert d240 <= res[False][1]
+    assert exact < 1e-5

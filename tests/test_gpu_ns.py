@@ -75,7 +75,7 @@ def test_rhs_div(gpu, O, n, prec):
     d = torch.empty_like(ud[0])
     ctx = gpu._abi.context()
     code = 1 if prec == "f64" else 0
-    ctx.call("ec_ns_rhs_div", code, n, l, visc, P(ud, 3), P(du, 3), gpu._abi.c_void_p(d.data_ptr()))
+    ctx.call("ec_ns_rhs_div", code, n, l, visc, P(ud, 3), P(du, 3), gpu._abi.c_void_p(d.data_ptr()), 0.0)
     for a in range(3):
         assert rel(host(du[a]), ref[a]) < tol
     # the divergence cancels: measure it against the size of its terms
@@ -87,11 +87,33 @@ def test_rhs_div(gpu, O, n, prec):
     g = NS.Grid(l, n)
     ps1, ps2 = NS.PoissonSolver(g, dt), NS.PoissonSolver(g, dt)
     ctx.call("ec_ns_pressure_solve", ps1.h, P(du, 3))
-    ctx.call("ec_ns_rhs_div", code, n, l, visc, P(ud, 3), P(du, 3), gpu._abi.c_void_p(ps2.p_ptr))
+    ctx.call("ec_ns_rhs_div", code, n, l, visc, P(ud, 3), P(du, 3), gpu._abi.c_void_p(ps2.p_ptr), 0.0)
     ctx.call("ec_poisson_solve", ps2.h)
     torch.cuda.synchronize()
     p1, p2 = ps1.p.data, ps2.p.data
     assert float((p1 - p2).abs().max()) <= (1e-12 if prec == "f64" else 1e-5) * float(p1.abs().max())
+
+
+@pytest.mark.parametrize("n", [16, 256])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_rhs_div_state_divergence(gpu, O, n, prec):
+    """d = div_h du + r div_h u: the divergence of the stage state folded into
+    the Poisson right-hand side (r = 6/dt at the last RK4 stage), against the
+    checker's divergence of du and of u (an unprojected random u, so div_h u
+    is of the size of its terms)."""
+    dt = torch.float64 if prec == "f64" else torch.float32
+    u = rand_u(O, n, 5 * n + 3)
+    visc, l, r = 0.011, 1.0, 6.0 / 1e-3
+    ref = O.rhs(u, visc, l)
+    dref = O.divergence(ref, l) + r * O.divergence(u, l)
+    ud = [dev(x, dt) for x in u]
+    du = [torch.empty_like(x) for x in ud]
+    d = torch.empty_like(ud[0])
+    ctx = gpu._abi.context()
+    ctx.call("ec_ns_rhs_div", 1 if prec == "f64" else 0, n, l, visc, P(ud, 3), P(du, 3),
+             gpu._abi.c_void_p(d.data_ptr()), r)
+    scale = (max(np.abs(x).max() for x in ref) + r * max(np.abs(x).max() for x in u)) * n
+    assert np.abs(host(d) - dref).max() / scale < (1e-13 if prec == "f64" else 1e-6)
 
 
 @pytest.mark.parametrize("n", [512, 1024])
@@ -106,7 +128,7 @@ def test_rhs_div_large_fp32(gpu, n):
     du = [torch.empty_like(x) for x in u]
     d = torch.empty_like(u[0])
     ctx = gpu._abi.context()
-    ctx.call("ec_ns_rhs_div", 0, n, l, visc, P(u, 3), P(du, 3), gpu._abi.c_void_p(d.data_ptr()))
+    ctx.call("ec_ns_rhs_div", 0, n, l, visc, P(u, 3), P(du, 3), gpu._abi.c_void_p(d.data_ptr()), 0.0)
     ref = [torch.empty_like(x) for x in u]
     ctx.call("ec_ns_rhs", 0, n, l, visc, P(u, 3), P(ref, 3), 0)
     torch.cuda.synchronize()
@@ -532,3 +554,62 @@ def test_invalid_arguments_raise(gpu):
         ps8 = NS.PoissonSolver(g8, torch.float64)
         u8 = NS.vectorfield(g8)
         ctx.call("ec_ns_spectrum", ps8.h, P([x.data for x in u8], 3), 0, (ctypes.c_double * 1)())
+
+
+@pytest.mark.parametrize("n", [16, 32])
+def test_steppers_fe_wray3_vs_checker(gpu, O, n):
+    """step_forwardeuler! and step_wray3! (src/N.jl:309-346) through the host
+    mirror (the fused RHS, axpy and the FFT projection on the GPU, FP64)
+    against the checker's steppers over 5 steps each, to 1e-12."""
+    NS = gpu.navierstokes
+    g = NS.Grid(1.0, n)
+    visc, dt = 0.01, 2e-3
+    u0 = O.randomfield(n, seed=n + 1)
+    ps = NS.PoissonSolver(g, torch.float64)
+    for name in ("fe", "wray3"):
+        u = tuple(NS.Field.from_numpy(g, x) for x in u0)
+        du, us = NS.vectorfield(g), NS.vectorfield(g)
+        ref = [x.copy(order="F") for x in u0]
+        for _ in range(5):
+            if name == "fe":
+                NS.step_forwardeuler_(u, du, ps, visc, dt)
+                O.step_forwardeuler(ref, visc, dt)
+            else:
+                NS.step_wray3_(u, du, us, ps, visc, dt)
+                O.step_wray3(ref, visc, dt)
+        for a in range(3):
+            assert rel(u[a].numpy(), ref[a]) < 1e-12, (name, a)
+
+
+def test_steppers_convergence_order_on_gpu(gpu, O):
+    """Observed order in dt of the GPU FP64 steppers over [0, 0.08] at n = 16
+    (the pins of the checker's steppers, run on the product path): forward
+    Euler 1, Wray RK3 3, the RK4 DNS step of DNSAidedLES (no LES grids) 4."""
+    NS, D = gpu.navierstokes, gpu.dnsaid
+    n, visc, T = 16, 0.01, 0.08
+    g = NS.Grid(1.0, n)
+    u0 = O.randomfield(n, seed=5)
+    ps = NS.PoissonSolver(g, torch.float64)
+
+    def run(kind, m):
+        dt = T / m
+        if kind == "rk4":
+            drv = D.DNSAidedLES(g, visc, [], dtype=torch.float64)
+            drv.set_state([dev(x) for x in u0])
+            drv.run(dt, m)
+            return [host(x) for x in drv.u0]
+        u = tuple(NS.Field.from_numpy(g, x) for x in u0)
+        du, us = NS.vectorfield(g), NS.vectorfield(g)
+        for _ in range(m):
+            if kind == "fe":
+                NS.step_forwardeuler_(u, du, ps, visc, dt)
+            else:
+                NS.step_wray3_(u, du, us, ps, visc, dt)
+        return [x.numpy() for x in u]
+
+    for kind, ms, lo, hi in (("fe", (8, 16, 32, 64), 0.9, 1.2), ("wray3", (4, 8, 16, 32), 2.8, 3.4),
+                             ("rk4", (4, 8, 16, 32), 3.8, 4.4)):
+        us = [run(kind, m) for m in ms]
+        d = [max(np.abs(a - b).max() for a, b in zip(us[i], us[i + 1])) for i in range(len(ms) - 1)]
+        p = np.log2(d[-2] / d[-1])
+        assert lo < p < hi, (kind, p, d)

@@ -517,3 +517,49 @@ def test_burgers_commutation(O, c):
     ps = O.burgers_filter_coarse(s, n // c, wp, -(c // 2))
     rhs = (np.roll(ps, -1) - ps) / H
     assert np.abs(lhs - rhs).max() < 1e-12 * np.abs(lhs).max()
+
+
+# ---------------------------------------------------------------- NS time steppers
+def _ns_order(O, step, ms, n=16, visc=0.01, T=0.08, seed=5):
+    """Observed order log2(|u_m - u_2m| / |u_2m - u_4m|) of a stepper over
+    [0, T] with m, 2m, 4m ... steps, from a divergence-free random field."""
+    u0 = O.randomfield(n, seed=seed)
+
+    def run(m):
+        u = [x.copy(order="F") for x in u0]
+        for _ in range(m):
+            step(u, T / m)
+        return u
+
+    us = [run(m) for m in ms]
+    d = [max(np.abs(a - b).max() for a, b in zip(us[i], us[i + 1])) for i in range(len(ms) - 1)]
+    return [math.log2(d[i] / d[i + 1]) for i in range(len(d) - 1)]
+
+
+def test_ns_forward_euler_first_order(O):
+    """step_forwardeuler! (src/N.jl:309-318): first order in dt."""
+    p = _ns_order(O, lambda u, dt: O.step_forwardeuler(u, 0.01, dt), (8, 16, 32, 64))
+    assert 0.9 < p[-1] < 1.2, p
+
+
+def test_ns_wray3_third_order(O):
+    """step_wray3! (src/N.jl:320-346), a = (8/15, 5/12, 3/4), b = (1/4, 0):
+    third order in dt (the projection at every stage keeps the stages on the
+    divergence-free subspace, where the scheme is the plain low-storage RK3)."""
+    p = _ns_order(O, lambda u, dt: O.step_wray3(u, 0.01, dt), (4, 8, 16, 32))
+    assert 2.8 < p[-1] < 3.4, p
+
+
+def test_ns_rk4_fourth_order_and_mutants(O, monkeypatch):
+    """The RK4 DNS step of dnsaid_rk4 (tableau of rk4_timestep!,
+    src/Burgers.jl:120-142, projected stage tendencies): fourth order in dt;
+    a wrong stage time (a3 = 1/2) or a wrong weight order (b = 1, 2, 1, 2)
+    drops it to first order."""
+    step = lambda u, dt: O.dnsaid_rk4(u, [], 0.01, dt, 1)
+    p = _ns_order(O, step, (4, 8, 16, 32))
+    assert 3.8 < p[-1] < 4.4, p
+    monkeypatch.setattr(O, "RK4_A", (0.5, 0.5, 0.5))
+    assert _ns_order(O, step, (4, 8, 16, 32))[-1] < 1.5
+    monkeypatch.setattr(O, "RK4_A", (0.5, 0.5, 1.0))
+    monkeypatch.setattr(O, "RK4_B", (1.0, 2.0, 1.0, 2.0))
+    assert _ns_order(O, step, (4, 8, 16, 32))[-1] < 1.5
