@@ -93,6 +93,8 @@ _SIGS = {
     "ec_ns_randomfield": (c_int, [c_void_p, c_void_p, c_int, c_double, c_double, c_u64, P3]),
     "ec_ns_spectrum": (c_int, [c_void_p, c_void_p, P3, c_int, ctypes.POINTER(c_double)]),
     "ec_randn": (c_int, [c_void_p, c_int, c_size_t, c_u64, c_u64, c_void_p]),
+    "ec_filter_apply": (c_int, [c_void_p, c_int, c_int, ctypes.POINTER(c_int), ctypes.POINTER(ctypes.POINTER(c_double)),
+                                c_void_p, c_void_p]),
     "ec_filter_coarse": (c_int, [c_void_p, c_int, c_int, c_int, c_double, c_double, P3, c_void_p,
                                  c_int, ctypes.POINTER(FilterQuantity)]),
     "ec_ns_closure": (c_int, [c_void_p, c_int, c_int, c_double, c_double, P3, c_void_p, c_int,

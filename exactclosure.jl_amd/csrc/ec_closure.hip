@@ -107,6 +107,74 @@ __global__ void k_filter_coarse(Src<T> src, int nbar, int c, QList ql) {
     q.out[J] = s;
 }
 
+// Separable form of k_filter_coarse for wide kernels (Gaussian-composed
+// filters: (2K+1)^3 taps per coarse point in the dense form): three passes,
+// each a 1D filter sampled at the coarse points of one axis,
+//   t1[J1, j, k]  = sum_t w1[t] src(c(J1+1)-1+off1+t, j, k)        (source on the fly)
+//   t2[J1, J2, k] = sum_t w2[t] t1[J1, c(J2+1)-1+off2+t, k]
+//   out[J]        = sum_t w3[t] t2[J1, J2, c(J3+1)-1+off3+t]
+// so a coarse point costs len1 + len2 + len3 multiply-adds per fine point
+// it covers instead of len1 len2 len3.  All sums FP64.
+template <typename T>
+__global__ void k_sep_x(Src<T> src, int source, int nbar, int c, int len, int off, const double *__restrict__ w,
+                        double *__restrict__ t1) {
+    const int n = src.n;
+    const size_t total = (size_t)nbar * n * n;
+    const size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (id >= total) return;
+    const int J1 = (int)(id % nbar);
+    const size_t jk = id / nbar;
+    const int j = (int)(jk % n), k = (int)(jk / n);
+    const int b = c * (J1 + 1) - 1 + off;
+    double s = 0.0;
+    for (int t = 0; t < len; t++) s += w[t] * src.eval(source, wrapn(b + t, n), j, k);
+    t1[id] = s;
+}
+
+__global__ void k_sep_y(int n, int nbar, int c, int len, int off, const double *__restrict__ w,
+                        const double *__restrict__ t1, double *__restrict__ t2) {
+    const size_t total = (size_t)nbar * nbar * n;
+    const size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (id >= total) return;
+    const int J1 = (int)(id % nbar), J2 = (int)((id / nbar) % nbar);
+    const size_t k = id / ((size_t)nbar * nbar);
+    const int b = c * (J2 + 1) - 1 + off;
+    double s = 0.0;
+    for (int t = 0; t < len; t++) s += w[t] * t1[(size_t)J1 + (size_t)nbar * ((size_t)wrapn(b + t, n) + (size_t)n * k)];
+    t2[id] = s;
+}
+
+__global__ void k_sep_z(int n, int nbar, int c, int len, int off, const double *__restrict__ w,
+                        const double *__restrict__ t2, double *__restrict__ out) {
+    const size_t total = (size_t)nbar * nbar * nbar;
+    const size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (id >= total) return;
+    const size_t J12 = id % ((size_t)nbar * nbar);
+    const int J3 = (int)(id / ((size_t)nbar * nbar));
+    const int b = c * (J3 + 1) - 1 + off;
+    double s = 0.0;
+    for (int t = 0; t < len; t++) s += w[t] * t2[J12 + (size_t)nbar * nbar * (size_t)wrapn(b + t, n)];
+    out[id] = s;
+}
+
+// applyfilter! (src/N.jl:639-653) with a kernelproduct of three centred 1D
+// kernels, as three 1D passes along x, y, z (periodic):
+//   out[i] = sum_{r=-R}^{R} w[R + r] in[i + r e_axis].
+template <typename TI, typename TO>
+__global__ void k_apply_axis(int n, int axis, int R, const double *__restrict__ w, const TI *__restrict__ in,
+                             TO *__restrict__ out) {
+    const size_t n1 = (size_t)n, total = n1 * n1 * n1;
+    const size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (id >= total) return;
+    const int i = (int)(id % n1), j = (int)((id / n1) % n1), k = (int)(id / (n1 * n1));
+    const int x0 = axis == 0 ? i : (axis == 1 ? j : k);
+    const size_t stride = axis == 0 ? 1 : (axis == 1 ? n1 : n1 * n1);
+    const size_t base = id - (size_t)x0 * stride;
+    double s = 0.0;
+    for (int r = -R; r <= R; r++) s += w[R + r] * (double)in[base + (size_t)wrapn(x0 + r, n) * stride];
+    out[id] = (TO)s;
+}
+
 // tau_ab = flux_ab - sigma_H(ubar)_ab on the LES grid, nonsym order.
 __global__ void k_commutator(int nb, double ih, double visc, const double *__restrict__ u1,
                              const double *__restrict__ u2, const double *__restrict__ u3, QList fl, QList tl) {
@@ -150,6 +218,51 @@ static const double *weights_on_device(ec_ctx *ctx, const double *w, int len, in
     return d;
 }
 
+// Scratch of the separable passes (t1, t2 and the three weight vectors) in
+// doubles for one quantity.
+static size_t sep_doubles(int n, int nbar, const int len[3]) {
+    return (size_t)nbar * n * n + (size_t)nbar * nbar * n + (size_t)len[0] + len[1] + len[2];
+}
+
+// The three separable passes of one quantity; ws holds sep_doubles(...)
+// doubles, the weights already in place at its end (w[a] device pointers).
+template <typename T>
+static int sep_launch(ec_ctx *ctx, const Src<T> &s, int nbar, int source, const int len[3], const int off[3],
+                      const double *const w[3], double *ws, double *out) {
+    const int n = s.n, c = n / nbar;
+    double *t1 = ws, *t2 = ws + (size_t)nbar * n * n;
+    const size_t m1 = (size_t)nbar * n * n, m2 = (size_t)nbar * nbar * n, m3 = (size_t)nbar * nbar * nbar;
+    k_sep_x<T><<<(unsigned)((m1 + 255) / 256), 256, 0, ctx->stream>>>(s, source, nbar, c, len[0], off[0], w[0], t1);
+    EC_CHECK_LAUNCH(ctx, "k_sep_x");
+    k_sep_y<<<(unsigned)((m2 + 255) / 256), 256, 0, ctx->stream>>>(n, nbar, c, len[1], off[1], w[1], t1, t2);
+    EC_CHECK_LAUNCH(ctx, "k_sep_y");
+    k_sep_z<<<(unsigned)((m3 + 255) / 256), 256, 0, ctx->stream>>>(n, nbar, c, len[2], off[2], w[2], t2, out);
+    EC_CHECK_LAUNCH(ctx, "k_sep_z");
+    return EC_OK;
+}
+
+// Context-owned workspace of at least `doubles` doubles (grown on demand;
+// freed with the context).
+static double *workspace(ec_ctx *ctx, size_t doubles, int *rc) {
+    if (ctx->ws_doubles < doubles) {
+        if (ctx->ws) {
+            (void)hipStreamSynchronize(ctx->stream);
+            (void)hipFree(ctx->ws);
+            ctx->ws = nullptr;
+            ctx->ws_doubles = 0;
+        }
+        if (hipMalloc(&ctx->ws, sizeof(double) * doubles) != hipSuccess) {
+            *rc = ec_fail(ctx, EC_E_OOM, "workspace of %zu doubles", doubles);
+            return nullptr;
+        }
+        ctx->ws_doubles = doubles;
+    }
+    return (double *)ctx->ws;
+}
+
+// Quantities whose dense tap count exceeds this take the separable passes.
+constexpr long EC_DENSE_TAPS = 64;
+
 template <typename T>
 static int filter_launch(ec_ctx *ctx, int n, int nbar, double l, double visc, void *const u[3], const void *p,
                          int nq, const ec_filter_quantity *q) {
@@ -161,25 +274,47 @@ static int filter_launch(ec_ctx *ctx, int n, int nbar, double l, double visc, vo
     s.visc = visc;
     const int c = n / nbar;
     const size_t nb3 = (size_t)nbar * nbar * nbar;
-    for (int q0 = 0; q0 < nq; q0 += MAXQ) {
+    // wide quantities: separable passes, one quantity at a time
+    std::vector<int> dense;
+    for (int m = 0; m < nq; m++) {
+        const ec_filter_quantity &src = q[m];
+        for (int a = 0; a < 3; a++) {
+            EC_REQUIRE(ctx, src.len[a] >= 1 && src.len[a] <= 4 * n && src.w[a], "filter quantity %d: bad axis %d", m, a);
+            EC_REQUIRE(ctx, src.off[a] >= -4 * n && src.off[a] <= 4 * n, "filter quantity %d: offset out of range", m);
+        }
+        EC_REQUIRE(ctx, src.out, "filter quantity %d: null output", m);
+        EC_REQUIRE(ctx, src.source >= 0 && src.source <= EC_SRC_K3, "filter quantity %d: bad source", m);
+        EC_REQUIRE(ctx, (src.source < EC_SRC_R11) || p, "pressure source without pressure field");
+        if ((long)src.len[0] * src.len[1] * src.len[2] <= EC_DENSE_TAPS) {
+            dense.push_back(m);
+            continue;
+        }
+        int rc = EC_OK;
+        double *ws = workspace(ctx, sep_doubles(n, nbar, src.len), &rc);
+        if (!ws) return rc;
+        const double *w[3];
+        for (int a = 0; a < 3; a++) {
+            w[a] = weights_on_device(ctx, src.w[a], src.len[a], &rc);
+            if (rc) return rc;
+        }
+        ec_prof_scope pr(ctx, "filter_sep", (double)sizeof(T) * 4.0 * ec_cube(n) + 8.0 * nb3);
+        if ((rc = sep_launch<T>(ctx, s, nbar, src.source, src.len, src.off, w, ws, (double *)src.out))) return rc;
+    }
+    const int ndense = (int)dense.size();
+    for (int q0 = 0; q0 < ndense; q0 += MAXQ) {
         QList ql;
-        ql.nq = std::min(MAXQ, nq - q0);
+        ql.nq = std::min(MAXQ, ndense - q0);
         for (int m = 0; m < ql.nq; m++) {
-            const ec_filter_quantity &src = q[q0 + m];
+            const ec_filter_quantity &src = q[dense[q0 + m]];
             QDesc &d = ql.q[m];
             d.source = src.source;
             for (int a = 0; a < 3; a++) {
-                EC_REQUIRE(ctx, src.len[a] >= 1 && src.len[a] <= n && src.w[a], "filter quantity %d: bad axis %d", q0 + m, a);
-                EC_REQUIRE(ctx, src.off[a] > -n && src.off[a] < n, "filter quantity %d: offset out of range", q0 + m);
                 d.len[a] = src.len[a];
                 d.off[a] = src.off[a];
                 int rc = EC_OK;
                 d.w[a] = weights_on_device(ctx, src.w[a], src.len[a], &rc);
                 if (rc) return rc;
             }
-            EC_REQUIRE(ctx, src.out, "filter quantity %d: null output", q0 + m);
-            EC_REQUIRE(ctx, src.source >= 0 && src.source <= EC_SRC_K3, "filter quantity %d: bad source", q0 + m);
-            EC_REQUIRE(ctx, (src.source < EC_SRC_R11) || p, "pressure source without pressure field");
             d.out = (double *)src.out;
         }
         dim3 grid((unsigned)((nb3 + 255) / 256), ql.nq);
@@ -188,6 +323,53 @@ static int filter_launch(ec_ctx *ctx, int n, int nbar, double l, double visc, vo
         EC_CHECK_LAUNCH(ctx, "k_filter_coarse");
     }
     return EC_OK;
+}
+
+// applyfilter!(out, in, kernelproduct(w1, w2, w3)) as three 1D passes; the
+// intermediates are FP64 in a second context workspace (2 n^3 doubles).
+template <typename T>
+static int apply_launch(ec_ctx *ctx, int n, const int len[3], const double *const w[3], const T *in, T *out) {
+    const size_t N = ec_cube(n);
+    if (ctx->ws2_doubles < 2 * N) {
+        if (ctx->ws2) {
+            (void)hipStreamSynchronize(ctx->stream);
+            (void)hipFree(ctx->ws2);
+            ctx->ws2 = nullptr;
+            ctx->ws2_doubles = 0;
+        }
+        if (hipMalloc(&ctx->ws2, sizeof(double) * 2 * N) != hipSuccess)
+            return ec_fail(ctx, EC_E_OOM, "ec_filter_apply: workspace of %zu doubles", 2 * N);
+        ctx->ws2_doubles = 2 * N;
+    }
+    double *ta = (double *)ctx->ws2, *tb = ta + N;
+    const double *wd[3];
+    for (int a = 0; a < 3; a++) {
+        int rc = EC_OK;
+        wd[a] = weights_on_device(ctx, w[a], len[a], &rc);
+        if (rc) return rc;
+    }
+    const unsigned blocks = (unsigned)((N + 255) / 256);
+    ec_prof_scope pr(ctx, "filter_apply", (double)(2 * sizeof(T) + 32) * N);
+    k_apply_axis<T, double><<<blocks, 256, 0, ctx->stream>>>(n, 0, len[0] / 2, wd[0], in, ta);
+    EC_CHECK_LAUNCH(ctx, "k_apply_axis");
+    k_apply_axis<double, double><<<blocks, 256, 0, ctx->stream>>>(n, 1, len[1] / 2, wd[1], ta, tb);
+    EC_CHECK_LAUNCH(ctx, "k_apply_axis");
+    k_apply_axis<double, T><<<blocks, 256, 0, ctx->stream>>>(n, 2, len[2] / 2, wd[2], tb, out);
+    EC_CHECK_LAUNCH(ctx, "k_apply_axis");
+    return EC_OK;
+}
+
+extern "C" int ec_filter_apply(ec_ctx *ctx, int dtype, int n, const int len[3], const double *const w[3],
+                               const void *in, void *out) {
+    EC_REQUIRE(ctx, ctx && len && w && in && out, "ec_filter_apply: null argument");
+    EC_REQUIRE(ctx, n >= 2, "ec_filter_apply: n=%d", n);
+    EC_REQUIRE(ctx, in != out, "ec_filter_apply: in place is not supported");
+    for (int a = 0; a < 3; a++) {
+        EC_REQUIRE(ctx, w[a] && len[a] >= 1 && (len[a] & 1), "ec_filter_apply: axis %d needs an odd, centred kernel", a);
+        EC_REQUIRE(ctx, len[a] / 2 <= 4 * n, "ec_filter_apply: axis %d kernel radius beyond 4 n", a);
+    }
+    return dtype == EC_F64 ? apply_launch<double>(ctx, n, len, w, (const double *)in, (double *)out)
+                           : apply_launch<float>(ctx, n, len, w, (const float *)in, (float *)out);
 }
 
 extern "C" int ec_filter_coarse(ec_ctx *ctx, int dtype, int n, int nbar, double l, double visc, void *const u[3],

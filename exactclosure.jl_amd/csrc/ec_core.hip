@@ -46,6 +46,8 @@ extern "C" int ec_destroy(ec_ctx *ctx) {
     }
     for (auto e : ctx->event_pool) hipEventDestroy(e);
     if (ctx->red_partial) hipFree(ctx->red_partial);
+    if (ctx->ws) hipFree(ctx->ws);
+    if (ctx->ws2) hipFree(ctx->ws2);
     if (ctx->red_host) hipHostFree(ctx->red_host);
     if (ctx->own_stream) hipStreamDestroy(ctx->stream);
     delete ctx;

@@ -37,6 +37,11 @@ struct ec_ctx {
     // the x/y/z block targets of the flux-form closure kernel.
     int fv_tz = 256;
     int cl_tx = 16, cl_ty = 16, cl_tz = 64;
+    // Workspace of the separable filter passes (grown on demand, FP64).
+    void *ws = nullptr;
+    size_t ws_doubles = 0;
+    void *ws2 = nullptr;
+    size_t ws2_doubles = 0;
 };
 
 // ---- error handling: no C++ exception crosses the C ABI ----

@@ -426,7 +426,12 @@ def filter_coarse(u, p, gbar, visc, quantities):
         qs.append(q)
         keep.append(k)
     arr = (_abi.FilterQuantity * len(qs))(*qs)
-    pp = ptr(p.data) if p is not None else None
+    if p is None:
+        pp = None
+    elif isinstance(p, Field):
+        pp = ptr(p.data)
+    else:
+        pp = _abi.c_void_p(p)  # a device pointer, e.g. PoissonSolver.p_ptr
     _ctx().call("ec_filter_coarse", _dt(u[0]), g.n, gbar.n, g.l, visc, ptrs([x.data for x in u], P3),
                 pp, len(qs), arr)
     return outs
@@ -453,6 +458,66 @@ def coarsegrain_(ubar, u, stag):
     """coarsegrain!(ubar, u, stag) (src/N.jl:670-684): point decimation."""
     one = KernelProduct([np.ones(1)] * 3)
     return applyfilter_coarsegrain(ubar, u, one, stag)
+
+
+def applyfilter_(ubar, u, kernel):
+    """applyfilter!(ubar, u, kernelproduct(w1, w2, w3)) (src/N.jl:627-653) on
+    the full grid, as three periodic 1D passes with FP64 sums."""
+    g = u.grid
+    ks = [np.ascontiguousarray(k, dtype=np.float64) for k in kernel.kernels]
+    lens = (_abi.c_int * 3)(*[len(k) for k in ks])
+    dp = _abi.ctypes.POINTER(_abi.c_double)
+    ws = (dp * 3)(*[k.ctypes.data_as(dp) for k in ks])
+    _ctx().call("ec_filter_apply", _dt(u), g.n, lens, ws, ptr(u.data), ptr(ubar.data))
+    return ubar
+
+
+# Staggered positions of get_positions (src/N.jl:999-1006): faces e_i, the
+# cell centre and the three edge positions corners3D = (F,T,T), (T,F,T), (T,T,F)
+CENTER = (False, False, False)
+CORNER23, CORNER31, CORNER12 = (False, True, True), (True, False, True), (True, True, False)
+
+
+def face(i):
+    return tuple(d == i for d in range(3))
+
+
+def dnsaid_quantities(c, FD, FDH):
+    """The coarse quantities of one DNS-aided LES step with the kernels of
+    get_kernels, as (name, source, [(w, off)] * 3) for filter_coarse:
+      ubar{a}    lazyfilter(u_a, GDH) at face a (apply_DHpi, src/N.jl:1228-1237);
+      sigDH{ab}  lazyfilter(sigma_ab, GDH) at the stress positions (get_sigDH,
+                 src/N.jl:1088-1115), symmetric order 11, 22, 33, 12, 23, 31;
+      rDHi{ab}   lazyfilter(r_ab, GDHi[b]) (get_rDHi, src/N.jl:1038-1073) in
+                 nonsym order 11, 21, 31, 12, 22, 32, 13, 23, 33, r = sigma + p I.
+    GDH = FDH x FDH x FDH, GDHi[i] = FD along i and FDH elsewhere."""
+    GDH = (FDH, FDH, FDH)
+    GDHi = [tuple(FD if d == i else FDH for d in range(3)) for i in range(3)]
+
+    def pieces(kernels, stag):
+        return [(np.asarray(k, dtype=np.float64), coarse_offset(c, s, len(k) // 2)) for k, s in zip(kernels, stag)]
+
+    q = []
+    for a in range(3):
+        q.append((f"ubar{a + 1}", f"u{a + 1}", pieces(GDH, face(a))))
+    for name, src, pos in (("11", "s11", CENTER), ("22", "s22", CENTER), ("33", "s33", CENTER),
+                           ("12", "s12", CORNER12), ("23", "s23", CORNER23), ("31", "s31", CORNER31)):
+        q.append((f"sigDH{name}", src, pieces(GDH, pos)))
+    # r_ab filtered with GDHi[b], b the derivative direction of -div^ns
+    for name, src, b, pos in (("11", "r11", 0, CENTER), ("21", "s12", 0, CORNER12), ("31", "s31", 0, CORNER31),
+                              ("12", "s12", 1, CORNER12), ("22", "r22", 1, CENTER), ("32", "s23", 1, CORNER23),
+                              ("13", "s31", 2, CORNER31), ("23", "s23", 2, CORNER23), ("33", "r33", 2, CENTER)):
+        q.append((f"rDHi{name}", src, pieces(GDHi[b], pos)))
+    return q
+
+
+def stress_positions(c):
+    """lazytensorcoarsegrain (src/N.jl:1117-1135) as one-tap quantities: the six
+    stresses of a fine field sampled at their LES positions."""
+    one = np.ones(1)
+    return [(f"s{name}", f"s{name}", [(one, coarse_offset(c, st, 0)) for st in pos])
+            for name, pos in (("11", CENTER), ("22", CENTER), ("33", CENTER), ("12", CORNER12),
+                              ("23", CORNER23), ("31", CORNER31))]
 
 
 def get_kernels(g_dns, g_les, delta_ratio, nsigma):

@@ -191,10 +191,18 @@ typedef struct {
     void *out;          /* FP64 device array, nbar^3 */
 } ec_filter_quantity;
 
-/* lazycoarsegrain(lazyfilter(.)) of many quantities in one launch
- * (get_rDHi :1038, get_sigDH :1088, apply_DHpi :1228) */
+/* lazycoarsegrain(lazyfilter(.)) of many quantities (get_rDHi :1038,
+ * get_sigDH :1088, apply_DHpi :1228).  Narrow kernels (at most 64 taps in
+ * all) in one dense launch; wider ones (Gaussian-composed) as three
+ * separable passes per quantity in a context-owned FP64 workspace of
+ * nbar n^2 + nbar^2 n doubles. */
 int ec_filter_coarse(ec_ctx *ctx, int dtype, int n, int nbar, double l, double visc,
                      void *const u[3], const void *p, int nq, const ec_filter_quantity *q);
+/* applyfilter!(out, in, kernelproduct(w1, w2, w3)) (src/N.jl:627-653) with
+ * centred odd-length 1D kernels (host arrays), as three periodic 1D passes,
+ * FP64 sums; in and out are n^3 fields of dtype (not in place). */
+int ec_filter_apply(ec_ctx *ctx, int dtype, int n, const int len[3], const double *const w[3], const void *in,
+                    void *out);
 
 /* Volume-average (kind 0) or face-average (kind 1) closure of one or more LES
  * grids from one DNS state, in FP64:

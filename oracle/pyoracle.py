@@ -771,9 +771,35 @@ def apply_DHpi(u, nbar, c, g=None, l=1.0):
     return ub
 
 
+def dnsaid_project(u, n_les, visc, delta_ratio=2.0, nsigma=2, twarm=0.1, tstop=0.1, cfl=0.4, l=1.0):
+    """dnsaid_project (src/N.jl:1239-1332) with its time control: forward-Euler
+    DNS warm-up to twarm, then the DNS and the five LES to tstop, each step
+    dt = min(propose_timestep(u_dns, visc, cfl), t_end - t).  Returns the LES
+    fields as dnsaid_project_fe and the number of steps of the main loop."""
+    t = 0.0
+    while t < twarm:
+        dt = min(propose_timestep(u, visc, cfl, l), twarm - t)
+        t += dt
+        step_forwardeuler(u, visc, dt, l)
+    state = {"t": 0.0, "k": 0}
+
+    def next_dt(uu):
+        if state["t"] >= tstop:
+            return None
+        dt = min(propose_timestep(uu, visc, cfl, l), tstop - state["t"])
+        state["t"] += dt
+        state["k"] += 1
+        return dt
+
+    les = dnsaid_project_fe(u, n_les, visc, None, next_dt, delta_ratio, nsigma, l)
+    return les, state["k"]
+
+
 def dnsaid_project_fe(u, n_les, visc, nsteps, dt, delta_ratio=0.0, nsigma=2, l=1.0):
     """dnsaid_project main loop (src/N.jl:1296-1332) with fixed dt and nsteps
-    (the reference chooses dt by propose_timestep and stops at t = 0.1).
+    (the reference chooses dt by propose_timestep and stops at t = 0.1; a
+    callable dt(u) giving the next step or None to stop does that, see
+    dnsaid_project).
 
     Returns the five LES fields and the filtered DNS, keyed like the reference:
     u_les, u_nomo, u_c, u_cf, u_cfd, u_cfd_symm.
@@ -793,7 +819,16 @@ def dnsaid_project_fe(u, n_les, visc, nsteps, dt, delta_ratio=0.0, nsigma=2, l=1
         FDH = composekernel(tophat(c), g)
     else:
         FDH = None
-    for _ in range(nsteps):
+    step_dt = dt if callable(dt) else None
+    k = 0
+    while True:
+        if step_dt is not None:
+            dt = step_dt(u)
+            if dt is None:
+                break
+        elif k >= nsteps:
+            break
+        k += 1
         du_dns = rhs(u, visc, l)
         p = project(du_dns, l)
         src = closure_sources(u, p, visc, l)
