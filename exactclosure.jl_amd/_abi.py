@@ -97,6 +97,8 @@ _SIGS = {
                                 c_void_p, c_void_p]),
     "ec_filter_coarse": (c_int, [c_void_p, c_int, c_int, c_int, c_double, c_double, P3, c_void_p,
                                  c_int, ctypes.POINTER(FilterQuantity)]),
+    "ec_filter_coarse_noalloc": (c_int, [c_void_p, c_int, c_int, c_int, c_double, c_double, P3, c_void_p,
+                                 c_int, ctypes.POINTER(FilterQuantity)]),
     "ec_ns_closure": (c_int, [c_void_p, c_int, c_int, c_double, c_double, P3, c_void_p, c_int,
                               ctypes.POINTER(ClosureSet)]),
     "ec_closure_set_tiles": (c_int, [c_void_p, c_int, c_int, c_int]),

@@ -10,6 +10,7 @@
 #include "ec_internal.h"
 
 #include <map>
+#include <mutex>
 
 namespace {
 
@@ -323,6 +324,112 @@ static int filter_launch(ec_ctx *ctx, int n, int nbar, double l, double visc, vo
         EC_CHECK_LAUNCH(ctx, "k_filter_coarse");
     }
     return EC_OK;
+}
+
+// ---- the filter without allocation (ec_filter_coarse_noalloc) ----
+// The separable passes of one quantity in a fixed device array: t1, t2 and
+// the three weight vectors of sep_doubles(n, nbar, len).  Each GPU holds its
+// own copy for the life of the process; every call on that GPU shares it.
+constexpr size_t EC_NOALLOC_DOUBLES = (size_t)4 << 20;  // 32 MiB
+constexpr int EC_NOALLOC_TAPS = 64;                       // per axis
+static __device__ __align__(256) double scratch[EC_NOALLOC_DOUBLES];
+
+namespace {
+struct WArg {
+    int len[3];
+    double w[3][EC_NOALLOC_TAPS];
+};
+// the weights reach the device as kernel arguments (no host copy)
+__global__ void k_put_weights(WArg a, double *__restrict__ dst) {
+    int o = 0;
+    for (int d = 0; d < 3; d++) {
+        for (int t = threadIdx.x; t < a.len[d]; t += blockDim.x) dst[o + t] = a.w[d][t];
+        o += a.len[d];
+    }
+}
+}  // namespace
+
+// The device address of this GPU's scratch, looked up once per device with
+// that device current (the host-side symbol itself is not a device address).
+static double *scratch_on_device(ec_ctx *ctx, int *rc) {
+    static std::mutex mu;
+    static double *addr[64] = {nullptr};
+    if (ctx->device < 0 || ctx->device >= 64) {
+        *rc = ec_fail(ctx, EC_E_UNSUPPORTED, "device %d beyond the scratch table", ctx->device);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[ctx->device]) {
+        int cur = 0;
+        if (hipGetDevice(&cur) != hipSuccess || (cur != ctx->device && hipSetDevice(ctx->device) != hipSuccess)) {
+            *rc = ec_fail(ctx, EC_E_HIP, "cannot make device %d current", ctx->device);
+            return nullptr;
+        }
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (cur != ctx->device) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !ptr) {
+            *rc = ec_fail(ctx, EC_E_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+            return nullptr;
+        }
+        addr[ctx->device] = (double *)ptr;
+    }
+    return addr[ctx->device];
+}
+
+template <typename T>
+static int noalloc_launch(ec_ctx *ctx, int n, int nbar, double l, double visc, void *const u[3], const void *p,
+                          int nq, const ec_filter_quantity *q) {
+    Src<T> s;
+    for (int a = 0; a < 3; a++) s.u[a] = (const T *)u[a];
+    s.p = (const T *)p;
+    s.n = n;
+    s.ih = n / l;
+    s.visc = visc;
+    int rc = EC_OK;
+    double *ws = scratch_on_device(ctx, &rc);
+    if (!ws) return rc;
+    for (int m = 0; m < nq; m++) {
+        const ec_filter_quantity &src = q[m];
+        WArg wa;
+        for (int a = 0; a < 3; a++) {
+            wa.len[a] = src.len[a];
+            for (int t = 0; t < src.len[a]; t++) wa.w[a][t] = src.w[a][t];
+        }
+        double *wdev = ws + (size_t)nbar * n * n + (size_t)nbar * nbar * n;
+        k_put_weights<<<1, 64, 0, ctx->stream>>>(wa, wdev);
+        EC_CHECK_LAUNCH(ctx, "k_put_weights");
+        const double *w[3] = {wdev, wdev + src.len[0], wdev + src.len[0] + src.len[1]};
+        if ((rc = sep_launch<T>(ctx, s, nbar, src.source, src.len, src.off, w, ws, (double *)src.out))) return rc;
+    }
+    return EC_OK;
+}
+
+extern "C" int ec_filter_coarse_noalloc(ec_ctx *ctx, int dtype, int n, int nbar, double l, double visc,
+                                        void *const u[3], const void *p, int nq, const ec_filter_quantity *q) {
+    EC_REQUIRE(ctx, ctx && u && (nq == 0 || q), "ec_filter_coarse_noalloc: null argument");
+    EC_REQUIRE(ctx, nbar >= 1 && n >= 2 && n % nbar == 0, "ec_filter_coarse_noalloc: n=%d not a multiple of nbar=%d",
+               n, nbar);
+    for (int m = 0; m < nq; m++) {
+        const ec_filter_quantity &src = q[m];
+        for (int a = 0; a < 3; a++) {
+            EC_REQUIRE(ctx, src.len[a] >= 1 && src.len[a] <= 4 * n && src.w[a], "filter quantity %d: bad axis %d", m, a);
+            EC_REQUIRE(ctx, src.len[a] <= EC_NOALLOC_TAPS,
+                       "ec_filter_coarse_noalloc: quantity %d axis %d has %d taps, more than the %d it holds", m, a,
+                       src.len[a], EC_NOALLOC_TAPS);
+            EC_REQUIRE(ctx, src.off[a] >= -4 * n && src.off[a] <= 4 * n, "filter quantity %d: offset out of range", m);
+        }
+        EC_REQUIRE(ctx, src.out, "filter quantity %d: null output", m);
+        EC_REQUIRE(ctx, src.source >= 0 && src.source <= EC_SRC_K3, "filter quantity %d: bad source", m);
+        EC_REQUIRE(ctx, (src.source < EC_SRC_R11) || p, "pressure source without pressure field");
+        const size_t need = sep_doubles(n, nbar, src.len);
+        EC_REQUIRE(ctx, need <= EC_NOALLOC_DOUBLES,
+                   "ec_filter_coarse_noalloc: n=%d, nbar=%d needs %zu doubles of scratch, more than its %zu "
+                   "(nbar n^2 + nbar^2 n + taps)", n, nbar, need, EC_NOALLOC_DOUBLES);
+    }
+    if (nq == 0) return EC_OK;
+    return dtype == EC_F64 ? noalloc_launch<double>(ctx, n, nbar, l, visc, u, p, nq, q)
+                           : noalloc_launch<float>(ctx, n, nbar, l, visc, u, p, nq, q);
 }
 
 // applyfilter!(out, in, kernelproduct(w1, w2, w3)) as three 1D passes; the

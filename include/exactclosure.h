@@ -198,6 +198,19 @@ typedef struct {
  * nbar n^2 + nbar^2 n doubles. */
 int ec_filter_coarse(ec_ctx *ctx, int dtype, int n, int nbar, double l, double visc,
                      void *const u[3], const void *p, int nq, const ec_filter_quantity *q);
+/* No counterpart in the reference: ec_filter_coarse for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops that
+ * forbid hipMalloc).  It calls no hipMalloc/hipFree and no allocating
+ * library routine: every quantity takes the separable passes in a fixed
+ * per-GPU scratch array of 4 Mi doubles (32 MiB, held until the process
+ * ends), with the weights passed to the device as kernel arguments.  Limits:
+ * at most 64 taps per axis and nbar n^2 + nbar^2 n + (taps) <= 4 Mi doubles
+ * per quantity (e.g. n = 256 with nbar <= 32); larger requests are refused
+ * with EC_E_INVALID before any launch.  Results equal ec_filter_coarse's
+ * separable path bit for bit.  All calls on one GPU share the scratch: calls
+ * that overlap on one GPU (several streams or threads) race. */
+int ec_filter_coarse_noalloc(ec_ctx *ctx, int dtype, int n, int nbar, double l, double visc,
+                             void *const u[3], const void *p, int nq, const ec_filter_quantity *q);
 /* applyfilter!(out, in, kernelproduct(w1, w2, w3)) (src/N.jl:627-653) with
  * centred odd-length 1D kernels (host arrays), as three periodic 1D passes,
  * FP64 sums; in and out are n^3 fields of dtype (not in place). */
