@@ -41,6 +41,9 @@ def parse():
     ap.add_argument("--no-closure-check", action="store_true")
     ap.add_argument("--fp64-steps", type=int, default=3,
                     help="timed steps of the plain FP64 form of the same step after the FP32 leg (0: skip)")
+    ap.add_argument("--reference-setup", action="store_true",
+                    help="run the reference's own comparison (dnsaid_project at n_dns 500, n_les 100, nu 2e-4, "
+                         "Delta = 2H Gaussian, forward Euler, CFL 0.4, warm-up and run to t = 0.1) instead")
     ap.add_argument("--precision-study", action="store_true",
                     help="run the 512^3 FP32-vs-FP64 study (nu = 1/4000, 50 steps, 64^3 and 128^3 closures) instead")
     ap.add_argument("--cpu-n", type=int, default=256)
@@ -222,6 +225,32 @@ def fp64_leg(ec, n, visc, sets, dt, steps):
     return ms
 
 
+def reference_setup_main(a):
+    """getsetup + dnsaid_project + compute_errors (navierstokes.jl:10-38): the
+    five LES errors of the reference's own comparison, FP64, on the GPU."""
+    sys.path.insert(0, ROOT)
+    import torch
+    import __graft_entry__ as ge
+
+    ec = ge.load_package()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = ec.dnsaid_project.reference_setup()
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    out = {"metric": "dnsaid_project (reference setup): LES errors against the filtered DNS",
+           "value": r["errors"]["u_cfd"], "unit": "relative L2 (exact closure)", "n_gpus": 1,
+           "steps": r["steps"], "warmup": 0, "ms_per_step": 1e3 * t / r["steps"], "higher_is_better": False,
+           "scaling": "weak", "vs_baseline": None, "dtype": "fp64",
+           "data": "synthetic: spectrum (k>0) k^(-5/3), E=1, counter-based normal variates (seed 0)",
+           "config": {"workload": "n_dns 500, n_les 100, nu 2e-4, Gaussian Delta = 2H (nsigma 2), forward Euler, "
+                                  "CFL 0.4, warm-up to t = 0.1, then to t = 0.1", **r["setup"]},
+           "errors": r["errors"], "wall_s": t,
+           "reference_pins": {"u_nomo": 0.263051, "u_c": 0.211564, "u_cf": 0.0642457, "u_cfd": 5.68942e-15,
+                              "u_cfd_symm": 0.0285891}}
+    print(json.dumps(out), flush=True)
+
+
 def precision_main(a):
     """The 512^3 FP32 pipeline against the plain FP64 form (velocity,
     divergence and closure errors after 50 steps from the same seed)."""
@@ -247,6 +276,8 @@ def main():
         return burgers_study_main(a)
     if a.precision_study and a.gpus == 1:
         return precision_main(a)
+    if a.reference_setup and a.gpus == 1:
+        return reference_setup_main(a)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); sharding the step over several "
               "GPUs is not part of this build", file=sys.stderr)
