@@ -55,6 +55,10 @@ _SIGS = {
     "ec_last_error": (ctypes.c_char_p, [c_void_p]),
     "ec_sync": (c_int, [c_void_p]),
     "ec_version": (c_int, []),
+    "ec_capture_begin": (c_int, [c_void_p]),
+    "ec_capture_end": (c_int, [c_void_p, ctypes.POINTER(c_void_p)]),
+    "ec_graph_launch": (c_int, [c_void_p, c_void_p]),
+    "ec_graph_destroy": (c_int, [c_void_p, c_void_p]),
     "ec_alloc": (c_int, [c_void_p, c_size_t, ctypes.POINTER(c_void_p)]),
     "ec_free": (c_int, [c_void_p, c_void_p]),
     "ec_h2d": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),

@@ -49,12 +49,68 @@ extern "C" int ec_destroy(ec_ctx *ctx) {
     if (ctx->ws) hipFree(ctx->ws);
     if (ctx->ws2) hipFree(ctx->ws2);
     if (ctx->red_host) hipHostFree(ctx->red_host);
+    if (ctx->capturing) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(ctx->cap_stream, &g);
+        if (g) hipGraphDestroy(g);
+        ctx->stream = ctx->user_stream;
+    }
+    if (ctx->cap_stream) hipStreamDestroy(ctx->cap_stream);
     if (ctx->own_stream) hipStreamDestroy(ctx->stream);
     delete ctx;
     return EC_OK;
 }
 
 extern "C" const char *ec_last_error(ec_ctx *ctx) { return ctx ? ctx->err : "null context"; }
+
+// ---------------------------------------------------------------- graph capture
+// The context's calls between ec_capture_begin and ec_capture_end are recorded
+// on a private stream (the legacy default stream cannot be captured) instead
+// of being run; the instantiated graph then replays them on the context's
+// stream, ordered with its other work.
+extern "C" int ec_capture_begin(ec_ctx *ctx) {
+    EC_REQUIRE(ctx, ctx, "null context");
+    EC_REQUIRE(ctx, !ctx->capturing, "ec_capture_begin: already capturing");
+    EC_REQUIRE(ctx, !ctx->prof, "ec_capture_begin: per-launch profiling is on");
+    if (!ctx->cap_stream) EC_HIP(ctx, hipStreamCreateWithFlags(&ctx->cap_stream, hipStreamNonBlocking));
+    // the captured work reads what the context's stream has queued so far
+    EC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    EC_HIP(ctx, hipStreamBeginCapture(ctx->cap_stream, hipStreamCaptureModeRelaxed));
+    ctx->user_stream = ctx->stream;
+    ctx->stream = ctx->cap_stream;
+    ctx->capturing = true;
+    return EC_OK;
+}
+
+extern "C" int ec_capture_end(ec_ctx *ctx, void **graph) {
+    EC_REQUIRE(ctx, ctx && graph, "ec_capture_end: null argument");
+    EC_REQUIRE(ctx, ctx->capturing, "ec_capture_end: not capturing");
+    *graph = nullptr;
+    ctx->stream = ctx->user_stream;
+    ctx->capturing = false;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->cap_stream, &g);
+    if (e != hipSuccess || !g) return ec_fail(ctx, EC_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    hipGraphExec_t ge = nullptr;
+    e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) return ec_fail(ctx, EC_E_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+    *graph = ge;
+    return EC_OK;
+}
+
+extern "C" int ec_graph_launch(ec_ctx *ctx, void *graph) {
+    EC_REQUIRE(ctx, ctx && graph, "ec_graph_launch: null argument");
+    EC_REQUIRE(ctx, !ctx->capturing, "ec_graph_launch: the context is capturing");
+    EC_HIP(ctx, hipGraphLaunch((hipGraphExec_t)graph, ctx->stream));
+    return EC_OK;
+}
+
+extern "C" int ec_graph_destroy(ec_ctx *ctx, void *graph) {
+    EC_REQUIRE(ctx, ctx, "null context");
+    if (graph) EC_HIP(ctx, hipGraphExecDestroy((hipGraphExec_t)graph));
+    return EC_OK;
+}
 
 extern "C" int ec_sync(ec_ctx *ctx) {
     EC_REQUIRE(ctx, ctx, "null context");

@@ -42,6 +42,11 @@ struct ec_ctx {
     size_t ws_doubles = 0;
     void *ws2 = nullptr;
     size_t ws2_doubles = 0;
+    // Graph capture (ec_capture_begin/end): while capturing, `stream` is the
+    // private capture stream and `user_stream` the context's own.
+    hipStream_t cap_stream = nullptr;
+    hipStream_t user_stream = nullptr;
+    bool capturing = false;
 };
 
 // ---- error handling: no C++ exception crosses the C ABI ----

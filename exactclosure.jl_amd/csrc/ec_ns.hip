@@ -336,16 +336,16 @@ __global__ void __launch_bounds__(256) k_rhs_march(int n, int ZC, T ih, T visc, 
 // du1 / du2 go through a double-buffered LDS exchange, one barrier per plane.
 // With rdh = r / h, d = div_h du + r div_h u: the divergence of the stage
 // state u is folded into the Poisson right-hand side (see ec_ns_rhs_div).
-template <typename T, int TX, int TY>
-__global__ void __launch_bounds__(256) k_rhs_div_march(int n, int ZC, T ih, T visc, const T *__restrict__ u1,
+template <typename T, int TX, int TY, int NT = 256>
+__global__ void __launch_bounds__(NT) k_rhs_div_march(int n, int ZC, T ih, T visc, const T *__restrict__ u1,
                                                        const T *__restrict__ u2, const T *__restrict__ u3,
                                                        T *__restrict__ du1, T *__restrict__ du2, T *__restrict__ du3,
                                                        T *__restrict__ dout, T rdh) {
     constexpr int HX = TX + 3, HY = TY + 3, HP = HX * HY;
-    constexpr int NLC = (HP + 255) / 256, NLD = 3 * NLC;  // loads per component and plane, all components
-    constexpr int CPT = TX * TY / 256;
+    constexpr int NLC = (HP + NT - 1) / NT, NLD = 3 * NLC;  // loads per component and plane, all components
+    constexpr int CPT = TX * TY / NT;
     constexpr int D1N = TY * (TX + 1), D2N = (TY + 1) * TX;
-    static_assert(TX * TY % 256 == 0 && 256 % TX == 0 && TX + TY <= 256, "tile shape");
+    static_assert(TX * TY % NT == 0 && NT % TX == 0 && NT % 64 == 0 && NT >= 256, "tile shape");
     __shared__ T ring[4][3 * HP];
     __shared__ T dx1[2][D1N], dx2[2][D2N];  // du1 at x in [x0-1, x0+TX), du2 at y in [y0-1, y0+TY)
     const int gx = n / TX, gy = n / TY;
@@ -364,7 +364,7 @@ __global__ void __launch_bounds__(256) k_rhs_div_march(int n, int ZC, T ih, T vi
     unsigned loff[NLC];
 #pragma unroll
     for (int jj = 0; jj < NLC; jj++) {
-        const int q = min(tid + 256 * jj, HP - 1), hx = q % HX, hy = q / HX;
+        const int q = min(tid + NT * jj, HP - 1), hx = q % HX, hy = q / HX;
         loff[jj] = ((unsigned)ec_wrap_any(x0 + hx - 2, n) + nn * (unsigned)ec_wrap_any(y0 + hy - 2, n)) *
                    (unsigned)sizeof(T);
     }
@@ -383,7 +383,7 @@ __global__ void __launch_bounds__(256) k_rhs_div_march(int n, int ZC, T ih, T vi
         for (int c = 0; c < 3; c++)
 #pragma unroll
             for (int jj = 0; jj < NLC; jj++)
-                if (HP % 256 == 0 || tid + 256 * jj < HP) ring[slot][c * HP + tid + 256 * jj] = r[c * NLC + jj];
+                if (HP % NT == 0 || tid + NT * jj < HP) ring[slot][c * HP + tid + NT * jj] = r[c * NLC + jj];
     };
     {
         T r[NLD];
@@ -411,7 +411,7 @@ __global__ void __launch_bounds__(256) k_rhs_div_march(int n, int ZC, T ih, T vi
     // (loop invariant); the plane base is uniform
     unsigned soff[CPT];
 #pragma unroll
-    for (int m = 0; m < CPT; m++) soff[m] = (unsigned)(cx + (int)nn * (cy0 + m * (256 / TX))) * (unsigned)sizeof(T);
+    for (int m = 0; m < CPT; m++) soff[m] = (unsigned)(cx + (int)nn * (cy0 + m * (NT / TX))) * (unsigned)sizeof(T);
     // plane k (ring index q = rr + 2): plane k + 2 arrives in `cur` (loaded one
     // plane earlier) and goes to the ring; plane k + 3 is loaded into `ahead`,
     // so two planes of loads are in flight while a plane is evaluated
@@ -424,14 +424,14 @@ __global__ void __launch_bounds__(256) k_rhs_div_march(int n, int ZC, T ih, T vi
         T a1[CPT], a2[CPT], a3[CPT], dv[CPT];
 #pragma unroll
         for (int m = 0; m < CPT; m++) {
-            const int cy = cy0 + m * (256 / TX);
+            const int cy = cy0 + m * (NT / TX);
             rhs_cell(pm, p0, pp, (cy + 2) * HX + (cx + 2), HX, HP, ih, visc, a1[m], a2[m], a3[m], dv[m]);
         }
         const int buf = (rr + 2) & 1;
         if (rr >= 0) {
 #pragma unroll
             for (int m = 0; m < CPT; m++) {
-                const int cy = cy0 + m * (256 / TX);
+                const int cy = cy0 + m * (NT / TX);
                 // streaming stores: the outputs are not re-read here, and keeping
                 // them out of L2 leaves it to the halo lines the neighbouring tiles share
                 ec_nt_store_at(r1, soff[m], a1[m]);
@@ -457,7 +457,7 @@ __global__ void __launch_bounds__(256) k_rhs_div_march(int n, int ZC, T ih, T vi
         if (rr >= 0) {
 #pragma unroll
             for (int m = 0; m < CPT; m++) {
-                const int cy = cy0 + m * (256 / TX);
+                const int cy = cy0 + m * (NT / TX);
                 ec_nt_store_at(rd, soff[m],
                                (a1[m] - dx1[buf][cy * (TX + 1) + cx]) * ih + (a2[m] - dx2[buf][cy * TX + cx]) * ih +
                                    (a3[m] - a3prev[m]) * ih + rdh * dv[m]);
@@ -716,7 +716,7 @@ extern "C" int ec_ns_rhs_div(ec_ctx *ctx, int dtype, int n, double l, double vis
             n, ZC, ih, visc, (const double *)u[0], (const double *)u[1], (const double *)u[2], (double *)du[0],
             (double *)du[1], (double *)du[2], (double *)d, rdiv * ih);
     else
-        k_rhs_div_march<float, 64, 16><<<tiles, 256, 0, ctx->stream>>>(
+        k_rhs_div_march<float, 64, 16, 512><<<tiles, 512, 0, ctx->stream>>>(
             n, ZC, (float)ih, (float)visc, (const float *)u[0], (const float *)u[1], (const float *)u[2],
             (float *)du[0], (float *)du[1], (float *)du[2], (float *)d, (float)(rdiv * ih));
     EC_CHECK_LAUNCH(ctx, "k_rhs_div_march");

@@ -92,6 +92,7 @@ class DNSAidedLES:
         # objects with .stage(st, dt), advanced after every stage of the forcing
         # form once the stage's exact closure is known (a posteriori model LES)
         self.stage_hooks = []
+        self._graphs = {}  # dt -> captured step (ec_capture_end)
 
     # -- state -------------------------------------------------------------
     def set_state(self, u):
@@ -160,6 +161,41 @@ class DNSAidedLES:
 
     # -- one RK4 step --------------------------------------------------------
     def step(self, dt):
+        g = self._graphs.get(dt) if self._graphs else None
+        if g is not None:
+            self.ctx.call("ec_graph_launch", g)
+            return
+        self._step_eager(dt)
+
+    def capture(self, dt):
+        """Record one RK4 step of time step dt into a HIP graph (ec_capture_*,
+        all of the step's launches, the LES grids included); step(dt) then
+        replays it with one launch.  The graph holds this driver's buffers and
+        dt; a stage hook (host work between stages) cannot be captured."""
+        assert not self.stage_hooks, "stage hooks run host code between the stages"
+        if dt in self._graphs:
+            return
+        self.ctx.call("ec_capture_begin")
+        g = _abi.c_void_p()
+        try:
+            self._step_eager(dt)
+        finally:
+            rc = self.ctx.lib.ec_capture_end(self.ctx.h, _abi.ctypes.byref(g))
+        self.ctx.check(rc, "ec_capture_end")
+        self._graphs[dt] = g
+
+    def release_graphs(self):
+        for g in self._graphs.values():
+            self.ctx.lib.ec_graph_destroy(self.ctx.h, g)
+        self._graphs = {}
+
+    def __del__(self):
+        try:
+            self.release_graphs()
+        except Exception:
+            pass
+
+    def _step_eager(self, dt):
         if self.closure == "forcing":
             return self._step_forcing(dt)
         c = self.ctx
@@ -186,13 +222,28 @@ class DNSAidedLES:
                    ptrs(self.du, P3), _abi.c_void_p(self.ps.p_ptr), ptrs(self.acc, P3))
 
     def _les_forcing(self, L, st, c=None):
-        """m = V k - F_H(P_H v) in FP64 for the stage state of LES grid L."""
+        """m = V k - F_H(P_H v) in FP64 for the stage state of LES grid L.
+
+        The face average maps a discretely divergence-free field to a coarse
+        divergence-free one (its coarse fluxes are face sums, so div_H V u is
+        the cell average of div_h u), and k is the projected DNS tendency: for
+        a face-average grid v is projected once per step, at stage 0 (in
+        place), and the later stage states v_s = v0 + a_s dt V k stay
+        projected up to the rounding of k's divergence.  The volume average
+        has no such property (P_H V u differs from V u at O(1)), so its v is
+        projected at every stage, as apply_DHpi does."""
         c = c or self.ctx
         gl = L.g
         v = L.v0 if st == 0 else L.vs
-        c.call("ec_ns_project_into", L.ps64.h, ptrs(v, P3), ptrs(L.ubar, P3))
+        if L.kind == "fa":
+            if st == 0:
+                c.call("ec_ns_project", L.ps64.h, ptrs(L.v0, P3))
+            ub = v
+        else:
+            c.call("ec_ns_project_into", L.ps64.h, ptrs(v, P3), ptrs(L.ubar, P3))
+            ub = L.ubar
         # m = V k - F_H(P_H v) = -F_H(ubar) + g_k, formed in the RHS pass
-        c.call("ec_ns_rhs_add", _abi.EC_F64, gl.n, gl.l, self.visc, ptrs(L.ubar, P3), ptrs(L.m, P3), -1.0,
+        c.call("ec_ns_rhs_add", _abi.EC_F64, gl.n, gl.l, self.visc, ptrs(ub, P3), ptrs(L.m, P3), -1.0,
                ptrs(L.g_k, P3))
         return L.m
 

@@ -55,6 +55,19 @@ const char *ec_last_error(ec_ctx *ctx);
 int ec_sync(ec_ctx *ctx);
 int ec_version(void);
 
+/* ---- graph capture (no reference counterpart; the reference launches every
+ * kernel of a step from Julia) ----
+ * Between ec_capture_begin and ec_capture_end the context's asynchronous calls
+ * are recorded, not run, into one HIP graph (e.g. a whole DNS-aided RK4 step,
+ * the ec_ns_dnsaid_step of the boundary); ec_graph_launch replays it on the
+ * context's stream.  Kernel arguments (sizes, dt, device pointers) are fixed
+ * at capture.  Synchronising calls (reductions, ec_d2h, spectrum) must not be
+ * captured; per-launch profiling must be off. */
+int ec_capture_begin(ec_ctx *ctx);
+int ec_capture_end(ec_ctx *ctx, void **graph);
+int ec_graph_launch(ec_ctx *ctx, void *graph);
+int ec_graph_destroy(ec_ctx *ctx, void *graph);
+
 /* ---- memory (replaces KernelAbstractions.zeros/allocate and Adapt moves) ---- */
 int ec_alloc(ec_ctx *ctx, size_t bytes, void **ptr);
 int ec_free(ec_ctx *ctx, void *ptr);

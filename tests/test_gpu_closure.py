@@ -269,3 +269,38 @@ def test_filter_vel_rk4_fused_large(gpu, n, stage):
             assert float((x - y).abs().max()) <= 1e-6 * float(x.abs().max())
     del du, p, state_a, state_b, oa, ob
     torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("n,sets,prec", [(256, [(16, "va"), (32, "va"), (64, "va"), (64, "fa")], "f32"),
+                                         (32, [(8, "va"), (4, "fa")], "f64")])
+def test_captured_step_replays_the_eager_step(gpu, O, n, sets, prec):
+    """A whole DNS-aided RK4 step (DNS kernels, every LES grid's forcing,
+    projection and update) captured into one HIP graph and replayed four
+    times, with eager work and synchronisation between the replays (a
+    reduction, an eager step of another driver), equals the eager step bit
+    for bit after every replay, for the DNS and every LES state."""
+    NS, D = gpu.navierstokes, gpu.dnsaid
+    dt_t = torch.float32 if prec == "f32" else torch.float64
+    g = NS.Grid(1.0, n)
+    u = O.randomfield(n, seed=n + 3) if n <= 64 else None
+    if u is None:
+        ps = NS.PoissonSolver(g, torch.float64)
+        u = [x.data for x in NS.randomfield(g, ps, "peak", 5.0, 0.5, seed=9)]
+    else:
+        u = [dev(x) for x in u]
+    eager = D.DNSAidedLES(g, 1.0 / 2000, sets, dtype=dt_t)
+    graph = D.DNSAidedLES(g, 1.0 / 2000, sets, dtype=dt_t)
+    eager.set_state([x.to(dt_t) for x in u])
+    graph.set_state([x.to(dt_t) for x in u])
+    dt = 2e-3 if n <= 64 else 5e-4
+    graph.capture(dt)
+    for _ in range(4):
+        eager.step(dt)
+        graph.step(dt)
+        graph.divergence_norm()  # eager, synchronising work between the replays
+        torch.cuda.synchronize()
+        for a in range(3):
+            assert torch.equal(eager.u0[a], graph.u0[a])
+            for Le, Lg in zip(eager.les, graph.les):
+                assert torch.equal(Le.u0[a], Lg.u0[a])
+    graph.release_graphs()

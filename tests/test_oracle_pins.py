@@ -563,3 +563,19 @@ def test_ns_rk4_fourth_order_and_mutants(O, monkeypatch):
     monkeypatch.setattr(O, "RK4_A", (0.5, 0.5, 1.0))
     monkeypatch.setattr(O, "RK4_B", (1.0, 2.0, 1.0, 2.0))
     assert _ns_order(O, step, (4, 8, 16, 32))[-1] < 1.5
+
+
+@pytest.mark.parametrize("c", [2, 3, 4])
+def test_face_average_keeps_divergence_free(O, c):
+    """The face average of a discretely divergence-free field is coarse
+    divergence-free (its coarse fluxes are face sums, so div_H V u is the cell
+    average of div_h u); the volume average is not (the projection of
+    apply_DHpi, src/N.jl:1228-1237, does real work there)."""
+    n = 8 * c
+    u = O.randomfield(n, seed=c)
+    div = {}
+    for kind in ("fa", "va"):
+        F = O.closure_filters(c, kind)
+        v = [O.filter_coarse(O._f(u[a]), n // c, *F[f"ubar{a + 1}"][1]) for a in range(3)]
+        div[kind] = np.abs(O.divergence(v)).max() / (max(np.abs(x).max() for x in v) * n // c)
+    assert div["fa"] < 1e-14 and div["va"] > 1e-3, div
