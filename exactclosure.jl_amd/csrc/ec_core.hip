@@ -48,6 +48,7 @@ extern "C" int ec_destroy(ec_ctx *ctx) {
     if (ctx->red_partial) hipFree(ctx->red_partial);
     if (ctx->ws) hipFree(ctx->ws);
     if (ctx->ws2) hipFree(ctx->ws2);
+    if (ctx->ws3) hipFree(ctx->ws3);
     if (ctx->red_host) hipHostFree(ctx->red_host);
     if (ctx->capturing) {
         hipGraph_t g = nullptr;

@@ -67,8 +67,40 @@ __device__ __forceinline__ double psum(const double *p, int stride, int f) {
     }
 }
 
+// The split face piece of a volume-average set over the c cells of coarse
+// cell J (p[0] its first fine cell, stride between cells): u = the part of
+// face J's window (around the cell's last face) inside the cell, l = the part
+// of face J-1's window (around the face below the cell) inside it, both / C.
+// u + l is the box of the cell.  Odd C: windows of C faces; even C: C + 1
+// faces with the two end faces at half weight, the middle face of the cell
+// shared half and half.
+template <int C>
+__device__ __forceinline__ void half_sums(const double *p, int stride, double &u, double &l) {
+    if constexpr (C & 1) {
+        constexpr int H = (C - 1) / 2;
+        double a = 0.0, b = 0.0;
+#pragma unroll
+        for (int t = 0; t < H; t++) b += p[t * stride];
+#pragma unroll
+        for (int t = H; t < C; t++) a += p[t * stride];
+        u = a * (1.0 / C);
+        l = b * (1.0 / C);
+    } else {
+        constexpr int M = C / 2 - 1;  // the shared middle face
+        double a = 0.0, b = 0.0;
+#pragma unroll
+        for (int t = 0; t < M; t++) b += p[t * stride];
+#pragma unroll
+        for (int t = M + 1; t < C; t++) a += p[t * stride];
+        const double m = 0.5 * p[M * stride];
+        u = (a + m) * (1.0 / C);
+        l = (b + m) * (1.0 / C);
+    }
+}
+
 struct FSet {
     double *out[3];
+    double *side[2];  // volume average: the lower-half partials of components 0 and 1 (below)
     int nb;
 };
 
@@ -83,19 +115,26 @@ struct Cfg {
     static constexpr int TX = TX_, TY = TY_, NT = NT_;
     static constexpr int c(int s) { return s == 0 ? C0 : s == 1 ? C1 : s == 2 ? C2 : C3; }
     static constexpr int kind(int s) { return s == 0 ? K0 : s == 1 ? K1 : s == 2 ? K2 : K3; }
-    // columns needed past the tile along a for component a (volume: face piece)
-    static constexpr int h(int s) { return c(s) && kind(s) == 0 ? c(s) / 2 : 0; }
-    static constexpr int hmax = std::max(std::max(h(0), h(1)), std::max(h(2), h(3)));
+    // planes needed past the chunk along z for component 2 (volume: face piece)
+    static constexpr int hz(int s) { return c(s) && kind(s) == 0 ? c(s) / 2 : 0; }
+    static constexpr int hmax = std::max(std::max(hz(0), hz(1)), std::max(hz(2), hz(3)));
+    // columns past the tile along x / y: none.  The face pieces of components 0
+    // and 1 of a volume-average set are split at the coarse-cell boundary
+    // instead: each tile sums, per coarse cell J, the part of face J's window
+    // inside cell J (U) and the part of face J-1's window inside cell J (L);
+    // face J is U_J + L_{J+1}, added by a coarse-grid pass after the launch.
+    static constexpr int h(int) { return 0; }
+    static constexpr bool split(int s) { return c(s) && kind(s) == 0; }
     static constexpr int lcm() {
         int l = 1;
         for (int s = 0; s < 4; s++)
             if (c(s)) l = std::lcm(l, c(s));
         return l;
     }
-    // columns of the tile: component 0 (TX + hmax) x TY, 1 TX x (TY + hmax),
-    // 2 TX x TY; each region padded to whole waves so a wave holds one component
+    // columns of the tile: TX x TY per component; each region padded to whole
+    // waves so a wave holds one component
     static constexpr int up64(int x) { return (x + 63) / 64 * 64; }
-    static constexpr int N0 = (TX + hmax) * TY, N1 = TX * (TY + hmax), N2 = TX * TY;
+    static constexpr int N0 = TX * TY, N1 = TX * TY, N2 = TX * TY;
     static constexpr int P0 = up64(N0), P1 = up64(N1), P2 = up64(N2);
     static constexpr int NCC = P0 + P1 + P2;
     static constexpr int NJ = (NCC + NT - 1) / NT;
@@ -111,14 +150,16 @@ struct Cfg {
             }
         return o;
     }
-    // x-partials per (set, component): TX/c columns of XR rows
-    static constexpr int XR = (TY + hmax) | 1;
+    // x-partials per (set, component): TX/c columns (two per coarse cell, U
+    // and L, for the split component 0) of XR rows
+    static constexpr int XR = TY | 1;
+    static constexpr int xcols(int s, int a) { return (TX / c(s)) * (split(s) && a == 0 ? 2 : 1); }
     static constexpr int xoff(int s, int a) {
         int o = zoff(4, 0);
         for (int t = 0; t < 4; t++)
             for (int b = 0; b < 3; b++) {
                 if (t == s && b == a) return o;
-                if (c(t)) o += (TX / c(t)) * XR;
+                if (c(t)) o += xcols(t, b) * XR;
             }
         return o;
     }
@@ -200,13 +241,20 @@ __device__ __forceinline__ void zupdate(int r, int ZC, int a, bool ok, int lx, i
 template <class G, int S, int A>
 __device__ __forceinline__ void xstage(double *lds, int tid) {
     constexpr int C = G::c(S), K = G::kind(S);
-    constexpr int rows = A == 1 ? G::TY + G::h(S) : G::TY, nix = G::TX / C;
+    constexpr int rows = G::TY, nix = G::TX / C;
     constexpr int PX = A == 0 ? (K == 0 ? 1 : 2) : 0;
     const int t = (tid + 64 * (S * 3 + A)) % G::NT;  // spread the stages over the waves
     for (int it = t; it < nix * rows; it += G::NT) {
         const int Ix = it / rows, row = it - Ix * rows;
         const double *zr = lds + G::zoff(S, A) + row * G::zw(S, A);
-        lds[G::xoff(S, A) + Ix * G::XR + row] = psum<C, PX>(zr, 1, C * (Ix + 1) - 1);
+        if constexpr (A == 0 && G::split(S)) {
+            double u, l;
+            half_sums<C>(zr + C * Ix, 1, u, l);
+            lds[G::xoff(S, A) + (2 * Ix) * G::XR + row] = u;
+            lds[G::xoff(S, A) + (2 * Ix + 1) * G::XR + row] = l;
+        } else {
+            lds[G::xoff(S, A) + Ix * G::XR + row] = psum<C, PX>(zr, 1, C * (Ix + 1) - 1);
+        }
     }
 }
 
@@ -220,9 +268,21 @@ __device__ __forceinline__ void ystage(const double *lds, int tid, int x0, int y
     const int t = (tid + 64 * (S * 3 + A)) % G::NT;
     for (int it = t; it < nix * niy; it += G::NT) {
         const int Iy = it / nix, Ix = it - Iy * nix;
-        const double v = psum<C, PY>(lds + G::xoff(S, A) + Ix * G::XR, 1, C * (Iy + 1) - 1);
         const size_t nb = (size_t)fs.nb;
-        fs.out[A][(size_t)(x0 / C + Ix) + nb * ((size_t)(y0 / C + Iy) + nb * (size_t)Kc)] = v;
+        const size_t J = (size_t)(x0 / C + Ix) + nb * ((size_t)(y0 / C + Iy) + nb * (size_t)Kc);
+        if constexpr (A == 0 && G::split(S)) {
+            // U and L of the x split, each boxed along y
+            fs.out[0][J] = psum<C, 0>(lds + G::xoff(S, 0) + (2 * Ix) * G::XR, 1, C * (Iy + 1) - 1);
+            fs.side[0][J] = psum<C, 0>(lds + G::xoff(S, 0) + (2 * Ix + 1) * G::XR, 1, C * (Iy + 1) - 1);
+        } else if constexpr (A == 1 && G::split(S)) {
+            // the face piece along y split at the coarse-cell boundary
+            double u, l;
+            half_sums<C>(lds + G::xoff(S, 1) + Ix * G::XR + C * Iy, 1, u, l);
+            fs.out[1][J] = u;
+            fs.side[1][J] = l;
+        } else {
+            fs.out[A][J] = psum<C, PY>(lds + G::xoff(S, A) + Ix * G::XR, 1, C * (Iy + 1) - 1);
+        }
     }
 }
 
@@ -450,7 +510,7 @@ __device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__re
         int e = tid + NT * j, lx = 0, ly = 0;
         bool valid = false;
         if (a == 0) {
-            valid = e < G::N0, lx = e % (TX + H), ly = e / (TX + H);
+            valid = e < G::N0, lx = e % TX, ly = e / TX;
         } else if (a == 1) {
             e -= G::P0;
             valid = e < G::N1, lx = e % TX, ly = e / TX;
@@ -498,6 +558,22 @@ __device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__re
         }
     };
     load(0, cv, cp, cqv);
+    // fused RK4 stage: the stage state (u0, and acc after stage 0) of plane r is
+    // loaded one plane ahead, with the filter's loads of that plane
+    T cu[NJ], ca[NJ];
+    auto rk_prefetch = [&](int r, T *uu, T *aa) {
+#pragma unroll
+        for (int j = 0; j < NJ; j++)
+            if (comp(j) != 3 && r < ZC && ok[j]) rk_load<T, RK < 0 ? 0 : RK>(comp(j), co[j] + (unsigned)(z0 + r) * n2, rk, uu[j], aa[j]);
+    };
+    auto rk_rotate = [&](const T *uu, const T *aa) {
+#pragma unroll
+        for (int j = 0; j < NJ; j++) {
+            cu[j] = uu[j];
+            ca[j] = aa[j];
+        }
+    };
+    if constexpr (RK >= 0) rk_prefetch(0, cu, ca);
     const int rend = ZC - 1 + H;
     // k = v - delta p in the field precision (one rounding at its scale); the
     // z-sums and everything after them are FP64
@@ -540,14 +616,11 @@ __device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__re
             auto plane = [&](auto qc) {
                 constexpr int Q = decltype(qc)::value;
                 const int r = r0 + Q;
+                // plane r + 1 (its filter loads and stage state) in flight through this plane
                 T nv[NJ], np[NJ], nq[NJ];
                 load(r + 1, nv, np, nq);
-                T ru[NJ], ra[NJ];
-                if constexpr (RK >= 0)
-#pragma unroll
-                    for (int j = 0; j < NJ; j++)
-                        if (comp(j) != 3 && r < ZC && ok[j] && clx[j] < TX && cly[j] < TY)
-                            rk_load<T, RK>(comp(j), co[j] + (unsigned)(z0 + r) * n2, rk, ru[j], ra[j]);
+                T nu[NJ], na[NJ];
+                if constexpr (RK >= 0) rk_prefetch(r + 1, nu, na);
 #pragma unroll
                 for (int j = 0; j < NJ; j++) {
                     const int a = comp(j);
@@ -555,14 +628,15 @@ __device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__re
                     const T kv = cv[j] - (cqv[j] - cp[j]) * ihT;
                     const double src = (double)kv;
                     if constexpr (RK >= 0)
-                        if (r < ZC && ok[j] && clx[j] < TX && cly[j] < TY)
-                            rk_store<T, RK>(a, co[j] + (unsigned)(z0 + r) * n2, kv, ru[j], ra[j], rk);
+                        if (r < ZC && ok[j])
+                            rk_store<T, RK>(a, co[j] + (unsigned)(z0 + r) * n2, kv, cu[j], ca[j], rk);
                     if (a == 0) nested_box<G, 0, Q>(r, ZC, ok[j], clx[j], cly[j], src, nst[j], lds);
                     else if (a == 1) nested_box<G, 1, Q>(r, ZC, ok[j], clx[j], cly[j], src, nst[j], lds);
                     else nested_face<G, Q>(r, ZC, ok[j], clx[j], cly[j], src, nst[j], lds);
                 }
                 if constexpr (Q == 3) emit(r);
                 rotate(nv, np, nq);
+                if constexpr (RK >= 0) rk_rotate(nu, na);
             };
             plane(std::integral_constant<int, 0>{});
             plane(std::integral_constant<int, 1>{});
@@ -573,12 +647,8 @@ __device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__re
         for (int r = 0; r <= rend; ++r) {
             T nv[NJ], np[NJ], nq[NJ];
             load(r + 1, nv, np, nq);
-            T ru[NJ], ra[NJ];
-            if constexpr (RK >= 0)
-#pragma unroll
-                for (int j = 0; j < NJ; j++)
-                    if (comp(j) != 3 && r < ZC && ok[j] && clx[j] < TX && cly[j] < TY)
-                        rk_load<T, RK>(comp(j), co[j] + (unsigned)(z0 + r) * n2, rk, ru[j], ra[j]);
+            T nu[NJ], na[NJ];
+            if constexpr (RK >= 0) rk_prefetch(r + 1, nu, na);
 #pragma unroll
             for (int j = 0; j < NJ; j++) {
                 const int a = comp(j);
@@ -586,8 +656,8 @@ __device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__re
                 const T kv = cv[j] - (cqv[j] - cp[j]) * ihT;
                 const double src = (double)kv;
                 if constexpr (RK >= 0)
-                    if (r < ZC && ok[j] && clx[j] < TX && cly[j] < TY)
-                        rk_store<T, RK>(a, co[j] + (unsigned)(z0 + r) * n2, kv, ru[j], ra[j], rk);
+                    if (r < ZC && ok[j])
+                        rk_store<T, RK>(a, co[j] + (unsigned)(z0 + r) * n2, kv, cu[j], ca[j], rk);
                 if constexpr (G::c(0) > 0) zupdate<G, 0>(r, ZC, a, ok[j], clx[j], cly[j], src, acc[j][0], lds);
                 if constexpr (G::c(1) > 0) zupdate<G, 1>(r, ZC, a, ok[j], clx[j], cly[j], src, acc[j][1], lds);
                 if constexpr (G::c(2) > 0) zupdate<G, 2>(r, ZC, a, ok[j], clx[j], cly[j], src, acc[j][2], lds);
@@ -595,6 +665,7 @@ __device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__re
             }
             emit(r);
             rotate(nv, np, nq);
+            if constexpr (RK >= 0) rk_rotate(nu, na);
         }
     }
 }
@@ -648,6 +719,20 @@ __global__ void __launch_bounds__(G::NT) k_filter_vel(const T *__restrict__ v1, 
     fv_dispatch<T, G, RK, 0>(code, v1, v2, v3, p, tl, lds, fsl, tid, R);
 }
 
+// Face J of a split component = its U partial (already in out) + the L
+// partial of the next coarse cell along the component's axis (periodic).
+__global__ void k_face_combine(int nb, double *__restrict__ out0, const double *__restrict__ side0,
+                               double *__restrict__ out1, const double *__restrict__ side1) {
+    const size_t n1 = (size_t)nb, total = n1 * n1 * n1;
+    const size_t J = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (J >= total) return;
+    const int i = (int)(J % n1), j = (int)((J / n1) % n1);
+    const size_t k = J / (n1 * n1);
+    const int ip = i + 1 == nb ? 0 : i + 1, jp = j + 1 == nb ? 0 : j + 1;
+    out0[J] += side0[(size_t)ip + n1 * ((size_t)j + n1 * k)];
+    out1[J] += side1[(size_t)i + n1 * ((size_t)jp + n1 * k)];
+}
+
 int largest_tile(int n, int L, int target) {
     int best = 0;
     for (int t = L; t <= n; t += L)
@@ -665,8 +750,9 @@ int launch_fv(ec_ctx *ctx, int n, double l, void *const v[3], const void *p, con
     static_assert(TX % L == 0 && TY % L == 0, "tile edges must be multiples of every compression");
     if (n % TX || n % TY || n % L) return EC_E_UNSUPPORTED;
     if ((double)n * n * n >= 4294967296.0) return EC_E_UNSUPPORTED;  // 32-bit element offsets
-    // every index read (up to n + hmax + 1 along each axis) lies within one period
-    if (G::hmax + 1 >= n) return EC_E_UNSUPPORTED;
+    // every index read (planes up to z0 + ZC + hmax + 1, columns up to x0 + TX,
+    // y0 + TY) lies within one period
+    if (G::hmax + 2 >= n) return EC_E_UNSUPPORTED;
     const int ZC = largest_tile(n, L, tz_target);
     if (!ZC) return EC_E_UNSUPPORTED;
     const size_t lds = sizeof(double) * (size_t)G::lds_doubles + sizeof(FSet) * 4;
@@ -681,9 +767,42 @@ int launch_fv(ec_ctx *ctx, int n, double l, void *const v[3], const void *p, con
                                     (int)lds));
     RkArgs R{};
     if (rk) R = *rk;
+    // the L partials of the split volume-average sets, in a context workspace
+    FArgs AA = A;
+    size_t need = 0;
+    for (int s = 0; s < 4; s++)
+        if (G::split(s)) need += 2 * ec_cube(AA.s[s].nb);
+    if (need > ctx->ws3_doubles) {
+        if (ctx->ws3) {
+            EC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            EC_HIP(ctx, hipFree(ctx->ws3));
+            ctx->ws3 = nullptr;
+            ctx->ws3_doubles = 0;
+        }
+        if (hipMalloc(&ctx->ws3, sizeof(double) * need) != hipSuccess)
+            return ec_fail(ctx, EC_E_OOM, "velocity filter: %zu doubles of partials", need);
+        ctx->ws3_doubles = need;
+    }
+    double *side = (double *)ctx->ws3;
+    for (int s = 0; s < 4; s++) {
+        AA.s[s].side[0] = AA.s[s].side[1] = nullptr;
+        if (G::split(s)) {
+            const size_t m = ec_cube(AA.s[s].nb);
+            AA.s[s].side[0] = side;
+            AA.s[s].side[1] = side + m;
+            side += 2 * m;
+        }
+    }
     k_filter_vel<T, G, RK><<<dim3((unsigned)tiles), G::NT, lds, ctx->stream>>>(
-        (const T *)v[0], (const T *)v[1], (const T *)v[2], (const T *)p, n, ZC, n / l, A, R);
+        (const T *)v[0], (const T *)v[1], (const T *)v[2], (const T *)p, n, ZC, n / l, AA, R);
     EC_CHECK_LAUNCH(ctx, "k_filter_vel");
+    for (int s = 0; s < 4; s++) {
+        if (!G::split(s)) continue;
+        const size_t m = ec_cube(AA.s[s].nb);
+        k_face_combine<<<(unsigned)((m + 255) / 256), 256, 0, ctx->stream>>>(AA.s[s].nb, AA.s[s].out[0], AA.s[s].side[0],
+                                                                              AA.s[s].out[1], AA.s[s].side[1]);
+        EC_CHECK_LAUNCH(ctx, "k_face_combine");
+    }
     return EC_OK;
 }
 

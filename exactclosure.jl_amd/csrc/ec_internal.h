@@ -42,6 +42,9 @@ struct ec_ctx {
     size_t ws_doubles = 0;
     void *ws2 = nullptr;
     size_t ws2_doubles = 0;
+    // The split-window partials of the velocity filter (FP64, grown on demand).
+    void *ws3 = nullptr;
+    size_t ws3_doubles = 0;
     // Graph capture (ec_capture_begin/end): while capturing, `stream` is the
     // private capture stream and `user_stream` the context's own.
     hipStream_t cap_stream = nullptr;
