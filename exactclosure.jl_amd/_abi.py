@@ -120,6 +120,10 @@ _SIGS = {
     "ec_burgers_batch_set_threads": (c_int, [c_void_p, c_void_p, c_int]),
     "ec_burgers_batch_run": (c_int, [c_void_p, c_void_p, c_int, c_int]),
     "ec_burgers_batch_get": (c_int, [c_void_p, c_void_p, c_int, c_int, ctypes.POINTER(c_double), c_int]),
+    "ec_burgers_randomfield": (c_int, [c_void_p, c_int, c_double, c_double, c_double, c_int, c_u64, c_void_p]),
+    "ec_burgers_dns_aided_les": (c_int, [c_void_p, c_int, c_int, c_double, c_double, c_double, c_double, c_int,
+                                         c_void_p, c_int, ctypes.POINTER(c_double), c_int, ctypes.POINTER(c_double),
+                                         c_void_p, ctypes.POINTER(c_int)]),
     "ec_burgers_batch_dt": (c_int, [c_void_p, c_void_p, c_int, ctypes.POINTER(c_double)]),
     "ec_burgers_rhs": (c_int, [c_void_p, c_int, c_int, c_double, c_double, c_void_p, c_void_p]),
     "ec_burgers_fractions": (c_int, [c_void_p, c_int, c_int, c_double, c_double, c_int, c_void_p, c_int,

@@ -151,6 +151,82 @@ def solve_smagorinsky(ustart, nH, theta, L=2 * math.pi, visc=5e-4, delta_ratio=2
     return out
 
 
+FIELDS = ("dns_ref", "dns_ref_fil", "dns_fil", "nomodel", "class_m", "class_p", "swapfil")
+LES_KEYS = ("nomodel", "class_m", "class_p", "swapfil")
+
+
+def getsetup(nsample=1000):
+    """getsetup (src/Burgers.jl:235-270): the reference's Burgers study."""
+    return {"L": 2 * math.pi, "nh": 100 * 3 ** 3 * 5, "nH": [100 * 3 ** k for k in (1, 2, 3)], "Delta_ratios": [2.0],
+            "visc": 5.0e-4, "kpeak": 10, "initialenergy": 2.0, "tstop": 0.1, "nsample": nsample,
+            "lesfiltertype": "gaussian"}
+
+
+def randomfield(n, kpeak, totalenergy, nsamples=1, seed=1, L=2 * math.pi):
+    """randomfield with bump_profile (src/Burgers.jl:192-202) for nsamples
+    samples (sample q seeded by seed + q, as the reference seeds Xoshiro(i)
+    per sample; the phases come from this library's counter-based hash):
+    a (nsamples, n) FP64 CUDA tensor."""
+    import torch
+
+    u = torch.empty((nsamples, n), dtype=torch.float64, device="cuda")
+    _abi.context().call("ec_burgers_randomfield", n, L, float(kpeak), float(totalenergy), nsamples, seed,
+                        _abi.c_void_p(u.data_ptr()))
+    return u
+
+
+def dns_aided_les(ustart, nH, visc, delta, tstop, cfl_factor=0.3, lesfiltertype="gaussian", L=2 * math.pi):
+    """dns_aided_les(ustart, gh, gH, visc; Delta, tstop, cfl_factor,
+    lesfiltertype) (src/Burgers.jl:350-445) for every row of ustart ((nsamples,
+    nh) FP64 CUDA tensor) in one launch: forward Euler with the CFL step, the
+    DNS and the nomodel, class_m, class_p and swapfil LES.  Returns a dict of
+    (nsamples, n) tensors keyed like the reference's named tuple, plus
+    "steps" (list of step counts)."""
+    import torch
+
+    assert ustart.dtype == torch.float64 and ustart.is_cuda and ustart.dim() == 2 and ustart.is_contiguous()
+    ns, nh = ustart.shape
+    delta_ratio = delta / (L / nH)
+    (Rl, wl), (K, wd) = les_kernels(nh, nH, L, delta_ratio, lesfiltertype)
+    wl = np.ascontiguousarray(wl, dtype=np.float64)
+    wd = np.ascontiguousarray(wd, dtype=np.float64)
+    out = torch.empty((ns, 2 * nh + 5 * nH), dtype=torch.float64, device="cuda")
+    steps = (ctypes.c_int * ns)()
+    dp = ctypes.POINTER(ctypes.c_double)
+    _abi.context().call("ec_burgers_dns_aided_les", nh, nH, L, visc, tstop, cfl_factor, ns,
+                        _abi.c_void_p(ustart.data_ptr()), Rl, wl.ctypes.data_as(dp), K, wd.ctypes.data_as(dp),
+                        _abi.c_void_p(out.data_ptr()), steps)
+    res = {"dns_ref": out[:, :nh], "dns_ref_fil": out[:, nh:2 * nh]}
+    for m, key in enumerate(FIELDS[2:]):
+        res[key] = out[:, 2 * nh + m * nH:2 * nh + (m + 1) * nH]
+    res["steps"] = list(steps)
+    return res
+
+
+def run_dns_aided_les(setup, seed=1):
+    """run_dns_aided_les (src/Burgers.jl:272-325): every (nH, Delta_ratio) of
+    the setup over nsample samples (sample i from randomfield seeded i), all
+    samples of one (nH, Delta_ratio) in one launch.  Returns a list of
+    {"nH", "Delta_ratio", "fields"} like the reference's saved results."""
+    out = []
+    ustart = randomfield(setup["nh"], setup["kpeak"], setup["initialenergy"], setup["nsample"], seed, setup["L"])
+    for nH in setup["nH"]:
+        for dr in setup["Delta_ratios"]:
+            H = setup["L"] / nH
+            f = dns_aided_les(ustart, nH, setup["visc"], H * dr, setup["tstop"], 0.3, setup["lesfiltertype"],
+                              setup["L"])
+            out.append({"nH": nH, "Delta_ratio": dr, "fields": f})
+    return out
+
+
+def compute_errors(series):
+    """compute_errors (src/Burgers.jl:909-917): for every LES model the
+    relative error norm(u - dns_fil) / norm(dns_fil) over all samples, one
+    entry per (nH, Delta_ratio) of the series (FP64)."""
+    return {key: [float((r["fields"][key] - r["fields"]["dns_fil"]).norm() / r["fields"]["dns_fil"].norm())
+                  for r in series] for key in LES_KEYS}
+
+
 class BurgersBatch:
     """count instances: n[i] points, compression c[i], RK4 with the exact
     closure and a no-model LES, FP32 or FP64 state, FP64 commutator."""

@@ -302,6 +302,26 @@ int ec_burgers_batch_get(ec_ctx *ctx, ec_burgers_batch *batch, int instance, int
 /* The time step of an instance (cfl, src/Burgers.jl:103-109, at its initial
  * state); read on the context stream after the work queued before it. */
 int ec_burgers_batch_dt(ec_ctx *ctx, ec_burgers_batch *batch, int instance, double *dt);
+/* randomfield with bump_profile (src/Burgers.jl:192-202) for nsamples samples
+ * (sample q seeded by seed0 + q, counter-based phases): u (device, nsamples x
+ * n FP64) = sum_k 2 (k/kp)^4 exp(-2 (k/kp)^2) cos(2 pi (k m / n + phi_k)),
+ * the irfft of the reference's coefficients, scaled to energy totalenergy
+ * (sum u^2 / 2 * L/n); modes past 6 kpeak (below 1e-27 of the peak) are
+ * omitted. */
+int ec_burgers_randomfield(ec_ctx *ctx, int n, double L, double kpeak, double totalenergy, int nsamples,
+                           uint64_t seed0, void *u);
+/* dns_aided_les (src/Burgers.jl:350-445) for nsamples DNS initial fields
+ * (ustart, device, nsamples x nh FP64), one persistent block per sample:
+ * forward Euler with dt = cfl_factor min(h / max|u|, h^2 / visc) cut at tstop,
+ * the DNS and the nomodel, class_m, class_p and swapfil LES with the LES
+ * kernel (Rl, wl) and the double kernel (K, wd) (host arrays).  out (device,
+ * per sample 2 nh + 5 nH doubles): dns_ref (nh), dns_ref_fil = D * u (nh),
+ * dns_fil (nH), nomodel, class_m, class_p, swapfil (nH each).  steps
+ * (host, nullable): the number of time steps of each sample.  nh <= 16384.
+ * Returns when the results are written (allocates a work area for the call). */
+int ec_burgers_dns_aided_les(ec_ctx *ctx, int nh, int nH, double L, double visc, double tstop, double cfl_factor,
+                             int nsamples, const void *ustart, int Rl, const double *wl, int K, const double *wd,
+                             void *out, int *steps);
 /* Single-instance operators for parity tests (src/Burgers.jl:89-142). */
 int ec_burgers_rhs(ec_ctx *ctx, int dtype, int n, double L, double visc, const void *u, void *du);
 /* compute_fractions (src/Burgers.jl:1022-1099) over nsamples FP64 DNS

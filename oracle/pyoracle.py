@@ -932,3 +932,90 @@ def burgers_dnsaid_rk4(u, c, visc, dt, nsteps, L=2 * math.pi):
     um = np.empty(nH)
     lib().or_burgers_dnsaid_rk4(len(u), c, L, visc, dt, nsteps, _p(u), _p(us), _p(um))
     return u, us, um
+
+
+_M64 = (1 << 64) - 1
+
+
+def _sm64(x):
+    x = (x + 0x9E3779B97F4A7C15) & _M64
+    x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+    x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & _M64
+    return x ^ (x >> 31)
+
+
+def _phase(seed, k):
+    """The counter-based uniform phase of mode k for a sample seed (splitmix64
+    finaliser of seed ^ finaliser(k), top 53 bits)."""
+    return (_sm64((seed ^ _sm64(k)) & _M64) >> 11) * 2.0 ** -53
+
+
+def burgers_randomfield(n, kpeak, totalenergy, seed, L=2 * math.pi):
+    """randomfield with bump_profile (src/Burgers.jl:192-202): c_k =
+    (k/kp)^4 exp(-2 (k/kp)^2 + 2 pi i phi_k) for k = 0 .. n/2, a = irfft(c n,
+    n) (numpy; the whole spectrum), u = sqrt(E / etot) a with etot = sum a^2 /
+    2 h.  The phases phi_k replace Julia's rand(Xoshiro(i)) draws."""
+    k = np.arange(n // 2 + 1)
+    amp = (k / kpeak) ** 4 * np.exp(-2 * (k / kpeak) ** 2)
+    ph = np.array([_phase(seed, int(kk)) for kk in k])
+    a = np.fft.irfft(amp * np.exp(2j * np.pi * ph) * n, n)
+    etot = np.sum(a ** 2) / 2 * (L / n)
+    return np.sqrt(totalenergy / etot) * a
+
+
+def burgers_dns_aided_les(ustart, nH, L, visc, les, dbl, tstop, cfl_factor=0.3):
+    """dns_aided_les (src/Burgers.jl:350-445) for one DNS initial field, FP64:
+    forward Euler with dt = min(cfl_factor min(h / max|u|, h^2 / visc), tstop -
+    t); per step the DNS stress s, fsu = C_c(F * s), vsu = C_c(D * s), vu =
+    C_f(D * u), svu = s_H(vu), sVU = C_c(s_h(D * u)); the LES stresses of
+    nomodel (s_H(u)), class_m (+ vsu - sVU), class_p (+ vsu - svu) and
+    swapfil (+ fsu - svu) and stepfromstress! for every LES and then the DNS
+    (src/Burgers.jl:338-347).  Filters and coarse-grain indices as in
+    burgers_fractions.  Returns (fields, steps)."""
+    u = np.array(ustart, dtype=np.float64)
+    n = len(u)
+    comp = n // nH
+    a = comp // 2
+    h, H = L / n, L / nH
+    Rl, wl = les
+    K, wd = dbl
+
+    def stress(x, sp):
+        xa = np.roll(x, 1)
+        return -((xa + x) ** 2) / 8 + visc * (x - xa) / sp
+
+    def filt(x, R, w):
+        out = np.zeros_like(x)
+        for r in range(-R, R + 1):
+            out += w[r + R] * np.roll(x, -r)
+        return out
+
+    J1 = np.arange(1, nH + 1)
+    centre = J1 * comp - a - 1
+    face = J1 * comp - 1
+    uH = filt(u, K, wd)[face]
+    les_u = {k: uH.copy() for k in ("nomodel", "class_m", "class_p", "swapfil")}
+    t, steps = 0.0, 0
+    while t < tstop:
+        dt = cfl_factor * min(h / np.abs(u).max(), h * h / visc)
+        dt = min(dt, tstop - t)
+        t += dt
+        steps += 1
+        s = stress(u, h)
+        fsu = filt(s, Rl, wl)[centre]
+        vsu = filt(s, K, wd)[centre]
+        VU = filt(u, K, wd)
+        vu = VU[face]
+        svu = stress(vu, H)
+        sVU = stress(VU, h)[centre]
+        sig = {"nomodel": stress(les_u["nomodel"], H),
+               "class_m": stress(les_u["class_m"], H) + vsu - sVU,
+               "class_p": stress(les_u["class_p"], H) + vsu - svu,
+               "swapfil": stress(les_u["swapfil"], H) + fsu - svu}
+        for key, sg in sig.items():
+            les_u[key] = les_u[key] + dt * ((np.roll(sg, -1) - sg) / H)
+        u = u + dt * ((np.roll(s, -1) - s) / h)
+    VU = filt(u, K, wd)
+    out = {"dns_ref": u, "dns_ref_fil": VU, "dns_fil": VU[face]}
+    out.update(les_u)
+    return out, steps

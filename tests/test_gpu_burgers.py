@@ -263,3 +263,40 @@ def test_study_entry_points_reject_bad_arguments(gpu):
     with pytest.raises(gpu._abi.ECError):
         ctx.call("ec_burgers_smag_solve", 18000, 9000, 1.0, 1e-3, 0.0, 0.1, 0.3, 2, gpu._abi.c_void_p(u2.data_ptr()),
                  1, w.ctypes.data_as(dp), gpu._abi.c_void_p(ub.data_ptr()))
+
+
+def test_burgers_randomfield_vs_checker(gpu, O):
+    """randomfield with bump_profile (src/Burgers.jl:192-202) for several
+    samples in one launch against the checker's numpy irfft of the whole
+    spectrum (the GPU sums modes up to 6 kpeak): 1e-13 of the field's maximum,
+    energy to 1e-12."""
+    B = gpu.burgers
+    n, kp, E, L = 1350, 10, 2.0, 2 * np.pi
+    u = B.randomfield(n, kp, E, nsamples=3, seed=7, L=L).cpu().numpy()
+    for q in range(3):
+        want = O.burgers_randomfield(n, kp, E, 7 + q, L)
+        assert rel(u[q], want) < 1e-13, q
+        assert abs(np.sum(u[q] ** 2) / 2 * (L / n) - E) < 1e-12 * E
+
+
+@pytest.mark.parametrize("nh,nH,kind,dr", [(1350, 30, "gaussian", 2.0), (1350, 90, "tophat", 1.0),
+                                           (1350, 270, "gaussian", 2.0), (2025, 135, "gaussian", 0.0)])
+def test_burgers_dns_aided_les_vs_checker(gpu, O, nh, nH, kind, dr):
+    """dns_aided_les (src/Burgers.jl:350-445), every sample in one persistent
+    launch, against the checker's per-sample loop: same number of CFL steps
+    and all seven fields (DNS, filtered DNS, the four LES) to 1e-11 of their
+    maximum after t = 0.01; the swap-filter LES equals the filtered DNS to
+    rounding (the exact closure)."""
+    B = gpu.burgers
+    L, visc, tstop = 2 * np.pi, 5e-4, 0.01
+    ustart = B.randomfield(nh, 10, 2.0, nsamples=2, seed=3, L=L)
+    got = B.dns_aided_les(ustart, nH, visc, dr * L / nH, tstop, 0.3, kind, L)
+    les, dbl = O.burgers_les_kernels(nh, nH, L, dr, kind)
+    u0 = ustart.cpu().numpy()
+    for q in range(2):
+        want, steps = O.burgers_dns_aided_les(u0[q], nH, L, visc, les, dbl, tstop)
+        assert got["steps"][q] == steps
+        for key in B.FIELDS:
+            assert rel(got[key][q].cpu().numpy(), want[key]) < 1e-11, (key, q)
+        f = got["dns_fil"][q]
+        assert float((got["swapfil"][q] - f).norm() / f.norm()) < 1e-14

@@ -579,3 +579,36 @@ def test_face_average_keeps_divergence_free(O, c):
         v = [O.filter_coarse(O._f(u[a]), n // c, *F[f"ubar{a + 1}"][1]) for a in range(3)]
         div[kind] = np.abs(O.divergence(v)).max() / (max(np.abs(x).max() for x in v) * n // c)
     assert div["fa"] < 1e-14 and div["va"] > 1e-3, div
+
+
+def test_burgers_bump_randomfield(O):
+    """randomfield with bump_profile (src/Burgers.jl:192-202): energy sum u^2/2 h
+    equals the requested one, the mean is zero, and the Fourier amplitudes
+    follow (k/kp)^4 exp(-2 (k/kp)^2) up to one common factor (a wrong profile
+    exponent, a lost factor 2 of the Hermitian half or a shifted k fails)."""
+    n, kp, E, L = 2048, 10, 2.0, 2 * np.pi
+    u = O.burgers_randomfield(n, kp, E, 5, L)
+    assert abs(np.sum(u ** 2) / 2 * (L / n) - E) < 1e-12 * E
+    assert abs(u.mean()) < 1e-13
+    c = np.abs(np.fft.rfft(u)) / n
+    k = np.arange(1, 31)  # amplitudes above 1e-5 of the peak (the rest sink into rounding)
+    prof = (k / kp) ** 4 * np.exp(-2 * (k / kp) ** 2)
+    ratio = c[1:31] / prof
+    assert np.abs(ratio / ratio[9] - 1).max() < 1e-8
+
+
+@pytest.mark.parametrize("nH,kind", [(30, "gaussian"), (90, "tophat"), (270, "gaussian")])
+def test_burgers_dns_aided_les_swap_exactness(O, nH, kind):
+    """dns_aided_les (src/Burgers.jl:350-445) with odd compressions: the
+    swap-filter LES equals the filtered DNS (dns_fil) to rounding after every
+    forward-Euler step (the exact closure), while no model, classic and
+    classic+flux stay O(1e-3) away; a swapped fsu/vsu or a centre one cell
+    off breaks the first."""
+    nh, L, visc = 1350, 2 * np.pi, 5e-4
+    u = O.burgers_randomfield(nh, 10, 2.0, 3, L)
+    les, dbl = O.burgers_les_kernels(nh, nH, L, 2.0, kind)
+    f, steps = O.burgers_dns_aided_les(u, nH, L, visc, les, dbl, 0.01)
+    err = {k: np.linalg.norm(f[k] - f["dns_fil"]) / np.linalg.norm(f["dns_fil"])
+           for k in ("nomodel", "class_m", "class_p", "swapfil")}
+    assert steps >= 10 and err["swapfil"] < 1e-14
+    assert min(err["nomodel"], err["class_m"], err["class_p"]) > 1e-4
