@@ -111,6 +111,8 @@ _SIGS = {
                                      c_int, c_double, P3, P3, P3]),
     "ec_filter_vel_set_tiles": (c_int, [c_void_p, c_int, c_int]),
     "ec_ns_commutator": (c_int, [c_void_p, c_int, c_double, c_double, P3, P9, P9]),
+    "ec_ns_sfs_filter": (c_int, [c_void_p, c_int, c_int, c_int, c_double, c_double, P3, c_void_p, c_int, P3, P9, P9]),
+    "ec_ns_tensordissipation": (c_int, [c_void_p, c_int, c_double, P9, P3, c_void_p]),
     "ec_ns_rk4_update": (c_int, [c_void_p, c_int, c_int, c_double, c_int, c_double, P3, P3, P3,
                                  c_void_p, P3]),
     "ec_burgers_batch_create": (c_int, [c_void_p, c_int, c_int, ctypes.POINTER(c_int),

@@ -9,6 +9,7 @@
 // heavily and is kept in double on the GPU.
 #include "ec_internal.h"
 
+#include <deque>
 #include <map>
 #include <mutex>
 
@@ -507,6 +508,30 @@ Piece centre(int c) {
     return {{0.5, 0.5}, -(c / 2)};
 }
 Piece point() { return {{1.0}, 0}; }
+
+// r_ab = sigma_ab + p delta_ab as a filter source
+const int rsrc[3][3] = {{EC_SRC_R11, EC_SRC_S12, EC_SRC_S31},
+                        {EC_SRC_S12, EC_SRC_R22, EC_SRC_S23},
+                        {EC_SRC_S31, EC_SRC_S23, EC_SRC_R33}};
+
+// A list of filter quantities whose weight vectors stay put while it grows.
+struct Quantities {
+    std::deque<Piece> keep;
+    std::vector<ec_filter_quantity> q;
+    void add(int source, const Piece (&ax)[3], void *out) {
+        if (!out) return;
+        ec_filter_quantity f;
+        f.source = source;
+        for (int d = 0; d < 3; d++) {
+            keep.push_back(ax[d]);
+            f.len[d] = (int)keep.back().w.size();
+            f.off[d] = keep.back().off;
+            f.w[d] = keep.back().w.data();
+        }
+        f.out = out;
+        q.push_back(f);
+    }
+};
 }  // namespace
 
 extern "C" int ec_ns_closure(ec_ctx *ctx, int dtype, int n, double l, double visc, void *const u[3], const void *p,
@@ -515,51 +540,28 @@ extern "C" int ec_ns_closure(ec_ctx *ctx, int dtype, int n, double l, double vis
     // one plane-marching launch for all sets when they fit the tiled scheme
     const int trc = ec_closure_tiled(ctx, dtype, n, l, visc, u, p, nsets, sets);
     if (trc != EC_E_UNSUPPORTED) return trc;
-    static const int rsrc[3][3] = {{EC_SRC_R11, EC_SRC_S12, EC_SRC_S31},
-                                   {EC_SRC_S12, EC_SRC_R22, EC_SRC_S23},
-                                   {EC_SRC_S31, EC_SRC_S23, EC_SRC_R33}};
     for (int s = 0; s < nsets; s++) {
         const ec_closure_set &S = sets[s];
         EC_REQUIRE(ctx, S.nbar >= 1 && n % S.nbar == 0, "closure set %d: n=%d not a multiple of nbar=%d", s, n, S.nbar);
         EC_REQUIRE(ctx, S.kind == 0 || S.kind == 1, "closure set %d: kind %d", s, S.kind);
         const int c = n / S.nbar;
-        std::vector<Piece> keep;
-        keep.reserve(64);
-        std::vector<ec_filter_quantity> q;
-        auto add = [&](int source, Piece a, Piece b, Piece cc, void *out) {
-            keep.push_back(a);
-            keep.push_back(b);
-            keep.push_back(cc);
-            ec_filter_quantity f;
-            f.source = source;
-            const Piece *pp[3] = {&keep[keep.size() - 3], &keep[keep.size() - 2], &keep[keep.size() - 1]};
-            for (int d = 0; d < 3; d++) {
-                f.len[d] = (int)pp[d]->w.size();
-                f.off[d] = pp[d]->off;
-                f.w[d] = pp[d]->w.data();
-            }
-            f.out = out;
-            q.push_back(f);
-        };
+        Quantities Q;
         for (int a = 0; a < 3; a++) {
-            if (!S.ubar[a]) continue;
             Piece ax[3];
             for (int d = 0; d < 3; d++) ax[d] = d == a ? (S.kind == 0 ? face(c) : point()) : box(c);
-            add(EC_SRC_U1 + a, ax[0], ax[1], ax[2], S.ubar[a]);
+            Q.add(EC_SRC_U1 + a, ax, S.ubar[a]);
         }
         for (int b = 0; b < 3; b++)
             for (int a = 0; a < 3; a++) {
-                void *out = S.flux[a + 3 * b];
-                if (!out) continue;
                 Piece ax[3];
                 for (int d = 0; d < 3; d++) {
                     if (a == b) ax[d] = d == a ? centre(c) : box(c);
                     else if (S.kind == 0) ax[d] = d == b ? point() : (d == a ? face(c) : box(c));
                     else ax[d] = (d == a || d == b) ? point() : box(c);
                 }
-                add(rsrc[a][b], ax[0], ax[1], ax[2], out);
+                Q.add(rsrc[a][b], ax, S.flux[a + 3 * b]);
             }
-        int rc = ec_filter_coarse(ctx, dtype, n, S.nbar, l, visc, u, p, (int)q.size(), q.data());
+        int rc = ec_filter_coarse(ctx, dtype, n, S.nbar, l, visc, u, p, (int)Q.q.size(), Q.q.data());
         if (rc) return rc;
     }
     return EC_OK;
@@ -579,5 +581,109 @@ extern "C" int ec_ns_commutator(ec_ctx *ctx, int nbar, double l, double visc, vo
     k_commutator<<<(unsigned)((nb3 + 255) / 256), 256, 0, ctx->stream>>>(
         nbar, nbar / l, visc, (const double *)ubar[0], (const double *)ubar[1], (const double *)ubar[2], fl, tl);
     EC_CHECK_LAUNCH(ctx, "k_commutator");
+    return EC_OK;
+}
+
+// ---- sub-filter stresses and their dissipation (src/NavierStokes.jl:470-593, process.jl:107-193) ----
+
+extern "C" int ec_ns_sfs_filter(ec_ctx *ctx, int dtype, int n, int nbar, double l, double visc, void *const u[3],
+                                const void *p, int kind, void *const fu[3], void *const fr9[9], void *const sr9[9]) {
+    EC_REQUIRE(ctx, ctx && u && p && fu && fr9 && sr9, "ec_ns_sfs_filter: null argument");
+    EC_REQUIRE(ctx, nbar >= 1 && n >= 2 && n % nbar == 0, "ec_ns_sfs_filter: n=%d not a multiple of nbar=%d", n,
+               nbar);
+    EC_REQUIRE(ctx, kind == 0 || kind == 1, "ec_ns_sfs_filter: kind %d (0 volume, 1 surface)", kind);
+    const int c = n / nbar;
+    Quantities Q;
+    // the LES velocity filter: volume average (c^2 box times the face window
+    // along a) or face average (c^2 box in the face plane)
+    for (int a = 0; a < 3; a++) {
+        Piece ax[3];
+        for (int d = 0; d < 3; d++) ax[d] = d == a ? (kind == 0 ? face(c) : point()) : box(c);
+        Q.add(EC_SRC_U1 + a, ax, fu[a]);
+    }
+    for (int b = 0; b < 3; b++)
+        for (int a = 0; a < 3; a++) {
+            Piece cl[3], sw[3];
+            for (int d = 0; d < 3; d++) {
+                if (a == b) {
+                    // volume: the c^3 box of the coarse cell; surface: the c^2
+                    // box of the planes normal to a through the coarse centre
+                    cl[d] = kind == 0 ? box(c) : (d == a ? centre(c) : box(c));
+                    sw[d] = d == a ? centre(c) : box(c);
+                } else if (kind == 0) {
+                    // classic: the volume around the coarse edge (a, b);
+                    // swap: the face normal to b, the derivative direction
+                    cl[d] = (d == a || d == b) ? face(c) : box(c);
+                    sw[d] = d == b ? point() : (d == a ? face(c) : box(c));
+                } else {
+                    // classic: the face normal to a (the filter of u_a);
+                    // swap: the line along the third direction
+                    cl[d] = d == a ? point() : (d == b ? face(c) : box(c));
+                    sw[d] = (d == a || d == b) ? point() : box(c);
+                }
+            }
+            Q.add(rsrc[a][b], cl, fr9[a + 3 * b]);
+            Q.add(rsrc[a][b], sw, sr9[a + 3 * b]);
+        }
+    return ec_filter_coarse(ctx, dtype, n, nbar, l, visc, u, p, (int)Q.q.size(), Q.q.data());
+}
+
+namespace {
+struct Tensor9 {
+    const double *s[9];
+};
+
+// d = sigma : grad u on the staggered grid, at the cell centres: the diagonal
+// products sigma_aa delta_a u_a sit at the centre; each off-diagonal product
+// sigma_ab delta_b u_a sits at its edge (a, b) and is averaged over the four
+// edges around the centre, so sum_I d[I] = sum_I u . (-div^ns sigma) exactly.
+__global__ void k_tensordissipation(int nb, double ih, Tensor9 S, const double *__restrict__ u1,
+                                    const double *__restrict__ u2, const double *__restrict__ u3,
+                                    double *__restrict__ d) {
+    const size_t nb3 = (size_t)nb * nb * nb;
+    const size_t I = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (I >= nb3) return;
+    const double *u[3] = {u1, u2, u3};
+    const int x[3] = {(int)(I % nb), (int)((I / nb) % nb), (int)(I / ((size_t)nb * nb))};
+    auto at = [nb](const int (&y)[3]) { return (size_t)y[0] + (size_t)nb * ((size_t)y[1] + (size_t)nb * y[2]); };
+    double acc = 0.0;
+    for (int a = 0; a < 3; a++) {
+        int m[3] = {x[0], x[1], x[2]};
+        m[a] = ec_dn(m[a], nb);
+        acc += S.s[4 * a][I] * ((u[a][I] - u[a][at(m)]) * ih);
+        for (int b = 0; b < 3; b++) {
+            if (b == a) continue;
+            double e = 0.0;
+            for (int sb = 0; sb < 2; sb++)
+                for (int sa = 0; sa < 2; sa++) {
+                    int J[3] = {x[0], x[1], x[2]};
+                    if (sa) J[a] = ec_dn(J[a], nb);
+                    if (sb) J[b] = ec_dn(J[b], nb);
+                    int Jb[3] = {J[0], J[1], J[2]};
+                    Jb[b] = ec_up(Jb[b], nb);
+                    const size_t j = at(J);
+                    e += S.s[a + 3 * b][j] * ((u[a][at(Jb)] - u[a][j]) * ih);
+                }
+            acc += 0.25 * e;
+        }
+    }
+    d[I] = acc;
+}
+}  // namespace
+
+extern "C" int ec_ns_tensordissipation(ec_ctx *ctx, int nbar, double l, void *const sigma9[9], void *const u[3],
+                                       void *d) {
+    EC_REQUIRE(ctx, ctx && sigma9 && u && d, "ec_ns_tensordissipation: null argument");
+    EC_REQUIRE(ctx, nbar >= 2, "ec_ns_tensordissipation: nbar=%d", nbar);
+    Tensor9 S;
+    for (int m = 0; m < 9; m++) {
+        EC_REQUIRE(ctx, sigma9[m], "ec_ns_tensordissipation: sigma component %d is null", m);
+        S.s[m] = (const double *)sigma9[m];
+    }
+    EC_REQUIRE(ctx, u[0] && u[1] && u[2], "ec_ns_tensordissipation: null velocity component");
+    const size_t nb3 = (size_t)nbar * nbar * nbar;
+    k_tensordissipation<<<(unsigned)((nb3 + 255) / 256), 256, 0, ctx->stream>>>(
+        nbar, nbar / l, S, (const double *)u[0], (const double *)u[1], (const double *)u[2], (double *)d);
+    EC_CHECK_LAUNCH(ctx, "k_tensordissipation");
     return EC_OK;
 }

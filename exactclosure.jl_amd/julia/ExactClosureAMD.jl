@@ -608,6 +608,82 @@ function compute_errors(uaid)
     return map(err, (; uaid.u_nomo, uaid.u_c, uaid.u_cf, uaid.u_cfd, uaid.u_cfd_symm))
 end
 
+# --- sub-filter stresses and their dissipation (src/NavierStokes.jl:470-593, process.jl:107-193) ---
+
+const NONSYM = ("11", "21", "31", "12", "22", "32", "13", "23", "33")
+
+function _sfs_tensors(u, g_les::Grid{3}, poisson_dns, poisson_les, visc, experiment::Symbol)
+    g = u[1].grid
+    rem(g.n, g_les.n) == 0 || throw(ArgumentError("n_les = $(g_les.n) does not divide n_dns = $(g.n)"))
+    du = vectorfield(g, AMDBackend(), eltype(u[1]))
+    rhs!(du, u, visc, false)
+    project!(du, poisson_dns)
+    nb = g_les.n
+    new() = HipArray{Float64}(undef, (nb, nb, nb))
+    fu, fr, sr = ntuple(_ -> new(), 3), ntuple(_ -> new(), 9), ntuple(_ -> new(), 9)
+    kind = experiment === :surfavg ? Cint(1) : Cint(0)
+    GC.@preserve u fu fr sr call(EC.ec_ns_sfs_filter, dcode(u[1]), g.n, nb, g.l, visc, ptrs(u), pressure(poisson_dns),
+                                 kind, ptrs(fu), ptrs(fr), ptrs(sr))
+    fuF = map(x -> Field(g_les, x), fu)
+    experiment === :project_volavg && project!(fuF, poisson_les)
+    dul = vectorfield(g_les, AMDBackend())
+    rhs!(dul, fuF, visc, false)
+    project!(dul, poisson_les)
+    pl = pressure(poisson_les)
+    # σ = filtered r - r_H(fu), r_H = σ_H(fu) + p_les I
+    σ_classic, σ_swapfil = map((fr, sr)) do q
+        τ = ntuple(_ -> new(), 9)
+        GC.@preserve fu q τ call(EC.ec_ns_commutator, nb, g_les.l, visc, ptrs(fu), ptrs(q), ptrs(τ))
+        foreach(a -> call(EC.ec_axpy, EC.EC_F64, length(τ[4a + 1]), -1.0, pl, τ[4a + 1]), 0:2)
+        τ
+    end
+    σ_swapfil_symm = ntuple(9) do m
+        x = copyto!(new(), σ_swapfil[m])
+        ab = NONSYM[m]
+        ab[1] == ab[2] || EC.axpby!(0.5, σ_swapfil[findfirst(==(reverse(ab)), NONSYM)], 0.5, x)
+        x
+    end
+    return (; σ_classic, σ_swapfil, σ_swapfil_symm, fu = fuF)
+end
+
+"""
+    sfs_tensors_volume(; ustart, g_dns, g_les, poisson_dns, poisson_les, viscosity, compression, doproject)
+
+sfs_tensors_volume (src/NavierStokes.jl:470-528): the classic, swap-filter
+and symmetrised swap-filter sub-filter stresses of the volume average (fu
+projected when `doproject`), 9-component FP64 device tensors in the order
+11, 21, 31, 12, 22, 32, 13, 23, 33, and the filtered velocity `fu`.
+"""
+sfs_tensors_volume(; ustart, g_dns = nothing, g_les, poisson_dns, poisson_les, viscosity, compression = nothing,
+                   doproject) =
+    _sfs_tensors(ustart, g_les, poisson_dns, poisson_les, viscosity, doproject ? :project_volavg : :volavg)
+
+"sfs_tensors_surface (src/NavierStokes.jl:530-593): as sfs_tensors_volume for the face average."
+sfs_tensors_surface(; ustart, g_dns = nothing, g_les, poisson_dns, poisson_les, viscosity, compression = nothing,
+                    doproject = false) =
+    _sfs_tensors(ustart, g_les, poisson_dns, poisson_les, viscosity, :surfavg)
+
+"Turbulox.tensordissipation_staggered!(d, σ, u) as process.jl:160-180 applies it: d = σ : ∇u at the centres (FP64)."
+function tensordissipation_staggered!(d::Field, σ, u)
+    g = d.grid
+    GC.@preserve d σ u call(EC.ec_ns_tensordissipation, g.n, g.l, ptrs(σ), ptrs(u), d.data)
+    return d
+end
+
+"""
+The dissipation fields of one LES grid and experiment (:volavg,
+:project_volavg or :surfavg) of process.jl:107-193: `(; classic, swapfil,
+swapfil_symm)` FP64 fields.  The kernel density estimate is left to the caller.
+"""
+function sfs_dissipation(u, g_les::Grid{3}, viscosity, experiment::Symbol; poisson_dns = nothing,
+                         poisson_les = nothing)
+    ps = something(poisson_dns, poissonsolver(u[1].grid, AMDBackend(), eltype(u[1])))
+    psl = something(poisson_les, poissonsolver(g_les, AMDBackend()))
+    t = _sfs_tensors(u, g_les, ps, psl, viscosity, experiment)
+    d(σ) = tensordissipation_staggered!(Field(g_les, AMDBackend()), σ, t.fu)
+    return (; classic = d(t.σ_classic), swapfil = d(t.σ_swapfil), swapfil_symm = d(t.σ_swapfil_symm))
+end
+
 end # module NavierStokes
 
 # ---------------------------------------------------------------------------

@@ -136,6 +136,10 @@ ec_closure_set_tiles(ctx, tx, ty, tz) =
     ccall((:ec_closure_set_tiles, libexactclosure), Cint, (Ptr{Cvoid}, Cint, Cint, Cint), ctx, tx, ty, tz)
 ec_ns_commutator(ctx, nbar, l, visc, ubar, flux9, tau9) =
     ccall((:ec_ns_commutator, libexactclosure), Cint, (Ptr{Cvoid}, Cint, Cdouble, Cdouble, Ref{NTuple{3,Ptr{Cvoid}}}, Ref{NTuple{9,Ptr{Cvoid}}}, Ref{NTuple{9,Ptr{Cvoid}}}), ctx, nbar, l, visc, ubar, flux9, tau9)
+ec_ns_sfs_filter(ctx, dtype, n, nbar, l, visc, u, p, kind, fu, fr9, sr9) =
+    ccall((:ec_ns_sfs_filter, libexactclosure), Cint, (Ptr{Cvoid}, Cint, Cint, Cint, Cdouble, Cdouble, Ref{NTuple{3,Ptr{Cvoid}}}, Ptr{Cvoid}, Cint, Ref{NTuple{3,Ptr{Cvoid}}}, Ref{NTuple{9,Ptr{Cvoid}}}, Ref{NTuple{9,Ptr{Cvoid}}}), ctx, dtype, n, nbar, l, visc, u, p, kind, fu, fr9, sr9)
+ec_ns_tensordissipation(ctx, nbar, l, sigma9, u, d) =
+    ccall((:ec_ns_tensordissipation, libexactclosure), Cint, (Ptr{Cvoid}, Cint, Cdouble, Ref{NTuple{9,Ptr{Cvoid}}}, Ref{NTuple{3,Ptr{Cvoid}}}, Ptr{Cvoid}), ctx, nbar, l, sigma9, u, d)
 ec_ns_rk4_update(ctx, dtype, n, l, stage, dt, u0, us, du, p, acc) =
     ccall((:ec_ns_rk4_update, libexactclosure), Cint, (Ptr{Cvoid}, Cint, Cint, Cdouble, Cint, Cdouble, Ref{NTuple{3,Ptr{Cvoid}}}, Ref{NTuple{3,Ptr{Cvoid}}}, Ref{NTuple{3,Ptr{Cvoid}}}, Ptr{Cvoid}, Ref{NTuple{3,Ptr{Cvoid}}}), ctx, dtype, n, l, stage, dt, u0, us, du, p, acc)
 ec_burgers_batch_create(ctx, dtype, count, n, c, L, visc, cfl, kmax, seed, batch) =

@@ -587,3 +587,109 @@ def compute_errors(ubar, uref):
     for a, b in zip(ubar, uref):
         s += diffsq(a, b) / sumsq(b)
     return math.sqrt(s / 3)
+
+
+# --------------------------------------------------------------------------
+# Sub-filter stresses and their dissipation (src/NavierStokes.jl:470-593,
+# process.jl:107-193)
+# --------------------------------------------------------------------------
+
+SFS_EXPERIMENTS = ("volavg", "project_volavg", "surfavg")
+SFS_TENSORS = ("classic", "swapfil", "swapfil_symm")
+
+
+def sfs_filter(u, p, g_les, visc, kind):
+    """ec_ns_sfs_filter: the DNS-side filters of sfs_tensors_volume (kind
+    "volume": volumefilter! of u and of r = sigma + p I, surfacefilter!(r,
+    false)) or sfs_tensors_surface ("surface": surfacefilter! of u,
+    surfacefilter!(r, true), linefilter!(r) off the diagonal) in one launch.
+    p: the DNS pressure (Field or device pointer).  Returns FP64 tensors
+    (fu[3], fr[9], sr[9]), nonsym order."""
+    import torch
+
+    g = u[0].grid
+    k = {"volume": 0, "surface": 1}[kind]
+    shape = (g_les.n,) * 3
+
+    def new(m):
+        return [torch.empty(shape, dtype=torch.float64, device="cuda") for _ in range(m)]
+
+    fu, fr, sr = new(3), new(9), new(9)
+    pp = ptr(p.data) if isinstance(p, Field) else _abi.c_void_p(p)
+    _ctx().call("ec_ns_sfs_filter", _dt(u[0]), g.n, g_les.n, g.l, visc, ptrs([x.data for x in u], P3), pp, k,
+                ptrs(fu, P3), ptrs(fr, P9), ptrs(sr, P9))
+    return fu, fr, sr
+
+
+def tensordissipation(sigma9, u):
+    """Turbulox.tensordissipation_staggered!(d, sigma, u) as used in
+    process.jl:160-180: d = sigma : grad u at the cell centres (FP64), sigma
+    in nonsym order, u FP64."""
+    import torch
+
+    g = u[0].grid
+    data = [x.data if isinstance(x, Field) else x for x in sigma9]
+    d = torch.empty((g.n,) * 3, dtype=torch.float64, device="cuda")
+    _ctx().call("ec_ns_tensordissipation", g.n, g.l, ptrs(data, P9), ptrs([x.data for x in u], P3), ptr(d))
+    return Field(g, d)
+
+
+def sfs_tensors(u, g_les, visc, experiment, ps_dns=None, ps_les=None):
+    """sfs_tensors_volume (experiment "volavg": doproject false,
+    "project_volavg": doproject true) and sfs_tensors_surface ("surfavg")
+    (src/NavierStokes.jl:470-593) of the DNS field u:
+      the DNS pressure from the projected right-hand side; fu the LES
+      velocity filter (projected for "project_volavg"); the LES pressure of
+      fu the same way, r_H(fu) = sigma_H(fu) + p_les I;
+      classic = fr - r_H(fu), swapfil = sr - r_H(fu) and its symmetric part.
+    Returns {"fu": 3 FP64 Fields, "classic"/"swapfil"/"swapfil_symm": 9 FP64
+    tensors each, nonsym order}.  ps_dns/ps_les: Poisson solvers of the two
+    grids (made here when omitted; ps_les is FP64)."""
+    import torch
+
+    if experiment not in SFS_EXPERIMENTS:
+        raise ValueError(f"experiment {experiment!r} (one of {SFS_EXPERIMENTS})")
+    g = u[0].grid
+    if g.n % g_les.n:
+        raise ValueError(f"n_les = {g_les.n} does not divide n_dns = {g.n}")
+    ps_dns = ps_dns or PoissonSolver(g, u[0].dtype)
+    ps_les = ps_les or PoissonSolver(g_les, torch.float64)
+    du = vectorfield(g, u[0].dtype)
+    rhs_(du, u, visc)
+    project_(du, ps_dns)
+    del du
+    fu, fr, sr = sfs_filter(u, ps_dns.p_ptr, g_les, visc, "surface" if experiment == "surfavg" else "volume")
+    fuF = tuple(Field(g_les, x) for x in fu)
+    if experiment == "project_volavg":
+        project_(fuF, ps_les)
+    dul = vectorfield(g_les, torch.float64)
+    rhs_(dul, fuF, visc)
+    project_(dul, ps_les)
+    pl = ps_les.p_ptr
+    out = {"fu": fuF}
+    ctx = _ctx()
+    for key, q in (("classic", fr), ("swapfil", sr)):
+        tau = [torch.empty_like(x) for x in q]
+        ctx.call("ec_ns_commutator", g_les.n, g_les.l, visc, ptrs(fu, P3), ptrs(q, P9), ptrs(tau, P9))
+        for a in range(3):  # r_H = sigma_H + p_les I on the diagonal
+            ctx.call("ec_axpy", _abi.EC_F64, tau[4 * a].numel(), -1.0, _abi.c_void_p(pl), ptr(tau[4 * a]))
+        out[key] = tau
+    sw = out["swapfil"]
+    symm = []
+    for m, ab in enumerate(NONSYM):
+        t = sw[m].clone()
+        if ab[0] != ab[1]:
+            ctx.call("ec_axpby", _abi.EC_F64, t.numel(), 0.5, ptr(sw[NONSYM.index(ab[::-1])]), 0.5, ptr(t))
+        symm.append(t)
+    out["swapfil_symm"] = symm
+    return out
+
+
+def sfs_dissipation(u, g_les, visc, experiment, ps_dns=None, ps_les=None):
+    """The dissipation fields of one LES grid and experiment of
+    process.jl:107-193: tensordissipation of sigma_classic, sigma_swapfil and
+    sigma_swapfil_symm with the filtered velocity fu (FP64 Fields).  The
+    kernel density estimate process.jl forms from them is left to the
+    caller."""
+    t = sfs_tensors(u, g_les, visc, experiment, ps_dns, ps_les)
+    return {k: tensordissipation(t[k], t["fu"]) for k in SFS_TENSORS}

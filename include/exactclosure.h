@@ -277,6 +277,27 @@ int ec_closure_set_tiles(ec_ctx *ctx, int tx, int ty, int tz);
 int ec_ns_commutator(ec_ctx *ctx, int nbar, double l, double visc, void *const ubar[3],
                      void *const flux9[9], void *const tau9[9]);
 
+/* ---- sub-filter stresses (src/NavierStokes.jl:470-593, process.jl:107-193) ----
+ * The DNS-side filters of sfs_tensors_volume (kind 0) and sfs_tensors_surface
+ * (kind 1) in one launch, r = sigma(u) + p I with the DNS pressure p, FP64
+ * outputs on the LES grid, tensors in nonsym order 11,21,31,12,22,32,13,23,33:
+ *   kind 0: fu = volumefilter!(u), fr = volumefilter!(r),
+ *           sr = surfacefilter!(r, false) (faces normal to the derivative
+ *           direction: the exact closure flux of the volume average);
+ *   kind 1: fu = surfacefilter!(u), fr = surfacefilter!(r, true) (faces normal
+ *           to the row direction), sr = the diagonal of fr and linefilter!(r)
+ *           off the diagonal (the exact closure flux of the face average).
+ * The filters are Turbulox's (not in the reference tree), re-derived for any
+ * compression: odd c top-hat windows, even c trapezoid windows at faces and
+ * two-point means at centres.  A null output component is skipped. */
+int ec_ns_sfs_filter(ec_ctx *ctx, int dtype, int n, int nbar, double l, double visc, void *const u[3],
+                     const void *p, int kind, void *const fu[3], void *const fr9[9], void *const sr9[9]);
+/* Turbulox.tensordissipation_staggered!(d, sigma, u) (process.jl:160-180):
+ * d = sigma : grad u at the LES cell centres, FP64; sigma in nonsym order,
+ * off-diagonal products averaged from the four edges around each centre so
+ * that the sum of d equals the sum of u . tensordivergence_nonsym(sigma). */
+int ec_ns_tensordissipation(ec_ctx *ctx, int nbar, double l, void *const sigma9[9], void *const u[3], void *d);
+
 /* ---- RK4 stage update fused with the pressure gradient ----
  * k = du - grad p.  stage 0..2: acc (= or +=) b_s k; us = u0 + a_{s+1} dt k.
  * stage 3: u0 += dt/6 (acc + k).  (rk4_timestep! tableau, src/Burgers.jl:120-142,

@@ -636,6 +636,109 @@ def commutator(q, ubar, visc, l=1.0):
     return [q["rDHi" + ab] - sym[ab] for ab in NONSYM_ORDER]
 
 
+# --------------------------------------------------------------------------
+# Sub-filter stresses and their dissipation (sfs_tensors_volume and
+# sfs_tensors_surface, src/NavierStokes.jl:470-593; process.jl:107-193).
+# The filters there are Turbulox's, a package outside the reference tree:
+# they are restated as the volume, face and line averages of the pieces above.
+# --------------------------------------------------------------------------
+
+def sfs_filters(c, kind):
+    """name -> (source, [(w, off)] * 3) of the DNS quantities of one
+    sfs_tensors_* call:
+      fu{a}   the LES velocity filter (kind 'volume': volumefilter!,
+              'surface': surfacefilter!);
+      fr{ab}  the classic filter of r = sigma + p I (volume: volumefilter!
+              of the tensor; surface: surfacefilter!(r, true), the face
+              normal to a, the filter of u_a);
+      sr{ab}  the exact-closure flux filter (volume: surfacefilter!(r,
+              false), faces normal to the derivative direction b; surface:
+              the diagonal of fr and linefilter!(r) off the diagonal), i.e.
+              the rDHi of closure_filters.
+    Tensors in nonsym order 11, 21, 31, 12, 22, 32, 13, 23, 33."""
+    base = closure_filters(c, "va" if kind == "volume" else "fa")
+    sym = {(0, 1): "s12", (1, 0): "s12", (1, 2): "s23", (2, 1): "s23", (2, 0): "s31", (0, 2): "s31"}
+    q = {}
+    for a in range(3):
+        q[f"fu{a + 1}"] = base[f"ubar{a + 1}"]
+    for ab in NONSYM_ORDER:
+        a, b = int(ab[0]) - 1, int(ab[1]) - 1
+        if a == b:
+            fs = [_box(c)] * 3 if kind == "volume" else base[f"rDHi{ab}"][1]
+            q[f"fr{ab}"] = (f"r{ab}", fs)
+        elif kind == "volume":
+            q[f"fr{ab}"] = (sym[(a, b)], [_face(c) if d in (a, b) else _box(c) for d in range(3)])
+        else:
+            q[f"fr{ab}"] = (sym[(a, b)], [_point() if d == a else (_face(c) if d == b else _box(c))
+                                          for d in range(3)])
+        q[f"sr{ab}"] = base[f"rDHi{ab}"]
+    return q
+
+
+def sfs_tensors(u, nbar, visc, experiment, l=1.0):
+    """sfs_tensors_volume (experiment 'volavg': doproject false;
+    'project_volavg': doproject true) and sfs_tensors_surface ('surfavg')
+    (src/NavierStokes.jl:470-593): the DNS pressure from the projected
+    right-hand side, the filtered velocity fu (projected for
+    'project_volavg'), the LES pressure of fu the same way, and
+    sigma_classic = fr - r_H(fu), sigma_swapfil = sr - r_H(fu) and its
+    symmetric part, r_H = sigma_H + p_les I.  Returns dict fu (3),
+    classic, swapfil, swapfil_symm (9 each, nonsym order)."""
+    u = [_f(x) for x in u]
+    n = u[0].shape[0]
+    c = n // nbar
+    assert c * nbar == n
+    kind = "surface" if experiment == "surfavg" else "volume"
+    assert experiment in ("volavg", "project_volavg", "surfavg"), experiment
+    du = rhs(u, visc, l)
+    p = project(du, l)
+    src = closure_sources(u, p, visc, l)
+    q = {name: filter_coarse(src[s], nbar, *fs) for name, (s, fs) in sfs_filters(c, kind).items()}
+    fu = [q["fu1"], q["fu2"], q["fu3"]]
+    if experiment == "project_volavg":
+        project(fu, l)
+    dul = rhs(fu, visc, l)
+    pl = project(dul, l)
+    s = stresstensor(fu, visc, l)
+    rh = {"11": s[0] + pl, "22": s[1] + pl, "33": s[2] + pl, "12": s[3], "21": s[3], "23": s[4], "32": s[4],
+          "31": s[5], "13": s[5]}
+    classic = [q["fr" + ab] - rh[ab] for ab in NONSYM_ORDER]
+    swapfil = [q["sr" + ab] - rh[ab] for ab in NONSYM_ORDER]
+    m = {ab: i for i, ab in enumerate(NONSYM_ORDER)}
+    symm = [(swapfil[m[ab]] + swapfil[m[ab[::-1]]]) / 2 for ab in NONSYM_ORDER]
+    return {"fu": fu, "classic": classic, "swapfil": swapfil, "swapfil_symm": symm}
+
+
+def tensordissipation(sigma9, u, l=1.0):
+    """Turbulox.tensordissipation_staggered!(d, sigma, u) (process.jl:160-180),
+    restated: d = sigma : grad u at the cell centres of the staggered grid.
+    sigma_aa delta_a u_a is formed at the centre; sigma_ab delta_b u_a at its
+    edge (a, b) (x_J + (e_a + e_b) h / 2, where tensordivergence_nonsym reads
+    sigma_ab) and averaged over the four edges around the centre."""
+    u = [_f(x) for x in u]
+    n = u[0].shape[0]
+    h = l / n
+    S = dict(zip(NONSYM_ORDER, [_f(x) for x in sigma9]))
+    d = field(n)
+    for a in range(3):
+        d += S[f"{a + 1}{a + 1}"] * ((u[a] - np.roll(u[a], 1, axis=a)) / h)
+        for b in range(3):
+            if b == a:
+                continue
+            e = S[f"{a + 1}{b + 1}"] * ((np.roll(u[a], -1, axis=b) - u[a]) / h)
+            d += (e + np.roll(e, 1, axis=a) + np.roll(e, 1, axis=b) + np.roll(np.roll(e, 1, axis=a), 1, axis=b)) / 4
+    return d
+
+
+def sfs_dissipation(u, nbar, visc, experiment, l=1.0):
+    """The dissipation fields of process.jl:107-193 for one LES grid and
+    experiment: tensordissipation of sigma_classic, sigma_swapfil and
+    sigma_swapfil_symm with the filtered velocity (the kernel density
+    estimate of process.jl is not part of this)."""
+    t = sfs_tensors(u, nbar, visc, experiment, l)
+    return {k: tensordissipation(t[k], t["fu"], l) for k in ("classic", "swapfil", "swapfil_symm")}
+
+
 def compute_errors(ubar, uref):
     """compute_errors (src/N.jl:1334-1349) for one LES field."""
     D = 3

@@ -1,0 +1,121 @@
+"""Sub-filter stresses and their dissipation (sfs_tensors_volume /
+sfs_tensors_surface, src/NavierStokes.jl:470-593; the dissipation fields of
+process.jl:107-193) on the GPU against the FP64 CPU checker, and the exact
+closure identity of the volume swap-filter stress at 128^3."""
+import numpy as np
+import pytest
+import torch
+
+from test_gpu_ns import dev, host
+
+pytestmark = pytest.mark.gpu
+
+NONSYM = ["11", "21", "31", "12", "22", "32", "13", "23", "33"]
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.mark.parametrize("experiment", ["volavg", "project_volavg", "surfavg"])
+@pytest.mark.parametrize("n,nb", [(24, 8), (24, 6), (20, 4)])
+def test_sfs_tensors_vs_checker(gpu, O, experiment, n, nb):
+    """fu and the three sub-filter stresses of every experiment, odd and even
+    compressions, FP64: 1e-11 (max-norm relative per component)."""
+    NS = gpu.navierstokes
+    visc = 0.01
+    u = O.randomfield(n, seed=3, totalenergy=1.0)
+    g, gl = NS.Grid(1.0, n), NS.Grid(1.0, nb)
+    got = NS.sfs_tensors([NS.Field(g, dev(x)) for x in u], gl, visc, experiment)
+    ref = O.sfs_tensors([x.copy(order="F") for x in u], nb, visc, experiment)
+    for a in range(3):
+        assert rel(host(got["fu"][a].data), ref["fu"][a]) < 1e-12, ("fu", a)
+    for k in ("classic", "swapfil", "swapfil_symm"):
+        scale = max(np.abs(x).max() for x in ref[k])
+        for m, ab in enumerate(NONSYM):
+            assert np.abs(host(got[k][m]) - ref[k][m]).max() < 1e-11 * scale, (k, ab)
+
+
+def test_tensordissipation_vs_checker(gpu, O):
+    NS = gpu.navierstokes
+    n = 18
+    rng = np.random.default_rng(5)
+    sig = [np.asfortranarray(rng.standard_normal((n, n, n))) for _ in range(9)]
+    u = [np.asfortranarray(rng.standard_normal((n, n, n))) for _ in range(3)]
+    g = NS.Grid(2.0, n)
+    d = NS.tensordissipation([dev(x) for x in sig], [NS.Field(g, dev(x)) for x in u])
+    want = O.tensordissipation(sig, u, 2.0)
+    assert rel(host(d.data), want) < 1e-13
+
+
+@pytest.mark.parametrize("experiment", ["volavg", "surfavg"])
+def test_sfs_dissipation_vs_checker(gpu, O, experiment):
+    """The end-to-end dissipation fields of one LES grid (process.jl:107-193)."""
+    NS = gpu.navierstokes
+    n, nb, visc = 30, 10, 0.005
+    u = O.randomfield(n, seed=8, totalenergy=1.0)
+    g, gl = NS.Grid(1.0, n), NS.Grid(1.0, nb)
+    got = NS.sfs_dissipation([NS.Field(g, dev(x)) for x in u], gl, visc, experiment)
+    ref = O.sfs_dissipation([x.copy(order="F") for x in u], nb, visc, experiment)
+    for k in ("classic", "swapfil", "swapfil_symm"):
+        assert rel(host(got[k].data), ref[k]) < 1e-10, k
+
+
+def test_sfs_volume_exact_closure_128(gpu):
+    """At 128^3 -> 32^3 (FP64): -div^ns sigma_swapfil = V(P du) - P_H du_les
+    to rounding (the volume swap-filter stress is the exact closure), the
+    classic stress misses it by O(1), and the dissipation fields sum to the
+    work of their stress divergence."""
+    NS = gpu.navierstokes
+    n, nb, visc = 128, 32, 1e-3
+    g, gl = NS.Grid(1.0, n), NS.Grid(1.0, nb)
+    ps = NS.PoissonSolver(g, torch.float64)
+    psl = NS.PoissonSolver(gl, torch.float64)
+    u = NS.randomfield(g, ps, "power", -5.0 / 3.0, 1.0, seed=21)
+    t = NS.sfs_tensors(u, gl, visc, "volavg", ps, psl)
+    du = NS.vectorfield(g, torch.float64)
+    NS.rhs_(du, u, visc)
+    NS.project_(du, ps)
+    # V(P du): the volume filter of the projected right-hand side (the fu
+    # output of sfs_filter applied to du; its tensor outputs are not used)
+    zero = torch.zeros((n, n, n), dtype=torch.float64, device="cuda")
+    fpdu, _, _ = NS.sfs_filter(du, zero.data_ptr(), gl, 0.0, "volume")
+    dul = NS.vectorfield(gl, torch.float64)
+    NS.rhs_(dul, t["fu"], visc)
+    NS.project_(dul, psl)
+    for key, exact in (("swapfil", True), ("classic", False)):
+        lhs = NS.vectorfield(gl, torch.float64)
+        NS.tensordivergence_nonsym_(lhs, t[key], False)
+        err = max(float((lhs[a].data - (fpdu[a] - dul[a].data)).abs().max() / fpdu[a].abs().max())
+                  for a in range(3))
+        print(f"\n[sfs 128->32 {key}] closure identity residual {err:.3e}")
+        assert (err < 1e-10) if exact else (err > 1e-3), (key, err)
+    for key in ("classic", "swapfil", "swapfil_symm"):
+        d = NS.tensordissipation(t[key], t["fu"])
+        w = NS.vectorfield(gl, torch.float64)
+        NS.tensordivergence_nonsym_(w, t[key], False)
+        work = sum(float((t["fu"][a].data * w[a].data).sum()) for a in range(3))
+        total = float(d.data.sum())
+        print(f"[sfs 128->32 {key}] mean dissipation {total / nb ** 3:.4e}")
+        assert abs(total - work) < 1e-10 * float(d.data.abs().sum()), key
+
+
+def test_sfs_rejects_bad_arguments(gpu):
+    NS, abi = gpu.navierstokes, gpu._abi
+    g = NS.Grid(1.0, 12)
+    u = NS.vectorfield(g, torch.float64)
+    with pytest.raises(ValueError):
+        NS.sfs_tensors(u, NS.Grid(1.0, 5), 0.01, "volavg")
+    with pytest.raises(ValueError):
+        NS.sfs_tensors(u, NS.Grid(1.0, 4), 0.01, "lineavg")
+    p = torch.zeros((12, 12, 12), dtype=torch.float64, device="cuda")
+    fu = [torch.empty((4, 4, 4), dtype=torch.float64, device="cuda") for _ in range(3)]
+    t9 = [torch.empty((4, 4, 4), dtype=torch.float64, device="cuda") for _ in range(9)]
+    ud = [x.data for x in u]
+    ctx = abi.context()
+    for nb, kind in ((4, 2), (5, 0), (4, -1)):
+        with pytest.raises(abi.ECError):
+            ctx.call("ec_ns_sfs_filter", abi.EC_F64, 12, nb, 1.0, 0.01, abi.ptrs(ud, abi.P3), abi.ptr(p), kind,
+                     abi.ptrs(fu, abi.P3), abi.ptrs(t9, abi.P9), abi.ptrs(t9, abi.P9))
+    with pytest.raises(abi.ECError):
+        ctx.call("ec_ns_tensordissipation", 1, 1.0, abi.ptrs(t9, abi.P9), abi.ptrs(fu, abi.P3), abi.ptr(fu[0]))

@@ -94,11 +94,12 @@ def test_every_library_call_has_the_c_arity():
     "stresstensor", "tensordivergence!", "tensordivergence_nonsym!", "divergence!", "pressuregradient!",
     "tophat", "gaussian", "composekernel", "kernelproduct", "get_kernels", "applyfilter!", "apply_ΔHπ",
     "getsetup", "dnsaid_project", "compute_errors", "spectrum", "dns_aided_les", "run_dns_aided_les",
-    "compute_fractions", "compute_dissipation", "solve_smagorinsky"])
+    "compute_fractions", "compute_dissipation", "solve_smagorinsky", "sfs_tensors_volume", "sfs_tensors_surface",
+    "tensordissipation_staggered!", "sfs_dissipation"])
 def test_reference_names_are_defined(name):
     txt = _module_text()
     pat = re.escape(name)
-    assert re.search(rf"(?m)^\s*(?:function\s+{pat}\(|{pat}\([^\n]*\)\s*=(?!=))", txt), name
+    assert re.search(rf"(?m)^\s*(?:function\s+{pat}\(|{pat}\([^)]*\)\s*=(?!=))", txt), name
 
 
 def test_entry_points_match_python_binding():

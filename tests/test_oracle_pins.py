@@ -612,3 +612,218 @@ def test_burgers_dns_aided_les_swap_exactness(O, nH, kind):
            for k in ("nomodel", "class_m", "class_p", "swapfil")}
     assert steps >= 10 and err["swapfil"] < 1e-14
     assert min(err["nomodel"], err["class_m"], err["class_p"]) > 1e-4
+
+
+# ---------------------------------------------------------------- SFS tensors and dissipation
+NONSYM = ["11", "21", "31", "12", "22", "32", "13", "23", "33"]
+
+
+def test_tensordissipation_summation_by_parts(O):
+    """sum_I d[I] = sum_I u . tensordivergence_nonsym(sigma): the staggered
+    sigma : grad u is minus the work of the staggered divergence, so a product
+    at the wrong edge, a derivative of the wrong sign or one cell off fails."""
+    n, l = 7, 2.0
+    rng = np.random.default_rng(3)
+    sig = [np.asfortranarray(rng.standard_normal((n, n, n))) for _ in range(9)]
+    u = rnd(n, 4)
+    d = O.tensordissipation(sig, u, l)
+    du = O.tensordivergence_nonsym(sig, l=l)
+    want = sum(float(np.sum(a * b)) for a, b in zip(u, du))
+    assert abs(d.sum() - want) < 1e-12 * np.abs(d).sum()
+
+
+@pytest.mark.parametrize("ab", NONSYM)
+def test_tensordissipation_locality(O, ab):
+    """A single nonzero sigma_ab at index J0 feeds exactly the centres its
+    position touches: the centre J0 for a diagonal entry (with delta_a u_a
+    there); for an edge x_J0 + (e_a + e_b) h/2 the four centres J0, J0 + e_a,
+    J0 + e_b, J0 + e_a + e_b in equal parts (with delta_b u_a on the edge)."""
+    n = 6
+    h = 1.0 / n
+    a, b = int(ab[0]) - 1, int(ab[1]) - 1
+    J0 = np.array([2, 3, 1])
+    sig = [np.asfortranarray(np.zeros((n, n, n))) for _ in range(9)]
+    sig[NONSYM.index(ab)][tuple(J0)] = 1.0
+    u = rnd(n, 5)
+    d = O.tensordissipation(sig, u)
+    ea, eb = np.eye(3, dtype=int)[a], np.eye(3, dtype=int)[b]
+    if a == b:
+        expect = {tuple(J0)}
+        val = (u[a][tuple(J0)] - u[a][tuple(J0 - ea)]) / h
+    else:
+        expect = {tuple(J0 + sa * ea + sb * eb) for sa in (0, 1) for sb in (0, 1)}
+        val = (u[a][tuple(J0 + eb)] - u[a][tuple(J0)]) / h / 4
+    got = {tuple(int(x) for x in I) for I in zip(*np.nonzero(np.abs(d) > 1e-14))}
+    assert got == expect
+    for I in expect:
+        assert abs(d[I] - val) < 1e-12 * abs(val)
+
+
+@pytest.mark.parametrize("experiment", ["volavg", "project_volavg", "surfavg"])
+def test_sfs_tensors_vanish_without_compression(O, experiment):
+    """With n_les = n_dns every filter is the identity and the LES pressure
+    is the DNS one: all three sub-filter stresses vanish (a window one cell
+    off, a missing pressure or a wrong source does not)."""
+    n = 6
+    u = O.randomfield(n, seed=2, totalenergy=1.0)
+    t = O.sfs_tensors(u, n, 0.01, experiment)
+    scale = max(np.abs(x).max() for x in O.stresstensor(u, 0.01))
+    for k in ("classic", "swapfil", "swapfil_symm"):
+        assert max(np.abs(x).max() for x in t[k]) < 1e-12 * scale, k
+
+
+@pytest.mark.parametrize("experiment,c", [("volavg", 2), ("volavg", 3), ("project_volavg", 2),
+                                          ("project_volavg", 3)])
+def test_sfs_swapfil_is_the_exact_closure(O, experiment, c):
+    """-div^ns sigma_swapfil = V(P du) - P_H du_les: the volume swap-filter
+    stress is the exact closure of the volume-filtered LES (P du the projected
+    DNS right-hand side, du_les the LES right-hand side of fu), which fixes
+    every window and position of the swap filter, its pressure and the sign
+    of the LES stress."""
+    n = 4 * c if c % 2 == 0 else 5 * c
+    nb = n // c
+    visc = 0.01
+    u = O.randomfield(n, seed=7, totalenergy=1.0)
+    t = O.sfs_tensors(u, nb, visc, experiment)
+    du = O.rhs(u, visc)
+    O.project(du)
+    F = O.closure_filters(c, "va")
+    fpdu = [O.filter_coarse(du[a], nb, *F[f"ubar{a + 1}"][1]) for a in range(3)]
+    dul = O.rhs(t["fu"], visc)
+    O.project(dul)
+    lhs = O.tensordivergence_nonsym(t["swapfil"])
+    for a in range(3):
+        assert np.abs(lhs[a] - (fpdu[a] - dul[a])).max() < 1e-11 * np.abs(fpdu[a]).max()
+    # the classic stress is not exact there
+    cl = O.tensordivergence_nonsym(t["classic"])
+    assert max(np.abs(cl[a] - (fpdu[a] - dul[a])).max() / np.abs(fpdu[a]).max() for a in range(3)) > 1e-3
+
+
+@pytest.mark.parametrize("c", [2, 3])
+def test_sfs_volume_filters_tile_the_domain(O, c):
+    """The volume filters (of u and of the tensor r) are partitions of unity
+    over the DNS grid: their LES means equal the DNS means, and the classic
+    filter of the symmetric r is symmetric."""
+    n = 4 * c
+    nb = n // c
+    rng = np.random.default_rng(c)
+    src = {k: np.asfortranarray(rng.standard_normal((n, n, n)))
+           for k in ("u1", "u2", "u3", "r11", "r22", "r33", "s12", "s23", "s31")}
+    q = {}
+    for name, (s, fs) in O.sfs_filters(c, "volume").items():
+        if name.startswith(("fu", "fr")):
+            q[name] = O.filter_coarse(src[s], nb, *fs)
+            assert abs(q[name].mean() - src[s].mean()) < 1e-13, name
+    for ab in NONSYM:
+        assert np.array_equal(q["fr" + ab], q["fr" + ab[::-1]]), ab
+
+
+@pytest.mark.parametrize("c", [2, 3])
+def test_sfs_surface_line_flux_is_exact_across_the_face(O, c):
+    """Face average: the transverse part of the swap flux (the line averages
+    off the diagonal) is exact, F(-sum_{b != a} delta_b r_ab) =
+    -sum_{b != a} delta_b^H (L r)_ab (the face sum of a transverse derivative
+    telescopes to the face's edges).  The normal part is not (the face
+    average of delta_a r_aa is a fine-scale difference), which is why the
+    surface swap-filter LES has an error at all."""
+    n = 4 * c if c % 2 == 0 else 5 * c
+    nb = n // c
+    rng = np.random.default_rng(11 + c)
+    z = np.asfortranarray(np.zeros((n, n, n)))
+    off = [np.asfortranarray(rng.standard_normal((n, n, n))) for _ in range(3)]  # r12, r23, r31
+    du = O.tensordivergence([z, z, z] + off)                                    # -div r, diagonal zero
+    F = O.closure_filters(c, "fa")
+    Fdu = [O.filter_coarse(du[a], nb, *F[f"ubar{a + 1}"][1]) for a in range(3)]
+    src = {"s12": off[0], "s23": off[1], "s31": off[2]}
+    q = O.sfs_filters(c, "surface")
+    zc = np.asfortranarray(np.zeros((nb, nb, nb)))
+    Lr = [zc if ab[0] == ab[1] else O.filter_coarse(src[q["sr" + ab][0]], nb, *q["sr" + ab][1]) for ab in NONSYM]
+    lhs = O.tensordivergence_nonsym(Lr)
+    for a in range(3):
+        assert np.abs(lhs[a] - Fdu[a]).max() < 1e-12 * np.abs(Fdu[a]).max()
+
+
+def _stagger(source):
+    """Fine position offset (in cells) of a source along each axis: 1 at a
+    face, 1/2 at a centre (u_a at face a; sigma_ab, a != b, at faces a and b;
+    the diagonal r_aa at the centre)."""
+    if source.startswith("u"):
+        a = int(source[1]) - 1
+        return [1.0 if d == a else 0.5 for d in range(3)]
+    ab = source[1:]
+    a, b = int(ab[0]) - 1, int(ab[1]) - 1
+    if a == b:
+        return [0.5] * 3
+    return [1.0 if d in (a, b) else 0.5 for d in range(3)]
+
+
+def _region(name, kind):
+    """(LES position offset in coarse cells, region width in coarse cells) per
+    axis of one sfs_tensors_* quantity, as the filter's definition states it:
+    volume filters average the coarse-cell-sized box around the LES position;
+    a surface filter is one plane normal to its direction (width 0); the
+    swap filters sample the plane normal to the derivative direction b
+    (volume) or the face normal to a and, off the diagonal, the line along
+    the third direction (surface)."""
+    if name.startswith("fu"):
+        a = int(name[2]) - 1
+        pos = [1.0 if d == a else 0.5 for d in range(3)]
+        width = [1.0] * 3 if kind == "volume" else [0.0 if d == a else 1.0 for d in range(3)]
+        return pos, width
+    ab = name[2:]
+    a, b = int(ab[0]) - 1, int(ab[1]) - 1
+    pos = [0.5] * 3 if a == b else [1.0 if d in (a, b) else 0.5 for d in range(3)]
+    if kind == "volume":
+        width = [1.0] * 3 if name.startswith("fr") else [0.0 if d == b else 1.0 for d in range(3)]
+    elif name.startswith("fr") or a == b:
+        width = [0.0 if d == a else 1.0 for d in range(3)]
+    else:
+        width = [0.0 if d in (a, b) else 1.0 for d in range(3)]
+    return pos, width
+
+
+def _region_weights(n, c, J, pos, width, s):
+    """Trapezoid weights of the fine samples x_i = i + s (cells) over the
+    region [X - w c/2, X + w c/2], X = c (J + pos): 1/c inside, 1/(2c) on the
+    ends; a plane (w = 0) takes the sample at X or the mean of the two at
+    X -+ 1/2."""
+    X = c * (J + pos)
+    w = {}
+    for i in range(-2 * n, 3 * n):
+        dx = abs(i + s - X)
+        if width > 0:
+            half = width * c / 2
+            wt = 1.0 / c if dx < half - 1e-9 else (0.5 / c if abs(dx - half) < 1e-9 else 0.0)
+        else:
+            wt = 1.0 if dx < 1e-9 else (0.5 if abs(dx - 0.5) < 1e-9 else 0.0)
+        if wt:
+            w[i] = w.get(i, 0.0) + wt
+    return w
+
+
+@pytest.mark.parametrize("kind", ["volume", "surface"])
+@pytest.mark.parametrize("c", [2, 3, 4])
+def test_sfs_filters_follow_their_regions(O, kind, c):
+    """Every quantity of sfs_filters on a field that varies along one axis
+    only equals the trapezoid average of the fine samples over the region
+    its definition names (position and width per axis), for every axis and
+    a random profile: this fixes each window, offset and end weight."""
+    n = 3 * c
+    nb = n // c
+    rng = np.random.default_rng(c)
+    for name, (source, fs) in O.sfs_filters(c, kind).items():
+        pos, width = _region(name, kind)
+        s = _stagger(source if not source.startswith("r") else "r" + name[2:])
+        for d in range(3):
+            prof = rng.standard_normal(n)
+            shape = [1, 1, 1]
+            shape[d] = n
+            f = np.asfortranarray(np.broadcast_to(prof.reshape(shape), (n, n, n)))
+            got = O.filter_coarse(f, nb, *fs)
+            idx = [0, 0, 0]
+            for J in range(nb):
+                idx[d] = J
+                wts = _region_weights(n, c, J, pos[d], width[d], s[d])
+                assert abs(sum(wts.values()) - 1) < 1e-12, (name, d)
+                want = sum(wt * prof[i % n] for i, wt in wts.items())
+                assert abs(got[tuple(idx)] - want) < 1e-12, (name, d, J)
