@@ -215,10 +215,25 @@ def rfft(x, poisson):
     import torch
 
     n = poisson.grid.n
+    _check_packed(x, (n, n, n), poisson)
     cdt = torch.complex128 if x.dtype == torch.float64 else torch.complex64
     out = torch.empty((n, n, n // 2 + 1), dtype=cdt, device="cuda")
     _ctx().call("ec_fft_r2c", poisson.h, ptr(x), ptr(out))
     return out
+
+
+def _check_packed(t, shape, poisson):
+    """The library reads and writes packed (contiguous) arrays of the solver's
+    precision; a strided view (e.g. a spectrum from another FFT library) is
+    refused rather than read with the wrong layout."""
+    import torch
+
+    if tuple(t.shape) != shape or not t.is_contiguous():
+        raise ValueError(f"expected a contiguous {shape} tensor, got shape {tuple(t.shape)} "
+                         f"strides {t.stride()} (call .contiguous())")
+    real = torch.float64 if poisson.torch_dtype == torch.float64 else torch.float32
+    if t.dtype not in (real, torch.complex128 if real == torch.float64 else torch.complex64):
+        raise ValueError(f"dtype {t.dtype} does not match the solver's {poisson.torch_dtype}")
 
 
 def irfft(X, poisson):
@@ -226,6 +241,7 @@ def irfft(X, poisson):
     import torch
 
     n = poisson.grid.n
+    _check_packed(X, (n, n, n // 2 + 1), poisson)
     rdt = torch.float64 if X.dtype == torch.complex128 else torch.float32
     out = torch.empty((n, n, n), dtype=rdt, device="cuda")
     _ctx().call("ec_fft_c2r", poisson.h, ptr(X), ptr(out))
