@@ -28,6 +28,12 @@ so every oracle function is pinned by checks chosen so a plausible mistake
                dissipation coefficients: the pinned parts times a closed-form
                coarse velocity gradient, mirror symmetry
   spectrum     full-FFT shells, Parseval, single modes
+  steppers     observed order under dt-halving: forward Euler 1, Wray RK3 3,
+               the NS RK4 tableau 4 (mutated tableaus fail)
+  SFS          volume swap-filter stress = exact closure; surface line flux
+               exact across the face; every filter window from its region
+               definition; all stresses vanish at n_les = n_dns; dissipation by
+               summation by parts and locality
 """
 import math
 
