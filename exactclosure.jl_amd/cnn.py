@@ -76,7 +76,10 @@ class CNNClosure:
     # -- forward / backward / update -----------------------------------------
     def forward(self, inputs, out=None):
         """Run the network on a batch; `out` (optional) receives the predictions
-        in physical units (3 FP64 fields per sample)."""
+        in physical units (3 FP64 fields per sample).  Inputs stored in another
+        precision (BF16 or FP32 pairs) are widened to FP64 per batch."""
+        inputs = [_f64(s) for s in inputs]
+        self._keep = inputs  # the widened batch must outlive the asynchronous launch
         n = inputs[0][0].shape[0]
         flat = [t for s in inputs for t in s]
         for t in flat:
@@ -90,6 +93,7 @@ class CNNClosure:
 
     def backward(self, targets):
         """Gradients of the mean squared error (network units) of the last forward; returns the loss."""
+        targets = [_f64(s) for s in targets]
         flat = [t for s in targets for t in s]
         loss = ctypes.c_double()
         self.ctx.call("ec_cnn_backward", self.h, _ptr_array(flat), float(self.out_scale), ctypes.byref(loss))
@@ -128,32 +132,38 @@ class CNNClosure:
         return hist
 
 
+def _f64(sample):
+    return [t if t.dtype == torch.float64 else t.double().contiguous() for t in sample]
+
+
 def _rms(samples):
     s, c = 0.0, 0
     for smp in samples:
         for t in smp:
-            s += float((t * t).sum())
+            s += float(t.double().square().sum())
             c += t.numel()
     return (s / c) ** 0.5
 
 
 def closure_error(pred, exact):
-    """Relative L2 error of a predicted closure against the exact one (FP64)."""
-    num = sum(float(((p - e) ** 2).sum()) for p, e in zip(pred, exact))
-    den = sum(float((e ** 2).sum()) for e in exact)
+    """Relative L2 error of a predicted closure against the exact one (FP64 sums)."""
+    num = sum(float((p.double() - e.double()).square().sum()) for p, e in zip(pred, exact))
+    den = sum(float(e.double().square().sum()) for e in exact)
     return (num / den) ** 0.5
 
 
-def generate_pairs(drv, set_index, dt, nsteps, every=1, log=None):
+def generate_pairs(drv, set_index, dt, nsteps, every=1, log=None, dtype=None):
     """Run the DNS-aided driver (forcing form) and collect (P_H V u, exact
-    closure forcing m) of LES set `set_index` after every `every` steps."""
+    closure forcing m) of LES set `set_index` after every `every` steps,
+    stored in `dtype` (default: the LES precision, FP64; BF16 quarters the
+    footprint of a large training set, 25 MB per pair at 128^3)."""
     X, Y = [], []
     L = drv.les[set_index]
     for s in range(nsteps):
         drv.step(dt)
         if (s + 1) % every == 0:
-            X.append([t.clone() for t in L.ubar])
-            Y.append([t.clone() for t in L.m])
+            X.append([t.to(dtype) if dtype else t.clone() for t in L.ubar])
+            Y.append([t.to(dtype) if dtype else t.clone() for t in L.m])
         if log and (s + 1) % 64 == 0:
             torch.cuda.synchronize()
             log(f"[pairs] step {s + 1}/{nsteps}: {len(X)} pairs")

@@ -705,16 +705,23 @@ extern "C" int ec_ns_rhs_div(ec_ctx *ctx, int dtype, int n, double l, double vis
     const double ih = n / l;
     const double b = dtype == EC_F64 ? 8 : 4;
     ec_prof_scope ps(ctx, "ns_rhs_div", 7.0 * ec_cube(n) * b);
-    // FP32 takes 64 x 16 tiles (halo rows 19/16 of the tile instead of 11/8;
-    // 78 KB of LDS, two blocks per CU; 3% faster at 1024^3); FP64 keeps 64 x 8,
-    // whose 88 KB already hold it to one block per CU
+    // FP32 takes 128 x 16 tiles when n allows (the 3-cell x halo touches two
+    // partial 128-byte lines per row of 4 instead of per row of 2; 150 KB of
+    // LDS, one block of 1024 threads per CU) and 64 x 16 otherwise (halo rows
+    // 19/16 of the tile instead of 11/8; 78 KB, two blocks per CU); FP64 keeps
+    // 64 x 8, whose 88 KB already hold it to one block per CU
     const int TY = dtype == EC_F64 ? 8 : 16;
     const int ZC = march_zc(n, TY);
-    const unsigned tiles = (unsigned)((n / 64) * (n / TY) * (n / ZC));
+    const bool wide = dtype == EC_F32 && n % 128 == 0 && n >= 512;
+    const unsigned tiles = (unsigned)((n / (wide ? 128 : 64)) * (n / TY) * (n / ZC));
     if (dtype == EC_F64)
         k_rhs_div_march<double, 64, 8><<<tiles, 256, 0, ctx->stream>>>(
             n, ZC, ih, visc, (const double *)u[0], (const double *)u[1], (const double *)u[2], (double *)du[0],
             (double *)du[1], (double *)du[2], (double *)d, rdiv * ih);
+    else if (wide)
+        k_rhs_div_march<float, 128, 16, 1024><<<tiles, 1024, 0, ctx->stream>>>(
+            n, ZC, (float)ih, (float)visc, (const float *)u[0], (const float *)u[1], (const float *)u[2],
+            (float *)du[0], (float *)du[1], (float *)du[2], (float *)d, (float)(rdiv * ih));
     else
         k_rhs_div_march<float, 64, 16, 512><<<tiles, 512, 0, ctx->stream>>>(
             n, ZC, (float)ih, (float)visc, (const float *)u[0], (const float *)u[1], (const float *)u[2],

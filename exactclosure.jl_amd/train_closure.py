@@ -1,11 +1,13 @@
 """Train the learned closure on pairs generated by the DNS-aided run itself.
 
-    python exactclosure.jl_amd/train_closure.py [--n 1024] [--nbar 128] [--pairs 256]
-        [--every 4] [--epochs 20] [--batch 8] [--lr 1e-3] [--post-steps 50]
+    python exactclosure.jl_amd/train_closure.py [--n 1024] [--nbar 128] [--pairs 2000]
+        [--every 1] [--epochs 20] [--batch 8] [--lr 1e-3] [--post-steps 500]
+        [--store bf16] [--fallback-tol auto] [--out results.json]
 
 1. DNS-aided LES (FP32 DNS, forcing-form exact closure, volume average) at
    n^3 with one LES grid of nbar^3; every `every` steps the pair
-   (P_H V u, exact closure forcing m) of that grid is kept on the GPU.
+   (P_H V u, exact closure forcing m) of that grid is kept on the GPU, in BF16
+   by default (2000 pairs at 128^3: 50 GB).
 2. The CNN (3 -> 32 -> 32 -> 32 -> 3, 3x3x3, GELU, periodic) is trained with
    Adam on 7/8 of the pairs (the earliest), in BF16 with FP32 accumulation.
 3. The closure error of the held-out (latest) pairs is measured against the exact
@@ -13,8 +15,9 @@
 4. A posteriori: from the filtered DNS state, three LES run `post-steps` RK4
    steps beside the DNS: no model, the network, and the network with the
    exact-closure fallback (a stage whose prediction misses the exact closure
-   by more than --fallback-tol uses the exact closure).
-Prints one JSON line with the results.
+   by more than --fallback-tol uses the exact closure; "auto" takes the median
+   held-out closure error, so the fallback is exercised).
+Prints one JSON line with the results (and writes it to --out when given).
 """
 from __future__ import annotations
 
@@ -32,13 +35,15 @@ def main():
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--nbar", type=int, default=128)
     ap.add_argument("--visc", type=float, default=1.0 / 8000)
-    ap.add_argument("--pairs", type=int, default=256)
-    ap.add_argument("--every", type=int, default=4)
+    ap.add_argument("--pairs", type=int, default=2000)
+    ap.add_argument("--every", type=int, default=1)
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--post-steps", type=int, default=50)
-    ap.add_argument("--fallback-tol", type=float, default=0.5)
+    ap.add_argument("--post-steps", type=int, default=500)
+    ap.add_argument("--fallback-tol", default="auto", help="relative closure error, or auto (held-out median)")
+    ap.add_argument("--store", choices=("bf16", "f32", "f64"), default="bf16", help="precision of the stored pairs")
+    ap.add_argument("--out", default=None, help="also write the JSON result to this file")
     ap.add_argument("--seed", type=int, default=123)
     a = ap.parse_args()
 
@@ -62,7 +67,8 @@ def main():
     drv.set_state(u32)
     del u32
     t0 = time.perf_counter()
-    X, Y = CN.generate_pairs(drv, 0, dt, a.pairs * a.every, every=a.every, log=log)
+    store = {"bf16": torch.bfloat16, "f32": torch.float32, "f64": None}[a.store]
+    X, Y = CN.generate_pairs(drv, 0, dt, a.pairs * a.every, every=a.every, log=log, dtype=store)
     torch.cuda.synchronize()
     t_gen = time.perf_counter() - t0
     log(f"[train] {len(X)} pairs in {t_gen:.1f} s ({a.pairs * a.every} DNS steps at {a.n}^3)")
@@ -73,31 +79,38 @@ def main():
     hist = net.fit(X[:ntr], Y[:ntr], epochs=a.epochs, batch=a.batch, lr=a.lr, seed=0, log=log)
     torch.cuda.synchronize()
     t_fit = time.perf_counter() - t0
-    errs = [CN.closure_error(net.predict(x), y) for x, y in zip(X[ntr:], Y[ntr:])]
+    errs = [CN.closure_error(net.predict(CN._f64(x)), y) for x, y in zip(X[ntr:], Y[ntr:])]
     log(f"[train] fit {t_fit:.1f} s, held-out closure error {sum(errs) / len(errs):.3f}")
     del X, Y
     torch.cuda.empty_cache()
+    tol = sorted(errs)[len(errs) // 2] if a.fallback_tol == "auto" else float(a.fallback_tol)
 
     plain = CN.LearnedLES(drv, 0, None)
     learned = CN.LearnedLES(drv, 0, net, fallback_tol=None)
-    guarded = CN.LearnedLES(drv, 0, net, fallback_tol=a.fallback_tol)
+    guarded = CN.LearnedLES(drv, 0, net, fallback_tol=tol)
     drv.stage_hooks = [plain, learned, guarded]
     post = []
     for s in range(a.post_steps):
         drv.step(dt)
-        if (s + 1) % max(1, a.post_steps // 5) == 0:
+        if (s + 1) % max(1, a.post_steps // 10) == 0:
             post.append({"step": s + 1, "no_model": plain.error(), "cnn": learned.error(),
                          "cnn_fallback": guarded.error()})
             log(f"[post] {post[-1]}")
     drv.stage_hooks = []
     flops = 2 * a.nbar ** 3 * 27 * (3 * 32 + 2 * 32 * 32 + 32 * 3)
     out = {"dns": a.n, "les": a.nbar, "pairs": len(errs) + ntr, "train_pairs": ntr, "epochs": a.epochs,
-           "batch": a.batch, "lr": a.lr, "generate_s": t_gen, "fit_s": t_fit,
+           "batch": a.batch, "lr": a.lr, "store": a.store, "generate_s": t_gen, "fit_s": t_fit,
            "train_tflops": 3 * flops * ntr * a.epochs / t_fit / 1e12,
            "loss_first_epoch": hist[0], "loss_last_epoch": hist[-1],
            "heldout_closure_error": sum(errs) / len(errs), "heldout_closure_error_no_model": 1.0,
-           "a_posteriori": post, "fallback_stages": guarded.fallbacks, "model_stages": guarded.stages}
-    print(json.dumps(out), flush=True)
+           "heldout_closure_error_range": [min(errs), max(errs)], "fallback_tol": tol,
+           "a_posteriori": post, "fallback_stages": guarded.fallbacks, "model_stages": guarded.stages,
+           "post_steps": a.post_steps, "dt": dt, "visc": a.visc}
+    line = json.dumps(out)
+    print(line, flush=True)
+    if a.out:
+        with open(a.out, "w") as fh:
+            fh.write(line + "\n")
 
 
 if __name__ == "__main__":

@@ -102,3 +102,36 @@ def test_cnn_learns_the_closure(gpu):
     drv.run(dt, 10)
     drv.stage_hooks = []
     assert learned.error() < plain.error(), (learned.error(), plain.error())
+
+
+def test_cnn_exact_closure_fallback(gpu):
+    """The exact-closure fallback of the a posteriori LES (the arbiter of the
+    learned closure): with a tolerance below every prediction's error each
+    stage falls back and the LES follows the driver's exact LES (the filtered
+    DNS) to FP64 rounding; with an infinite tolerance no stage does and the LES
+    equals the pure network LES bit for bit.  Pairs stored in BF16, as the
+    large training set is, train the network."""
+    NS, D, CN = gpu.navierstokes, gpu.dnsaid, gpu.cnn
+    n, visc = 64, 1.0 / 1000
+    g = NS.Grid(1.0, n)
+    ps = NS.PoissonSolver(g, torch.float64)
+    u = NS.randomfield(g, ps, seed=9)
+    dt = NS.propose_timestep(u, visc, 0.4)
+    drv = D.DNSAidedLES(g, visc, [(16, "va")], dtype=torch.float64)
+    drv.set_state(u)
+    X, Y = CN.generate_pairs(drv, 0, dt, 16, dtype=torch.bfloat16)
+    assert X[0][0].dtype == torch.bfloat16 and Y[0][0].dtype == torch.bfloat16
+    net = CN.CNNClosure(seed=3)
+    hist = net.fit(X, Y, epochs=10, batch=4, lr=2e-3, seed=0)
+    assert hist[-1] < hist[0]
+    learned = CN.LearnedLES(drv, 0, net, fallback_tol=None)
+    every = CN.LearnedLES(drv, 0, net, fallback_tol=0.0)
+    never = CN.LearnedLES(drv, 0, net, fallback_tol=float("inf"))
+    drv.stage_hooks = [learned, every, never]
+    drv.run(dt, 5)
+    drv.stage_hooks = []
+    print(f"\n[fallback] learned {learned.error():.3e}, always {every.error():.3e}, never {never.error():.3e}")
+    assert every.fallbacks == every.stages == 20
+    assert every.error() < 1e-12
+    assert never.fallbacks == 0 and never.stages == 20
+    assert never.error() == learned.error() and learned.error() > 1e-6
