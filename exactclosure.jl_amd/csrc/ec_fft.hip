@@ -735,7 +735,12 @@ __global__ void __launch_bounds__(TK *NTHR) k_fft_cols_ct(cplx<T> *__restrict__ 
 #pragma unroll
     for (int idx = threadIdx.x; idx < TOTAL; idx += TK * NTHR) {
         const int cc = idx % TK, e = idx / TK;
-        if (full || col0 + cc < ncols) ec_nt_store(base + cc + e * elem_stride, lds[pad8(e) * TK + cc]);
+        if (full || col0 + cc < ncols) {
+            // rows shorter than a 128-byte line: plain stores, so that L2 joins
+            // the halves the neighbouring tile writes before the line goes out
+            if constexpr (TK * sizeof(cplx<T>) < 128) base[cc + e * elem_stride] = lds[pad8(e) * TK + cc];
+            else ec_nt_store(base + cc + e * elem_stride, lds[pad8(e) * TK + cc]);
+        }
     }
 }
 
