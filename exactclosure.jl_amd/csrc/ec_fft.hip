@@ -788,7 +788,10 @@ __global__ void __launch_bounds__(TK * 64) k_fft_cols_w(cplx<T> *__restrict__ da
 #pragma unroll
     for (int q = 0; q < PER; q++) {
         const int idx = threadIdx.x + q * TK * 64, cc = idx % TK, e = idx / TK;
-        if (full || col0 + cc < ncols) ec_nt_store(base + cc + e * elem_stride, lds[cc * CS + pad8(e)]);
+        if (full || col0 + cc < ncols) {
+            if constexpr (TK * sizeof(cplx<T>) < 128) base[cc + e * elem_stride] = lds[cc * CS + pad8(e)];
+            else ec_nt_store(base + cc + e * elem_stride, lds[cc * CS + pad8(e)]);
+        }
     }
 }
 

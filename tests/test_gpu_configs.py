@@ -176,3 +176,53 @@ def test_fp32_divergence_bounded_over_many_steps(gpu, n):
     assert d240 <= 2.5 * d24
     assert d240 <= res[False][1]
     assert exact < 1e-5
+
+
+def test_config_1024_fp32_step_graph(gpu):
+    """The benchmark's own step at its size: 1024^3 FP32 DNS with the 64^3,
+    128^3, 256^3 volume-average and 256^3 face-average LES grids, RK4 at the
+    CFL time step, four graph-replayed steps.  Size-independent properties:
+    every LES stays on the filtered DNS (exactness, FP32 rounding), the DNS
+    stays divergence-free (<= 1e-6), its energy decays (no forcing), and the
+    FP32 closure of the final state matches the all-FP64 evaluation to 1e-5."""
+    NS, D = gpu.navierstokes, gpu.dnsaid
+    n, visc = 1024, 1.0 / 8000
+    sets = [(64, "va"), (128, "va"), (256, "va"), (256, "fa")]
+    g = NS.Grid(1.0, n)
+    ps64 = NS.PoissonSolver(g, torch.float64)
+    u64 = NS.randomfield(g, ps64, "peak", 5.0, 0.5, seed=123)
+    dt = NS.propose_timestep(u64, visc, 0.4)
+    u32 = [x.data.float() for x in u64]
+    del u64, ps64
+    torch.cuda.empty_cache()
+    drv = D.DNSAidedLES(g, visc, sets, dtype=torch.float32)
+    drv.set_state(u32)
+    del u32
+    e0 = sum(NS.sumsq(x) for x in drv.u0)
+    drv.capture(dt)
+    for _ in range(4):
+        drv.step(dt)
+    les = drv.les_errors()
+    div = drv.divergence_norm()
+    e4 = sum(NS.sumsq(x) for x in drv.u0)
+    taus32 = drv.closure_terms()
+    u_state = [x.double() for x in drv.u0]
+    drv.release_graphs()
+    del drv
+    torch.cuda.empty_cache()
+    d64 = D.DNSAidedLES(g, visc, sets, dtype=torch.float64)
+    taus64 = d64.closure_terms(u_state)
+    num = den = 0.0
+    for s32, s64 in zip(taus32, taus64):
+        for x, y in zip(s32, s64):
+            num += float(((x - y) ** 2).sum())
+            den += float((y ** 2).sum())
+    closure = (num / den) ** 0.5
+    print(f"\n[config 1024^3 FP32 step] LES vs filtered DNS {les}, divergence {div:.3e}, "
+          f"energy ratio {e4 / e0:.6f}, closure rel err vs FP64 {closure:.3e}")
+    del d64, taus64, taus32, u_state
+    torch.cuda.empty_cache()
+    assert max(les) < 1e-6
+    assert div < 1e-6
+    assert 0.99 < e4 / e0 < 1.0
+    assert closure < 1e-5
