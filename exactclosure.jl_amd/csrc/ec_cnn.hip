@@ -46,14 +46,17 @@ __device__ __forceinline__ float bf16_float(uint16_t u) { return __uint_as_float
 
 // ---------------------------------------------------------------------------
 // forward / dgrad convolution
-// tile 8 x 8 x 4 points (256 threads = 4 waves, wave w owns z = w, two
-// M-blocks of 8 x 4 points); halo tile 10 x 10 x 6 in LDS, all weights in LDS
-constexpr int FX = 8, FY = 8, FZ = 4, FHX = 10, FHY = 10, FHZ = 6, FHP = FHX * FHY * FHZ;
+// tile 8 x 8 x 8 points (512 threads = 8 waves, wave w owns z = w, two
+// M-blocks of 8 x 4 points); halo tile 10 x 10 x 10 in LDS, all weights in LDS
+// (146 KB for 32 channels: one block of eight waves per CU, the weights loaded
+// once per 512 points)
+constexpr int FX = 8, FY = 8, FZ = 8, FHX = 10, FHY = 10, FHZ = 10, FHP = FHX * FHY * FHZ;
+constexpr int FT = 64 * FZ;  // threads per block: one wave per z-plane of the tile
 
 enum { CONV_HIDDEN = 0, CONV_OUT = 1, CONV_DGRAD = 2 };
 
 template <int CP, int MODE>
-__global__ void __launch_bounds__(256) k_conv(const uint16_t *__restrict__ X, const uint16_t *__restrict__ W,
+__global__ void __launch_bounds__(FT) k_conv(const uint16_t *__restrict__ X, const uint16_t *__restrict__ W,
                                               const float *__restrict__ bias, int n, int tiles_per_sample,
                                               uint16_t *__restrict__ out_a, uint16_t *__restrict__ out_b,
                                               float *__restrict__ out_f, const uint16_t *__restrict__ pre_in) {
@@ -70,11 +73,11 @@ __global__ void __launch_bounds__(256) k_conv(const uint16_t *__restrict__ X, co
     const size_t n3 = (size_t)n * n * n, sbase = (size_t)s * n3;
 
     constexpr int CH8 = CP / 8;  // 16-byte chunks per point
-    for (int i = tid; i < 27 * CH * CH8; i += 256) {
+    for (int i = tid; i < 27 * CH * CH8; i += FT) {
         const int row = i / CH8, part = i - row * CH8;
         *(uint4 *)(ws + row * SP + part * 8) = ((const uint4 *)W)[i];
     }
-    for (int i = tid; i < FHP * CH8; i += 256) {
+    for (int i = tid; i < FHP * CH8; i += FT) {
         const int pt = i / CH8, part = i - pt * CH8;
         const int hx = pt % FHX, hy = (pt / FHX) % FHY, hz = pt / (FHX * FHY);
         const size_t g = (size_t)wrapn(x0 + hx - 1, n) + (size_t)n * ((size_t)wrapn(y0 + hy - 1, n) +
@@ -104,23 +107,73 @@ __global__ void __launch_bounds__(256) k_conv(const uint16_t *__restrict__ X, co
     // D layout: lane l, register r -> point row i = 8 (r / 4) + 4 kh + r % 4, channel m
     const int co = m;
     const float bco = (MODE == CONV_DGRAD) ? 0.0f : bias[co];
+    if constexpr (MODE == CONV_OUT) {
 #pragma unroll
-    for (int mb = 0; mb < 2; mb++) {
+        for (int mb = 0; mb < 2; mb++) {
 #pragma unroll
-        for (int r = 0; r < 16; r++) {
-            const int i = 8 * (r >> 2) + 4 * kh + (r & 3);
-            const int x = x0 + (i & 7), y = y0 + 4 * mb + (i >> 3), z = z0 + w;
-            const size_t p = sbase + (size_t)x + (size_t)n * ((size_t)y + (size_t)n * (size_t)z);
-            const float v = (mb ? acc1[r] : acc0[r]) + bco;
-            if constexpr (MODE == CONV_HIDDEN) {
-                out_a[p * CH + co] = bf16_bits(v);        // pre-activation
-                out_b[p * CH + co] = bf16_bits(gelu(v));  // activation
-            } else if constexpr (MODE == CONV_OUT) {
-                if (co < 3) out_f[p * 4 + co] = v;
-            } else {
-                out_a[p * CH + co] = bf16_bits(v * gelu_grad(bf16_float(pre_in[p * CH + co])));
+            for (int r = 0; r < 16; r++) {
+                const int i = 8 * (r >> 2) + 4 * kh + (r & 3);
+                const int x = x0 + (i & 7), y = y0 + 4 * mb + (i >> 3), z = z0 + w;
+                const size_t p = sbase + (size_t)x + (size_t)n * ((size_t)y + (size_t)n * (size_t)z);
+                if (co < 3) out_f[p * 4 + co] = (mb ? acc1[r] : acc0[r]) + bco;
             }
         }
+        return;
+    }
+    // BF16 outputs go through LDS: the wave's 64 points x 32 channels (4 KB,
+    // a wave-private slice of the halo tile, free once every wave is past the
+    // taps) are written in the D layout and stored as 16-byte chunks, four per
+    // lane (64 contiguous bytes per point, eight points per row of the tile)
+    // instead of 32 two-byte stores per lane and output.
+    static_assert(FZ * 64 * CH <= FHP * (16 + 8), "staging slices fit the halo tile");
+    uint16_t *stg = xs + w * (64 * CH);
+    auto chunk_addr = [&](int c) {  // chunk c of the wave: point q = c / 4, 8 channels from 8 (c % 4)
+        const int q = c >> 2;
+        const int x = x0 + (q & 7), y = y0 + (q >> 3), z = z0 + w;
+        return (sbase + (size_t)x + (size_t)n * ((size_t)y + (size_t)n * (size_t)z)) * CH + (size_t)(c & 3) * 8;
+    };
+    auto flush = [&](uint16_t *dst) {  // stg -> global, after a barrier
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int c = l + 64 * k;
+            *(uint4 *)(dst + chunk_addr(c)) = *(const uint4 *)(stg + c * 8);
+        }
+    };
+    __syncthreads();  // every wave is past the taps: the halo tile is free
+    if constexpr (MODE == CONV_DGRAD) {
+        // the saved pre-activations of the same points, staged the same way
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int c = l + 64 * k;
+            *(uint4 *)(stg + c * 8) = *(const uint4 *)(pre_in + chunk_addr(c));
+        }
+        __syncthreads();
+    }
+    float v[2][16];
+#pragma unroll
+    for (int mb = 0; mb < 2; mb++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int q = 32 * mb + 8 * (r >> 2) + 4 * kh + (r & 3);
+            v[mb][r] = (mb ? acc1[r] : acc0[r]) + bco;
+            if constexpr (MODE == CONV_DGRAD) v[mb][r] *= gelu_grad(bf16_float(stg[q * CH + co]));
+        }
+    if constexpr (MODE == CONV_DGRAD) __syncthreads();
+#pragma unroll
+    for (int mb = 0; mb < 2; mb++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) stg[(32 * mb + 8 * (r >> 2) + 4 * kh + (r & 3)) * CH + co] = bf16_bits(v[mb][r]);
+    __syncthreads();
+    flush(out_a);  // pre-activation (hidden) or input gradient (dgrad)
+    if constexpr (MODE == CONV_HIDDEN) {
+        __syncthreads();
+#pragma unroll
+        for (int mb = 0; mb < 2; mb++)
+#pragma unroll
+            for (int r = 0; r < 16; r++)
+                stg[(32 * mb + 8 * (r >> 2) + 4 * kh + (r & 3)) * CH + co] = bf16_bits(gelu(v[mb][r]));
+        __syncthreads();
+        flush(out_b);  // activation
     }
 }
 
@@ -129,6 +182,10 @@ __global__ void __launch_bounds__(256) k_conv(const uint16_t *__restrict__ X, co
 // 24) and dY stored channel-major in LDS so eight consecutive x-points of one
 // channel are one 16-byte read.  Wave w accumulates taps w, w + 4, ...
 constexpr int WX = 16, WY = 8, WZ = 4, WP = WX * WY * WZ, WHX = 24, WHY = 10, WHZ = 6, WHP = WHX * WHY * WHZ;
+// channel strides in LDS: one 16-byte pad per channel row, so that the 32
+// lanes of an MFMA operand read (one channel each) fall on distinct banks
+// (unpadded, the dY rows are 1024 bytes apart and every lane hits the same banks)
+constexpr int WPS = WP + 8, WHPS = WHP + 8;
 
 // eight bf16 starting at element offset q8 * 8 + rr of a 16-byte aligned row
 template <int RR>
@@ -155,8 +212,8 @@ template <int CP, int CY, int DX>
 __device__ __forceinline__ void wgrad_tap(const uint16_t *xs, const uint16_t *ys, int m, int kh, int dy, int dz,
                                           f32x16 &acc) {
     const bool aok = m < CY, bok = m < CP;
-    const uint16_t *yrow = ys + (aok ? m : 0) * WP;
-    const uint16_t *xch = xs + (bok ? m : 0) * WHP;
+    const uint16_t *yrow = ys + (aok ? m : 0) * WPS;
+    const uint16_t *xch = xs + (bok ? m : 0) * WHPS;
     const bf16x8 zero = {};
 #pragma unroll 4
     for (int kc = 0; kc < WY * WZ; kc++) {
@@ -176,7 +233,7 @@ __global__ void __launch_bounds__(256) k_wgrad(const uint16_t *__restrict__ X, c
                                                float *__restrict__ wpart) {
     extern __shared__ uint16_t smem_cnn[];
     uint16_t *xs = smem_cnn;             // [CP][WHZ][WHY][WHX]
-    uint16_t *ys = smem_cnn + CP * WHP;  // [CY][WZ][WY][WX]
+    uint16_t *ys = smem_cnn + CP * WHPS;  // [CY][WZ][WY][WX], channel stride WPS
     const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, m = l & 31, kh = l >> 5;
     const int gx = n / WX, gy = n / WY;
     const size_t n3 = (size_t)n * n * n;
@@ -198,7 +255,7 @@ __global__ void __launch_bounds__(256) k_wgrad(const uint16_t *__restrict__ X, c
             const uint16_t *e = (const uint16_t *)&v;
             const int o = (hz * WHY + hy) * WHX + hx;
 #pragma unroll
-            for (int c = 0; c < 8; c++) xs[(part * 8 + c) * WHP + o] = e[c];
+            for (int c = 0; c < 8; c++) xs[(part * 8 + c) * WHPS + o] = e[c];
         }
         for (int i = tid; i < WP * (CY / 8); i += 256) {
             const int pt = i / (CY / 8), part = i - pt * (CY / 8);
@@ -207,7 +264,7 @@ __global__ void __launch_bounds__(256) k_wgrad(const uint16_t *__restrict__ X, c
             const uint4 v = *(const uint4 *)(dY + (sbase + g) * CY + part * 8);
             const uint16_t *e = (const uint16_t *)&v;
 #pragma unroll
-            for (int c = 0; c < 8; c++) ys[(part * 8 + c) * WP + pt] = e[c];
+            for (int c = 0; c < 8; c++) ys[(part * 8 + c) * WPS + pt] = e[c];
         }
         __syncthreads();
 #pragma unroll
@@ -535,7 +592,7 @@ static int launch_conv(ec_ctx *ctx, const uint16_t *X, const uint16_t *W, const 
     const size_t lds = (size_t)(FHP + 27 * CH) * (CP + 8) * 2;
     EC_HIP(ctx, hipFuncSetAttribute((const void *)k_conv<CP, MODE>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)lds));
-    k_conv<CP, MODE><<<tps * batch, 256, lds, ctx->stream>>>(X, W, bias, n, tps, oa, ob, of, pre_in);
+    k_conv<CP, MODE><<<tps * batch, FT, lds, ctx->stream>>>(X, W, bias, n, tps, oa, ob, of, pre_in);
     EC_CHECK_LAUNCH(ctx, "k_conv");
     return EC_OK;
 }
@@ -545,7 +602,7 @@ static int launch_wgrad(ec_ctx *ctx, ec_cnn *c, const uint16_t *X, const uint16_
                         float *gb, int cin, int cout) {
     const int tps = (n / WX) * (n / WY) * (n / WZ), total = tps * batch;
     const int blocks = std::min(total, c->wblocks);
-    const size_t lds = (size_t)(CP * WHP + CY * WP) * 2;
+    const size_t lds = (size_t)(CP * WHPS + CY * WPS) * 2;
     EC_HIP(ctx, hipFuncSetAttribute((const void *)k_wgrad<CP, CY>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)lds));
     k_wgrad<CP, CY><<<blocks, 256, lds, ctx->stream>>>(X, dY, n, tps, total, c->wpart);
