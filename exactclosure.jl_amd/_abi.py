@@ -109,12 +109,16 @@ _SIGS = {
     "ec_ns_filter_vel": (c_int, [c_void_p, c_int, c_int, c_double, P3, c_void_p, c_int, ctypes.POINTER(VelSet)]),
     "ec_ns_filter_vel_rk4": (c_int, [c_void_p, c_int, c_int, c_double, P3, c_void_p, c_int, ctypes.POINTER(VelSet),
                                      c_int, c_double, P3, P3, P3]),
+    "ec_ns_filter_vel_rk4_states": (c_int, [c_void_p, c_int, c_int, c_double, P3, c_void_p, c_int,
+                                            ctypes.POINTER(VelSet), c_int, c_double, P3, P9]),
     "ec_filter_vel_set_tiles": (c_int, [c_void_p, c_int, c_int]),
     "ec_ns_commutator": (c_int, [c_void_p, c_int, c_double, c_double, P3, P9, P9]),
     "ec_ns_sfs_filter": (c_int, [c_void_p, c_int, c_int, c_int, c_double, c_double, P3, c_void_p, c_int, P3, P9, P9]),
     "ec_ns_tensordissipation": (c_int, [c_void_p, c_int, c_double, P9, P3, c_void_p]),
     "ec_ns_rk4_update": (c_int, [c_void_p, c_int, c_int, c_double, c_int, c_double, P3, P3, P3,
                                  c_void_p, P3]),
+    "ec_ns_rk4_update_states": (c_int, [c_void_p, c_int, c_int, c_double, c_int, c_double, P3, P9, P3,
+                                        c_void_p]),
     "ec_burgers_batch_create": (c_int, [c_void_p, c_int, c_int, ctypes.POINTER(c_int),
                                         ctypes.POINTER(c_int), c_double, c_double, c_double, c_int,
                                         c_u64, ctypes.POINTER(c_void_p)]),

@@ -102,8 +102,8 @@ def tune_filter_vel(tuner, drv, dt=None, candidates=(64, 128, 256, 512)):
     and isolated launches have ranked the chunks differently from whole steps
     (stage 0 alone once chose 512 planes, 2 ms per step slower in the step
     than 256).  Without dt, one run is stages 0, 1 and 2 with dt = 0,
-    which write only the stage state us and acc (the next step's stage 0
-    rewrites them) and leave the DNS state u0 untouched."""
+    which write only stage buffers (us and acc, or the stored states u1..u3;
+    the next step rewrites them) and leave the DNS state u0 untouched."""
     from . import _abi
     from ._abi import P3, ptrs
 
@@ -113,9 +113,15 @@ def tune_filter_vel(tuner, drv, dt=None, candidates=(64, 128, 256, 512)):
 
     def run_stages():
         for st in range(3):
-            ctx.call("ec_ns_filter_vel_rk4", drv.dt_code, n, l, ptrs(drv.du, P3), _abi.c_void_p(drv.ps.p_ptr),
-                     len(drv.les), drv._vel_sets(outs), st, 0.0, ptrs(drv.u0, P3), ptrs(drv.us, P3),
-                     ptrs(drv.acc, P3))
+            if drv.rk_states:
+                ust = ptrs([x for s in drv._stage_states() for x in s], _abi.P9)
+                ctx.call("ec_ns_filter_vel_rk4_states", drv.dt_code, n, l, ptrs(drv.du, P3),
+                         _abi.c_void_p(drv.ps.p_ptr), len(drv.les), drv._vel_sets(outs), st, 0.0, ptrs(drv.u0, P3),
+                         ust)
+            else:
+                ctx.call("ec_ns_filter_vel_rk4", drv.dt_code, n, l, ptrs(drv.du, P3), _abi.c_void_p(drv.ps.p_ptr),
+                         len(drv.les), drv._vel_sets(outs), st, 0.0, ptrs(drv.u0, P3), ptrs(drv.us, P3),
+                         ptrs(drv.acc, P3))
 
     run = run_stages if dt is None else (lambda: drv.step(dt))
     cands = [z for z in candidates if n % z == 0] or [n]

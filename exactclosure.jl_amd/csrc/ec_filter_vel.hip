@@ -459,30 +459,50 @@ struct FvTile {
 // already forms k_a = du_a - delta_a p for every column, so the interior cells
 // of the tile also take the stage update of ec_ns_rk4_update (same arithmetic,
 // same FP32 k), and du and p are read once per stage instead of twice.
+// RK = 0..3: stage RK with the accumulator acc (ec_ns_rk4_update);
+// RK = 4..7: stage RK - 4 of the stored-state form (ec_ns_rk4_update_states),
+// whose stages 0..2 write u1, u2, u3 (ust) and whose last stage combines them.
 struct RkArgs {
     void *u0[3], *us[3], *acc[3];
+    void *ust[3][3];
     double dt;
 };
 
+// values of the stage state read per cell: u0, with acc after stage 0 of the
+// accumulator form, with u1, u2, u3 at the last stage of the stored-state form
+template <int RK>
+constexpr int rk_nv() {
+    return RK == 7 ? 4 : ((RK >= 1 && RK <= 3) ? 2 : 1);
+}
+
 // loads of the stage state of one cell, issued at the top of a plane so that
 // they arrive during the plane's filter work
-template <typename T, int STAGE>
-__device__ __forceinline__ void rk_load(int a, unsigned I, const RkArgs &rk, T &u, T &ac) {
-    u = ((const T *)rk.u0[a])[I];
-    if constexpr (STAGE != 0) ac = ((const T *)rk.acc[a])[I];
+template <typename T, int RK>
+__device__ __forceinline__ void rk_load(int a, unsigned I, const RkArgs &rk, T *v) {
+    v[0] = ((const T *)rk.u0[a])[I];
+    if constexpr (RK >= 1 && RK <= 3) v[1] = ((const T *)rk.acc[a])[I];
+    if constexpr (RK == 7) {
+        v[1] = ((const T *)rk.ust[0][a])[I];
+        v[2] = ((const T *)rk.ust[1][a])[I];
+        v[3] = ((const T *)rk.ust[2][a])[I];
+    }
 }
-template <typename T, int STAGE>
-__device__ __forceinline__ void rk_store(int a, unsigned I, T k, T u, T ac, const RkArgs &rk) {
+template <typename T, int RK>
+__device__ __forceinline__ void rk_store(int a, unsigned I, T k, const T *v, const RkArgs &rk) {
     const T dt = (T)rk.dt;
-    T *acc = (T *)rk.acc[a];
-    if constexpr (STAGE == 0) {
-        ec_nt_store(acc + I, k);
+    const T u = v[0];
+    if constexpr (RK == 0) {
+        ec_nt_store((T *)rk.acc[a] + I, k);
         ec_nt_store((T *)rk.us[a] + I, u + (dt / 2) * k);
-    } else if constexpr (STAGE == 1 || STAGE == 2) {
-        ec_nt_store(acc + I, ac + 2 * k);
-        ec_nt_store((T *)rk.us[a] + I, u + (STAGE == 1 ? dt / 2 : dt) * k);
+    } else if constexpr (RK == 1 || RK == 2) {
+        ec_nt_store((T *)rk.acc[a] + I, v[1] + 2 * k);
+        ec_nt_store((T *)rk.us[a] + I, u + (RK == 1 ? dt / 2 : dt) * k);
+    } else if constexpr (RK == 3) {
+        ec_nt_store((T *)rk.u0[a] + I, u + (dt / 6) * (v[1] + k));
+    } else if constexpr (RK < 7) {
+        ec_nt_store((T *)rk.ust[RK - 4][a] + I, u + ec_rk4_stage_coef<T>(RK - 4, dt) * k);
     } else {
-        ec_nt_store((T *)rk.u0[a] + I, u + (dt / 6) * (ac + k));
+        ec_nt_store((T *)rk.u0[a] + I, ec_rk4_combine<T>(u, v[1], v[2], v[3], k, dt));
     }
 }
 
@@ -558,22 +578,22 @@ __device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__re
         }
     };
     load(0, cv, cp, cqv);
-    // fused RK4 stage: the stage state (u0, and acc after stage 0) of plane r is
+    // fused RK4 stage: the stage state of plane r (rk_nv values per cell) is
     // loaded one plane ahead, with the filter's loads of that plane
-    T cu[NJ], ca[NJ];
-    auto rk_prefetch = [&](int r, T *uu, T *aa) {
+    constexpr int NV = rk_nv<RK>();
+    T rkv[NJ][NV];
+    auto rk_prefetch = [&](int r, T (*vv)[NV]) {
 #pragma unroll
         for (int j = 0; j < NJ; j++)
-            if (comp(j) != 3 && r < ZC && ok[j]) rk_load<T, RK < 0 ? 0 : RK>(comp(j), co[j] + (unsigned)(z0 + r) * n2, rk, uu[j], aa[j]);
+            if (comp(j) != 3 && r < ZC && ok[j]) rk_load<T, RK>(comp(j), co[j] + (unsigned)(z0 + r) * n2, rk, vv[j]);
     };
-    auto rk_rotate = [&](const T *uu, const T *aa) {
+    auto rk_rotate = [&](T (*vv)[NV]) {
 #pragma unroll
-        for (int j = 0; j < NJ; j++) {
-            cu[j] = uu[j];
-            ca[j] = aa[j];
-        }
+        for (int j = 0; j < NJ; j++)
+#pragma unroll
+            for (int q = 0; q < NV; q++) rkv[j][q] = vv[j][q];
     };
-    if constexpr (RK >= 0) rk_prefetch(0, cu, ca);
+    if constexpr (RK >= 0) rk_prefetch(0, rkv);
     const int rend = ZC - 1 + H;
     // k = v - delta p in the field precision (one rounding at its scale); the
     // z-sums and everything after them are FP64
@@ -619,8 +639,8 @@ __device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__re
                 // plane r + 1 (its filter loads and stage state) in flight through this plane
                 T nv[NJ], np[NJ], nq[NJ];
                 load(r + 1, nv, np, nq);
-                T nu[NJ], na[NJ];
-                if constexpr (RK >= 0) rk_prefetch(r + 1, nu, na);
+                T rkn[NJ][NV];
+                if constexpr (RK >= 0) rk_prefetch(r + 1, rkn);
 #pragma unroll
                 for (int j = 0; j < NJ; j++) {
                     const int a = comp(j);
@@ -629,14 +649,14 @@ __device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__re
                     const double src = (double)kv;
                     if constexpr (RK >= 0)
                         if (r < ZC && ok[j])
-                            rk_store<T, RK>(a, co[j] + (unsigned)(z0 + r) * n2, kv, cu[j], ca[j], rk);
+                            rk_store<T, RK>(a, co[j] + (unsigned)(z0 + r) * n2, kv, rkv[j], rk);
                     if (a == 0) nested_box<G, 0, Q>(r, ZC, ok[j], clx[j], cly[j], src, nst[j], lds);
                     else if (a == 1) nested_box<G, 1, Q>(r, ZC, ok[j], clx[j], cly[j], src, nst[j], lds);
                     else nested_face<G, Q>(r, ZC, ok[j], clx[j], cly[j], src, nst[j], lds);
                 }
                 if constexpr (Q == 3) emit(r);
                 rotate(nv, np, nq);
-                if constexpr (RK >= 0) rk_rotate(nu, na);
+                if constexpr (RK >= 0) rk_rotate(rkn);
             };
             plane(std::integral_constant<int, 0>{});
             plane(std::integral_constant<int, 1>{});
@@ -647,8 +667,8 @@ __device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__re
         for (int r = 0; r <= rend; ++r) {
             T nv[NJ], np[NJ], nq[NJ];
             load(r + 1, nv, np, nq);
-            T nu[NJ], na[NJ];
-            if constexpr (RK >= 0) rk_prefetch(r + 1, nu, na);
+            T rkn[NJ][NV];
+            if constexpr (RK >= 0) rk_prefetch(r + 1, rkn);
 #pragma unroll
             for (int j = 0; j < NJ; j++) {
                 const int a = comp(j);
@@ -657,7 +677,7 @@ __device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__re
                 const double src = (double)kv;
                 if constexpr (RK >= 0)
                     if (r < ZC && ok[j])
-                        rk_store<T, RK>(a, co[j] + (unsigned)(z0 + r) * n2, kv, cu[j], ca[j], rk);
+                        rk_store<T, RK>(a, co[j] + (unsigned)(z0 + r) * n2, kv, rkv[j], rk);
                 if constexpr (G::c(0) > 0) zupdate<G, 0>(r, ZC, a, ok[j], clx[j], cly[j], src, acc[j][0], lds);
                 if constexpr (G::c(1) > 0) zupdate<G, 1>(r, ZC, a, ok[j], clx[j], cly[j], src, acc[j][1], lds);
                 if constexpr (G::c(2) > 0) zupdate<G, 2>(r, ZC, a, ok[j], clx[j], cly[j], src, acc[j][2], lds);
@@ -665,7 +685,7 @@ __device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__re
             }
             emit(r);
             rotate(nv, np, nq);
-            if constexpr (RK >= 0) rk_rotate(nu, na);
+            if constexpr (RK >= 0) rk_rotate(rkn);
         }
     }
 }
@@ -686,7 +706,7 @@ __device__ __forceinline__ void fv_dispatch(int code, const T *__restrict__ v1, 
     }
 }
 
-// RK < 0: the filter alone; RK = 0..3: with the fused DNS RK4 stage RK
+// RK < 0: the filter alone; RK = 0..7: with the fused DNS RK4 stage (RkArgs)
 template <typename T, class G, int RK>
 __global__ void __launch_bounds__(G::NT) k_filter_vel(const T *__restrict__ v1, const T *__restrict__ v2,
                                                      const T *__restrict__ v3, const T *__restrict__ p, int n, int ZC,
@@ -759,8 +779,11 @@ int launch_fv(ec_ctx *ctx, int n, double l, void *const v[3], const void *p, con
     if (lds > 150 * 1024) return EC_E_UNSUPPORTED;
     const long tiles = (long)(n / TX) * (n / TY) * (n / ZC);
     if (tiles > 0x7fffffffL) return EC_E_UNSUPPORTED;
-    // fused stage: reads u0 (and acc after stage 0), writes us and acc (u0 at stage 3)
-    constexpr double rk_units = RK < 0 ? 0.0 : (RK == 0 || RK == 3 ? 9.0 : 12.0);
+    // fused stage: reads u0 (and acc after stage 0), writes us and acc (u0 at
+    // stage 3); stored-state form: reads u0, writes one state, or reads u0, u1,
+    // u2, u3 and writes u0 at the last stage
+    constexpr double rk_units =
+        RK < 0 ? 0.0 : (RK == 0 || RK == 3 ? 9.0 : (RK <= 2 ? 12.0 : (RK == 7 ? 15.0 : 6.0)));
     ec_prof_scope pr(ctx, RK < 0 ? "filter_vel" : "filter_vel_rk4",
                      sizeof(T) * ((p ? 4.0 : 3.0) + rk_units) * ec_cube(n));
     EC_HIP(ctx, hipFuncSetAttribute((const void *)k_filter_vel<T, G, RK>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -824,7 +847,7 @@ int launch_single(ec_ctx *ctx, int n, double l, void *const v[3], const void *p,
 // EC_E_UNSUPPORTED when the list or the grid takes another path.
 template <typename T>
 int filter_vel_rk4_t(ec_ctx *ctx, int n, double l, void *const v[3], const void *p, int nsets,
-                     const ec_vel_set *sets, int tz, int stage, const RkArgs &rk) {
+                     const ec_vel_set *sets, int tz, int stage, const RkArgs &rk, bool states) {
     if (nsets != 4 || !p) return EC_E_UNSUPPORTED;
     FArgs A;
     int cs[4], ks[4];
@@ -838,11 +861,15 @@ int filter_vel_rk4_t(ec_ctx *ctx, int n, double l, void *const v[3], const void 
           ks[3] == 1))
         return EC_E_UNSUPPORTED;
     typedef Cfg<32, 16, 512, 16, 0, 8, 0, 4, 0, 4, 1> G4;
-    switch (stage) {
+    switch (stage + (states ? 4 : 0)) {
         case 0: return launch_fv<T, G4, 0>(ctx, n, l, v, p, A, tz, &rk);
         case 1: return launch_fv<T, G4, 1>(ctx, n, l, v, p, A, tz, &rk);
         case 2: return launch_fv<T, G4, 2>(ctx, n, l, v, p, A, tz, &rk);
-        default: return launch_fv<T, G4, 3>(ctx, n, l, v, p, A, tz, &rk);
+        case 3: return launch_fv<T, G4, 3>(ctx, n, l, v, p, A, tz, &rk);
+        case 4: return launch_fv<T, G4, 4>(ctx, n, l, v, p, A, tz, &rk);
+        case 5: return launch_fv<T, G4, 5>(ctx, n, l, v, p, A, tz, &rk);
+        case 6: return launch_fv<T, G4, 6>(ctx, n, l, v, p, A, tz, &rk);
+        default: return launch_fv<T, G4, 7>(ctx, n, l, v, p, A, tz, &rk);
     }
 }
 
@@ -951,17 +978,44 @@ extern "C" int ec_ns_filter_vel_rk4(ec_ctx *ctx, int dtype, int n, double l, voi
         EC_REQUIRE(ctx, sets[s].kind == 0 || sets[s].kind == 1, "set %d: kind %d", s, sets[s].kind);
         for (int a = 0; a < 3; a++) EC_REQUIRE(ctx, sets[s].out[a], "set %d: null output", s);
     }
-    RkArgs rk;
+    RkArgs rk{};
     for (int a = 0; a < 3; a++) {
         rk.u0[a] = u0[a];
         rk.us[a] = us ? us[a] : nullptr;
         rk.acc[a] = acc[a];
     }
     rk.dt = dt;
-    int rc = dtype == EC_F64 ? filter_vel_rk4_t<double>(ctx, n, l, du, p, nsets, sets, ctx->fv_tz, stage, rk)
-                             : filter_vel_rk4_t<float>(ctx, n, l, du, p, nsets, sets, ctx->fv_tz, stage, rk);
+    int rc = dtype == EC_F64 ? filter_vel_rk4_t<double>(ctx, n, l, du, p, nsets, sets, ctx->fv_tz, stage, rk, false)
+                             : filter_vel_rk4_t<float>(ctx, n, l, du, p, nsets, sets, ctx->fv_tz, stage, rk, false);
     if (rc != EC_E_UNSUPPORTED) return rc;
     // any other set list: the filter, then the stage update
     if ((rc = ec_ns_filter_vel(ctx, dtype, n, l, du, p, nsets, sets))) return rc;
     return ec_ns_rk4_update(ctx, dtype, n, l, stage, dt, u0, us, du, p, acc);
+}
+
+extern "C" int ec_ns_filter_vel_rk4_states(ec_ctx *ctx, int dtype, int n, double l, void *const du[3],
+                                           const void *p, int nsets, const ec_vel_set *sets, int stage, double dt,
+                                           void *const u0[3], void *const ust[9]) {
+    EC_REQUIRE(ctx, ctx && du && p && u0 && ust && (nsets == 0 || sets), "ec_ns_filter_vel_rk4_states: null argument");
+    EC_REQUIRE(ctx, stage >= 0 && stage <= 3, "ec_ns_filter_vel_rk4_states: stage %d out of range", stage);
+    for (int q = 0; q < 9; q++)
+        if (stage == 3 || q / 3 == stage)
+            EC_REQUIRE(ctx, ust[q], "ec_ns_filter_vel_rk4_states: null stage state %d", q);
+    for (int s = 0; s < nsets; s++) {
+        EC_REQUIRE(ctx, sets[s].nbar >= 1 && n % sets[s].nbar == 0, "set %d: n=%d not a multiple of nbar=%d", s, n,
+                   sets[s].nbar);
+        EC_REQUIRE(ctx, sets[s].kind == 0 || sets[s].kind == 1, "set %d: kind %d", s, sets[s].kind);
+        for (int a = 0; a < 3; a++) EC_REQUIRE(ctx, sets[s].out[a], "set %d: null output", s);
+    }
+    RkArgs rk{};
+    for (int a = 0; a < 3; a++) {
+        rk.u0[a] = u0[a];
+        for (int s = 0; s < 3; s++) rk.ust[s][a] = ust[3 * s + a];
+    }
+    rk.dt = dt;
+    int rc = dtype == EC_F64 ? filter_vel_rk4_t<double>(ctx, n, l, du, p, nsets, sets, ctx->fv_tz, stage, rk, true)
+                             : filter_vel_rk4_t<float>(ctx, n, l, du, p, nsets, sets, ctx->fv_tz, stage, rk, true);
+    if (rc != EC_E_UNSUPPORTED) return rc;
+    if ((rc = ec_ns_filter_vel(ctx, dtype, n, l, du, p, nsets, sets))) return rc;
+    return ec_ns_rk4_update_states(ctx, dtype, n, l, stage, dt, u0, ust, du, p);
 }

@@ -110,6 +110,22 @@ __device__ __forceinline__ void ec_nt_store16(VT *p, const VT &v) {  // a 16-byt
     __builtin_memcpy(&w, &v, 16);
     __builtin_nontemporal_store(w, reinterpret_cast<u4 *>(p));
 }
+// RK4 in stored stage states (ec_ns_rk4_update_states): the states
+// u_{s+1} = u0 + c_s dt k_s (c = 1/2, 1/2, 1) carry k1, k2, k3, so the new
+// state u0 + dt/6 (k1 + 2 k2 + 2 k3 + k4) of rk4_timestep! is
+// u0 + ((u1 - u0) + 2 (u2 - u0) + (u3 - u0)) / 3 + dt/6 k4.  The differences
+// of neighbouring states are formed first (exact where they are within a
+// factor 2 of each other), so the sum meets u0 once, as u0 + dt/6 (acc + k4)
+// does.
+template <typename T>
+__device__ __forceinline__ T ec_rk4_stage_coef(int stage, T dt) {
+    return stage == 2 ? dt : dt / 2;
+}
+template <typename T>
+__device__ __forceinline__ T ec_rk4_combine(T u0, T u1, T u2, T u3, T k, T dt) {
+    const T d = (u1 - u0) + 2 * (u2 - u0) + (u3 - u0);
+    return u0 + (d * (T)(1.0 / 3.0) + (dt / 6) * k);
+}
 // Non-temporal stores through a raw buffer descriptor of a uniform base: the
 // address is the descriptor's base plus a 32-bit byte offset, so a kernel that
 // writes many fields at the same offsets (a plane of a tile) forms no per-lane

@@ -4,6 +4,8 @@
 // consecutive values of each face array; the neighbour loads hit L1/L2.
 #include "ec_internal.h"
 
+#include <type_traits>
+
 namespace {
 
 struct Grid3 {
@@ -868,6 +870,107 @@ __global__ void __launch_bounds__(256) k_rk4_update_vec(int n, T ih, T dt, Ptr3 
     }
 }
 
+// The stage update of RK4 in stored stage states (ec_rk4_combine): k =
+// du - grad p; stage s < 3 writes u_{s+1} = u0 + c_s dt k into its own
+// buffer, the last stage reads u1, u2, u3 and writes the new u0.  16-byte
+// accesses as k_rk4_update_vec, or one cell per thread (VEC false) where the
+// fields are not 16-byte aligned.
+template <typename T, int STAGE, bool VEC>
+__global__ void __launch_bounds__(256) k_rk4_states(int n, T ih, T dt, Ptr3 u0p, Ptr3 u1p, Ptr3 u2p, Ptr3 u3p,
+                                                    Ptr3 dup, const T *__restrict__ p, size_t nvec) {
+    typedef typename std::conditional<VEC, typename Vec16<T>::type, T>::type VT;
+    constexpr int V = VEC ? Vec16<T>::V : 1;
+    const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (t >= nvec) return;
+    const unsigned nx = (unsigned)n / V;
+    const unsigned iv = (unsigned)(t % nx);
+    const size_t row = t / nx;  // j + n k
+    const unsigned j = (unsigned)(row % (unsigned)n), k = (unsigned)(row / (unsigned)n);
+    const size_t n1 = (size_t)n;
+    const size_t I = (size_t)iv * V + n1 * row;
+    const size_t Iy = (size_t)iv * V + n1 * ((j + 1 == (unsigned)n ? 0 : j + 1) + n1 * k);
+    const size_t Iz = (size_t)iv * V + n1 * (j + n1 * (k + 1 == (unsigned)n ? 0 : k + 1));
+    const unsigned xl = iv * V + V;  // x + 1 of the last cell
+    const T pxl = p[(xl == (unsigned)n ? 0 : xl) + n1 * row];
+    const VT p0 = *(const VT *)(p + I), py = *(const VT *)(p + Iy), pz = *(const VT *)(p + Iz);
+    const T *p0s = (const T *)&p0, *pys = (const T *)&py, *pzs = (const T *)&pz;
+    T gp[3][V];
+#pragma unroll
+    for (int e = 0; e < V; e++) {
+        const T pxp = e + 1 < V ? p0s[e + 1] : pxl;
+        gp[0][e] = (pxp - p0s[e]) * ih;
+        gp[1][e] = (pys[e] - p0s[e]) * ih;
+        gp[2][e] = (pzs[e] - p0s[e]) * ih;
+    }
+    // all loads first (the fields may alias as far as the compiler knows)
+    VT d[3], uv[3], s1[3], s2[3], s3[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        d[a] = *(const VT *)((const T *)dup.p[a] + I);
+        uv[a] = *(const VT *)((const T *)u0p.p[a] + I);
+        if constexpr (STAGE == 3) {
+            s1[a] = *(const VT *)((const T *)u1p.p[a] + I);
+            s2[a] = *(const VT *)((const T *)u2p.p[a] + I);
+            s3[a] = *(const VT *)((const T *)u3p.p[a] + I);
+        }
+    }
+    const Ptr3 &outp = STAGE == 0 ? u1p : (STAGE == 1 ? u2p : (STAGE == 2 ? u3p : u0p));
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const T *ds = (const T *)&d[a], *us_ = (const T *)&uv[a];
+        const T *a1 = (const T *)&s1[a], *a2 = (const T *)&s2[a], *a3 = (const T *)&s3[a];
+        VT o;
+        T *r = (T *)&o;
+#pragma unroll
+        for (int e = 0; e < V; e++) {
+            const T kk = ds[e] - gp[a][e];
+            if constexpr (STAGE < 3) r[e] = us_[e] + ec_rk4_stage_coef<T>(STAGE, dt) * kk;
+            else r[e] = ec_rk4_combine<T>(us_[e], a1[e], a2[e], a3[e], kk, dt);
+        }
+        if constexpr (VEC) ec_nt_store16((VT *)((T *)outp.p[a] + I), o);
+        else ec_nt_store((T *)outp.p[a] + I, r[0]);
+    }
+}
+
+template <typename T>
+static int rk4_states_launch(ec_ctx *ctx, int n, double l, int stage, double dt, void *const u0[3],
+                             void *const ust[9], void *const du[3], const void *p) {
+    const T ih = (T)(n / l);
+    Ptr3 U0{{u0[0], u0[1], u0[2]}}, DU{{du[0], du[1], du[2]}};
+    Ptr3 S[3];
+    for (int s = 0; s < 3; s++)
+        for (int a = 0; a < 3; a++) S[s].p[a] = ust[3 * s + a];
+    constexpr int V = Vec16<T>::V;
+    bool aligned = n % V == 0 && ((uintptr_t)p % 16) == 0;
+    for (int a = 0; a < 3; a++) {
+        aligned = aligned && ((uintptr_t)u0[a] % 16) == 0 && ((uintptr_t)du[a] % 16) == 0;
+        for (int s = 0; s < 3; s++) aligned = aligned && ((uintptr_t)ust[3 * s + a] % 16) == 0;
+    }
+    const size_t nvec = ec_cube(n) / (aligned ? V : 1);
+    const unsigned blocks = (unsigned)((nvec + 255) / 256);
+#define EC_RK4S(ST, VEC)                                                                              \
+    k_rk4_states<T, ST, VEC><<<blocks, 256, 0, ctx->stream>>>(n, ih, (T)dt, U0, S[0], S[1], S[2], DU, \
+                                                               (const T *)p, nvec)
+    if (aligned) {
+        switch (stage) {
+            case 0: EC_RK4S(0, true); break;
+            case 1: EC_RK4S(1, true); break;
+            case 2: EC_RK4S(2, true); break;
+            default: EC_RK4S(3, true); break;
+        }
+    } else {
+        switch (stage) {
+            case 0: EC_RK4S(0, false); break;
+            case 1: EC_RK4S(1, false); break;
+            case 2: EC_RK4S(2, false); break;
+            default: EC_RK4S(3, false); break;
+        }
+    }
+#undef EC_RK4S
+    EC_CHECK_LAUNCH(ctx, "k_rk4_states");
+    return EC_OK;
+}
+
 template <typename T>
 static int rk4_launch(ec_ctx *ctx, int n, double l, int stage, double dt, void *const u0[3], void *const us[3],
                       void *const du[3], const void *p, void *const acc[3]) {
@@ -914,4 +1017,22 @@ extern "C" int ec_ns_rk4_update(ec_ctx *ctx, int dtype, int n, double l, int sta
     ec_prof_scope ps(ctx, "rk4_update", units * ec_cube(n) * b);
     return dtype == EC_F64 ? rk4_launch<double>(ctx, n, l, stage, dt, u0, us, du, p, acc)
                            : rk4_launch<float>(ctx, n, l, stage, dt, u0, us, du, p, acc);
+}
+
+extern "C" int ec_ns_rk4_update_states(ec_ctx *ctx, int dtype, int n, double l, int stage, double dt,
+                                       void *const u0[3], void *const ust[9], void *const du[3], const void *p) {
+    EC_REQUIRE(ctx, ctx && u0 && ust && du && p, "ec_ns_rk4_update_states: null argument");
+    EC_REQUIRE(ctx, stage >= 0 && stage <= 3, "ec_ns_rk4_update_states: stage %d out of range", stage);
+    for (int q = 0; q < 9; q++) {
+        // the buffers a stage touches: its own state (stage < 3), all three at stage 3
+        if (stage == 3 || q / 3 == stage) EC_REQUIRE(ctx, ust[q], "ec_ns_rk4_update_states: null stage state %d", q);
+    }
+    EC_CHECK_FIELD(ctx, n);
+    EC_REQUIRE(ctx, ec_cube(n) / 256 < 2147483647.0, "ec_ns_rk4_update_states: n=%d too large", n);
+    const double b = dtype == EC_F64 ? 8 : 4;
+    // reads p, du(3), u0(3) (+ u1, u2, u3 at stage 3); writes one state (3)
+    const double units = stage == 3 ? 19 : 10;
+    ec_prof_scope ps(ctx, "rk4_update", units * ec_cube(n) * b);
+    return dtype == EC_F64 ? rk4_states_launch<double>(ctx, n, l, stage, dt, u0, ust, du, p)
+                           : rk4_states_launch<float>(ctx, n, l, stage, dt, u0, ust, du, p);
 }

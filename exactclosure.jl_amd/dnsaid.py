@@ -69,9 +69,15 @@ class DNSAidedLES:
     (face average).  Memory at n = 1024 FP32: 12 DNS fields + p + half
     spectrum ~ 60 GB."""
 
-    def __init__(self, g_dns, visc, sets, dtype=torch.float32, closure="forcing", clean_divergence=True):
+    def __init__(self, g_dns, visc, sets, dtype=torch.float32, closure="forcing", clean_divergence=True,
+                 rk_states=True):
         assert closure in ("forcing", "flux")
         self.closure = closure
+        # the DNS RK4 in stored stage states (ec_ns_rk4_update_states: stages
+        # 0..2 write u1, u2, u3, the last stage combines them; 49 fields of
+        # state traffic per step against 58 with an accumulator), forcing form
+        # only; False keeps the accumulator form (ec_ns_rk4_update)
+        self.rk_states = bool(rk_states) and closure == "forcing"
         # fold 6/dt div_h u_3 into the last stage's Poisson right-hand side, so
         # that the step removes the divergence the state carried in (FP32
         # rounding of the projected stage tendencies would otherwise add up
@@ -85,9 +91,10 @@ class DNSAidedLES:
         n = g_dns.n
         self.ps = NS.PoissonSolver(g_dns, dtype)
         self.u0 = _vec(n, dtype)
-        self.us = _vec(n, dtype)
+        self.us = _vec(n, dtype)   # stage state (u1 of the stored-state form)
         self.du = _vec(n, dtype)
-        self.acc = _vec(n, dtype)
+        self.acc = _vec(n, dtype)  # accumulator (u2 of the stored-state form)
+        self.u3 = _vec(n, dtype) if self.rk_states else None
         self.les = [LESGrid(g_dns, nbar, kind, dtype, closure) for nbar, kind in sets]
         # objects with .stage(st, dt), advanced after every stage of the forcing
         # form once the stage's exact closure is known (a posteriori model LES)
@@ -247,11 +254,17 @@ class DNSAidedLES:
                ptrs(L.g_k, P3))
         return L.m
 
+    def _stage_states(self):
+        """[u1, u2, u3] of the stored-state RK4 (the buffers us, acc, u3)."""
+        return [self.us, self.acc, self.u3]
+
     def _step_forcing(self, dt):
         c = self.ctx
         n, l = self.g.n, self.g.l
+        ust = self._stage_states() if self.rk_states else None
+        ust_p = ptrs([x for s in ust for x in s], _abi.P9) if ust else None
         for st in range(4):
-            u = self.u0 if st == 0 else self.us
+            u = self.u0 if st == 0 else (ust[st - 1] if ust else self.us)
             # du = -div sigma(u) and its divergence in one pass, then the FFT solve
             # at the last stage u_new = u0 + dt (acc + k / 6): with d = div du +
             # (6/dt) div u_3, div k = -(6/dt) div u_3 cancels the divergence
@@ -263,10 +276,18 @@ class DNSAidedLES:
             # V_s(du - grad p) for every LES grid and the DNS RK4 stage in one
             # pass over du and p (nothing below reads the DNS state)
             if self.les:
-                c.call("ec_ns_filter_vel_rk4", self.dt_code, n, l, ptrs(self.du, P3), _abi.c_void_p(self.ps.p_ptr),
-                       len(self.les), self._vel_sets([L.g_k for L in self.les]), st, dt, ptrs(self.u0, P3),
-                       ptrs(self.us, P3), ptrs(self.acc, P3))
+                sets = self._vel_sets([L.g_k for L in self.les])
+                if ust:
+                    c.call("ec_ns_filter_vel_rk4_states", self.dt_code, n, l, ptrs(self.du, P3),
+                           _abi.c_void_p(self.ps.p_ptr), len(self.les), sets, st, dt, ptrs(self.u0, P3), ust_p)
+                else:
+                    c.call("ec_ns_filter_vel_rk4", self.dt_code, n, l, ptrs(self.du, P3),
+                           _abi.c_void_p(self.ps.p_ptr), len(self.les), sets, st, dt, ptrs(self.u0, P3),
+                           ptrs(self.us, P3), ptrs(self.acc, P3))
                 self._les_stage(st, dt, c)
+            elif ust:
+                c.call("ec_ns_rk4_update_states", self.dt_code, n, l, st, dt, ptrs(self.u0, P3), ust_p,
+                       ptrs(self.du, P3), _abi.c_void_p(self.ps.p_ptr))
             else:
                 c.call("ec_ns_rk4_update", self.dt_code, n, l, st, dt, ptrs(self.u0, P3), ptrs(self.us, P3),
                        ptrs(self.du, P3), _abi.c_void_p(self.ps.p_ptr), ptrs(self.acc, P3))

@@ -130,6 +130,8 @@ ec_ns_filter_vel(ctx, dtype, n, l, v, p, nsets, sets) =
     ccall((:ec_ns_filter_vel, libexactclosure), Cint, (Ptr{Cvoid}, Cint, Cint, Cdouble, Ref{NTuple{3,Ptr{Cvoid}}}, Ptr{Cvoid}, Cint, Ptr{VelSet}), ctx, dtype, n, l, v, p, nsets, sets)
 ec_ns_filter_vel_rk4(ctx, dtype, n, l, du, p, nsets, sets, stage, dt, u0, us, acc) =
     ccall((:ec_ns_filter_vel_rk4, libexactclosure), Cint, (Ptr{Cvoid}, Cint, Cint, Cdouble, Ref{NTuple{3,Ptr{Cvoid}}}, Ptr{Cvoid}, Cint, Ptr{VelSet}, Cint, Cdouble, Ref{NTuple{3,Ptr{Cvoid}}}, Ref{NTuple{3,Ptr{Cvoid}}}, Ref{NTuple{3,Ptr{Cvoid}}}), ctx, dtype, n, l, du, p, nsets, sets, stage, dt, u0, us, acc)
+ec_ns_filter_vel_rk4_states(ctx, dtype, n, l, du, p, nsets, sets, stage, dt, u0, ust) =
+    ccall((:ec_ns_filter_vel_rk4_states, libexactclosure), Cint, (Ptr{Cvoid}, Cint, Cint, Cdouble, Ref{NTuple{3,Ptr{Cvoid}}}, Ptr{Cvoid}, Cint, Ptr{VelSet}, Cint, Cdouble, Ref{NTuple{3,Ptr{Cvoid}}}, Ref{NTuple{9,Ptr{Cvoid}}}), ctx, dtype, n, l, du, p, nsets, sets, stage, dt, u0, ust)
 ec_filter_vel_set_tiles(ctx, tx, tz) =
     ccall((:ec_filter_vel_set_tiles, libexactclosure), Cint, (Ptr{Cvoid}, Cint, Cint), ctx, tx, tz)
 ec_closure_set_tiles(ctx, tx, ty, tz) =
@@ -142,6 +144,8 @@ ec_ns_tensordissipation(ctx, nbar, l, sigma9, u, d) =
     ccall((:ec_ns_tensordissipation, libexactclosure), Cint, (Ptr{Cvoid}, Cint, Cdouble, Ref{NTuple{9,Ptr{Cvoid}}}, Ref{NTuple{3,Ptr{Cvoid}}}, Ptr{Cvoid}), ctx, nbar, l, sigma9, u, d)
 ec_ns_rk4_update(ctx, dtype, n, l, stage, dt, u0, us, du, p, acc) =
     ccall((:ec_ns_rk4_update, libexactclosure), Cint, (Ptr{Cvoid}, Cint, Cint, Cdouble, Cint, Cdouble, Ref{NTuple{3,Ptr{Cvoid}}}, Ref{NTuple{3,Ptr{Cvoid}}}, Ref{NTuple{3,Ptr{Cvoid}}}, Ptr{Cvoid}, Ref{NTuple{3,Ptr{Cvoid}}}), ctx, dtype, n, l, stage, dt, u0, us, du, p, acc)
+ec_ns_rk4_update_states(ctx, dtype, n, l, stage, dt, u0, ust, du, p) =
+    ccall((:ec_ns_rk4_update_states, libexactclosure), Cint, (Ptr{Cvoid}, Cint, Cint, Cdouble, Cint, Cdouble, Ref{NTuple{3,Ptr{Cvoid}}}, Ref{NTuple{9,Ptr{Cvoid}}}, Ref{NTuple{3,Ptr{Cvoid}}}, Ptr{Cvoid}), ctx, dtype, n, l, stage, dt, u0, ust, du, p)
 ec_burgers_batch_create(ctx, dtype, count, n, c, L, visc, cfl, kmax, seed, batch) =
     ccall((:ec_burgers_batch_create, libexactclosure), Cint, (Ptr{Cvoid}, Cint, Cint, Ptr{Cint}, Ptr{Cint}, Cdouble, Cdouble, Cdouble, Cint, UInt64, Ptr{Ptr{Cvoid}}), ctx, dtype, count, n, c, L, visc, cfl, kmax, seed, batch)
 ec_burgers_batch_destroy(ctx, batch) =

@@ -265,6 +265,12 @@ int ec_ns_filter_vel(ec_ctx *ctx, int dtype, int n, double l, void *const v[3], 
 int ec_ns_filter_vel_rk4(ec_ctx *ctx, int dtype, int n, double l, void *const du[3], const void *p,
                          int nsets, const ec_vel_set *sets, int stage, double dt, void *const u0[3],
                          void *const us[3], void *const acc[3]);
+/* ec_ns_filter_vel of du - grad p followed by ec_ns_rk4_update_states(stage,
+ * dt, u0, ust, du, p), fused like ec_ns_filter_vel_rk4.  No counterpart entry
+ * in the reference. */
+int ec_ns_filter_vel_rk4_states(ec_ctx *ctx, int dtype, int n, double l, void *const du[3],
+                                const void *p, int nsets, const ec_vel_set *sets, int stage, double dt,
+                                void *const u0[3], void *const ust[9]);
 /* Target block size (x = y, z) of ec_ns_filter_vel and ec_ns_filter_vel_rk4 on
  * this context (default z 256); set by the tile autotuner. */
 int ec_filter_vel_set_tiles(ec_ctx *ctx, int tx, int tz);
@@ -305,6 +311,17 @@ int ec_ns_tensordissipation(ec_ctx *ctx, int nbar, double l, void *const sigma9[
 int ec_ns_rk4_update(ec_ctx *ctx, int dtype, int n, double l, int stage, double dt,
                      void *const u0[3], void *const us[3], void *const du[3], const void *p,
                      void *const acc[3]);
+/* The same RK4 step in stored stage states (no accumulator): k = du - grad p;
+ * stage s = 0..2 writes its state u_{s+1} = u0 + c_s dt k (c = 1/2, 1/2, 1)
+ * to ust[3s..3s+2]; stage 3 reads the three states and writes
+ * u0 = u0 + ((u1 - u0) + 2 (u2 - u0) + (u3 - u0)) / 3 + dt/6 k, which is
+ * u0 + dt/6 (k1 + 2 k2 + 2 k3 + k4) of rk4_timestep! (src/Burgers.jl:120-142;
+ * projection at every stage as in step_wray3!, src/N.jl:320-346).  The states
+ * u1..u3 equal the stage states us of ec_ns_rk4_update bit for bit.  Per step
+ * the update moves 49 fields instead of 58 (no accumulator read and written
+ * at every stage). */
+int ec_ns_rk4_update_states(ec_ctx *ctx, int dtype, int n, double l, int stage, double dt,
+                            void *const u0[3], void *const ust[9], void *const du[3], const void *p);
 
 /* ---- batched 1D Burgers DNS-aided LES (src/Burgers.jl:89-142,350-445) ----
  * count independent instances, instance i: n[i] points, compression c[i],

@@ -271,6 +271,75 @@ def test_filter_vel_rk4_fused_large(gpu, n, stage):
     torch.cuda.empty_cache()
 
 
+@pytest.mark.parametrize("n,prec,sets", [(64, "f64", [(16, 0), (8, 0), (4, 0), (4, 1)]),
+                                         (48, "f32", [(3, 0), (3, 1), (16, 1), (2, 0)]),
+                                         (512, "f32", [(16, 0), (8, 0), (4, 0), (4, 1)]),
+                                         (1024, "f32", [(16, 0), (8, 0), (4, 0), (4, 1)])])
+def test_filter_vel_rk4_states_fused(gpu, n, prec, sets):
+    """ec_ns_filter_vel_rk4_states over the four stages of a step equals
+    ec_ns_filter_vel followed by ec_ns_rk4_update_states on the same fields:
+    the filter outputs bit for bit, the stage states u1, u2, u3 and the new
+    u0 to rounding (FMA contraction may differ between the kernels; fused
+    for the volume-16/8/4 + face-4 list, at the benchmark's 256-plane chunks
+    for n >= 512; the other list takes the two calls).  Each stage gets its
+    own du and p."""
+    dt = torch.float64 if prec == "f64" else torch.float32
+    code = 1 if prec == "f64" else 0
+    gen = torch.Generator(device="cuda").manual_seed(3 * n + 1)
+    f = lambda: torch.randn((n, n, n), device="cuda", dtype=dt, generator=gen)  # noqa: E731
+    u0 = [f() for _ in range(3)]
+    ua, ub = [x.clone() for x in u0], [x.clone() for x in u0]
+    del u0
+    sta = [[torch.empty_like(ua[0]) for _ in range(3)] for _ in range(3)]
+    stb = [[torch.empty_like(ua[0]) for _ in range(3)] for _ in range(3)]
+    pa, pb = P([x for s_ in sta for x in s_], 9), P([x for s_ in stb for x in s_], 9)
+
+    def outsets():
+        outs = []
+        arr = (gpu._abi.VelSet * len(sets))()
+        for i, (c, kind) in enumerate(sets):
+            o = [torch.empty((n // c,) * 3, dtype=torch.float64, device="cuda") for _ in range(3)]
+            outs.append(o)
+            arr[i].nbar = n // c
+            arr[i].kind = kind
+            for a in range(3):
+                arr[i].out[a] = o[a].data_ptr()
+        return outs, arr
+
+    ctx = gpu._abi.context()
+    step = 1e-3 if n <= 64 else 1e-4
+    small = n <= 64
+    tol = 1e-14 if prec == "f64" else 1e-6
+    if small:
+        ctx.call("ec_filter_vel_set_tiles", 16, 16)
+    try:
+        for st in range(4):
+            du = [f() for _ in range(3)]
+            p = f()
+            oa, arra = outsets()
+            ob, arrb = outsets()
+            ctx.call("ec_ns_filter_vel", code, n, 1.0, P(du, 3), p.data_ptr(), len(sets), arra)
+            ctx.call("ec_ns_rk4_update_states", code, n, 1.0, st, step, P(ua, 3), pa, P(du, 3), p.data_ptr())
+            ctx.call("ec_ns_filter_vel_rk4_states", code, n, 1.0, P(du, 3), gpu._abi.c_void_p(p.data_ptr()),
+                     len(sets), arrb, st, step, P(ub, 3), pb)
+            torch.cuda.synchronize()
+            for x, y in zip(oa, ob):
+                for a in range(3):
+                    assert torch.equal(x[a], y[a]), (st, a)
+            if st < 3:
+                for a in range(3):
+                    x, y = sta[st][a], stb[st][a]
+                    assert float((x - y).abs().max()) <= tol * float(x.abs().max()), (st, a)
+            del du, p, oa, ob
+    finally:
+        if small:
+            ctx.call("ec_filter_vel_set_tiles", 16, 256)
+    for x, y in zip(ua, ub):
+        assert float((x - y).abs().max()) <= tol * float(x.abs().max())
+    del ua, ub, sta, stb
+    torch.cuda.empty_cache()
+
+
 @pytest.mark.parametrize("n,sets,prec", [(256, [(16, "va"), (32, "va"), (64, "va"), (64, "fa")], "f32"),
                                          (32, [(8, "va"), (4, "fa")], "f64")])
 def test_captured_step_replays_the_eager_step(gpu, O, n, sets, prec):

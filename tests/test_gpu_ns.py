@@ -442,6 +442,56 @@ def test_rk4_update_large_fp32(gpu, stage):
     torch.cuda.empty_cache()
 
 
+@pytest.mark.parametrize("n,prec", [(12, "f64"), (13, "f64"), (64, "f32"), (512, "f32")])
+def test_rk4_update_states_matches_accumulator_form(gpu, O, n, prec):
+    """Four stages of the stored-state RK4 (ec_ns_rk4_update_states) against
+    the accumulator form (ec_ns_rk4_update) fed the same stage tendencies
+    (du, p per stage): the stage states u1, u2, u3 equal the accumulator
+    form's us of that stage bit for bit (same arithmetic), and the new u0
+    agrees to rounding of the stored states (FP64 1e-14; FP32 2 ulp of |u|).
+    n = 13 takes the one-cell-per-thread path (not 16-byte aligned rows), the
+    others the 16-byte one.  At n = 12 the new u0 is also checked against the
+    checker's pressure gradient and the textbook tableau in FP64."""
+    dt_t = torch.float64 if prec == "f64" else torch.float32
+    code = 1 if prec == "f64" else 0
+    step, l = 0.01, 1.0
+    gen = torch.Generator(device="cuda").manual_seed(7 * n)
+    f = lambda: torch.randn((n, n, n), device="cuda", dtype=dt_t, generator=gen)  # noqa: E731
+    u0 = [f() for _ in range(3)]
+    dus = [[f() for _ in range(3)] for _ in range(4)]
+    ps_ = [f() for _ in range(4)]
+    ctx = gpu._abi.context()
+    # accumulator form
+    ua, us, acc = [x.clone() for x in u0], [torch.empty_like(x) for x in u0], [torch.empty_like(x) for x in u0]
+    us_seen = []
+    # stored-state form
+    ub = [x.clone() for x in u0]
+    ust = [[torch.empty_like(x) for x in u0] for _ in range(3)]
+    ust_p = P([x for s_ in ust for x in s_], 9)
+    for st in range(4):
+        ctx.call("ec_ns_rk4_update", code, n, l, st, step, P(ua, 3), P(us, 3), P(dus[st], 3), ps_[st].data_ptr(),
+                 P(acc, 3))
+        if st < 3:
+            us_seen.append([x.clone() for x in us])
+        ctx.call("ec_ns_rk4_update_states", code, n, l, st, step, P(ub, 3), ust_p, P(dus[st], 3),
+                 ps_[st].data_ptr())
+    torch.cuda.synchronize()
+    for st in range(3):
+        for a in range(3):
+            assert torch.equal(ust[st][a], us_seen[st][a]), (st, a)
+    tol = 1e-14 if prec == "f64" else 2.5e-7
+    for a in range(3):
+        assert float((ub[a] - ua[a]).abs().max()) <= tol * float(ua[a].abs().max()), a
+    if n == 12:
+        # the textbook tableau on the checker's k = du - grad p of every stage
+        ks = [O.pressuregradient([host(x) for x in dus[st]], host(ps_[st])) for st in range(4)]
+        for a in range(3):
+            want = host(u0[a]) + step / 6 * (ks[0][a] + 2 * ks[1][a] + 2 * ks[2][a] + ks[3][a])
+            assert rel(host(ub[a]), want) < 1e-13
+    del u0, dus, ps_, ua, us, acc, ub, ust, us_seen
+    torch.cuda.empty_cache()
+
+
 @pytest.mark.parametrize("n,prec", [(16, "f64"), (64, "f64"), (256, "f32")])
 def test_project_into(gpu, n, prec):
     """dst = P src leaves src unchanged and equals copy + in-place projection
