@@ -384,6 +384,18 @@ constexpr int ct_radix(int m, int maxr = 16) {
 // LDS padding: one spare element every 2^EC_FFT_PAD_SHIFT elements
 __device__ __forceinline__ constexpr int pad8(int e) { return e + (e >> EC_FFT_PAD_SHIFT); }
 constexpr int padded_len(int n) { return n + (n >> EC_FFT_PAD_SHIFT); }
+// pad8(x + r S) = pad8(x) + pad_off<S>(r) for the accesses of the power-of-two
+// stages: with P = 2^EC_FFT_PAD_SHIFT and S, P powers of two, either S is a
+// multiple of P, or S divides P and x + r S stays inside the period that
+// r S starts in (reads: x = b < S; writes: x = idxD = S R m + jm, jm < S,
+// with S R a multiple or a divisor of P; tile rows: x = e0 < S).  So every
+// LDS access is a per-thread base plus a compile-time offset, with no
+// per-element index arithmetic.
+template <int S>
+__device__ __forceinline__ constexpr int pad_off(int r) {
+    static_assert((S & (S - 1)) == 0, "power-of-two stride");
+    return r * S + ((r * S) >> EC_FFT_PAD_SHIFT);
+}
 
 // Ordering between the stages of a line: a block barrier, or (WAVE) when one
 // wave owns the line, only the completion of its own LDS operations (a wave's
@@ -398,6 +410,7 @@ __device__ __forceinline__ void line_sync() {
 // MAXR caps the radix (a 1024-thread block keeps radix 8 to stay within 128 VGPRs)
 template <typename T, int N, int NS, int NTHR, int LS, int SIGN, int MAXR = 16, bool WAVE = false>
 __device__ __forceinline__ void ct_stages(cplx<T> *lds, int t, const cplx<T> *__restrict__ tw) {
+    static_assert((N & (N - 1)) == 0, "power-of-two length");
     if constexpr (NS < N) {
         constexpr int R = ct_radix(N / NS, MAXR);
         constexpr int NR = N / R;
@@ -409,9 +422,10 @@ __device__ __forceinline__ void ct_stages(cplx<T> *lds, int t, const cplx<T> *__
             const int b = t + q * NTHR;
             if (NR % NTHR == 0 || b < NR) {
                 const int jm = b & (NS - 1);
+                const int pb = pad8(b) * LS;
 #pragma unroll
                 for (int r = 0; r < R; r++) {
-                    cplx<T> x = lds[pad8(b + r * NR) * LS];
+                    cplx<T> x = lds[pb + pad_off<NR>(r) * LS];
                     if (NS > 1 && r > 0) {
                         cplx<T> w = tw[r * jm * TSTEP];
                         if (SIGN > 0) w.y = -w.y;
@@ -428,9 +442,9 @@ __device__ __forceinline__ void ct_stages(cplx<T> *lds, int t, const cplx<T> *__
             const int b = t + q * NTHR;
             if (NR % NTHR == 0 || b < NR) {
                 const int jm = b & (NS - 1);
-                const int idxD = (b - jm) * R + jm;
+                const int pd = pad8((b - jm) * R + jm) * LS;
 #pragma unroll
-                for (int r = 0; r < R; r++) lds[pad8(idxD + r * NS) * LS] = v[q][r];
+                for (int r = 0; r < R; r++) lds[pd + pad_off<NS>(r) * LS] = v[q][r];
             }
         }
         line_sync<WAVE>();
@@ -502,9 +516,10 @@ struct RegStages<T, N, NS, NTHR, MAXR, true> {
                         if (r != low) pw[r] = cmul(pw[r - low], pw[low]);
                     }
                 }
+                const int pb = pad8(b);
 #pragma unroll
                 for (int r = 0; r < R; r++) {
-                    cplx<T> x = lds[pad8(b + r * NR)];
+                    cplx<T> x = lds[pb + pad_off<NR>(r)];
                     if (NS > 1 && r > 0) {
                         cplx<T> ww = pw[r];
                         if (SIGN > 0) ww.y = -ww.y;
@@ -521,9 +536,9 @@ struct RegStages<T, N, NS, NTHR, MAXR, true> {
             const int b = t + q * NTHR;
             if (NR % NTHR == 0 || b < NR) {
                 const int jm = b & (NS - 1);
-                const int idxD = (b - jm) * R + jm;
+                const int pd = pad8((b - jm) * R + jm);
 #pragma unroll
-                for (int r = 0; r < R; r++) lds[pad8(idxD + r * NS)] = v[q][r];
+                for (int r = 0; r < R; r++) lds[pd + pad_off<NS>(r)] = v[q][r];
             }
         }
         __syncthreads();
@@ -560,7 +575,7 @@ __global__ void __launch_bounds__(xblock(NTHR)) k_fft_x_r2c_ct(const T *__restri
         if (g0 + (long)g * W >= npairs) break;  // uniform over the block
         const long pr = g0 + (long)g * W + pair;
 #pragma unroll
-        for (int q = 0; q < PER; q++) line[pad8(t + q * NTHR)] = cplx<T>{ra[q], rb[q]};
+        for (int q = 0; q < PER; q++) line[pad8(t) + pad_off<NTHR>(q)] = cplx<T>{ra[q], rb[q]};
         if (g + 1 < G) fetch(pr + W);
         __syncthreads();
         st.template run<-1>(line, t);
@@ -636,7 +651,7 @@ __global__ void __launch_bounds__(xblock(NTHR)) k_fft_x_c2r_ct(const cplx<T> *__
 #pragma unroll
         for (int q = 0; q < PER; q++) {
             const int e = t + q * NTHR;
-            const cplx<T> z = line[pad8(e)];
+            const cplx<T> z = line[pad8(t) + pad_off<NTHR>(q)];
             if (v0) ec_nt_store(oa + e, z.x * scale);
             if (v1) ec_nt_store(ob + e, z.y * scale);
         }
@@ -670,14 +685,22 @@ __global__ void __launch_bounds__(TK *NTHR) k_fft_cols_ct(cplx<T> *__restrict__ 
     const int outer = L / ntiles;
     const int c = threadIdx.x % TK, t = threadIdx.x / TK;
     cplx<T> *base = data + col0 + (long)outer * outer_stride;
-    const bool full = col0 + TK <= ncols;
     constexpr int TOTAL = N * TK;
     constexpr int TBQ = (N + TK * NTHR - 1) / (TK * NTHR);  // table entries per thread
     cplx<T> *twl = lds + (size_t)padded_len(N) * TK;         // MODE 1: N twiddles, then N eigenvalues
     T *eigl = reinterpret_cast<T *>(twl + N);
+    // element q of a thread is (column cc, row e0 + q NTHR): a uniform row
+    // base plus one loop-invariant 32-bit byte offset per lane, and in LDS a
+    // per-thread base plus a compile-time offset
+    static_assert(TOTAL % (TK * NTHR) == 0, "whole rows per thread");
+    constexpr int PER = TOTAL / (TK * NTHR);
+    const int cc = threadIdx.x % TK, e0 = threadIdx.x / TK;
+    const bool cok = col0 + cc < ncols;
+    const unsigned voff = (unsigned)((cc + (long)e0 * elem_stride) * (long)sizeof(cplx<T>));
+    const long qstep = (long)NTHR * elem_stride * (long)sizeof(cplx<T>);
+    const int lrow = pad8(e0) * TK + cc;
     // all loads of the tile (and of the tables) in flight at once
     {
-        constexpr int PER = TOTAL / (TK * NTHR);
         cplx<T> tmp[PER];
         cplx<T> twr[TBQ];
         T eir[TBQ];
@@ -692,10 +715,8 @@ __global__ void __launch_bounds__(TK *NTHR) k_fft_cols_ct(cplx<T> *__restrict__ 
             }
         }
 #pragma unroll
-        for (int q = 0; q < PER; q++) {
-            const int idx = threadIdx.x + q * TK * NTHR, cc = idx % TK, e = idx / TK;
-            tmp[q] = (full || col0 + cc < ncols) ? base[cc + e * elem_stride] : cplx<T>{T(0), T(0)};
-        }
+        for (int q = 0; q < PER; q++)
+            tmp[q] = cok ? *(const cplx<T> *)((const char *)base + q * qstep + voff) : cplx<T>{T(0), T(0)};
         if constexpr (MODE == 1) {
 #pragma unroll
             for (int q = 0; q < TBQ; q++) {
@@ -707,10 +728,7 @@ __global__ void __launch_bounds__(TK *NTHR) k_fft_cols_ct(cplx<T> *__restrict__ 
             }
         }
 #pragma unroll
-        for (int q = 0; q < PER; q++) {
-            const int idx = threadIdx.x + q * TK * NTHR, cc = idx % TK, e = idx / TK;
-            lds[pad8(e) * TK + cc] = tmp[q];
-        }
+        for (int q = 0; q < PER; q++) lds[lrow + pad_off<NTHR>(q) * TK] = tmp[q];
     }
     __syncthreads();
     if (MODE == 0) {
@@ -719,27 +737,30 @@ __global__ void __launch_bounds__(TK *NTHR) k_fft_cols_ct(cplx<T> *__restrict__ 
         ct_stages<T, N, 1, NTHR, TK, -1, (TK * NTHR > 512 ? 8 : 16)>(lds + c, t, twl);
         const int kx = col0 + c;
         const T sxy = (kx < ncols ? eigl[kx] : T(0)) + eigl[outer];
+        const int lt = pad8(t) * TK + c;
 #pragma unroll
-        for (int e = t; e < N; e += NTHR) {
+        for (int k = 0; k < N / NTHR; k++) {
+            const int e = t + k * NTHR;
             const T denom = poisson_a * (sxy + eigl[e]);
             const bool zero_mode = (kx == 0 && outer == 0 && e == 0);
             const T f = zero_mode ? T(0) : poisson_factor(scale, denom);
-            cplx<T> z = lds[pad8(e) * TK + c];
+            cplx<T> z = lds[lt + pad_off<NTHR>(k) * TK];
             z.x *= f;
             z.y *= f;
-            lds[pad8(e) * TK + c] = z;
+            lds[lt + pad_off<NTHR>(k) * TK] = z;
         }
         __syncthreads();
         ct_stages<T, N, 1, NTHR, TK, +1, (TK * NTHR > 512 ? 8 : 16)>(lds + c, t, twl);
     }
+    if (cok) {
 #pragma unroll
-    for (int idx = threadIdx.x; idx < TOTAL; idx += TK * NTHR) {
-        const int cc = idx % TK, e = idx / TK;
-        if (full || col0 + cc < ncols) {
+        for (int q = 0; q < PER; q++) {
+            cplx<T> *dst = (cplx<T> *)((char *)base + q * qstep + voff);
+            const cplx<T> z = lds[lrow + pad_off<NTHR>(q) * TK];
             // rows shorter than a 128-byte line: plain stores, so that L2 joins
             // the halves the neighbouring tile writes before the line goes out
-            if constexpr (TK * sizeof(cplx<T>) < 128) base[cc + e * elem_stride] = lds[pad8(e) * TK + cc];
-            else ec_nt_store(base + cc + e * elem_stride, lds[pad8(e) * TK + cc]);
+            if constexpr (TK * sizeof(cplx<T>) < 128) *dst = z;
+            else ec_nt_store(dst, z);
         }
     }
 }
@@ -768,29 +789,31 @@ __global__ void __launch_bounds__(TK * 64) k_fft_cols_w(cplx<T> *__restrict__ da
     const int outer = L / ntiles;
     const int c = threadIdx.x / 64, t = threadIdx.x % 64;
     cplx<T> *base = data + col0 + (long)outer * outer_stride;
-    const bool full = col0 + TK <= ncols;
+    // element q of a thread: column cc, row e0 + 64 q (uniform row base, one
+    // 32-bit lane offset; LDS base plus a compile-time offset)
+    const int cc = threadIdx.x % TK, e0 = threadIdx.x / TK;
+    const bool cok = col0 + cc < ncols;
+    const unsigned voff = (unsigned)((cc + (long)e0 * elem_stride) * (long)sizeof(cplx<T>));
+    const long qstep = 64L * elem_stride * (long)sizeof(cplx<T>);
+    const int lrow = cc * CS + pad8(e0);
     {
         cplx<T> tmp[PER];
 #pragma unroll
-        for (int q = 0; q < PER; q++) {
-            const int idx = threadIdx.x + q * TK * 64, cc = idx % TK, e = idx / TK;
-            tmp[q] = (full || col0 + cc < ncols) ? base[cc + e * elem_stride] : cplx<T>{T(0), T(0)};
-        }
+        for (int q = 0; q < PER; q++)
+            tmp[q] = cok ? *(const cplx<T> *)((const char *)base + q * qstep + voff) : cplx<T>{T(0), T(0)};
 #pragma unroll
-        for (int q = 0; q < PER; q++) {
-            const int idx = threadIdx.x + q * TK * 64, cc = idx % TK, e = idx / TK;
-            lds[cc * CS + pad8(e)] = tmp[q];
-        }
+        for (int q = 0; q < PER; q++) lds[lrow + pad_off<64>(q)] = tmp[q];
     }
     __syncthreads();
     ct_stages<T, N, 1, 64, 1, SIGN, 16, true>(lds + c * CS, t, tw);
     __syncthreads();
+    if (cok) {
 #pragma unroll
-    for (int q = 0; q < PER; q++) {
-        const int idx = threadIdx.x + q * TK * 64, cc = idx % TK, e = idx / TK;
-        if (full || col0 + cc < ncols) {
-            if constexpr (TK * sizeof(cplx<T>) < 128) base[cc + e * elem_stride] = lds[cc * CS + pad8(e)];
-            else ec_nt_store(base + cc + e * elem_stride, lds[cc * CS + pad8(e)]);
+        for (int q = 0; q < PER; q++) {
+            cplx<T> *dst = (cplx<T> *)((char *)base + q * qstep + voff);
+            const cplx<T> z = lds[lrow + pad_off<64>(q)];
+            if constexpr (TK * sizeof(cplx<T>) < 128) *dst = z;
+            else ec_nt_store(dst, z);
         }
     }
 }
@@ -1190,6 +1213,8 @@ extern "C" int ec_poisson_solve(ec_ctx *ctx, ec_poisson *ps) {
 
 extern "C" int ec_fft_r2c(ec_ctx *ctx, ec_poisson *ps, const void *in, void *out) {
     EC_REQUIRE(ctx, ctx && ps && in && out, "ec_fft_r2c: null argument");
+    EC_REQUIRE(ctx, (uintptr_t)out % (ps->dtype == EC_F64 ? 16 : 8) == 0,
+               "ec_fft_r2c: spectrum not aligned to its complex element size");
     // a caller's buffer is the packed (nh, n, n) spectrum; the solver's own is pitched
     const int sp = out == ps->spec ? ps->pitch : ps->nh;
     int rc;

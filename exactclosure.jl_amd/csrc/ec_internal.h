@@ -91,8 +91,10 @@ struct ec_prof_scope {
 };
 
 // ---- complex helpers ----
+// Aligned to its size, so that a complex value moves as one 8- (FP32) or
+// 16-byte (FP64) access; LDS accesses then take 16-bit immediate offsets.
 template <typename T>
-struct cplx {
+struct alignas(2 * sizeof(T)) cplx {
     T x, y;
 };
 
