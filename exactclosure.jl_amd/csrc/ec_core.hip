@@ -28,6 +28,8 @@ extern "C" int ec_create(ec_ctx **out, int device, void *stream) {
     ctx->stream = (hipStream_t)stream;  // NULL: the device's default stream
     ctx->own_stream = false;
     ctx->red_cap = 4096;
+    if (hipDeviceGetAttribute(&ctx->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || ctx->ncu < 1)
+        ctx->ncu = 256;
     if (hipMalloc(&ctx->red_partial, sizeof(double) * ctx->red_cap) != hipSuccess ||
         hipHostMalloc(&ctx->red_host, sizeof(double) * 4) != hipSuccess) {
         delete ctx;

@@ -12,6 +12,7 @@
 // Twiddles and sin^2 eigenvalue tables are computed on the host in FP64.
 #include "ec_internal.h"
 
+#include <algorithm>
 #include <cmath>
 
 namespace {
@@ -407,9 +408,16 @@ __device__ __forceinline__ void line_sync() {
     else __syncthreads();
 }
 
+// A hook on the values the first stage reads: pre(e, x) for element e (the
+// Poisson pass scales the spectrum as its inverse transform loads it).
+struct NoPre {
+    template <typename X>
+    __device__ __forceinline__ X operator()(int, X x) const { return x; }
+};
+
 // MAXR caps the radix (a 1024-thread block keeps radix 8 to stay within 128 VGPRs)
-template <typename T, int N, int NS, int NTHR, int LS, int SIGN, int MAXR = 16, bool WAVE = false>
-__device__ __forceinline__ void ct_stages(cplx<T> *lds, int t, const cplx<T> *__restrict__ tw) {
+template <typename T, int N, int NS, int NTHR, int LS, int SIGN, int MAXR = 16, bool WAVE = false, class PRE = NoPre>
+__device__ __forceinline__ void ct_stages(cplx<T> *lds, int t, const cplx<T> *__restrict__ tw, const PRE &pre = PRE()) {
     static_assert((N & (N - 1)) == 0, "power-of-two length");
     if constexpr (NS < N) {
         constexpr int R = ct_radix(N / NS, MAXR);
@@ -426,6 +434,7 @@ __device__ __forceinline__ void ct_stages(cplx<T> *lds, int t, const cplx<T> *__
 #pragma unroll
                 for (int r = 0; r < R; r++) {
                     cplx<T> x = lds[pb + pad_off<NR>(r) * LS];
+                    if constexpr (NS == 1) x = pre(b + r * NR, x);
                     if (NS > 1 && r > 0) {
                         cplx<T> w = tw[r * jm * TSTEP];
                         if (SIGN > 0) w.y = -w.y;
@@ -448,7 +457,7 @@ __device__ __forceinline__ void ct_stages(cplx<T> *lds, int t, const cplx<T> *__
             }
         }
         line_sync<WAVE>();
-        ct_stages<T, N, NS * R, NTHR, LS, SIGN, MAXR, WAVE>(lds, t, tw);
+        ct_stages<T, N, NS * R, NTHR, LS, SIGN, MAXR, WAVE, NoPre>(lds, t, tw);
     }
 }
 
@@ -667,6 +676,22 @@ __global__ void __launch_bounds__(xblock(NTHR)) k_fft_x_c2r_ct(const cplx<T> *__
 // global memory they are small scattered loads, issued between the stages and
 // between the two transforms, whose latency under the streaming traffic of
 // the other blocks left the pass at half the speed of a single transform.
+// The Poisson scale of bin (kx, ky, kz) = 1 / (a (s_x + s_y + s_z)), s the
+// eigenvalues sin^2(pi k / N) (kx, ky fixed per column: sxy = s_x + s_y), and 0
+// for the zero mode; applied by the inverse transform as it loads bin kz = e.
+template <typename T>
+struct PoissonScale {
+    T a, scale, sxy;
+    const T *eig;   // LDS
+    bool origin;    // the column holds the zero mode (kx = ky = 0)
+    __device__ __forceinline__ cplx<T> operator()(int e, cplx<T> z) const {
+        const T denom = a * (sxy + eig[e]);
+        const T f = (origin && e == 0) ? T(0) : poisson_factor(scale, denom);
+        return cplx<T>{z.x * f, z.y * f};
+    }
+};
+__device__ __forceinline__ bool kx_valid(int kx, int ncols) { return kx < ncols; }
+
 constexpr size_t lds_cols_ct(int n, int tk, int mode, size_t tsize) {
     return (size_t)padded_len(n) * tk * 2 * tsize + (mode == 1 ? (size_t)n * 3 * tsize : 0);
 }
@@ -735,22 +760,10 @@ __global__ void __launch_bounds__(TK *NTHR) k_fft_cols_ct(cplx<T> *__restrict__ 
         ct_stages<T, N, 1, NTHR, TK, SIGN>(lds + c, t, tw);
     } else {
         ct_stages<T, N, 1, NTHR, TK, -1, (TK * NTHR > 512 ? 8 : 16)>(lds + c, t, twl);
-        const int kx = col0 + c;
-        const T sxy = (kx < ncols ? eigl[kx] : T(0)) + eigl[outer];
-        const int lt = pad8(t) * TK + c;
-#pragma unroll
-        for (int k = 0; k < N / NTHR; k++) {
-            const int e = t + k * NTHR;
-            const T denom = poisson_a * (sxy + eigl[e]);
-            const bool zero_mode = (kx == 0 && outer == 0 && e == 0);
-            const T f = zero_mode ? T(0) : poisson_factor(scale, denom);
-            cplx<T> z = lds[lt + pad_off<NTHR>(k) * TK];
-            z.x *= f;
-            z.y *= f;
-            lds[lt + pad_off<NTHR>(k) * TK] = z;
-        }
-        __syncthreads();
-        ct_stages<T, N, 1, NTHR, TK, +1, (TK * NTHR > 512 ? 8 : 16)>(lds + c, t, twl);
+        // the Poisson scale and zero mode, applied as the inverse loads each bin
+        const PoissonScale<T> ps{poisson_a, scale, (kx_valid(col0 + c, ncols) ? eigl[col0 + c] : T(0)) + eigl[outer],
+                                 eigl, col0 + c == 0 && outer == 0};
+        ct_stages<T, N, 1, NTHR, TK, +1, (TK * NTHR > 512 ? 8 : 16)>(lds + c, t, twl, ps);
     }
     if (cok) {
 #pragma unroll
@@ -818,6 +831,73 @@ __global__ void __launch_bounds__(TK * 64) k_fft_cols_w(cplx<T> *__restrict__ da
     }
 }
 
+// Persistent form of the plain column pass above (same tiles, threads and
+// stages, so the same bits).  A block that owns one tile loads it, transforms
+// it and stores it one phase after the other, and with one or two blocks per
+// CU the CU's memory traffic stops while its tiles are transformed.  Here the
+// grid holds as many blocks as fit on the device at once; each walks the
+// tiles L = blockIdx.x, + gridDim.x, ... and, as soon as tile L is in LDS,
+// issues the global loads of its next tile into registers, which arrive
+// while tile L is transformed and written back.  The twiddles are staged in
+// LDS once per block: a global table load among the stages would wait, in the
+// in-order vector-memory counter, for the whole prefetch issued before it.
+// (The fused Poisson pass stays with one block per tile: its two transforms
+// and a 16-value prefetch per lane need more than the 128 VGPRs of a
+// 1024-thread block.)
+template <typename T, int N, int TK, int SIGN>
+__global__ void __launch_bounds__(TK * 64) k_fft_cols_wp(cplx<T> *__restrict__ data, int ncols, int ntiles, int ntot,
+                                                        long elem_stride, long outer_stride,
+                                                        const cplx<T> *__restrict__ tw) {
+    constexpr int CS = col_stride(N, TK), TOTAL = N * TK, NT = TK * 64, PER = TOTAL / NT;
+    extern __shared__ char smem_raw[];
+    cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
+    cplx<T> *twl = lds + (size_t)CS * TK;  // the N twiddles
+    for (int i = threadIdx.x; i < N; i += NT) twl[i] = tw[i];
+    const int c = threadIdx.x / 64, t = threadIdx.x % 64;
+    const int cc = threadIdx.x % TK, e0 = threadIdx.x / TK;
+    const unsigned voff = (unsigned)((cc + (long)e0 * elem_stride) * (long)sizeof(cplx<T>));
+    const long qstep = 64L * elem_stride * (long)sizeof(cplx<T>);
+    const int lrow = cc * CS + pad8(e0);
+    cplx<T> buf[PER];
+    auto load = [&](int L) {
+        const int col0 = (L % ntiles) * TK, outer = L / ntiles;
+        const char *base = (const char *)(data + col0 + (long)outer * outer_stride);
+        const bool ok = col0 + cc < ncols;
+#pragma unroll
+        for (int q = 0; q < PER; q++)
+            buf[q] = ok ? *(const cplx<T> *)(base + q * qstep + voff) : cplx<T>{T(0), T(0)};
+    };
+    int L = blockIdx.x;
+    if (L < ntot) load(L);
+    for (; L < ntot; L += gridDim.x) {
+#pragma unroll
+        for (int q = 0; q < PER; q++) lds[lrow + pad_off<64>(q)] = buf[q];
+        __syncthreads();
+        if (L + (int)gridDim.x < ntot) load(L + gridDim.x);
+        const int col0 = (L % ntiles) * TK, outer = L / ntiles;
+        ct_stages<T, N, 1, 64, 1, SIGN, 16, true>(lds + c * CS, t, twl);
+        __syncthreads();
+        if (col0 + cc < ncols) {
+            char *base = (char *)(data + col0 + (long)outer * outer_stride);
+#pragma unroll
+            for (int q = 0; q < PER; q++) {
+                cplx<T> *dst = (cplx<T> *)(base + q * qstep + voff);
+                const cplx<T> z = lds[lrow + pad_off<64>(q)];
+                if constexpr (TK * sizeof(cplx<T>) < 128) *dst = z;
+                else ec_nt_store(dst, z);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// blocks of a persistent launch: as many as fit on the device at once (by
+// LDS per block), at most one per tile
+inline long persistent_grid(ec_ctx *ctx, size_t lds_bytes, long ntot) {
+    const long per_cu = std::max<long>(1, (long)(160 * 1024 / std::max<size_t>(lds_bytes, 1)));
+    return std::min<long>(ntot, (long)std::max(1, ctx->ncu) * per_cu);
+}
+
 // groups per block of the x passes: enough blocks to fill the device several times
 inline int x_groups(long npairs, int W) {
     if (npairs / (16L * W) >= 8192) return 16;
@@ -881,6 +961,18 @@ int cols_w(ec_ctx *ctx, ec_poisson *ps, void *spec, int axis, int pitch) {
     const long outer_stride = axis == 1 ? (long)pitch * N : (long)pitch;
     const int ntiles = (nh + TK - 1) / TK;
     const size_t lds = (size_t)col_stride(N, TK) * TK * sizeof(cplx<T>);
+    const long ntot = (long)ntiles * N;
+    if (ctx->fft_cols_persistent && ntot <= 0x7fffffffL) {
+        const size_t ldsp = lds + (size_t)N * sizeof(cplx<T>);  // + the twiddle table
+        if (ldsp <= 160 * 1024) {
+            EC_HIP(ctx, hipFuncSetAttribute((const void *)k_fft_cols_wp<T, N, TK, SIGN>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsp));
+            k_fft_cols_wp<T, N, TK, SIGN><<<(unsigned)persistent_grid(ctx, ldsp, ntot), TK * 64, ldsp, ctx->stream>>>(
+                (cplx<T> *)spec, nh, ntiles, (int)ntot, elem_stride, outer_stride, (const cplx<T> *)ps->twiddle);
+            EC_CHECK_LAUNCH(ctx, "k_fft_cols_wp");
+            return EC_OK;
+        }
+    }
     EC_HIP(ctx, hipFuncSetAttribute((const void *)k_fft_cols_w<T, N, TK, SIGN>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     k_fft_cols_w<T, N, TK, SIGN><<<(unsigned)(ntiles * N), TK * 64, lds, ctx->stream>>>(
@@ -1106,6 +1198,12 @@ int project_into_t(ec_ctx *ctx, ec_poisson *ps, const void *const src[3], void *
 }
 
 }  // namespace
+
+extern "C" int ec_fft_set_cols_persistent(ec_ctx *ctx, int on) {
+    EC_REQUIRE(ctx, ctx && (on == 0 || on == 1), "ec_fft_set_cols_persistent: bad argument");
+    ctx->fft_cols_persistent = on;
+    return EC_OK;
+}
 
 extern "C" int ec_poisson_create(ec_ctx *ctx, int dtype, int n, double l, ec_poisson **out) {
     EC_REQUIRE(ctx, ctx && out, "ec_poisson_create: null argument");

@@ -36,6 +36,11 @@ struct ec_ctx {
     // z planes per block of the velocity filter (256 is fastest at 1024^3) and
     // the x/y/z block targets of the flux-form closure kernel.
     int fv_tz = 256;
+    // compute units of the device (persistent launches size their grid by it)
+    int ncu = 0;
+    // column passes of large FP32 spectra: persistent blocks that prefetch
+    // their next tile (1) or one block per tile (0); the same bits either way
+    int fft_cols_persistent = 1;
     int cl_tx = 16, cl_ty = 16, cl_tz = 64;
     // Workspace of the separable filter passes (grown on demand, FP64).
     void *ws = nullptr;

@@ -326,6 +326,47 @@ def test_poisson_plane_waves_large(gpu, n):
     torch.cuda.empty_cache()
 
 
+@pytest.mark.parametrize("n", [512, 1024])
+def test_fft_cols_persistent_same_bits(gpu, n):
+    """The persistent column passes (blocks that prefetch their next tile) and
+    the one-block-per-tile passes run the same stages on the same values:
+    forward and inverse transforms and the fused Poisson solve agree bit for
+    bit.  Also times the Poisson solve both ways."""
+    NS = gpu.navierstokes
+    ctx = gpu._abi.context()
+    g = NS.Grid(1.0, n)
+    ps = NS.PoissonSolver(g, torch.float32)
+    gen = torch.Generator(device="cuda").manual_seed(n + 1)
+    x = torch.randn((n, n, n), dtype=torch.float32, device="cuda", generator=gen)
+    out, ms = {}, {}
+    try:
+        for on in (1, 0):
+            ctx.call("ec_fft_set_cols_persistent", on)
+            X = NS.rfft(x, ps)
+            back = NS.irfft(X, ps)
+            ctx.call("ec_d2d", gpu._abi.c_void_p(ps.p_ptr), gpu._abi.c_void_p(x.data_ptr()), x.numel() * 4)
+            ctx.call("ec_poisson_solve", ps.h)
+            p = ps.p.data
+            torch.cuda.synchronize()
+            t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            t0.record()
+            for _ in range(3):
+                ctx.call("ec_poisson_solve", ps.h)
+            t1.record()
+            t1.synchronize()
+            ms[on] = t0.elapsed_time(t1) / 3
+            out[on] = (X, back, p)
+            del X, back, p
+    finally:
+        ctx.call("ec_fft_set_cols_persistent", 1)
+    print(f"\n[poisson solve n={n} fp32] persistent column passes {ms[1]:.3f} ms, one block per tile {ms[0]:.3f} ms")
+    for a, b in zip(out[1], out[0]):
+        assert torch.equal(a, b)
+    assert float((out[1][1] - x).abs().max() / x.abs().max()) < 3e-6
+    del out, x
+    torch.cuda.empty_cache()
+
+
 @pytest.mark.parametrize("n", SIZES + [64])
 def test_project_fp64(gpu, O, n):
     NS = gpu.navierstokes
