@@ -875,7 +875,7 @@ __global__ void __launch_bounds__(256) k_rk4_update_vec(int n, T ih, T dt, Ptr3 
 // buffer, the last stage reads u1, u2, u3 and writes the new u0.  16-byte
 // accesses as k_rk4_update_vec, or one cell per thread (VEC false) where the
 // fields are not 16-byte aligned.
-template <typename T, int STAGE, bool VEC>
+template <typename T, int STAGE, bool VEC, bool HASP = true>
 __global__ void __launch_bounds__(256) k_rk4_states(int n, T ih, T dt, Ptr3 u0p, Ptr3 u1p, Ptr3 u2p, Ptr3 u3p,
                                                     Ptr3 dup, const T *__restrict__ p, size_t nvec) {
     typedef typename std::conditional<VEC, typename Vec16<T>::type, T>::type VT;
@@ -890,17 +890,19 @@ __global__ void __launch_bounds__(256) k_rk4_states(int n, T ih, T dt, Ptr3 u0p,
     const size_t I = (size_t)iv * V + n1 * row;
     const size_t Iy = (size_t)iv * V + n1 * ((j + 1 == (unsigned)n ? 0 : j + 1) + n1 * k);
     const size_t Iz = (size_t)iv * V + n1 * (j + n1 * (k + 1 == (unsigned)n ? 0 : k + 1));
-    const unsigned xl = iv * V + V;  // x + 1 of the last cell
-    const T pxl = p[(xl == (unsigned)n ? 0 : xl) + n1 * row];
-    const VT p0 = *(const VT *)(p + I), py = *(const VT *)(p + Iy), pz = *(const VT *)(p + Iz);
-    const T *p0s = (const T *)&p0, *pys = (const T *)&py, *pzs = (const T *)&pz;
     T gp[3][V];
+    if constexpr (HASP) {
+        const unsigned xl = iv * V + V;  // x + 1 of the last cell
+        const T pxl = p[(xl == (unsigned)n ? 0 : xl) + n1 * row];
+        const VT p0 = *(const VT *)(p + I), py = *(const VT *)(p + Iy), pz = *(const VT *)(p + Iz);
+        const T *p0s = (const T *)&p0, *pys = (const T *)&py, *pzs = (const T *)&pz;
 #pragma unroll
-    for (int e = 0; e < V; e++) {
-        const T pxp = e + 1 < V ? p0s[e + 1] : pxl;
-        gp[0][e] = (pxp - p0s[e]) * ih;
-        gp[1][e] = (pys[e] - p0s[e]) * ih;
-        gp[2][e] = (pzs[e] - p0s[e]) * ih;
+        for (int e = 0; e < V; e++) {
+            const T pxp = e + 1 < V ? p0s[e + 1] : pxl;
+            gp[0][e] = (pxp - p0s[e]) * ih;
+            gp[1][e] = (pys[e] - p0s[e]) * ih;
+            gp[2][e] = (pzs[e] - p0s[e]) * ih;
+        }
     }
     // all loads first (the fields may alias as far as the compiler knows)
     VT d[3], uv[3], s1[3], s2[3], s3[3];
@@ -923,7 +925,8 @@ __global__ void __launch_bounds__(256) k_rk4_states(int n, T ih, T dt, Ptr3 u0p,
         T *r = (T *)&o;
 #pragma unroll
         for (int e = 0; e < V; e++) {
-            const T kk = ds[e] - gp[a][e];
+            T kk = ds[e];
+            if constexpr (HASP) kk = ds[e] - gp[a][e];
             if constexpr (STAGE < 3) r[e] = us_[e] + ec_rk4_stage_coef<T>(STAGE, dt) * kk;
             else r[e] = ec_rk4_combine<T>(us_[e], a1[e], a2[e], a3[e], kk, dt);
         }
@@ -948,9 +951,16 @@ static int rk4_states_launch(ec_ctx *ctx, int n, double l, int stage, double dt,
     }
     const size_t nvec = ec_cube(n) / (aligned ? V : 1);
     const unsigned blocks = (unsigned)((nvec + 255) / 256);
-#define EC_RK4S(ST, VEC)                                                                              \
-    k_rk4_states<T, ST, VEC><<<blocks, 256, 0, ctx->stream>>>(n, ih, (T)dt, U0, S[0], S[1], S[2], DU, \
-                                                               (const T *)p, nvec)
+    // without p, k = du (a tendency that is already projected)
+#define EC_RK4S(ST, VEC)                                                                                          \
+    do {                                                                                                          \
+        if (p)                                                                                                    \
+            k_rk4_states<T, ST, VEC, true><<<blocks, 256, 0, ctx->stream>>>(n, ih, (T)dt, U0, S[0], S[1], S[2], DU, \
+                                                                             (const T *)p, nvec);                 \
+        else                                                                                                      \
+            k_rk4_states<T, ST, VEC, false><<<blocks, 256, 0, ctx->stream>>>(n, ih, (T)dt, U0, S[0], S[1], S[2],  \
+                                                                              DU, nullptr, nvec);                 \
+    } while (0)
     if (aligned) {
         switch (stage) {
             case 0: EC_RK4S(0, true); break;
@@ -1021,7 +1031,7 @@ extern "C" int ec_ns_rk4_update(ec_ctx *ctx, int dtype, int n, double l, int sta
 
 extern "C" int ec_ns_rk4_update_states(ec_ctx *ctx, int dtype, int n, double l, int stage, double dt,
                                        void *const u0[3], void *const ust[9], void *const du[3], const void *p) {
-    EC_REQUIRE(ctx, ctx && u0 && ust && du && p, "ec_ns_rk4_update_states: null argument");
+    EC_REQUIRE(ctx, ctx && u0 && ust && du, "ec_ns_rk4_update_states: null argument");
     EC_REQUIRE(ctx, stage >= 0 && stage <= 3, "ec_ns_rk4_update_states: stage %d out of range", stage);
     for (int q = 0; q < 9; q++) {
         // the buffers a stage touches: its own state (stage < 3), all three at stage 3
@@ -1031,7 +1041,7 @@ extern "C" int ec_ns_rk4_update_states(ec_ctx *ctx, int dtype, int n, double l, 
     EC_REQUIRE(ctx, ec_cube(n) / 256 < 2147483647.0, "ec_ns_rk4_update_states: n=%d too large", n);
     const double b = dtype == EC_F64 ? 8 : 4;
     // reads p, du(3), u0(3) (+ u1, u2, u3 at stage 3); writes one state (3)
-    const double units = stage == 3 ? 19 : 10;
+    const double units = (stage == 3 ? 19 : 10) - (p ? 0 : 1);
     ec_prof_scope ps(ctx, "rk4_update", units * ec_cube(n) * b);
     return dtype == EC_F64 ? rk4_states_launch<double>(ctx, n, l, stage, dt, u0, ust, du, p)
                            : rk4_states_launch<float>(ctx, n, l, stage, dt, u0, ust, du, p);

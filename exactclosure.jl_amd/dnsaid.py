@@ -40,26 +40,42 @@ def _vec(n, dtype):
 
 
 class LESGrid:
-    def __init__(self, g_dns, nbar, kind, dtype, closure="forcing"):
+    def __init__(self, g_dns, nbar, kind, dtype, closure="forcing", rk_states=False):
         self.g = NS.Grid(g_dns.l, nbar)
         self.kind = kind
         self.dtype = dtype
         self.cset = NS.ClosureSet(g_dns, nbar, kind) if closure == "flux" else None
+        # RK4 of v = V u (FP64) in stored stage states: vs and vacc hold v1
+        # and v2, v3 the third.  The FP32 LES state keeps the accumulator form:
+        # its stored states would carry FP32 rounding into the new state (the
+        # LES then sits ~1.5x further from the filtered DNS) for a small saving
+        self.rk_states = bool(rk_states) and closure == "forcing"
         f64 = torch.float64
         if closure == "forcing":
             self.v0 = _vec(nbar, f64)    # V u at the step start (unprojected)
             self.vs = _vec(nbar, f64)    # V u at the current stage
             self.vacc = _vec(nbar, f64)
+            self.v3 = _vec(nbar, f64) if self.rk_states else None
             self.g_k = _vec(nbar, f64)   # V k of the current stage
             self.ubar = _vec(nbar, f64)  # P_H V u
             self.m = _vec(nbar, f64)     # closure forcing V k - F_H(ubar)
-            self.zero64 = torch.zeros((nbar,) * 3, dtype=f64, device="cuda")
+            self.zero64 = None if self.rk_states else torch.zeros((nbar,) * 3, dtype=f64, device="cuda")
         self.ps = NS.PoissonSolver(self.g, dtype)
         self.ps64 = self.ps if dtype == torch.float64 else NS.PoissonSolver(self.g, torch.float64)
         self.u0 = _vec(nbar, dtype)
         self.us = _vec(nbar, dtype)
         self.du = _vec(nbar, dtype)
         self.acc = _vec(nbar, dtype)
+
+    def u_stage(self, st):
+        """The LES state that stage st evaluates."""
+        return self.u0 if st == 0 else self.us
+
+    def v_stage(self, st):
+        """v = V u at stage st."""
+        if st == 0:
+            return self.v0
+        return [self.vs, self.vacc, self.v3][st - 1] if self.rk_states else self.vs
 
 
 class DNSAidedLES:
@@ -95,7 +111,7 @@ class DNSAidedLES:
         self.du = _vec(n, dtype)
         self.acc = _vec(n, dtype)  # accumulator (u2 of the stored-state form)
         self.u3 = _vec(n, dtype) if self.rk_states else None
-        self.les = [LESGrid(g_dns, nbar, kind, dtype, closure) for nbar, kind in sets]
+        self.les = [LESGrid(g_dns, nbar, kind, dtype, closure, self.rk_states) for nbar, kind in sets]
         # objects with .stage(st, dt), advanced after every stage of the forcing
         # form once the stage's exact closure is known (a posteriori model LES)
         self.stage_hooks = []
@@ -241,7 +257,7 @@ class DNSAidedLES:
         projected at every stage, as apply_DHpi does."""
         c = c or self.ctx
         gl = L.g
-        v = L.v0 if st == 0 else L.vs
+        v = L.v_stage(st)
         if L.kind == "fa":
             if st == 0:
                 c.call("ec_ns_project", L.ps64.h, ptrs(L.v0, P3))
@@ -301,16 +317,22 @@ class DNSAidedLES:
         for L in self.les:
             gl = L.g
             m = self._les_forcing(L, st, c)
-            ul = L.u0 if st == 0 else L.us
+            ul = L.u_stage(st)
             # du = F_H(u) + m in one pass (the sum in FP64, rounded once)
             c.call("ec_ns_rhs_add", self.dt_code, gl.n, gl.l, self.visc, ptrs(ul, P3), ptrs(L.du, P3), 1.0,
                    ptrs(m, P3))
             c.call("ec_ns_pressure_solve", L.ps.h, ptrs(L.du, P3))
             c.call("ec_ns_rk4_update", self.dt_code, gl.n, gl.l, st, dt, ptrs(L.u0, P3), ptrs(L.us, P3),
                    ptrs(L.du, P3), _abi.c_void_p(L.ps.p_ptr), ptrs(L.acc, P3))
-            # v = V u follows the DNS through the same tableau (V k is already projected)
-            c.call("ec_ns_rk4_update", _abi.EC_F64, gl.n, gl.l, st, dt, ptrs(L.v0, P3), ptrs(L.vs, P3),
-                   ptrs(L.g_k, P3), ptr(L.zero64), ptrs(L.vacc, P3))
+            # v = V u follows the DNS through the same tableau (V k is already
+            # projected: no pressure)
+            if L.rk_states:
+                vst = ptrs(L.vs + L.vacc + L.v3, _abi.P9)
+                c.call("ec_ns_rk4_update_states", _abi.EC_F64, gl.n, gl.l, st, dt, ptrs(L.v0, P3), vst,
+                       ptrs(L.g_k, P3), None)
+            else:
+                c.call("ec_ns_rk4_update", _abi.EC_F64, gl.n, gl.l, st, dt, ptrs(L.v0, P3), ptrs(L.vs, P3),
+                       ptrs(L.g_k, P3), ptr(L.zero64), ptrs(L.vacc, P3))
 
     def run(self, dt, nsteps):
         for _ in range(nsteps):

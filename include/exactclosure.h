@@ -326,7 +326,7 @@ int ec_ns_rk4_update(ec_ctx *ctx, int dtype, int n, double l, int stage, double 
  * projection at every stage as in step_wray3!, src/N.jl:320-346).  The states
  * u1..u3 equal the stage states us of ec_ns_rk4_update bit for bit.  Per step
  * the update moves 49 fields instead of 58 (no accumulator read and written
- * at every stage). */
+ * at every stage).  p may be NULL: k = du (a tendency already projected). */
 int ec_ns_rk4_update_states(ec_ctx *ctx, int dtype, int n, double l, int stage, double dt,
                             void *const u0[3], void *const ust[9], void *const du[3], const void *p);
 

@@ -533,6 +533,36 @@ def test_rk4_update_states_matches_accumulator_form(gpu, O, n, prec):
     torch.cuda.empty_cache()
 
 
+@pytest.mark.parametrize("n", [12, 13, 64])
+def test_rk4_update_states_without_pressure(gpu, n):
+    """p = NULL: k = du.  Four stages against the accumulator form given a
+    zero pressure field (the form the LES grids' filtered velocity used): the
+    stage states bit for bit, the new state to FP64 rounding."""
+    step = 0.01
+    gen = torch.Generator(device="cuda").manual_seed(11 * n)
+    f = lambda: torch.randn((n, n, n), device="cuda", dtype=torch.float64, generator=gen)  # noqa: E731
+    u0 = [f() for _ in range(3)]
+    dus = [[f() for _ in range(3)] for _ in range(4)]
+    zero = torch.zeros((n, n, n), device="cuda", dtype=torch.float64)
+    ctx = gpu._abi.context()
+    ua, us, acc = [x.clone() for x in u0], [torch.empty_like(x) for x in u0], [torch.empty_like(x) for x in u0]
+    ub = [x.clone() for x in u0]
+    ust = [[torch.empty_like(x) for x in u0] for _ in range(3)]
+    ust_p = P([x for s_ in ust for x in s_], 9)
+    for st in range(4):
+        ctx.call("ec_ns_rk4_update", 1, n, 1.0, st, step, P(ua, 3), P(us, 3), P(dus[st], 3), zero.data_ptr(),
+                 P(acc, 3))
+        ctx.call("ec_ns_rk4_update_states", 1, n, 1.0, st, step, P(ub, 3), ust_p, P(dus[st], 3), None)
+        torch.cuda.synchronize()
+        if st < 3:
+            for a in range(3):
+                assert torch.equal(ust[st][a], us[a]), (st, a)
+    for a in range(3):
+        want = u0[a] + step / 6 * (dus[0][a] + 2 * dus[1][a] + 2 * dus[2][a] + dus[3][a])
+        assert float((ub[a] - want).abs().max()) <= 1e-14 * float(want.abs().max())
+        assert float((ub[a] - ua[a]).abs().max()) <= 1e-14 * float(ua[a].abs().max())
+
+
 @pytest.mark.parametrize("n,prec", [(16, "f64"), (64, "f64"), (256, "f32")])
 def test_project_into(gpu, n, prec):
     """dst = P src leaves src unchanged and equals copy + in-place projection
