@@ -216,6 +216,33 @@ def cpu_baseline(n=256, nsteps=5):
                       f"({sps:.4f} steps/s at {n}^3), scaled by (n/1024)^3"}
 
 
+def copy_ceiling_gbs(abi, nbytes=1 << 31, reps=5):
+    """Attainable streaming rate of this device: a plain device-to-device copy
+    of nbytes (read nbytes, write nbytes) in GB/s, the faster of torch's copy
+    kernel and the runtime's (hipMemcpyAsync through ec_d2d), each the mean of
+    reps copies between two synchronisations.  The step's kernels each read
+    and write several fields, so this (not the 8 TB/s of the data sheet) is the
+    rate a perfectly streaming kernel reaches here."""
+    import torch
+
+    ctx = abi.context()
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    best = 0.0
+    for f in (lambda: y.copy_(x),
+              lambda: ctx.call("ec_d2d", abi.c_void_p(y.data_ptr()), abi.c_void_p(x.data_ptr()), nbytes)):
+        f()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            f()
+        torch.cuda.synchronize()
+        best = max(best, 2.0 * nbytes * reps / (time.perf_counter() - t0) / 1e9)
+    del x, y
+    torch.cuda.empty_cache()
+    return best
+
+
 def fp64_leg(ec, n, visc, sets, dt, steps):
     """The plain FP64 form of the same DNS-aided step (the same kernels with
     FP64 fields), on the same seeded initial field: mean ms per step over
@@ -314,6 +341,7 @@ def main():
         sets.append((n // int(c), kind))
     g = NS.Grid(1.0, n)
 
+    copy_gbs = copy_ceiling_gbs(abi)
     if a.copy_calibration:
         # a plain streaming kernel of known traffic for the HBM counters: reads
         # 1 GiB and writes 1 GiB (element-wise negation of 2^28 floats)
@@ -486,7 +514,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dname, "achieved": d_achieved, "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": d_achieved / PEAK_HBM_GBS, "launches": dcount,
                      "avg_ms": d_avg_ms, "bytes_per_launch": dbytes / dcount,
-                     "traffic": traffic_from_profile(dname) if n == 1024 and a.dtype == "f32" else None},
+                     "traffic": traffic_from_profile(dname) if n == 1024 and a.dtype == "f32" else None,
+                     "copy_ceiling": copy_gbs, "frac_of_copy_ceiling": d_achieved / copy_gbs},
         "autotuned": tuned,
         "kernels": {k: {"launches": c, "ms": round(tms, 3), "gbs": round((b / c) / (tms / c * 1e-3) / 1e9, 1)}
                     for k, (c, tms, b) in per.items()},
