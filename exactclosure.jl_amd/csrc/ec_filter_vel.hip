@@ -244,7 +244,10 @@ __device__ __forceinline__ void xstage(double *lds, int tid) {
     constexpr int rows = G::TY, nix = G::TX / C;
     constexpr int PX = A == 0 ? (K == 0 ? 1 : 2) : 0;
     const int t = (tid + 64 * (S * 3 + A)) % G::NT;  // spread the stages over the waves
-    for (int it = t; it < nix * rows; it += G::NT) {
+    // at most one item per thread where the stage fits the block: a guarded
+    // single iteration instead of a loop (see fv_march on wait counts)
+    constexpr int STEP = nix * rows <= G::NT ? 0x7fffffff : G::NT;
+    for (int it = t; it < nix * rows; it += STEP) {
         const int Ix = it / rows, row = it - Ix * rows;
         const double *zr = lds + G::zoff(S, A) + row * G::zw(S, A);
         if constexpr (A == 0 && G::split(S)) {
@@ -266,7 +269,8 @@ __device__ __forceinline__ void ystage(const double *lds, int tid, int x0, int y
     constexpr int nix = G::TX / C, niy = G::TY / C;
     constexpr int PY = A == 1 ? (K == 0 ? 1 : 2) : 0;
     const int t = (tid + 64 * (S * 3 + A)) % G::NT;
-    for (int it = t; it < nix * niy; it += G::NT) {
+    constexpr int STEP = nix * niy <= G::NT ? 0x7fffffff : G::NT;
+    for (int it = t; it < nix * niy; it += STEP) {
         const int Iy = it / nix, Ix = it - Iy * nix;
         const size_t nb = (size_t)fs.nb;
         const size_t J = (size_t)(x0 / C + Ix) + nb * ((size_t)(y0 / C + Iy) + nb * (size_t)Kc);
@@ -510,7 +514,13 @@ __device__ __forceinline__ void rk_store(int a, unsigned I, T k, const T *v, con
 // column and window logic are compile-time, so a plane is straight-line code
 // (loads, FP64 z-updates) up to the uniform emit branch.
 // Element offsets are 32-bit: the host requires n^3 < 2^32.
-template <typename T, class G, int CODE, int RK>
+// The plane body has no branch around a global load: the pressure's presence
+// is a template flag, slots are known valid where the tile has no padding,
+// and stage-state loads past the chunk read a wrapped plane (unused).  With a
+// load under a branch, the compiler's wait-count insertion met a join it could
+// not resolve and waited for every outstanding load (vmcnt(0)) before the
+// plane's arithmetic, the next plane's prefetch included.
+template <typename T, class G, int CODE, int RK, bool HASP>
 __device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__restrict__ v2, const T *__restrict__ v3,
                                          const T *__restrict__ p, const FvTile &tl, double *lds, const FSet *fsl,
                                          int tid, const RkArgs &rk) {
@@ -539,7 +549,7 @@ __device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__re
             valid = e < G::N2, lx = e % TX, ly = e / TX;
         }
         if (!valid) lx = ly = 0;  // padding: reads column (0, 0), writes nothing
-        ok[j] = valid;
+        ok[j] = G::NCC == G::NJ * G::NT && G::P0 == G::N0 && G::P1 == G::N1 && G::P2 == G::N2 ? true : valid;
         clx[j] = lx;
         cly[j] = ly;
         const unsigned xo = (unsigned)wrap1(x0 + lx, n), yo = (unsigned)wrap1(y0 + ly, n);
@@ -569,7 +579,7 @@ __device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__re
             if (a == 3) continue;
             const T *vp = a == 0 ? v1 : (a == 1 ? v2 : v3);
             vv[j] = vp[co[j] + kk];
-            if (p) {
+            if constexpr (HASP) {
                 pp[j] = p[co[j] + kk];
                 qq[j] = p[cq[j] + (a == 2 ? k1 : kk)];
             } else {
@@ -585,7 +595,7 @@ __device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__re
     auto rk_prefetch = [&](int r, T (*vv)[NV]) {
 #pragma unroll
         for (int j = 0; j < NJ; j++)
-            if (comp(j) != 3 && r < ZC && ok[j]) rk_load<T, RK>(comp(j), co[j] + (unsigned)(z0 + r) * n2, rk, vv[j]);
+            if (comp(j) != 3) rk_load<T, RK>(comp(j), co[j] + (unsigned)wrap1(z0 + r, n) * n2, rk, vv[j]);
     };
     auto rk_rotate = [&](T (*vv)[NV]) {
 #pragma unroll
@@ -631,8 +641,12 @@ __device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__re
         // four planes per iteration (ZC + H is a multiple of 4): the window
         // phase of each is a compile-time constant, and only the phase-3 plane
         // can emit
-        static_assert(G::hmax % 4 == 0, "nested march: whole 4-plane groups");
-        for (int r0 = 0; r0 <= rend; r0 += 4) {
+        static_assert(G::hmax % 4 == 0 && G::lcm() % 4 == 0, "nested march: whole 4-plane groups");
+        // the chunk's planes (ZC, a multiple of 4) take the stage update; the
+        // halo planes after them (to ZC + hmax) only feed the face windows, so
+        // the plane body needs no per-plane store condition
+        auto group = [&](int r0, auto st) {
+            constexpr bool STORE = decltype(st)::value;
             auto plane = [&](auto qc) {
                 constexpr int Q = decltype(qc)::value;
                 const int r = r0 + Q;
@@ -640,29 +654,30 @@ __device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__re
                 T nv[NJ], np[NJ], nq[NJ];
                 load(r + 1, nv, np, nq);
                 T rkn[NJ][NV];
-                if constexpr (RK >= 0) rk_prefetch(r + 1, rkn);
+                if constexpr (RK >= 0 && STORE) rk_prefetch(r + 1, rkn);
 #pragma unroll
                 for (int j = 0; j < NJ; j++) {
                     const int a = comp(j);
                     if (a == 3) continue;
                     const T kv = cv[j] - (cqv[j] - cp[j]) * ihT;
                     const double src = (double)kv;
-                    if constexpr (RK >= 0)
-                        if (r < ZC && ok[j])
-                            rk_store<T, RK>(a, co[j] + (unsigned)(z0 + r) * n2, kv, rkv[j], rk);
+                    if constexpr (RK >= 0 && STORE)
+                        if (ok[j]) rk_store<T, RK>(a, co[j] + (unsigned)(z0 + r) * n2, kv, rkv[j], rk);
                     if (a == 0) nested_box<G, 0, Q>(r, ZC, ok[j], clx[j], cly[j], src, nst[j], lds);
                     else if (a == 1) nested_box<G, 1, Q>(r, ZC, ok[j], clx[j], cly[j], src, nst[j], lds);
                     else nested_face<G, Q>(r, ZC, ok[j], clx[j], cly[j], src, nst[j], lds);
                 }
                 if constexpr (Q == 3) emit(r);
                 rotate(nv, np, nq);
-                if constexpr (RK >= 0) rk_rotate(rkn);
+                if constexpr (RK >= 0 && STORE) rk_rotate(rkn);
             };
             plane(std::integral_constant<int, 0>{});
             plane(std::integral_constant<int, 1>{});
             plane(std::integral_constant<int, 2>{});
             plane(std::integral_constant<int, 3>{});
-        }
+        };
+        for (int r0 = 0; r0 < ZC; r0 += 4) group(r0, std::true_type{});
+        for (int r0 = ZC; r0 <= rend; r0 += 4) group(r0, std::false_type{});
     } else {
         for (int r = 0; r <= rend; ++r) {
             T nv[NJ], np[NJ], nq[NJ];
@@ -691,23 +706,24 @@ __device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__re
 }
 
 // one instantiation of the march per distinct slot-component code
-template <typename T, class G, int RK, int W>
+template <typename T, class G, int RK, bool HASP, int W>
 __device__ __forceinline__ void fv_dispatch(int code, const T *__restrict__ v1, const T *__restrict__ v2,
                                             const T *__restrict__ v3, const T *__restrict__ p, const FvTile &tl,
                                             double *lds, const FSet *fsl, int tid, const RkArgs &rk) {
     if constexpr (W < G::NT / 64) {
         if constexpr (first_code<G>(W)) {
             if (code == wave_code<G>(W)) {
-                fv_march<T, G, wave_code<G>(W), RK>(v1, v2, v3, p, tl, lds, fsl, tid, rk);
+                fv_march<T, G, wave_code<G>(W), RK, HASP>(v1, v2, v3, p, tl, lds, fsl, tid, rk);
                 return;
             }
         }
-        fv_dispatch<T, G, RK, W + 1>(code, v1, v2, v3, p, tl, lds, fsl, tid, rk);
+        fv_dispatch<T, G, RK, HASP, W + 1>(code, v1, v2, v3, p, tl, lds, fsl, tid, rk);
     }
 }
 
-// RK < 0: the filter alone; RK = 0..7: with the fused DNS RK4 stage (RkArgs)
-template <typename T, class G, int RK>
+// RK < 0: the filter alone; RK = 0..7: with the fused DNS RK4 stage (RkArgs).
+// HASP: the source is v - delta p (else v).
+template <typename T, class G, int RK, bool HASP = true>
 __global__ void __launch_bounds__(G::NT) k_filter_vel(const T *__restrict__ v1, const T *__restrict__ v2,
                                                      const T *__restrict__ v3, const T *__restrict__ p, int n, int ZC,
                                                      double ih, FArgs A, RkArgs R) {
@@ -736,7 +752,7 @@ __global__ void __launch_bounds__(G::NT) k_filter_vel(const T *__restrict__ v1, 
         const int c = e < G::P0 ? 0 : (e < G::P0 + G::P1 ? 1 : (e < G::NCC ? 2 : 3));
         code |= c << (2 * j);
     }
-    fv_dispatch<T, G, RK, 0>(code, v1, v2, v3, p, tl, lds, fsl, tid, R);
+    fv_dispatch<T, G, RK, HASP, 0>(code, v1, v2, v3, p, tl, lds, fsl, tid, R);
 }
 
 // Face J of a split component = its U partial (already in out) + the L
@@ -786,8 +802,10 @@ int launch_fv(ec_ctx *ctx, int n, double l, void *const v[3], const void *p, con
         RK < 0 ? 0.0 : (RK == 0 || RK == 3 ? 9.0 : (RK <= 2 ? 12.0 : (RK == 7 ? 15.0 : 6.0)));
     ec_prof_scope pr(ctx, RK < 0 ? "filter_vel" : "filter_vel_rk4",
                      sizeof(T) * ((p ? 4.0 : 3.0) + rk_units) * ec_cube(n));
-    EC_HIP(ctx, hipFuncSetAttribute((const void *)k_filter_vel<T, G, RK>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)lds));
+    EC_HIP(ctx, hipFuncSetAttribute((const void *)k_filter_vel<T, G, RK, true>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    EC_HIP(ctx, hipFuncSetAttribute((const void *)k_filter_vel<T, G, RK, false>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     RkArgs R{};
     if (rk) R = *rk;
     // the L partials of the split volume-average sets, in a context workspace
@@ -816,8 +834,12 @@ int launch_fv(ec_ctx *ctx, int n, double l, void *const v[3], const void *p, con
             side += 2 * m;
         }
     }
-    k_filter_vel<T, G, RK><<<dim3((unsigned)tiles), G::NT, lds, ctx->stream>>>(
-        (const T *)v[0], (const T *)v[1], (const T *)v[2], (const T *)p, n, ZC, n / l, AA, R);
+    if (p)
+        k_filter_vel<T, G, RK, true><<<dim3((unsigned)tiles), G::NT, lds, ctx->stream>>>(
+            (const T *)v[0], (const T *)v[1], (const T *)v[2], (const T *)p, n, ZC, n / l, AA, R);
+    else
+        k_filter_vel<T, G, RK, false><<<dim3((unsigned)tiles), G::NT, lds, ctx->stream>>>(
+            (const T *)v[0], (const T *)v[1], (const T *)v[2], nullptr, n, ZC, n / l, AA, R);
     EC_CHECK_LAUNCH(ctx, "k_filter_vel");
     for (int s = 0; s < 4; s++) {
         if (!G::split(s)) continue;
