@@ -246,7 +246,7 @@ __device__ __forceinline__ void rhs_cell(const T *pm, const T *p0, const T *pp, 
 // plane k+2 are issued before the stencil of plane k and land after it, so
 // every u value is read from HBM once per tile (the plain kernel re-reads the
 // neighbouring planes, about 4x the compulsory traffic).
-template <typename T, int TX, int TY>
+template <typename T, int TX, int TY, int DOADD>
 __global__ void __launch_bounds__(256) k_rhs_march(int n, int ZC, T ih, T visc, const T *__restrict__ u1,
                                                    const T *__restrict__ u2, const T *__restrict__ u3,
                                                    T *__restrict__ du1, T *__restrict__ du2, T *__restrict__ du3,
@@ -300,10 +300,30 @@ __global__ void __launch_bounds__(256) k_rhs_march(int n, int ZC, T ih, T visc, 
     }
     __syncthreads();
     const int cx = tid % TX, cy0 = tid / TX;
+    // the addend of the plane's cells (DOADD 1: du itself, 2: the FP64 ad.a)
+    typedef typename std::conditional<DOADD == 2, double, T>::type AT;
     for (int rr = 0; rr < ZC; ++rr) {
         const int k = z0 + rr;
+        // the addend loads go out before the next plane's: the in-order memory
+        // counter then lets the plane wait for them without waiting for the
+        // next plane (and DOADD is a template parameter: a load under a
+        // runtime branch would make the waits drain everything in flight)
+        AT add[CPT][3];
+#pragma unroll
+        for (int m = 0; m < CPT; m++) {
+            const size_t I = (size_t)(x0 + cx) + (size_t)nn * (size_t)(y0 + cy0 + m * (256 / TX)) + n2 * (size_t)k;
+            if constexpr (DOADD == 1) {
+                add[m][0] = du1[I];
+                add[m][1] = du2[I];
+                add[m][2] = du3[I];
+            } else if constexpr (DOADD == 2) {
+                add[m][0] = ad.a1[I];
+                add[m][1] = ad.a2[I];
+                add[m][2] = ad.a3[I];
+            }
+        }
         T nxt[NLD];
-        if (rr + 1 < ZC) load(k + 2, nxt);
+        load(k + 2, nxt);  // unconditional: see k_rhs_div_march
         const T *pm = ring[rr & 3], *p0 = ring[(rr + 1) & 3], *pp = ring[(rr + 2) & 3];
 #pragma unroll
         for (int m = 0; m < CPT; m++) {
@@ -312,14 +332,14 @@ __global__ void __launch_bounds__(256) k_rhs_march(int n, int ZC, T ih, T visc, 
             T a1, a2, a3;
             rhs_cell(pm, p0, pp, o, HX, HP, ih, visc, a1, a2, a3);
             const size_t I = (size_t)(x0 + cx) + (size_t)nn * (size_t)(y0 + cy) + n2 * (size_t)k;
-            if (doadd == 1) {
-                a1 += du1[I];
-                a2 += du2[I];
-                a3 += du3[I];
-            } else if (doadd == 2) {
-                a1 = (T)(ad.s * (double)a1 + ad.a1[I]);
-                a2 = (T)(ad.s * (double)a2 + ad.a2[I]);
-                a3 = (T)(ad.s * (double)a3 + ad.a3[I]);
+            if constexpr (DOADD == 1) {
+                a1 += add[m][0];
+                a2 += add[m][1];
+                a3 += add[m][2];
+            } else if constexpr (DOADD == 2) {
+                a1 = (T)(ad.s * (double)a1 + add[m][0]);
+                a2 = (T)(ad.s * (double)a2 + add[m][1]);
+                a3 = (T)(ad.s * (double)a3 + add[m][2]);
             }
             du1[I] = a1;
             du2[I] = a2;
@@ -420,7 +440,10 @@ __global__ void __launch_bounds__(NT) k_rhs_div_march(int n, int ZC, T ih, T vis
     auto plane = [&](int rr, T *cur, T *ahead) {
         const int k = z0 + rr, q = rr + 2;
         const size_t pk = n2 * (size_t)k + (size_t)x0 + (size_t)nn * (size_t)y0;  // tile corner in plane k
-        if (rr + 2 < ZC) load(k + 3, ahead);
+        // unconditional (the plane index wraps; past the chunk the values go
+        // unused): a load under a branch leaves the wait counts of the join
+        // to the path without it, which drains the other plane in flight
+        load(k + 3, ahead);
         const auto r1 = ec_rsrc(du1 + pk), r2 = ec_rsrc(du2 + pk), r3 = ec_rsrc(du3 + pk), rd = ec_rsrc(dout + pk);
         const T *pm = ring[(q - 1) & 3], *p0 = ring[q & 3], *pp = ring[(q + 1) & 3];
         T a1[CPT], a2[CPT], a3[CPT], dv[CPT];
@@ -645,14 +668,20 @@ static int rhs_launch(ec_ctx *ctx, int dtype, int n, double l, double visc, void
         // z-marching tiles of 64 x 8 columns
         const int ZC = march_zc(n);
         const unsigned tiles = (unsigned)((n / 64) * (n / 8) * (n / ZC));
-        if (dtype == EC_F64)
-            k_rhs_march<double, 64, 8><<<tiles, 256, 0, ctx->stream>>>(
-                n, ZC, ih, visc, (const double *)u[0], (const double *)u[1], (const double *)u[2], (double *)du[0],
-                (double *)du[1], (double *)du[2], doadd, ad);
-        else
-            k_rhs_march<float, 64, 8><<<tiles, 256, 0, ctx->stream>>>(
-                n, ZC, (float)ih, (float)visc, (const float *)u[0], (const float *)u[1], (const float *)u[2],
-                (float *)du[0], (float *)du[1], (float *)du[2], doadd, ad);
+#define EC_RHS_MARCH(T, D)                                                                                       \
+    k_rhs_march<T, 64, 8, D><<<tiles, 256, 0, ctx->stream>>>(n, ZC, (T)ih, (T)visc, (const T *)u[0], (const T *)u[1], \
+                                                             (const T *)u[2], (T *)du[0], (T *)du[1], (T *)du[2],    \
+                                                             doadd, ad)
+        if (dtype == EC_F64) {
+            if (doadd == 1) EC_RHS_MARCH(double, 1);
+            else if (doadd == 2) EC_RHS_MARCH(double, 2);
+            else EC_RHS_MARCH(double, 0);
+        } else {
+            if (doadd == 1) EC_RHS_MARCH(float, 1);
+            else if (doadd == 2) EC_RHS_MARCH(float, 2);
+            else EC_RHS_MARCH(float, 0);
+        }
+#undef EC_RHS_MARCH
         EC_CHECK_LAUNCH(ctx, "k_rhs_march");
         return EC_OK;
     }
