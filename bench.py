@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--closure-tiles", default="16,16,128", help="tx,ty,tz block targets of the closure kernel")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--eager", action="store_true", help="time eager launches instead of one graph replay per step")
+    ap.add_argument("--les-stream", type=int, default=1, choices=[0, 1],
+                    help="1: the LES grids' stage work on a second stream beside the DNS passes")
     ap.add_argument("--burgers", action="store_true",
                     help="time the batched Burgers workload (4096 instances, 1000 RK4 steps) instead")
     ap.add_argument("--burgers-steps", type=int, default=1000)
@@ -362,7 +364,7 @@ def main():
     torch.cuda.empty_cache()
     log(f"[bench] IC {n}^3 ready in {time.perf_counter() - t0:.1f}s, dt={dt:.3e}")
 
-    drv = D.DNSAidedLES(g, a.visc, sets, dtype=dtype)
+    drv = D.DNSAidedLES(g, a.visc, sets, dtype=dtype, les_stream=bool(a.les_stream))
     if a.closure_tiles:
         drv.ctx.call("ec_closure_set_tiles", *[int(x) for x in a.closure_tiles.split(",")])
     drv.set_state(u_init)
@@ -395,7 +397,10 @@ def main():
     # per-launch HIP events (ec_profile_*) on the context stream: the same
     # step issued eagerly, K more steps, for the kernel table and the roofline
     drv.release_graphs()
-    ctx.call("ec_profile_begin")
+    # (the LES stage work runs on the driver's second context when it has one)
+    pctx = [ctx] + ([drv.les_ctx] if drv.les_ctx is not None else [])
+    for c in pctx:
+        c.call("ec_profile_begin")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(a.steps):
@@ -403,12 +408,18 @@ def main():
     torch.cuda.synchronize()
     t_eager = time.perf_counter() - t0
     cap = 100000
-    names = (abi.ctypes.c_char_p * cap)()
-    ms = (abi.c_double * cap)()
-    by = (abi.c_double * cap)()
-    cnt = abi.c_int()
-    ctx.call("ec_profile_end", names, ms, by, cap, abi.ctypes.byref(cnt))
-    m = min(cnt.value, cap)
+    names, ms, by = [], [], []
+    for c in pctx:
+        nm_c = (abi.ctypes.c_char_p * cap)()
+        ms_c = (abi.c_double * cap)()
+        by_c = (abi.c_double * cap)()
+        cnt = abi.c_int()
+        c.call("ec_profile_end", nm_c, ms_c, by_c, cap, abi.ctypes.byref(cnt))
+        k = min(cnt.value, cap)
+        names += [nm_c[i] for i in range(k)]
+        ms += ms_c[:k]
+        by += by_c[:k]
+    m = len(names)
     per = {}
     for i in range(m):
         k = names[i].decode()

@@ -59,6 +59,7 @@ _SIGS = {
     "ec_capture_end": (c_int, [c_void_p, ctypes.POINTER(c_void_p)]),
     "ec_graph_launch": (c_int, [c_void_p, c_void_p]),
     "ec_graph_destroy": (c_int, [c_void_p, c_void_p]),
+    "ec_stream_after": (c_int, [c_void_p, c_void_p]),
     "ec_alloc": (c_int, [c_void_p, c_size_t, ctypes.POINTER(c_void_p)]),
     "ec_free": (c_int, [c_void_p, c_void_p]),
     "ec_h2d": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),

@@ -58,6 +58,7 @@ extern "C" int ec_destroy(ec_ctx *ctx) {
         if (g) hipGraphDestroy(g);
         ctx->stream = ctx->user_stream;
     }
+    if (ctx->order_event) hipEventDestroy(ctx->order_event);
     if (ctx->cap_stream) hipStreamDestroy(ctx->cap_stream);
     if (ctx->own_stream) hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -112,6 +113,17 @@ extern "C" int ec_graph_launch(ec_ctx *ctx, void *graph) {
 extern "C" int ec_graph_destroy(ec_ctx *ctx, void *graph) {
     EC_REQUIRE(ctx, ctx, "null context");
     if (graph) EC_HIP(ctx, hipGraphExecDestroy((hipGraphExec_t)graph));
+    return EC_OK;
+}
+
+extern "C" int ec_stream_after(ec_ctx *ctx, ec_ctx *other) {
+    EC_REQUIRE(ctx, ctx && other && other != ctx, "ec_stream_after: needs two different contexts");
+    EC_REQUIRE(ctx, other->device == ctx->device, "ec_stream_after: contexts of devices %d and %d", ctx->device,
+               other->device);
+    EC_REQUIRE(ctx, other->stream != ctx->stream, "ec_stream_after: both contexts order on one stream");
+    if (!other->order_event) EC_HIP(ctx, hipEventCreateWithFlags(&other->order_event, hipEventDisableTiming));
+    EC_HIP(ctx, hipEventRecord(other->order_event, other->stream));
+    EC_HIP(ctx, hipStreamWaitEvent(ctx->stream, other->order_event, 0));
     return EC_OK;
 }
 

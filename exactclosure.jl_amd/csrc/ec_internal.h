@@ -55,6 +55,8 @@ struct ec_ctx {
     hipStream_t cap_stream = nullptr;
     hipStream_t user_stream = nullptr;
     bool capturing = false;
+    // recorded on this context's stream by ec_stream_after(other, this)
+    hipEvent_t order_event = nullptr;
 };
 
 // ---- error handling: no C++ exception crosses the C ABI ----

@@ -86,7 +86,7 @@ class DNSAidedLES:
     spectrum ~ 60 GB."""
 
     def __init__(self, g_dns, visc, sets, dtype=torch.float32, closure="forcing", clean_divergence=True,
-                 rk_states=True):
+                 rk_states=True, les_stream=True):
         assert closure in ("forcing", "flux")
         self.closure = closure
         # the DNS RK4 in stored stage states (ec_ns_rk4_update_states: stages
@@ -116,6 +116,17 @@ class DNSAidedLES:
         # form once the stage's exact closure is known (a posteriori model LES)
         self.stage_hooks = []
         self._graphs = {}  # dt -> captured step (ec_capture_end)
+        # les_stream (forcing form): the LES grids' stage work runs on a second
+        # stream, beside the DNS right-hand side and Poisson solve of the next
+        # stage (it only needs the filter outputs of its own stage, and the
+        # next filter waits for it); in a captured step it is a parallel
+        # branch of the graph.  Same kernels on the same values: same bits;
+        # 1% off the 1024^3 step (the LES launches fill what the DNS passes
+        # leave of the device).  The last stage's LES work ends the step.
+        self.les_ctx = None
+        if les_stream and closure == "forcing" and self.les:
+            self._les_torch_stream = torch.cuda.Stream()
+            self.les_ctx = _abi.Context(0, self._les_torch_stream.cuda_stream)
 
     # -- state -------------------------------------------------------------
     def set_state(self, u):
@@ -292,6 +303,10 @@ class DNSAidedLES:
             # V_s(du - grad p) for every LES grid and the DNS RK4 stage in one
             # pass over du and p (nothing below reads the DNS state)
             if self.les:
+                # (stage hooks read the LES state between stages: one stream then)
+                lc = self.les_ctx if not self.stage_hooks else None
+                if lc is not None and st > 0:
+                    c.call("ec_stream_after", lc.h)  # the previous LES stage has read g_k
                 sets = self._vel_sets([L.g_k for L in self.les])
                 if ust:
                     c.call("ec_ns_filter_vel_rk4_states", self.dt_code, n, l, ptrs(self.du, P3),
@@ -300,7 +315,13 @@ class DNSAidedLES:
                     c.call("ec_ns_filter_vel_rk4", self.dt_code, n, l, ptrs(self.du, P3),
                            _abi.c_void_p(self.ps.p_ptr), len(self.les), sets, st, dt, ptrs(self.u0, P3),
                            ptrs(self.us, P3), ptrs(self.acc, P3))
-                self._les_stage(st, dt, c)
+                if lc is None:
+                    self._les_stage(st, dt, c)
+                else:
+                    lc.call("ec_stream_after", c.h)  # this stage's filter outputs are written
+                    self._les_stage(st, dt, lc)
+                    if st == 3:
+                        c.call("ec_stream_after", lc.h)  # the step ends with the LES stages
             elif ust:
                 c.call("ec_ns_rk4_update_states", self.dt_code, n, l, st, dt, ptrs(self.u0, P3), ust_p,
                        ptrs(self.du, P3), _abi.c_void_p(self.ps.p_ptr))

@@ -46,6 +46,8 @@ ec_graph_launch(ctx, graph) =
     ccall((:ec_graph_launch, libexactclosure), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), ctx, graph)
 ec_graph_destroy(ctx, graph) =
     ccall((:ec_graph_destroy, libexactclosure), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), ctx, graph)
+ec_stream_after(ctx, other) =
+    ccall((:ec_stream_after, libexactclosure), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), ctx, other)
 ec_alloc(ctx, bytes, ptr) =
     ccall((:ec_alloc, libexactclosure), Cint, (Ptr{Cvoid}, Csize_t, Ptr{Ptr{Cvoid}}), ctx, bytes, ptr)
 ec_free(ctx, ptr) =

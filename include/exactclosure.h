@@ -67,6 +67,13 @@ int ec_capture_begin(ec_ctx *ctx);
 int ec_capture_end(ec_ctx *ctx, void **graph);
 int ec_graph_launch(ec_ctx *ctx, void *graph);
 int ec_graph_destroy(ec_ctx *ctx, void *graph);
+/* Stream order between two contexts of one device (no reference counterpart):
+ * the calls made on ctx after this one run after everything queued on other
+ * so far (an event recorded on other's stream, waited on by ctx's).  Inside a
+ * capture of either context the other stream joins the same graph as a
+ * parallel branch; every branch must be joined back into the capturing
+ * context's stream this way before ec_capture_end. */
+int ec_stream_after(ec_ctx *ctx, ec_ctx *other);
 
 /* ---- memory (replaces KernelAbstractions.zeros/allocate and Adapt moves) ---- */
 int ec_alloc(ec_ctx *ctx, size_t bytes, void **ptr);

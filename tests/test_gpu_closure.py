@@ -342,6 +342,51 @@ def test_filter_vel_rk4_states_fused(gpu, n, prec, sets):
 
 @pytest.mark.parametrize("n,sets,prec", [(256, [(16, "va"), (32, "va"), (64, "va"), (64, "fa")], "f32"),
                                          (32, [(8, "va"), (4, "fa")], "f64")])
+def test_les_stream_same_bits(gpu, O, n, sets, prec):
+    """The LES stage work on a second stream (ec_stream_after orders it after
+    its stage's filter and the next filter after it) gives the one-stream
+    step's values bit for bit, eagerly and as a captured graph whose LES work
+    is a parallel branch (replayed with eager, synchronising work between the
+    replays), for the DNS state and every LES state."""
+    NS, D = gpu.navierstokes, gpu.dnsaid
+    dt_t = torch.float32 if prec == "f32" else torch.float64
+    g = NS.Grid(1.0, n)
+    u = O.randomfield(n, seed=n + 5) if n <= 64 else None
+    if u is None:
+        ps = NS.PoissonSolver(g, torch.float64)
+        u = [x.data for x in NS.randomfield(g, ps, "peak", 5.0, 0.5, seed=11)]
+    else:
+        u = [dev(x) for x in u]
+    one = D.DNSAidedLES(g, 1.0 / 2000, sets, dtype=dt_t, les_stream=False)
+    two = D.DNSAidedLES(g, 1.0 / 2000, sets, dtype=dt_t, les_stream=True)
+    assert two.les_ctx is not None and one.les_ctx is None
+    one.set_state([x.to(dt_t) for x in u])
+    two.set_state([x.to(dt_t) for x in u])
+    dt = 2e-3 if n <= 64 else 5e-4
+
+    def same():
+        torch.cuda.synchronize()
+        for a in range(3):
+            assert torch.equal(one.u0[a], two.u0[a])
+            for L1, L2 in zip(one.les, two.les):
+                assert torch.equal(L1.u0[a], L2.u0[a])
+                assert torch.equal(L1.v0[a], L2.v0[a])
+
+    for _ in range(2):
+        one.step(dt)
+        two.step(dt)
+        same()
+    two.capture(dt)
+    for _ in range(3):
+        one.step(dt)
+        two.step(dt)
+        two.divergence_norm()  # eager, synchronising work between the replays
+        same()
+    two.release_graphs()
+
+
+@pytest.mark.parametrize("n,sets,prec", [(256, [(16, "va"), (32, "va"), (64, "va"), (64, "fa")], "f32"),
+                                         (32, [(8, "va"), (4, "fa")], "f64")])
 def test_captured_step_replays_the_eager_step(gpu, O, n, sets, prec):
     """A whole DNS-aided RK4 step (DNS kernels, every LES grid's forcing,
     projection and update) captured into one HIP graph and replayed four
