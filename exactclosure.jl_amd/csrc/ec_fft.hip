@@ -574,8 +574,8 @@ __global__ void __launch_bounds__(xblock(NTHR)) k_fft_x_r2c_ct(const T *__restri
 #pragma unroll
         for (int q = 0; q < PER; q++) {
             const int e = t + q * NTHR;
-            ra[q] = v0 ? a[e] : T(0);
-            rb[q] = v1 ? b[e] : T(0);
+            ra[q] = v0 ? ec_nt_load(a + e) : T(0);
+            rb[q] = v1 ? ec_nt_load(b + e) : T(0);
         }
     };
     const long g0 = (long)blockIdx.x * G * W;  // first pair of the block
@@ -626,8 +626,8 @@ __global__ void __launch_bounds__(xblock(NTHR)) k_fft_x_c2r_ct(const cplx<T> *__
         for (int q = 0; q < KQ; q++) {
             const int k = t + q * NTHR;
             const bool kin = q < KQ - 1 || k < NH;
-            ra[q] = v0 && kin ? ia[k] : cplx<T>{T(0), T(0)};
-            rb[q] = v1 && kin ? ib[k] : cplx<T>{T(0), T(0)};
+            ra[q] = v0 && kin ? ec_nt_load(ia + k) : cplx<T>{T(0), T(0)};
+            rb[q] = v1 && kin ? ec_nt_load(ib + k) : cplx<T>{T(0), T(0)};
         }
     };
     const long g0 = (long)blockIdx.x * G * W;

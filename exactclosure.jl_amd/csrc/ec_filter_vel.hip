@@ -481,14 +481,16 @@ constexpr int rk_nv() {
 
 // loads of the stage state of one cell, issued at the top of a plane so that
 // they arrive during the plane's filter work
+// (read once: non-temporal, so that L2 keeps the pressure lines the
+// neighbouring tiles share)
 template <typename T, int RK>
 __device__ __forceinline__ void rk_load(int a, unsigned I, const RkArgs &rk, T *v) {
-    v[0] = ((const T *)rk.u0[a])[I];
-    if constexpr (RK >= 1 && RK <= 3) v[1] = ((const T *)rk.acc[a])[I];
+    v[0] = __builtin_nontemporal_load((const T *)rk.u0[a] + I);
+    if constexpr (RK >= 1 && RK <= 3) v[1] = __builtin_nontemporal_load((const T *)rk.acc[a] + I);
     if constexpr (RK == 7) {
-        v[1] = ((const T *)rk.ust[0][a])[I];
-        v[2] = ((const T *)rk.ust[1][a])[I];
-        v[3] = ((const T *)rk.ust[2][a])[I];
+        v[1] = __builtin_nontemporal_load((const T *)rk.ust[0][a] + I);
+        v[2] = __builtin_nontemporal_load((const T *)rk.ust[1][a] + I);
+        v[3] = __builtin_nontemporal_load((const T *)rk.ust[2][a] + I);
     }
 }
 template <typename T, int RK>
@@ -578,7 +580,9 @@ __device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__re
             const int a = comp(j);
             if (a == 3) continue;
             const T *vp = a == 0 ? v1 : (a == 1 ? v2 : v3);
-            vv[j] = vp[co[j] + kk];
+            // the filtered field is read once (non-temporal); the pressure and
+            // its x/y neighbours are shared with the neighbouring tiles
+            vv[j] = __builtin_nontemporal_load(vp + co[j] + kk);
             if constexpr (HASP) {
                 pp[j] = p[co[j] + kk];
                 qq[j] = p[cq[j] + (a == 2 ? k1 : kk)];

@@ -111,6 +111,33 @@ template <typename T>
 __device__ __forceinline__ void ec_nt_store(T *p, T v) {
     __builtin_nontemporal_store(v, p);
 }
+// Streaming loads for inputs a kernel reads exactly once: they leave L2 to
+// the lines that neighbouring blocks share.
+template <typename T>
+__device__ __forceinline__ T ec_nt_load(const T *p) {
+    return __builtin_nontemporal_load(p);
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> ec_nt_load(const cplx<T> *p) {  // one 8- or 16-byte access
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    const v2 v = __builtin_nontemporal_load(reinterpret_cast<const v2 *>(p));
+    return cplx<T>{v.x, v.y};
+}
+template <typename VT>
+__device__ __forceinline__ VT ec_nt_load16(const VT *p) {  // a 16-byte vector (float4, double2)
+    static_assert(sizeof(VT) == 16, "16-byte vector");
+    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    const u4 w = __builtin_nontemporal_load(reinterpret_cast<const u4 *>(p));
+    VT v;
+    __builtin_memcpy(&v, &w, 16);
+    return v;
+}
+// a value or a 16-byte vector, read once
+template <typename VT>
+__device__ __forceinline__ VT ec_nt_ld(const VT *p) {
+    if constexpr (sizeof(VT) == 16) return ec_nt_load16(p);
+    else return ec_nt_load(p);
+}
 template <typename VT>
 __device__ __forceinline__ void ec_nt_store16(VT *p, const VT &v) {  // a 16-byte vector (float4, double2)
     static_assert(sizeof(VT) == 16, "16-byte vector");

@@ -316,10 +316,10 @@ __global__ void __launch_bounds__(256) k_rhs_march(int n, int ZC, T ih, T visc, 
                 add[m][0] = du1[I];
                 add[m][1] = du2[I];
                 add[m][2] = du3[I];
-            } else if constexpr (DOADD == 2) {
-                add[m][0] = ad.a1[I];
-                add[m][1] = ad.a2[I];
-                add[m][2] = ad.a3[I];
+            } else if constexpr (DOADD == 2) {  // (read once: non-temporal)
+                add[m][0] = ec_nt_load(ad.a1 + I);
+                add[m][1] = ec_nt_load(ad.a2 + I);
+                add[m][2] = ec_nt_load(ad.a3 + I);
             }
         }
         T nxt[NLD];
@@ -857,13 +857,14 @@ __global__ void __launch_bounds__(256) k_rk4_update_vec(int n, T ih, T dt, Ptr3 
         gp[2][e] = (pzs[e] - p0s[e]) * ih;
     }
     // all loads first: the three components' fields may alias as far as the
-    // compiler knows, so loads placed after a store would wait for it
+    // compiler knows, so loads placed after a store would wait for it (read
+    // once: non-temporal; p's neighbour rows are shared and stay cached)
     VT d[3], uv[3], av[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        d[a] = *(const VT *)((const T *)dup.p[a] + I);
-        uv[a] = *(const VT *)((const T *)u0p.p[a] + I);
-        if (STAGE != 0) av[a] = *(const VT *)((const T *)accp.p[a] + I);
+        d[a] = ec_nt_ld((const VT *)((const T *)dup.p[a] + I));
+        uv[a] = ec_nt_ld((const VT *)((const T *)u0p.p[a] + I));
+        if (STAGE != 0) av[a] = ec_nt_ld((const VT *)((const T *)accp.p[a] + I));
     }
 #pragma unroll
     for (int a = 0; a < 3; a++) {
@@ -933,16 +934,17 @@ __global__ void __launch_bounds__(256) k_rk4_states(int n, T ih, T dt, Ptr3 u0p,
             gp[2][e] = (pzs[e] - p0s[e]) * ih;
         }
     }
-    // all loads first (the fields may alias as far as the compiler knows)
+    // all loads first (the fields may alias as far as the compiler knows);
+    // read once: non-temporal
     VT d[3], uv[3], s1[3], s2[3], s3[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        d[a] = *(const VT *)((const T *)dup.p[a] + I);
-        uv[a] = *(const VT *)((const T *)u0p.p[a] + I);
+        d[a] = ec_nt_ld((const VT *)((const T *)dup.p[a] + I));
+        uv[a] = ec_nt_ld((const VT *)((const T *)u0p.p[a] + I));
         if constexpr (STAGE == 3) {
-            s1[a] = *(const VT *)((const T *)u1p.p[a] + I);
-            s2[a] = *(const VT *)((const T *)u2p.p[a] + I);
-            s3[a] = *(const VT *)((const T *)u3p.p[a] + I);
+            s1[a] = ec_nt_ld((const VT *)((const T *)u1p.p[a] + I));
+            s2[a] = ec_nt_ld((const VT *)((const T *)u2p.p[a] + I));
+            s3[a] = ec_nt_ld((const VT *)((const T *)u3p.p[a] + I));
         }
     }
     const Ptr3 &outp = STAGE == 0 ? u1p : (STAGE == 1 ? u2p : (STAGE == 2 ? u3p : u0p));
