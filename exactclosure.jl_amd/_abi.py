@@ -114,6 +114,7 @@ _SIGS = {
                                             ctypes.POINTER(VelSet), c_int, c_double, P3, P9]),
     "ec_filter_vel_set_tiles": (c_int, [c_void_p, c_int, c_int]),
     "ec_fft_set_cols_persistent": (c_int, [c_void_p, c_int]),
+    "ec_poisson_set_z_tridiagonal": (c_int, [c_void_p, c_int]),
     "ec_ns_commutator": (c_int, [c_void_p, c_int, c_double, c_double, P3, P9, P9]),
     "ec_ns_sfs_filter": (c_int, [c_void_p, c_int, c_int, c_int, c_double, c_double, P3, c_void_p, c_int, P3, P9, P9]),
     "ec_ns_tensordissipation": (c_int, [c_void_p, c_int, c_double, P9, P3, c_void_p]),

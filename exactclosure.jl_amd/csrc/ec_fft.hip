@@ -778,6 +778,188 @@ __global__ void __launch_bounds__(TK *NTHR) k_fft_cols_ct(cplx<T> *__restrict__ 
     }
 }
 
+// The z pass of the FP32 Poisson solve as the tridiagonal solve it is.  After
+// the x and y transforms, column (kx, ky) of the half spectrum holds D_z, and
+// the z transform, the division by the eigenvalues a (s_x + s_y + s_z) of the
+// discrete Laplacian and the inverse z transform solve, exactly, the cyclic
+// system (P_{z+1} - 2 P_z + P_{z-1}) / h^2 + c P_z = D_z with
+// c = a (s_x + s_y) = -4 (s_x + s_y) / h^2 (the z modes are its eigenvectors).
+// With beta = 2 - c h^2 = 2 + 4 s, s = s_x + s_y, and r the root of
+// r^2 - beta r + 1 = 0 in (0, 1], r = 1 / (1 + 2 s + 2 sqrt(s (1 + s))), the
+// operator factors as S+ + S- - beta = -(1/r) (1 - r S+)(1 - r S-), so
+//     Q_z = F_z + r Q_{z+1},   P_z = Q_z + r P_{z-1}   (cyclic in z),
+// with F = -r h^2 D / N^2 (N^2: the inverse x and y transforms are not
+// normalised).  The tile of TK columns is staged in LDS as in the transform
+// pass (whole row segments per lane group); thread (column c, block t) then
+// runs each recursion over its V = N/64 consecutive planes from zero, and the
+// value entering each block comes from a scan over the 64 blocks of the
+// column (factor R = r^V per block): within a wave over its 64/TK blocks by
+// shuffles, across the TK waves through LDS, cyclic through 1 / (1 - r^N).
+// Running sums, carries and scans are FP64 (the recursions damp rounding by r;
+// 1 / (1 - r) reaches ~N / (2 pi) for the lowest modes); the planes stay FP32
+// between the passes, each rounded once at its own scale.  The zero column
+// (kx = ky = 0, r = 1, whose zero mode the transforms drop) keeps its input
+// for the transform pass, run on that one column afterwards.  About a fifth of
+// the two transforms' arithmetic.
+template <int N, int TK>
+__global__ void __launch_bounds__(64 * TK) __attribute__((amdgpu_waves_per_eu(4))) k_z_poisson_tri(
+    cplx<float> *__restrict__ data, int ncols, int ntiles, long elem_stride, long outer_stride, double h2n) {
+    using T = float;
+    typedef cplx<double> cd;
+    constexpr int V = N / 64, BW = 64 / TK, NW = TK;  // planes per block, blocks per wave and column, waves
+    constexpr int LB = BW == 16 ? 4 : (BW == 8 ? 3 : (BW == 4 ? 2 : 1));
+    static_assert(N % 64 == 0 && V >= 1 && V <= 16 && (1 << LB) == BW, "64 blocks of V planes per column");
+    extern __shared__ char smem_raw[];
+    cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
+    cd *gsc = reinterpret_cast<cd *>(lds + (size_t)padded_len(N) * TK);  // [2][NW waves][TK columns]
+    const int L = (int)ec_xcd_tile(blockIdx.x, gridDim.x);
+    const int col0 = (L % ntiles) * TK, outer = L / ntiles;
+    cplx<T> *base = data + col0 + (long)outer * outer_stride;
+    // tile in and out as in k_fft_cols_ct: element q of a thread is column cc,
+    // row e0 + 64 q (whole row segments per TK lanes)
+    const int cc = threadIdx.x % TK, e0 = threadIdx.x / TK;
+    const bool cok = col0 + cc < ncols;
+    const unsigned voff = (unsigned)((cc + (long)e0 * elem_stride) * (long)sizeof(cplx<T>));
+    const long qstep = 64L * elem_stride * (long)sizeof(cplx<T>);
+    const int lrow = pad8(e0) * TK + cc;
+    {
+        cplx<T> tmp[V];
+#pragma unroll
+        for (int q = 0; q < V; q++)
+            tmp[q] = cok ? *(const cplx<T> *)((const char *)base + q * qstep + voff) : cplx<T>{T(0), T(0)};
+#pragma unroll
+        for (int q = 0; q < V; q++) lds[lrow + pad_off<64>(q) * TK] = tmp[q];
+    }
+    __syncthreads();
+    // the same thread now owns column c = cc, planes V t .. V t + V - 1 with
+    // t = e0 = BW w + j (w the wave, j = lane / TK)
+    const int c = cc, t = e0, w = t / BW, j = t % BW, lane = threadIdx.x & 63;
+    const int kx = col0 + c;
+    const bool solve = kx < ncols && !(kx == 0 && outer == 0);
+    cplx<T> *col = lds + pad8(V * t) * TK + c;  // plane V t + v at col[v * TK]
+    const double sx = sinpi((double)kx / N), sy = sinpi((double)outer / N);
+    const double sxy = solve ? sx * sx + sy * sy : 1.0;
+    const double r = 1.0 / (1.0 + 2.0 * sxy + 2.0 * sqrt(sxy * (1.0 + sxy)));
+    double Rk[5];  // R^(2^k), R = r^V
+    Rk[0] = r;
+#pragma unroll
+    for (int v = 1; v < V; v++) Rk[0] *= r;
+#pragma unroll
+    for (int k = 1; k < 5; k++) Rk[k] = Rk[k - 1] * Rk[k - 1];
+    const double RW = Rk[LB];  // R^BW: one wave's blocks
+    double RN = RW;            // R^64 = r^N
+#pragma unroll
+    for (int k = LB; k < 6; k++) RN *= RN;
+    const double cyc = 1.0 / (1.0 - RN);
+    auto rpow = [&](int e) {  // R^e, 0 <= e <= 16
+        double p = 1.0;
+#pragma unroll
+        for (int k = 0; k < 5; k++)
+            if (e & (1 << k)) p *= Rk[k];
+        return p;
+    };
+    auto shfl = [&](const cd &z, int src) { return cd{__shfl(z.x, src, 64), __shfl(z.y, src, 64)}; };
+    auto fma_c = [](const cd &acc, double a, const cd &z) { return cd{fma(a, z.x, acc.x), fma(a, z.y, acc.y)}; };
+    // the planes stay FP32 in registers between the passes; every running sum,
+    // carry and scan is FP64 (each stored value is rounded once, at its own
+    // scale)
+    cplx<T> x[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) x[v] = col[v * TK];
+    // Q_z = F_z + r Q_{z+1}: from the block's end with zero, E = the block's first value
+    cd acc{0.0, 0.0};
+    {
+        const double f = -r * h2n;
+#pragma unroll
+        for (int v = V - 1; v >= 0; v--) {
+            acc = cd{fma(f, (double)x[v].x, r * acc.x), fma(f, (double)x[v].y, r * acc.y)};
+            x[v] = cplx<T>{(T)acc.x, (T)acc.y};
+        }
+    }
+    // suffix over the wave's BW blocks: X_j = sum_{i <= BW-1-j} R^i E_{j+i}
+    cd X = acc;
+#pragma unroll
+    for (int k = 0; k < LB; k++) {
+        const int s = 1 << k;
+        const cd o = shfl(X, (lane + TK * s) & 63);
+        if (j + s < BW) X = fma_c(X, Rk[k], o);
+    }
+    if (j == 0) gsc[w * TK + c] = X;  // G_w: the wave's blocks as one transfer
+    const cd Xn = shfl(X, (lane + TK) & 63);
+    __syncthreads();
+    // Q at the end of this wave's blocks: sum_i (R^BW)^i G_{w+1+i} / (1 - R^64)
+    cd Cw{0.0, 0.0};
+    {
+        double f = 1.0;
+#pragma unroll
+        for (int i = 0; i < NW; i++) {
+            Cw = fma_c(Cw, f, gsc[((w + 1 + i) % NW) * TK + c]);
+            f *= RW;
+        }
+        Cw = cd{Cw.x * cyc, Cw.y * cyc};
+    }
+    // Q_z = l_z + r^(V - v) C with C the Q entering this block (at plane
+    // V (t + 1)); then P_z = Q_z + r P_{z-1} from the block's start with zero,
+    // E' = the block's last value
+    {
+        const cd C = j == BW - 1 ? Cw : fma_c(Xn, rpow(BW - 1 - j), Cw);
+        const double ri = 1.0 / r;
+        double p = Rk[0];  // r^(V - v)
+        acc = cd{0.0, 0.0};
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            const cd q = fma_c(cd{(double)x[v].x, (double)x[v].y}, p, C);
+            p *= ri;
+            asm volatile("" : "+v"(p));  // formed here, not all ahead (they would hold registers)
+            acc = fma_c(q, r, acc);
+            x[v] = cplx<T>{(T)acc.x, (T)acc.y};
+        }
+    }
+    cd Y = acc;
+#pragma unroll
+    for (int k = 0; k < LB; k++) {
+        const int s = 1 << k;
+        const cd o = shfl(Y, (lane - TK * s) & 63);
+        if (j - s >= 0) Y = fma_c(Y, Rk[k], o);
+    }
+    if (j == BW - 1) gsc[NW * TK + w * TK + c] = Y;  // H_w
+    const cd Yp = shfl(Y, (lane - TK) & 63);
+    __syncthreads();
+    cd Cp{0.0, 0.0};
+    {
+        double f = 1.0;
+#pragma unroll
+        for (int i = 0; i < NW; i++) {
+            Cp = fma_c(Cp, f, gsc[NW * TK + ((w + NW - 1 - i) % NW) * TK + c]);
+            f *= RW;
+        }
+        Cp = cd{Cp.x * cyc, Cp.y * cyc};
+    }
+    if (solve) {
+        // P entering this block (P at plane V t - 1)
+        const cd C = j == 0 ? Cp : fma_c(Yp, rpow(j), Cp);
+        double p = r;  // r^(v + 1)
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            const cd z = fma_c(cd{(double)x[v].x, (double)x[v].y}, p, C);
+            p *= r;
+            asm volatile("" : "+v"(p));
+            col[v * TK] = cplx<T>{(T)z.x, (T)z.y};
+        }
+    }
+    __syncthreads();
+    // the zero column keeps its input for the transform pass that solves it
+    // (the store addresses are formed again here: held from the loads they
+    // would occupy 2 V registers through the solve)
+    unsigned soff = voff;
+    asm volatile("" : "+v"(soff));
+    if (cok && !(col0 + cc == 0 && outer == 0)) {
+#pragma unroll
+        for (int q = 0; q < V; q++)
+            *(cplx<T> *)((char *)base + q * qstep + soff) = lds[lrow + pad_off<64>(q) * TK];
+    }
+}
+
 // Plain (non-Poisson) column transforms of large FP32 grids with one wave per
 // column: the tile of TK columns is staged column-major (column stride CS,
 // CS = 64 / TK mod 32, so the row-wise transposing writes share banks at most
@@ -981,9 +1163,40 @@ int cols_w(ec_ctx *ctx, ec_poisson *ps, void *spec, int axis, int pitch) {
     return EC_OK;
 }
 
+// the FP32 z Poisson pass as tridiagonal solves (k_z_poisson_tri), then the
+// zero column by the transform pass on that one column
+// (8-column tiles: 64-byte row segments; 4-column tiles, four blocks per CU,
+// made the 1024^3 step 1% slower)
+template <int N>
+int z_poisson_tri(ec_ctx *ctx, ec_poisson *ps, void *spec, int pitch) {
+    constexpr int TK = 8, TT = 8;  // TK: the transform pass's tile, TT: the tridiagonal tile
+    const int nh = ps->nh, ntt = (nh + TT - 1) / TT;
+    const long elem_stride = (long)pitch * N, outer_stride = pitch;
+    constexpr size_t lds = (size_t)padded_len(N) * TT * sizeof(cplx<float>) + 2 * 64 * sizeof(cplx<double>);
+    static_assert(lds <= 80 * 1024, "two blocks per CU");
+    const double h = ps->l / N;
+    EC_HIP(ctx, hipFuncSetAttribute((const void *)k_z_poisson_tri<N, TT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds));
+    k_z_poisson_tri<N, TT><<<(unsigned)(ntt * N), 64 * TT, lds, ctx->stream>>>(
+        (cplx<float> *)spec, nh, ntt, elem_stride, outer_stride, h * h / ((double)N * N));
+    EC_CHECK_LAUNCH(ctx, "k_z_poisson_tri");
+    // column (0, 0): one block of the transform pass with one valid column
+    constexpr int NTHR = N / 16;
+    constexpr size_t ldsz = lds_cols_ct(N, TK, 1, sizeof(float));
+    EC_HIP(ctx, hipFuncSetAttribute((const void *)k_fft_cols_ct<float, N, TK, NTHR, 1, -1>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsz));
+    k_fft_cols_ct<float, N, TK, NTHR, 1, -1><<<1, TK * NTHR, ldsz, ctx->stream>>>(
+        (cplx<float> *)spec, 1, 1, elem_stride, outer_stride, (const cplx<float> *)ps->twiddle,
+        (const float *)ps->eig, (float)(-4.0 / (h * h)), (float)(1.0 / ((double)N * N * N)));
+    EC_CHECK_LAUNCH(ctx, "k_fft_cols_ct");
+    return EC_OK;
+}
+
 template <typename T, int N, int MODE, int SIGN>
 int cols_ct(ec_ctx *ctx, ec_poisson *ps, void *spec, int axis, int pitch) {
     if constexpr (sizeof(T) == 4 && N >= 512 && MODE == 0) return cols_w<T, N, SIGN>(ctx, ps, spec, axis, pitch);
+    if constexpr (sizeof(T) == 4 && N >= 64 && MODE == 1)
+        if (ctx->z_poisson_tri && axis == 2) return z_poisson_tri<N>(ctx, ps, spec, pitch);
     constexpr int NTHR = N / 16;
     // one 128-byte line of complex values per row; the fused Poisson pass (two
     // transforms) takes 8 columns so that 512-thread blocks can use radix 16
@@ -1198,6 +1411,12 @@ int project_into_t(ec_ctx *ctx, ec_poisson *ps, const void *const src[3], void *
 }
 
 }  // namespace
+
+extern "C" int ec_poisson_set_z_tridiagonal(ec_ctx *ctx, int on) {
+    EC_REQUIRE(ctx, ctx && (on == 0 || on == 1), "ec_poisson_set_z_tridiagonal: bad argument");
+    ctx->z_poisson_tri = on;
+    return EC_OK;
+}
 
 extern "C" int ec_fft_set_cols_persistent(ec_ctx *ctx, int on) {
     EC_REQUIRE(ctx, ctx && (on == 0 || on == 1), "ec_fft_set_cols_persistent: bad argument");

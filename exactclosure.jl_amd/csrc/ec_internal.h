@@ -41,6 +41,9 @@ struct ec_ctx {
     // column passes of large FP32 spectra: persistent blocks that prefetch
     // their next tile (1) or one block per tile (0); the same bits either way
     int fft_cols_persistent = 1;
+    // the z pass of FP32 Poisson solves: tridiagonal solves per column (1) or
+    // the forward transform, scale and inverse transform (0)
+    int z_poisson_tri = 1;
     int cl_tx = 16, cl_ty = 16, cl_tz = 64;
     // Workspace of the separable filter passes (grown on demand, FP64).
     void *ws = nullptr;

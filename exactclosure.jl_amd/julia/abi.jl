@@ -136,6 +136,8 @@ ec_ns_filter_vel_rk4_states(ctx, dtype, n, l, du, p, nsets, sets, stage, dt, u0,
     ccall((:ec_ns_filter_vel_rk4_states, libexactclosure), Cint, (Ptr{Cvoid}, Cint, Cint, Cdouble, Ref{NTuple{3,Ptr{Cvoid}}}, Ptr{Cvoid}, Cint, Ptr{VelSet}, Cint, Cdouble, Ref{NTuple{3,Ptr{Cvoid}}}, Ref{NTuple{9,Ptr{Cvoid}}}), ctx, dtype, n, l, du, p, nsets, sets, stage, dt, u0, ust)
 ec_fft_set_cols_persistent(ctx, on) =
     ccall((:ec_fft_set_cols_persistent, libexactclosure), Cint, (Ptr{Cvoid}, Cint), ctx, on)
+ec_poisson_set_z_tridiagonal(ctx, on) =
+    ccall((:ec_poisson_set_z_tridiagonal, libexactclosure), Cint, (Ptr{Cvoid}, Cint), ctx, on)
 ec_filter_vel_set_tiles(ctx, tx, tz) =
     ccall((:ec_filter_vel_set_tiles, libexactclosure), Cint, (Ptr{Cvoid}, Cint, Cint), ctx, tx, tz)
 ec_closure_set_tiles(ctx, tx, ty, tz) =

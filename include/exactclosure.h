@@ -278,13 +278,21 @@ int ec_ns_filter_vel_rk4(ec_ctx *ctx, int dtype, int n, double l, void *const du
 int ec_ns_filter_vel_rk4_states(ec_ctx *ctx, int dtype, int n, double l, void *const du[3],
                                 const void *p, int nsets, const ec_vel_set *sets, int stage, double dt,
                                 void *const u0[3], void *const ust[9]);
-/* Column passes (y, and z with the Poisson scale) of FP32 spectra with
- * n >= 512: persistent blocks, as many as fit on the device, each
- * prefetching its next column tile while it transforms the current one
- * (on = 1, the default), or one block per tile (on = 0).  Both give the same
- * bits.  A launch-shape setting of this context with no counterpart in the
- * reference (the internals of plan_rfft, src/N.jl:217). */
+/* Plain column passes (y) of FP32 spectra with n >= 512: persistent blocks,
+ * as many as fit on the device, each prefetching its next column tile while
+ * it transforms the current one (on = 1, the default), or one block per tile
+ * (on = 0).  Both give the same bits.  A launch-shape setting of this context
+ * with no counterpart in the reference (the internals of plan_rfft,
+ * src/N.jl:217). */
 int ec_fft_set_cols_persistent(ec_ctx *ctx, int on);
+/* The z pass of FP32 Poisson solves (poissonsolver, src/N.jl:202-220; n a
+ * power of two >= 64): on = 1 (the default) solves each (kx, ky) column's
+ * cyclic tridiagonal system (the discrete d^2/dz^2 plus the column's x/y
+ * eigenvalues) directly, by two first-order recursions in FP64; on = 0 runs the
+ * forward z transform, the division by the eigenvalues and the inverse
+ * transform.  The same solution up to rounding (the direct solve is the more
+ * accurate: FP64 arithmetic); a setting of this context. */
+int ec_poisson_set_z_tridiagonal(ec_ctx *ctx, int on);
 /* Target block size (x = y, z) of ec_ns_filter_vel and ec_ns_filter_vel_rk4 on
  * this context (default z 256); set by the tile autotuner. */
 int ec_filter_vel_set_tiles(ec_ctx *ctx, int tx, int tz);

@@ -1,6 +1,8 @@
 """GPU parity of the Navier-Stokes operators and the FFT projection against the
 FP64 CPU checker (same seeded inputs).  FP64 kernels must agree to ~1e-12
 relative; FP32 kernels to the FP32 tolerances written at each test."""
+import math
+
 import numpy as np
 import pytest
 import torch
@@ -323,6 +325,62 @@ def test_poisson_plane_waves_large(gpu, n):
         err = float((got.double() * lam - w).abs().max())
         assert err < 2e-6 + 5e-9 * abs(lam) / lmin, (a, b, c, err)
         del w, f, got
+    torch.cuda.empty_cache()
+
+
+
+@pytest.mark.parametrize("n", [64, 128, 256, 512, 1024])
+def test_z_poisson_tridiagonal(gpu, n):
+    """The FP32 Poisson solve with its z pass as per-column cyclic tridiagonal
+    solves (two first-order recursions in FP64) against the transform z pass,
+    both against the FP64 solve of the same right-hand side: the direct form
+    is at least as close to it.  A single mode on the column (kx, ky) = (0, 1),
+    the slowest-decaying recursion (r = 1 - O(1/n)), comes back as d / lambda
+    with lambda the discrete Laplacian's eigenvalue.  Times both forms."""
+    NS, abi = gpu.navierstokes, gpu._abi
+    ctx = abi.context()
+    g = NS.Grid(1.0, n)
+    ps = NS.PoissonSolver(g, torch.float32)
+    ps64 = NS.PoissonSolver(g, torch.float64)
+    gen = torch.Generator(device="cuda").manual_seed(n + 7)
+    d = torch.randn((n, n, n), dtype=torch.float32, device="cuda", generator=gen)
+    d -= d.mean()
+    h = 1.0 / n
+
+    def solve(src, solver, nbytes):
+        ctx.call("ec_d2d", abi.c_void_p(solver.p_ptr), abi.c_void_p(src.data_ptr()), nbytes)
+        ctx.call("ec_poisson_solve", solver.h)
+        return solver.p.data.clone()
+
+    ref = solve(d.double(), ps64, d.numel() * 8)
+    ar = torch.arange(n, device="cuda", dtype=torch.float64)
+    # index order (x, y, z) of the fields: mode 1 along y and z
+    mode = torch.cos(2 * math.pi * (ar.view(1, n, 1) + ar.view(1, 1, n)) / n).expand(n, n, n).contiguous()
+    lam = -4.0 / h**2 * 2 * math.sin(math.pi / n) ** 2
+    out, ms, err, merr = {}, {}, {}, {}
+    try:
+        for on in (1, 0):
+            ctx.call("ec_poisson_set_z_tridiagonal", on)
+            out[on] = solve(d, ps, d.numel() * 4)
+            err[on] = float((out[on].double() - ref).abs().max() / ref.abs().max())
+            pm = solve(mode.float(), ps, d.numel() * 4)
+            merr[on] = float((pm.double() - mode / lam).abs().max() / (1.0 / abs(lam)))
+            del pm
+            torch.cuda.synchronize()
+            t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            t0.record()
+            for _ in range(3):
+                ctx.call("ec_poisson_solve", ps.h)
+            t1.record()
+            t1.synchronize()
+            ms[on] = t0.elapsed_time(t1) / 3
+    finally:
+        ctx.call("ec_poisson_set_z_tridiagonal", 1)
+    print(f"\n[poisson n={n} fp32] vs FP64: tridiagonal z {err[1]:.2e} ({ms[1]:.3f} ms), transform z {err[0]:.2e} "
+          f"({ms[0]:.3f} ms); mode (0,1,1): {merr[1]:.2e} / {merr[0]:.2e}")
+    assert err[1] < 2e-6 and err[1] <= 1.5 * err[0] + 1e-7
+    assert merr[1] < 1e-5 and merr[0] < 1e-5
+    del out, ref, d, mode
     torch.cuda.empty_cache()
 
 
