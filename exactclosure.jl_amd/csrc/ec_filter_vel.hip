@@ -573,12 +573,19 @@ __device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__re
 
     // register double buffer: v, p and the p neighbour of plane r
     T cv[NJ], cp[NJ], cqv[NJ];
-    auto load = [&](int r, T *vv, T *pp, T *qq) {
+    // HALO: a plane past the chunk, which only component 2's face windows read
+    // (the box windows of components 0 and 1 close inside the chunk)
+    auto load = [&](int r, T *vv, T *pp, T *qq, auto halo) {
+        constexpr bool HALO = decltype(halo)::value;
         const unsigned kk = (unsigned)wrap1(z0 + r, n) * n2, k1 = (unsigned)wrap1(z0 + r + 1, n) * n2;
 #pragma unroll
         for (int j = 0; j < NJ; j++) {
             const int a = comp(j);
             if (a == 3) continue;
+            if (HALO && a != 2) {  // (defined values, never read)
+                vv[j] = pp[j] = qq[j] = T(0);
+                continue;
+            }
             const T *vp = a == 0 ? v1 : (a == 1 ? v2 : v3);
             // the filtered field is read once (non-temporal); the pressure and
             // its x/y neighbours are shared with the neighbouring tiles
@@ -591,7 +598,7 @@ __device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__re
             }
         }
     };
-    load(0, cv, cp, cqv);
+    load(0, cv, cp, cqv, std::false_type{});
     // fused RK4 stage: the stage state of plane r (rk_nv values per cell) is
     // loaded one plane ahead, with the filter's loads of that plane
     constexpr int NV = rk_nv<RK>();
@@ -656,13 +663,13 @@ __device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__re
                 const int r = r0 + Q;
                 // plane r + 1 (its filter loads and stage state) in flight through this plane
                 T nv[NJ], np[NJ], nq[NJ];
-                load(r + 1, nv, np, nq);
+                load(r + 1, nv, np, nq, std::integral_constant<bool, !STORE>{});
                 T rkn[NJ][NV];
                 if constexpr (RK >= 0 && STORE) rk_prefetch(r + 1, rkn);
 #pragma unroll
                 for (int j = 0; j < NJ; j++) {
                     const int a = comp(j);
-                    if (a == 3) continue;
+                    if (a == 3 || (!STORE && a != 2)) continue;
                     const T kv = cv[j] - (cqv[j] - cp[j]) * ihT;
                     const double src = (double)kv;
                     if constexpr (RK >= 0 && STORE)
@@ -685,7 +692,7 @@ __device__ __forceinline__ void fv_march(const T *__restrict__ v1, const T *__re
     } else {
         for (int r = 0; r <= rend; ++r) {
             T nv[NJ], np[NJ], nq[NJ];
-            load(r + 1, nv, np, nq);
+            load(r + 1, nv, np, nq, std::false_type{});
             T rkn[NJ][NV];
             if constexpr (RK >= 0) rk_prefetch(r + 1, rkn);
 #pragma unroll
