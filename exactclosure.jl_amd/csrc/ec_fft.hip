@@ -1195,7 +1195,8 @@ int z_poisson_tri(ec_ctx *ctx, ec_poisson *ps, void *spec, int pitch) {
 template <typename T, int N, int MODE, int SIGN>
 int cols_ct(ec_ctx *ctx, ec_poisson *ps, void *spec, int axis, int pitch) {
     if constexpr (sizeof(T) == 4 && N >= 512 && MODE == 0) return cols_w<T, N, SIGN>(ctx, ps, spec, axis, pitch);
-    if constexpr (sizeof(T) == 4 && N >= 64 && MODE == 1)
+    // (below 512 the transform pass is as fast, and takes one launch instead of two)
+    if constexpr (sizeof(T) == 4 && N >= 512 && MODE == 1)
         if (ctx->z_poisson_tri && axis == 2) return z_poisson_tri<N>(ctx, ps, spec, pitch);
     constexpr int NTHR = N / 16;
     // one 128-byte line of complex values per row; the fused Poisson pass (two

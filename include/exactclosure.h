@@ -286,7 +286,7 @@ int ec_ns_filter_vel_rk4_states(ec_ctx *ctx, int dtype, int n, double l, void *c
  * src/N.jl:217). */
 int ec_fft_set_cols_persistent(ec_ctx *ctx, int on);
 /* The z pass of FP32 Poisson solves (poissonsolver, src/N.jl:202-220; n a
- * power of two >= 64): on = 1 (the default) solves each (kx, ky) column's
+ * power of two >= 512): on = 1 (the default) solves each (kx, ky) column's
  * cyclic tridiagonal system (the discrete d^2/dz^2 plus the column's x/y
  * eigenvalues) directly, by two first-order recursions in FP64; on = 0 runs the
  * forward z transform, the division by the eigenvalues and the inverse

@@ -329,7 +329,7 @@ def test_poisson_plane_waves_large(gpu, n):
 
 
 
-@pytest.mark.parametrize("n", [64, 128, 256, 512, 1024])
+@pytest.mark.parametrize("n", [512, 1024])
 def test_z_poisson_tridiagonal(gpu, n):
     """The FP32 Poisson solve with its z pass as per-column cyclic tridiagonal
     solves (two first-order recursions in FP64) against the transform z pass,
