@@ -340,6 +340,35 @@ def test_filter_vel_rk4_states_fused(gpu, n, prec, sets):
     torch.cuda.empty_cache()
 
 
+def test_stream_after_orders_and_refuses(gpu):
+    """ec_stream_after: work queued on one context after the call runs after
+    what the other context had queued (a copy on the second stream reads the
+    result of a long kernel sequence on the first), and the call refuses a
+    context paired with itself or with a context on the same stream."""
+    abi = gpu._abi
+    a = abi.context()
+    s2 = torch.cuda.Stream()
+    b = abi.Context(0, s2.cuda_stream)
+    c = abi.Context(0, a.stream)
+    x = torch.zeros(1 << 24, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(50):  # many dependent passes on the first stream
+        a.call("ec_axpby", abi.EC_F64, x.numel(), 0.0, abi.c_void_p(x.data_ptr()), 1.0, abi.c_void_p(x.data_ptr()))
+    a.call("ec_axpby", abi.EC_F64, x.numel(), 0.0, abi.c_void_p(x.data_ptr()), 1.0, abi.c_void_p(x.data_ptr()))
+    ones = torch.ones_like(x)
+    torch.cuda.synchronize()
+    a.call("ec_axpby", abi.EC_F64, x.numel(), 1.0, abi.c_void_p(ones.data_ptr()), 1.0, abi.c_void_p(x.data_ptr()))
+    b.call("ec_stream_after", a.h)
+    b.call("ec_d2d", abi.c_void_p(y.data_ptr()), abi.c_void_p(x.data_ptr()), x.numel() * 8)
+    b.call("ec_sync")
+    assert bool((y == 1.0).all())
+    for bad in (a, c):
+        with pytest.raises(abi.ECError):
+            bad.call("ec_stream_after", a.h)
+    del x, y, ones
+
+
 @pytest.mark.parametrize("n,sets,prec", [(256, [(16, "va"), (32, "va"), (64, "va"), (64, "fa")], "f32"),
                                          (32, [(8, "va"), (4, "fa")], "f64")])
 def test_les_stream_same_bits(gpu, O, n, sets, prec):
