@@ -108,7 +108,8 @@ __global__ void __launch_bounds__(NT) k_burgers_persistent(const BInst *__restri
     extern __shared__ char smem_raw[];
     __shared__ int s_idx;
     __shared__ double s_red[16];
-    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int tid = threadIdx.x;
+    constexpr int nthr = NT;  // the block size (launched with NT threads): point offsets are compile-time
     for (;;) {
         if (tid == 0) s_idx = atomicAdd(counter, 1);
         __syncthreads();
@@ -119,9 +120,11 @@ __global__ void __launch_bounds__(NT) k_burgers_persistent(const BInst *__restri
         const BInst I = inst[id];
         const int n = I.n, c = I.c, nH = n / c;
         // coarse face averages V u (FP64) first, so every array stays aligned
+        // us[-1] and us[n] are ghost copies of us[n - 1] and us[0], written
+        // with them, so the stencil reads need no wrap-around selects
         double *vbs = reinterpret_cast<double *>(smem_raw);
-        T *us = reinterpret_cast<T *>(vbs + nH);
-        T *ul = us + n;
+        T *us = reinterpret_cast<T *>(vbs + nH) + 1;
+        T *ul = us + n + 1;
         T *um = ul + nH;
         const double h = L / n, H = L / nH;
         const T ih = (T)(1.0 / h), iH = (T)(1.0 / H), tv = (T)visc;
@@ -135,6 +138,8 @@ __global__ void __launch_bounds__(NT) k_burgers_persistent(const BInst *__restri
             if (i < n) {
                 u0[q] = U[I.off + i];
                 us[i] = u0[q];
+                if (i == n - 1) us[-1] = u0[q];
+                if (i == 0) us[n] = u0[q];
                 umax = fmax(umax, fabs((double)u0[q]));
             }
         }
@@ -175,7 +180,7 @@ __global__ void __launch_bounds__(NT) k_burgers_persistent(const BInst *__restri
                 for (int q = 0; q < PT; q++) {
                     const int i = tid + q * nthr;
                     if (i < n) {
-                        const T a = us[i == 0 ? n - 1 : i - 1], b = us[i], cc = us[i + 1 == n ? 0 : i + 1];
+                        const T a = us[i - 1], b = us[i], cc = us[i + 1];
                         kd[q] = (bstress(b, cc, tv, ih) - bstress(a, b, tv, ih)) * ih;
                     }
                 }
@@ -216,12 +221,16 @@ __global__ void __launch_bounds__(NT) k_burgers_persistent(const BInst *__restri
                     const int i = tid + q * nthr;
                     if (i < n) {
                         acc[q] = st == 0 ? kd[q] : acc[q] + bw * kd[q];
+                        T v;
                         if (st < 3) {
-                            us[i] = u0[q] + a_next * kd[q];
+                            v = u0[q] + a_next * kd[q];
                         } else {
                             u0[q] = u0[q] + tdt / 6 * acc[q];
-                            us[i] = u0[q];
+                            v = u0[q];
                         }
+                        us[i] = v;
+                        if (i == n - 1) us[-1] = v;
+                        if (i == 0) us[n] = v;
                     }
                 }
 #pragma unroll
@@ -298,8 +307,9 @@ static int pt_max(int threads) { return std::min(64, 32768 / threads); }
 
 template <typename T>
 static size_t batch_lds(const ec_burgers_batch *b) {
-    // V u at the coarse faces (FP64), then the DNS and the two LES stage states
-    return sizeof(double) * (size_t)b->nHmax + sizeof(T) * ((size_t)b->nmax + 2 * (size_t)b->nHmax);
+    // V u at the coarse faces (FP64), then the DNS stage state with a ghost
+    // point at either end and the two LES stage states
+    return sizeof(double) * (size_t)b->nHmax + sizeof(T) * ((size_t)b->nmax + 2 + 2 * (size_t)b->nHmax);
 }
 
 template <typename T, int PT, int NT, int PTL>
