@@ -1,4 +1,4 @@
-"""Build libexactclosure.so (gfx950) and the FP64 CPU checker library.
+"""Build libexactclosure.so (gfx950) and the FP64 CPU oracle library.
 
     python exactclosure.jl_amd/build.py            # build the HIP library
     python exactclosure.jl_amd/build.py --oracle   # also the oracle (test infrastructure)

@@ -418,7 +418,7 @@ extern "C" int ec_reduce_diffsq(ec_ctx *ctx, int dtype, size_t count, const void
 
 // ---------------------------------------------------------------- random numbers
 // Counter-based normals: splitmix64 hashing + Box-Muller, the same integer
-// recipe the FP64 CPU checker uses, so both produce the same variates up to
+// recipe the FP64 CPU oracle uses, so both produce the same variates up to
 // the last-ulp differences of log/cos.
 __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
     x += 0x9E3779B97F4A7C15ULL;

@@ -1,7 +1,7 @@
 /*
  * FP64 CPU oracle for the DNS-aided LES hot path of ExactClosure.jl.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker: the parity tests,
+ * TEST INFRASTRUCTURE ONLY.  The parity tests,
  * the smoke check and the CPU-baseline leg of the benchmark call it.  The
  * product path (libexactclosure, the HIP kernels) never links or calls it.
  *

@@ -1,7 +1,7 @@
 """FP64 CPU oracle for the DNS-aided LES hot path (TEST INFRASTRUCTURE ONLY).
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
-``bench.py`` may import this module, and only as the checker.  The product
+``bench.py`` may import this module, and only to check the product against it.  The product
 (``exactclosure.jl_amd``) never imports it.
 
 Fields are Fortran-ordered float64 arrays of shape (n, n, n) indexed [i, j, k]

@@ -1,4 +1,4 @@
-"""Filter weights pinned by their mathematics (CPU), for the checker's
+"""Filter weights pinned by their mathematics (CPU), for the oracle's
 restatement and the host mirror alike, so that neither side can carry a
 mistake the other shares: normalisation, symmetry, the Gaussian's discrete
 second moment sigma^2 = Delta^2 / 12, Julia's ties-to-even radius rounding,
@@ -132,7 +132,7 @@ def test_burgers_les_kernels(ec, O, nh, nH, dr, kind):
     4.5 and 7.5 round to 4 and 8 -> 5 and 9), the Gaussian has variance
     (Delta/h)^2 / 12 up to its 3-sigma truncation, and the double kernel has the
     variance of the finite-volume kernel plus that of the LES kernel; the
-    checker's and the mirror's kernels agree."""
+    oracle's and the mirror's kernels agree."""
     L = 2 * math.pi
     (Ro, wo), (Ko, do) = O.burgers_les_kernels(nh, nH, L, dr, kind)
     (Rm, wm), (Km, dm) = ec.burgers.les_kernels(nh, nH, L, dr, kind)

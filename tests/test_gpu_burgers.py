@@ -1,5 +1,5 @@
 """GPU parity of the batched Burgers DNS-aided LES persistent kernel against the
-FP64 CPU checker (per instance), graph replay against eager launches, and the
+FP64 CPU oracle (per instance), graph replay against eager launches, and the
 exactness of the swap-filter LES (src/Burgers.jl:1087-1089 identity)."""
 import numpy as np
 import pytest
@@ -28,7 +28,7 @@ def test_burgers_rhs(gpu, O):
 
 
 @pytest.mark.parametrize("threads", [128, 256, 1024])
-def test_batch_fp64_matches_checker(gpu, O, threads):
+def test_batch_fp64_matches_oracle(gpu, O, threads):
     B = gpu.burgers
     n, c = _small_batch()
     visc, L, seed, steps = 5e-3, 2 * np.pi, 11, 40
@@ -61,7 +61,7 @@ def test_batch_fp64_matches_checker(gpu, O, threads):
 def test_batch_points_per_thread_builds(gpu, O, prec, threads, n, c):
     """Builds with several points per thread (the register arrays of the DNS
     state and of the LES states in their PT/2 and PT/4 widths) against the
-    checker; FP64 to 1e-12, FP32 to 2e-5 after 10 steps."""
+    oracle; FP64 to 1e-12, FP32 to 2e-5 after 10 steps."""
     B = gpu.burgers
     visc, L, seed, steps = 5e-4, 2 * np.pi, 21, 10
     b = B.BurgersBatch(n, c, visc=visc, L=L, cfl=0.3, seed=seed, dtype=prec)
@@ -126,10 +126,10 @@ def test_invalid_batch_rejected(gpu):
                                              (13500, 300, "gaussian", 2.0, 6), (9000, 10, "tophat", 8.0, 2)])
 def test_burgers_fractions(gpu, O, n, nH, kind, dr, ns):
     """compute_fractions (src/Burgers.jl:1022-1099) on the GPU against the
-    checker on the same snapshots, FP64: the mean fractions to 1e-12 (the last
+    oracle on the same snapshots, FP64: the mean fractions to 1e-12 (the last
     case is the reference's own setup, nh = 13500, nH = 300, Delta = 2H,
     Gaussian, on a few samples); the GPU path builds its kernels with the host
-    mirror of tophat/gaussian_weights and build_doublekernel, the checker with
+    mirror of tophat/gaussian_weights and build_doublekernel, the oracle with
     its own restatement of them.  The fine pass stages each
     block's stretch of the row in LDS; the 9000 / 10 case (double kernel of
     8101 taps) is wider than that stage and takes the direct pass."""
@@ -153,7 +153,7 @@ def test_burgers_fractions(gpu, O, n, nH, kind, dr, ns):
                                              (13500, 2700, "tophat", 4.0, 3)])
 def test_burgers_dissipation(gpu, O, n, nH, kind, dr, ns):
     """compute_dissipation (src/Burgers.jl:447-516) on the GPU against the
-    checker on the same snapshots, FP64, every coefficient of every model to
+    oracle on the same snapshots, FP64, every coefficient of every model to
     1e-10 of the model's largest (the parts are differences of O(1) stresses
     filtered by sums of up to 2K + 1 terms that the GPU contracts into FMAs, so
     they carry about 1e-15 absolute rounding against values near 1e-3); the
@@ -186,7 +186,7 @@ def _smag_samples(n, ns, seed, L):
                                           (4000, 8, "tophat", 6.0), (9000, 10, "tophat", 8.0)])
 def test_smagorinsky_fit(gpu, O, n, nH, kind, dr):
     """The four sums of the Smagorinsky fit (smagorinsky.jl:24-36) against the
-    checker on the same snapshots (1e-10 relative; FP64 sums in another
+    oracle on the same snapshots (1e-10 relative; FP64 sums in another
     order); theta from them; the reference's setup sizes included.  The last
     two cases have kernels wider than the LDS-staged passes hold (K = 1750:
     direct fit pass; K = 4050: direct fine and fit passes)."""
@@ -216,7 +216,7 @@ def test_smagorinsky_fit(gpu, O, n, nH, kind, dr):
                                             (13500, 300, "gaussian", 2.0, 1e-3), (13500, 2700, "gaussian", 2.0, 4e-8)])
 def test_smagorinsky_solve(gpu, O, n, nH, kind, dr, C):
     """solve_smagorinsky (src/Burgers.jl:682-772) on the GPU, every sample in
-    one launch, against the checker's per-sample loop (same CFL steps, the
+    one launch, against the oracle's per-sample loop (same CFL steps, the
     reference's accumulating RK4 stages), FP64 to 1e-11; nH = 2700 (eleven
     points per thread) with C near the fitted theta^2 (Delta^2 + H^2) of that
     grid (the CFL step ignores the eddy viscosity, so a C far above it blows
@@ -265,9 +265,9 @@ def test_study_entry_points_reject_bad_arguments(gpu):
                  1, w.ctypes.data_as(dp), gpu._abi.c_void_p(ub.data_ptr()))
 
 
-def test_burgers_randomfield_vs_checker(gpu, O):
+def test_burgers_randomfield_vs_oracle(gpu, O):
     """randomfield with bump_profile (src/Burgers.jl:192-202) for several
-    samples in one launch against the checker's numpy irfft of the whole
+    samples in one launch against the oracle's numpy irfft of the whole
     spectrum (the GPU sums modes up to 6 kpeak): 1e-13 of the field's maximum,
     energy to 1e-12."""
     B = gpu.burgers
@@ -281,9 +281,9 @@ def test_burgers_randomfield_vs_checker(gpu, O):
 
 @pytest.mark.parametrize("nh,nH,kind,dr", [(1350, 30, "gaussian", 2.0), (1350, 90, "tophat", 1.0),
                                            (1350, 270, "gaussian", 2.0), (2025, 135, "gaussian", 0.0)])
-def test_burgers_dns_aided_les_vs_checker(gpu, O, nh, nH, kind, dr):
+def test_burgers_dns_aided_les_vs_oracle(gpu, O, nh, nH, kind, dr):
     """dns_aided_les (src/Burgers.jl:350-445), every sample in one persistent
-    launch, against the checker's per-sample loop: same number of CFL steps
+    launch, against the oracle's per-sample loop: same number of CFL steps
     and all seven fields (DNS, filtered DNS, the four LES) to 1e-11 of their
     maximum after t = 0.01; the swap-filter LES equals the filtered DNS to
     rounding (the exact closure)."""

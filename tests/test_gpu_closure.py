@@ -1,5 +1,5 @@
 """GPU parity of the filters, the exact-closure fluxes and the RK4 DNS-aided
-LES driver against the FP64 CPU checker, plus the exactness invariant
+LES driver against the FP64 CPU oracle, plus the exactness invariant
 (navierstokes.jl:33: the classic+flux+div LES equals the filtered DNS to
 rounding) and the FP32-pipeline closure error against FP64 (gate 1e-5)."""
 import numpy as np
@@ -35,7 +35,7 @@ def test_closure_quantities(gpu, O, n, c, kind):
                                     (64, [(16, "va"), (8, "va"), (4, "va"), (4, "fa")])])
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_filter_vel(gpu, O, n, sets, prec):
-    """V(u) and V(k), k = du - grad p, against the checker's separable filters;
+    """V(u) and V(k), k = du - grad p, against the oracle's separable filters;
     and V(k) = -div_H^ns (S r) (the flux form) on the same data."""
     NS = gpu.navierstokes
     dt = torch.float64 if prec == "f64" else torch.float32
@@ -94,7 +94,7 @@ def test_filter_coarse_gaussian(gpu, O):
     FD = NS.gaussian(g, 2 * NS.spacing(gb), 2)
     FDH = NS.composekernel(NS.tophat(g, c), FD)
     u = O.randomfield(n, seed=3)
-    # the checker builds its kernel with its own gaussian / composekernel / tophat
+    # the oracle builds its kernel with its own gaussian / composekernel / tophat
     OFDH = O.composekernel(O.tophat(c), O.gaussian(1.0 / n, 2.0 * c / n, 2))
     ref = O.coarsegrain(O.applyfilter(u[0], OFDH, OFDH, OFDH), n // c, (True, False, False))
     ub = NS.Field(gb)

@@ -2,11 +2,11 @@
 
 * 256^3 FP64 DNS-aided RK4 (nu = 1/2000, dt = 1e-3, 20 steps, 32^3
   volume-average exact closure): the DNS and the LES against the FP64 CPU
-  checker's RK4 DNS-aided step run on the same initial field;
+  oracle's RK4 DNS-aided step run on the same initial field;
 * 4096 batched Burgers instances (N in {2048, 4096, 8192, 16384}, 40/30/20/10
   %, c in {4, 8, 16, 32, 64}, nu = 5e-4), 1000 RK4 steps in one
   graph-replayed persistent launch at the 1024-thread build: every size
-  class and compression checked against the checker;
+  class and compression checked against the oracle;
 * 512^3 FP32 pipeline against the plain FP64 form (nu = 1/4000, 50 steps,
   64^3 and 128^3 exact closures): velocity, divergence and closure errors;
 * the FP32 divergence over many steps (the state's divergence is removed
@@ -27,10 +27,10 @@ def rel(a, b):
     return float(np.abs(a - b).max() / np.abs(b).max())
 
 
-def test_config_256_fp64_rk4_20_steps_vs_checker(gpu, O):
+def test_config_256_fp64_rk4_20_steps_vs_oracle(gpu, O):
     """256^3 FP64, 20 RK4 steps of dt = 1e-3 with the 32^3 volume-average
-    closure at every stage, from the checker's random field (seed 123): the
-    DNS state and the LES state agree with the checker's dnsaid_rk4 to 1e-10
+    closure at every stage, from the oracle's random field (seed 123): the
+    DNS state and the LES state agree with the oracle's dnsaid_rk4 to 1e-10
     (max-norm relative; the FFTs differ in their rounding), and the LES equals
     the filtered DNS to rounding (the exactness invariant)."""
     NS, D = gpu.navierstokes, gpu.dnsaid
@@ -68,7 +68,7 @@ def test_config_burgers_4096_instances_1000_steps(gpu, O):
     1024 threads (16 points per thread for the 16384-point instances, LES
     arrays of 4 points per thread).  For one instance of every (size,
     compression) pair the DNS, exact-closure LES and no-model LES are checked
-    against the checker's FP64 RK4 from the same initial field and dt
+    against the oracle's FP64 RK4 from the same initial field and dt
     (FP32 rounding over 1000 steps, measured 1.4e-5 of the field's maximum:
     bound 1e-4), and the exact-closure LES against the filter of the FP32
     DNS (exactness, measured 1.0e-5: bound 1e-4)."""
@@ -99,9 +99,9 @@ def test_config_burgers_4096_instances_1000_steps(gpu, O):
     b.close()
 
 
-def test_config_burgers_1000_steps_fp64_vs_checker(gpu, O):
+def test_config_burgers_1000_steps_fp64_vs_oracle(gpu, O):
     """The same 1000-step graph-replayed launch in FP64, checked to 1e-9
-    against the checker: the algorithm, not only the FP32 rounding, at the
+    against the oracle: the algorithm, not only the FP32 rounding, at the
     full length.  FP64 state takes twice the LDS, so the classes run as two
     batches whose per-block state fits (8 nmax + 24 nHmax bytes <= 160 KiB):
     every class up to 8192 points, then the 16384-point classes with LES

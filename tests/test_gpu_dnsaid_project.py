@@ -1,5 +1,5 @@
 """The reference's five-variant DNS-aided LES comparison (dnsaid_project,
-src/N.jl:1239-1332) on the GPU against the FP64 CPU checker: forward Euler,
+src/N.jl:1239-1332) on the GPU against the FP64 CPU oracle: forward Euler,
 the Gaussian-composed filters of get_kernels (Delta = 2H, nsigma = 2), the
 no-model, classic, classic+flux, exact (classic+flux+div) and symmetrised LES,
 with fixed steps and with the reference's CFL time control; and the error
@@ -27,10 +27,10 @@ def _gpu_run(gpu, u, n, nl, visc, dr, nsigma):
 
 
 @pytest.mark.parametrize("n,nl,dr", [(15, 3, 2.0), (30, 6, 2.0), (27, 9, 2.0), (25, 5, 0.0)])
-def test_dnsaid_project_fixed_steps_vs_checker(gpu, O, n, nl, dr):
-    """Six forward-Euler steps of dt = 2e-3 from the checker's k^(-5/3)
+def test_dnsaid_project_fixed_steps_vs_oracle(gpu, O, n, nl, dr):
+    """Six forward-Euler steps of dt = 2e-3 from the oracle's k^(-5/3)
     field: every LES variant and the final apply_DHpi agree with the
-    checker's dnsaid_project_fe to 1e-11 (max-norm relative)."""
+    oracle's dnsaid_project_fe to 1e-11 (max-norm relative)."""
     visc, dt, steps = 2e-4, 2e-3, 6
     u = O.randomfield(n, seed=1, profile="power", param=-5 / 3, totalenergy=1.0)
     run = _gpu_run(gpu, u, n, nl, visc, dr, 2)
@@ -43,10 +43,10 @@ def test_dnsaid_project_fixed_steps_vs_checker(gpu, O, n, nl, dr):
             assert rel(host(got[k][a].data), ref[k][a]) < 1e-11, (k, a)
 
 
-def test_dnsaid_project_cfl_time_control_vs_checker(gpu, O):
+def test_dnsaid_project_cfl_time_control_vs_oracle(gpu, O):
     """The reference's time control (forward-Euler warm-up, then dt =
     min(propose_timestep, tstop - t) each step, CFL 0.4): same number of
-    steps and the same five LES fields as the checker's dnsaid_project."""
+    steps and the same five LES fields as the oracle's dnsaid_project."""
     n, nl, visc = 30, 6, 2e-4
     u = O.randomfield(n, seed=2, profile="power", param=-5 / 3, totalenergy=1.0)
     run = _gpu_run(gpu, u, n, nl, visc, 2.0, 2)
@@ -76,7 +76,7 @@ def test_dnsaid_project_error_ordering(gpu):
 
 def test_separable_filter_matches_dense(gpu, O):
     """ec_filter_coarse with kernels wider than the dense launch takes (the
-    separable passes) against the checker's dense lazycoarsegrain(lazyfilter)
+    separable passes) against the oracle's dense lazycoarsegrain(lazyfilter)
     on every source type: velocity, stress, r = sigma + p and pressure."""
     NS = gpu.navierstokes
     n, nl = 40, 8
@@ -96,8 +96,8 @@ def test_separable_filter_matches_dense(gpu, O):
         assert rel(host(o), want) < 1e-12, name
 
 
-def test_filter_apply_matches_checker(gpu, O):
-    """ec_filter_apply (three 1D passes) against the checker's dense
+def test_filter_apply_matches_oracle(gpu, O):
+    """ec_filter_apply (three 1D passes) against the oracle's dense
     applyfilter! with the Gaussian-composed kernel, FP64 and FP32."""
     NS = gpu.navierstokes
     n = 20

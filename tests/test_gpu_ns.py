@@ -1,5 +1,5 @@
 """GPU parity of the Navier-Stokes operators and the FFT projection against the
-FP64 CPU checker (same seeded inputs).  FP64 kernels must agree to ~1e-12
+FP64 CPU oracle (same seeded inputs).  FP64 kernels must agree to ~1e-12
 relative; FP32 kernels to the FP32 tolerances written at each test."""
 import math
 
@@ -64,7 +64,7 @@ def test_rhs_marching_tiles_fp64(gpu, O, n):
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_rhs_div(gpu, O, n, prec):
     """Fused du = -div sigma(u), d = div_h du (marching kernel for n % 64 == 0,
-    the two plain kernels otherwise) against the checker's rhs and divergence;
+    the two plain kernels otherwise) against the oracle's rhs and divergence;
     then the FFT solve on d gives the pressure of ec_ns_pressure_solve."""
     dt = torch.float64 if prec == "f64" else torch.float32
     tol = 1e-12 if prec == "f64" else 2e-5
@@ -101,7 +101,7 @@ def test_rhs_div(gpu, O, n, prec):
 def test_rhs_div_state_divergence(gpu, O, n, prec):
     """d = div_h du + r div_h u: the divergence of the stage state folded into
     the Poisson right-hand side (r = 6/dt at the last RK4 stage), against the
-    checker's divergence of du and of u (an unprojected random u, so div_h u
+    oracle's divergence of du and of u (an unprojected random u, so div_h u
     is of the size of its terms)."""
     dt = torch.float64 if prec == "f64" else torch.float32
     u = rand_u(O, n, 5 * n + 3)
@@ -153,7 +153,7 @@ def test_rhs_div_large_fp32(gpu, n):
 @pytest.mark.parametrize("s", [1.0, -1.0])
 def test_rhs_add(gpu, O, n, prec, s):
     """du = s F(u) + add (FP64 addend, one rounding) on the plain kernel (16)
-    and the marching kernel (256): against the checker's s rhs + add, and
+    and the marching kernel (256): against the oracle's s rhs + add, and
     bitwise against the two-pass form it replaces (the RHS written, then an
     FP64 axpy of the addend: ec_axpby for FP64 fields, ec_axpy2 for FP32)."""
     dt = torch.float64 if prec == "f64" else torch.float32
@@ -550,7 +550,7 @@ def test_rk4_update_states_matches_accumulator_form(gpu, O, n, prec):
     agrees to rounding of the stored states (FP64 1e-14; FP32 2 ulp of |u|).
     n = 13 takes the one-cell-per-thread path (not 16-byte aligned rows), the
     others the 16-byte one.  At n = 12 the new u0 is also checked against the
-    checker's pressure gradient and the textbook tableau in FP64."""
+    oracle's pressure gradient and the textbook tableau in FP64."""
     dt_t = torch.float64 if prec == "f64" else torch.float32
     code = 1 if prec == "f64" else 0
     step, l = 0.01, 1.0
@@ -582,7 +582,7 @@ def test_rk4_update_states_matches_accumulator_form(gpu, O, n, prec):
     for a in range(3):
         assert float((ub[a] - ua[a]).abs().max()) <= tol * float(ua[a].abs().max()), a
     if n == 12:
-        # the textbook tableau on the checker's k = du - grad p of every stage
+        # the textbook tableau on the oracle's k = du - grad p of every stage
         ks = [O.pressuregradient([host(x) for x in dus[st]], host(ps_[st])) for st in range(4)]
         for a in range(3):
             want = host(u0[a]) + step / 6 * (ks[0][a] + 2 * ks[1][a] + 2 * ks[2][a] + ks[3][a])
@@ -658,7 +658,7 @@ def test_axpby(gpu, prec):
 @pytest.mark.parametrize("n,prec", [(16, "f64"), (32, "f64"), (24, "f64"), (64, "f32")])
 def test_spectrum(gpu, O, n, prec):
     """spectrum(u, spectral_stuff(grid), poisson) (src/N.jl:547-590) against the
-    checker's shell sums on the same field: the shells 1..n/3 and their
+    oracle's shell sums on the same field: the shells 1..n/3 and their
     wavenumbers 2 pi / l * shell; all shells through the C entry point."""
     NS = gpu.navierstokes
     dt = torch.float64 if prec == "f64" else torch.float32
@@ -736,10 +736,10 @@ def test_invalid_arguments_raise(gpu):
 
 
 @pytest.mark.parametrize("n", [16, 32])
-def test_steppers_fe_wray3_vs_checker(gpu, O, n):
+def test_steppers_fe_wray3_vs_oracle(gpu, O, n):
     """step_forwardeuler! and step_wray3! (src/N.jl:309-346) through the host
     mirror (the fused RHS, axpy and the FFT projection on the GPU, FP64)
-    against the checker's steppers over 5 steps each, to 1e-12."""
+    against the oracle's steppers over 5 steps each, to 1e-12."""
     NS = gpu.navierstokes
     g = NS.Grid(1.0, n)
     visc, dt = 0.01, 2e-3
@@ -762,7 +762,7 @@ def test_steppers_fe_wray3_vs_checker(gpu, O, n):
 
 def test_steppers_convergence_order_on_gpu(gpu, O):
     """Observed order in dt of the GPU FP64 steppers over [0, 0.08] at n = 16
-    (the pins of the checker's steppers, run on the product path): forward
+    (the pins of the oracle's steppers, run on the product path): forward
     Euler 1, Wray RK3 3, the RK4 DNS step of DNSAidedLES (no LES grids) 4."""
     NS, D = gpu.navierstokes, gpu.dnsaid
     n, visc, T = 16, 0.01, 0.08

@@ -1,6 +1,6 @@
 """Sub-filter stresses and their dissipation (sfs_tensors_volume /
 sfs_tensors_surface, src/NavierStokes.jl:470-593; the dissipation fields of
-process.jl:107-193) on the GPU against the FP64 CPU checker, and the exact
+process.jl:107-193) on the GPU against the FP64 CPU oracle, and the exact
 closure identity of the volume swap-filter stress at 128^3."""
 import numpy as np
 import pytest
@@ -19,7 +19,7 @@ def rel(a, b):
 
 @pytest.mark.parametrize("experiment", ["volavg", "project_volavg", "surfavg"])
 @pytest.mark.parametrize("n,nb", [(24, 8), (24, 6), (20, 4)])
-def test_sfs_tensors_vs_checker(gpu, O, experiment, n, nb):
+def test_sfs_tensors_vs_oracle(gpu, O, experiment, n, nb):
     """fu and the three sub-filter stresses of every experiment, odd and even
     compressions, FP64: 1e-11 (max-norm relative per component)."""
     NS = gpu.navierstokes
@@ -36,7 +36,7 @@ def test_sfs_tensors_vs_checker(gpu, O, experiment, n, nb):
             assert np.abs(host(got[k][m]) - ref[k][m]).max() < 1e-11 * scale, (k, ab)
 
 
-def test_tensordissipation_vs_checker(gpu, O):
+def test_tensordissipation_vs_oracle(gpu, O):
     NS = gpu.navierstokes
     n = 18
     rng = np.random.default_rng(5)
@@ -49,7 +49,7 @@ def test_tensordissipation_vs_checker(gpu, O):
 
 
 @pytest.mark.parametrize("experiment", ["volavg", "surfavg"])
-def test_sfs_dissipation_vs_checker(gpu, O, experiment):
+def test_sfs_dissipation_vs_oracle(gpu, O, experiment):
     """The end-to-end dissipation fields of one LES grid (process.jl:107-193)."""
     NS = gpu.navierstokes
     n, nb, visc = 30, 10, 0.005

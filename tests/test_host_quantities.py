@@ -1,14 +1,14 @@
 """Host logic of the DNS-aided quantities (no GPU): the filter pieces the
 host mirror hands to ec_filter_coarse for apply_DHpi, get_sigDH and get_rDHi
 (src/N.jl:1038-1115, 1228-1237), built from get_kernels' centred kernels and
-the coarse-grain positions, equal the checker's independently composed
+the coarse-grain positions, equal the oracle's independently composed
 pieces (face / box / centre / point pieces convolved with the Gaussian)."""
 import numpy as np
 import pytest
 
 
 @pytest.mark.parametrize("c,dr", [(3, 0.0), (5, 2.0), (7, 1.5), (3, 2.0)])
-def test_dnsaid_quantities_match_checker_pieces(ec, O, c, dr):
+def test_dnsaid_quantities_match_oracle_pieces(ec, O, c, dr):
     NS = ec.navierstokes
     n = 12 * c
     g, gl = NS.Grid(1.0, n), NS.Grid(1.0, n // c)

@@ -1,4 +1,4 @@
-"""Pins of the FP64 CPU checker (oracle/) by what the mathematics fixes.
+"""Pins of the FP64 CPU oracle (oracle/) by what the mathematics fixes.
 
 The reference holds no golden vectors and cannot run here (Julia is absent),
 so every oracle function is pinned by checks chosen so a plausible mistake
