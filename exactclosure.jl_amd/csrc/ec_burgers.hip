@@ -49,30 +49,42 @@ __global__ void k_burgers_ic(const BInst *__restrict__ inst, int count, int kmax
 }
 
 // Volume-average filter of the DNS face values at coarse face J (odd c:
-// top-hat of c faces; even c: trapezoid over c+1 faces).
-template <typename T>
-__device__ __forceinline__ double vavg(const T *u, int n, int c, int J) {
+// top-hat of c faces; even c: trapezoid over c+1 faces).  `at` maps a point
+// index in [0, n) to its storage index.
+template <typename T, typename At>
+__device__ __forceinline__ double vavg(const T *u, int n, int c, int J, At at) {
     const int f = c * (J + 1) - 1;
     const int h = c / 2;
     double s = 0.0;
     if (c & 1) {
+#pragma unroll 4
         for (int t = -h; t <= h; t++) {
             int m = f + t;
             m = m < 0 ? m + n : (m >= n ? m - n : m);
-            s += (double)u[m];
+            s += (double)u[at(m)];
         }
         return s / c;
     }
+#pragma unroll 4
     for (int t = -h + 1; t <= h - 1; t++) {
         int m = f + t;
         m = m < 0 ? m + n : (m >= n ? m - n : m);
-        s += (double)u[m];
+        s += (double)u[at(m)];
     }
     int a = f - h, b = f + h;
     a = a < 0 ? a + n : (a >= n ? a - n : a);
     b = b < 0 ? b + n : (b >= n ? b - n : b);
-    return (s + 0.5 * ((double)u[a] + (double)u[b])) / c;
+    return (s + 0.5 * ((double)u[at(a)] + (double)u[at(b)])) / c;
 }
+
+// The DNS stage state in LDS skips one word after every 2^kPadShift points:
+// point m is stored at pad_at(m) = m + (m >> kPadShift).  The face averages
+// and the commutator read it at a stride of c points, which would put the
+// lanes of a wave on a few LDS banks for power-of-two c; with the skip they
+// fall on distinct banks.  pad_at(-1) = -2 and pad_at(n) hold ghost copies
+// of points n - 1 and 0.
+constexpr int kPadShift = 5;
+__device__ __forceinline__ int pad_at(int m) { return m + (m >> kPadShift); }
 
 template <typename T>
 __global__ void k_burgers_les_init(const BInst *__restrict__ inst, int count, const T *__restrict__ U,
@@ -82,7 +94,7 @@ __global__ void k_burgers_les_init(const BInst *__restrict__ inst, int count, co
     const BInst I = inst[b];
     const int nH = I.n / I.c;
     for (int J = blockIdx.x * blockDim.x + threadIdx.x; J < nH; J += gridDim.x * blockDim.x) {
-        const T v = (T)vavg(U + I.off, I.n, I.c, J);
+        const T v = (T)vavg(U + I.off, I.n, I.c, J, [](int m) { return m; });
         UL[I.offl + J] = v;
         UM[I.offl + J] = v;
     }
@@ -119,13 +131,25 @@ __global__ void __launch_bounds__(NT) k_burgers_persistent(const BInst *__restri
         const int id = order[w];
         const BInst I = inst[id];
         const int n = I.n, c = I.c, nH = n / c;
-        // coarse face averages V u (FP64) first, so every array stays aligned
-        // us[-1] and us[n] are ghost copies of us[n - 1] and us[0], written
-        // with them, so the stencil reads need no wrap-around selects
+        // coarse face averages V u (FP64) first, so every array stays aligned;
+        // then the DNS stage state, point m at us[pad_at(m)], with ghost
+        // copies of points n - 1 and 0 at pad_at(-1) and pad_at(n), so the
+        // stencil reads need no wrap-around selects
         double *vbs = reinterpret_cast<double *>(smem_raw);
-        T *us = reinterpret_cast<T *>(vbs + nH) + 1;
-        T *ul = us + n + 1;
+        T *us = reinterpret_cast<T *>(vbs + nH) + 2;
+        T *ul = us + pad_at(n) + 1;
         T *um = ul + nH;
+        // this thread's points tid + q NT sit at us[ps + q * PSTR]; their left
+        // and right neighbours one word further, or two where the skipped
+        // word lies between (every 2^kPadShift-th lane)
+        static_assert(NT % (1 << kPadShift) == 0, "block size a multiple of the pad period");
+        constexpr int PSTR = NT + (NT >> kPadShift);
+        constexpr int PMASK = (1 << kPadShift) - 1;
+        const int ps = pad_at(tid);
+        T *const usq = us + ps;
+        const T *const usl = usq - ((tid & PMASK) == 0 ? 2 : 1);
+        const T *const usr = usq + ((tid & PMASK) == PMASK ? 2 : 1);
+        T *const gl = us + pad_at(-1), *const gr = us + pad_at(n);
         const double h = L / n, H = L / nH;
         const T ih = (T)(1.0 / h), iH = (T)(1.0 / H), tv = (T)visc;
 
@@ -137,9 +161,9 @@ __global__ void __launch_bounds__(NT) k_burgers_persistent(const BInst *__restri
             const int i = tid + q * nthr;
             if (i < n) {
                 u0[q] = U[I.off + i];
-                us[i] = u0[q];
-                if (i == n - 1) us[-1] = u0[q];
-                if (i == 0) us[n] = u0[q];
+                usq[q * PSTR] = u0[q];
+                if (i == n - 1) *gl = u0[q];
+                if (i == 0) *gr = u0[q];
                 umax = fmax(umax, fabs((double)u0[q]));
             }
         }
@@ -173,14 +197,14 @@ __global__ void __launch_bounds__(NT) k_burgers_persistent(const BInst *__restri
             for (int st = 0; st < 4; st++) {
                 // --- coarse face averages of the stage state, once per face (the
                 // commutator of coarse point J reads faces J - 1 and J)
-                for (int J = tid; J < nH; J += nthr) vbs[J] = vavg(us, n, c, J);
+                for (int J = tid; J < nH; J += nthr) vbs[J] = vavg(us, n, c, J, pad_at);
                 // --- DNS right-hand side (rhs!, src/Burgers.jl:103-109)
                 T kd[PT];
 #pragma unroll
                 for (int q = 0; q < PT; q++) {
                     const int i = tid + q * nthr;
                     if (i < n) {
-                        const T a = us[i - 1], b = us[i], cc = us[i + 1];
+                        const T a = usl[q * PSTR], b = usq[q * PSTR], cc = usr[q * PSTR];
                         kd[q] = (bstress(b, cc, tv, ih) - bstress(a, b, tv, ih)) * ih;
                     }
                 }
@@ -189,10 +213,10 @@ __global__ void __launch_bounds__(NT) k_burgers_persistent(const BInst *__restri
                 auto tau_at = [&](int Jx) {
                     const double vb = vbs[Jx], va = vbs[Jx == 0 ? nH - 1 : Jx - 1];
                     const int m0i = c * (Jx + 1) - 1 - c / 2;
+                    // m lies in [0, n) (c / 2 - 1 <= m0i, m0i + 1 <= n - c / 2);
+                    // point m - 1 = -1 is the ghost copy of point n - 1
                     auto sfine = [&](int m) {
-                        const int mm = m < 0 ? m + n : (m >= n ? m - n : m);
-                        const int ml = mm == 0 ? n - 1 : mm - 1;
-                        const double a = (double)us[ml], b = (double)us[mm];
+                        const double a = (double)us[pad_at(m - 1)], b = (double)us[pad_at(m)];
                         return -((a + b) * (a + b)) / 8 + visc * (b - a) / h;
                     };
                     const double fsu = (c & 1) ? sfine(m0i) : 0.5 * (sfine(m0i) + sfine(m0i + 1));
@@ -228,9 +252,9 @@ __global__ void __launch_bounds__(NT) k_burgers_persistent(const BInst *__restri
                             u0[q] = u0[q] + tdt / 6 * acc[q];
                             v = u0[q];
                         }
-                        us[i] = v;
-                        if (i == n - 1) us[-1] = v;
-                        if (i == 0) us[n] = v;
+                        usq[q * PSTR] = v;
+                        if (i == n - 1) *gl = v;
+                        if (i == 0) *gr = v;
                     }
                 }
 #pragma unroll
@@ -307,9 +331,10 @@ static int pt_max(int threads) { return std::min(64, 32768 / threads); }
 
 template <typename T>
 static size_t batch_lds(const ec_burgers_batch *b) {
-    // V u at the coarse faces (FP64), then the DNS stage state with a ghost
-    // point at either end and the two LES stage states
-    return sizeof(double) * (size_t)b->nHmax + sizeof(T) * ((size_t)b->nmax + 2 + 2 * (size_t)b->nHmax);
+    // V u at the coarse faces (FP64), then the DNS stage state (the skipped
+    // words, a ghost point at either end) and the two LES stage states
+    const size_t dns = (size_t)b->nmax + ((size_t)b->nmax >> kPadShift) + 3;
+    return sizeof(double) * (size_t)b->nHmax + sizeof(T) * (dns + 2 * (size_t)b->nHmax);
 }
 
 template <typename T, int PT, int NT, int PTL>
