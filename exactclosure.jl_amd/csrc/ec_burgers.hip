@@ -100,6 +100,18 @@ __global__ void k_burgers_les_init(const BInst *__restrict__ inst, int count, co
     }
 }
 
+// x / y correctly rounded, from r = RN(1 / y) and two corrections with exact
+// FMA residuals: x r is within two ulps of x / y, the first correction makes
+// it faithful, and then q + (x - y q) r rounds to RN(x / y) (Markstein,
+// IBM J. Res. Dev. 34 (1990) 111-119).  For finite x and normal y, r, x / y.
+__device__ __forceinline__ double div_by(double x, double y, double r) {
+    double q = x * r;
+    double e = fma(-q, y, x);
+    q = fma(e, r, q);
+    e = fma(-q, y, x);
+    return fma(e, r, q);
+}
+
 template <typename T>
 __device__ __forceinline__ T bstress(T a, T b, T visc, T ih) {  // stress (src/Burgers.jl:89-94)
     return -((a + b) * (a + b)) / 8 + visc * (b - a) * ih;
@@ -120,6 +132,9 @@ __global__ void __launch_bounds__(NT) k_burgers_persistent(const BInst *__restri
     extern __shared__ char smem_raw[];
     __shared__ int s_idx;
     __shared__ double s_red[16];
+    // the LES face fluxes of lane 0 of each wave, for lane 63 of the wave before
+    constexpr int NWV = NT / 64;
+    __shared__ T s_xs[NWV * PTL], s_xm[NWV * PTL];
     const int tid = threadIdx.x;
     constexpr int nthr = NT;  // the block size (launched with NT threads): point offsets are compile-time
     for (;;) {
@@ -151,6 +166,7 @@ __global__ void __launch_bounds__(NT) k_burgers_persistent(const BInst *__restri
         const T *const usr = usq + ((tid & PMASK) == PMASK ? 2 : 1);
         T *const gl = us + pad_at(-1), *const gr = us + pad_at(n);
         const double h = L / n, H = L / nH;
+        const double rh = 1.0 / h, rH = 1.0 / H;
         const T ih = (T)(1.0 / h), iH = (T)(1.0 / H), tv = (T)visc;
 
         T u0[PT], acc[PT];
@@ -198,7 +214,42 @@ __global__ void __launch_bounds__(NT) k_burgers_persistent(const BInst *__restri
                 // --- coarse face averages of the stage state, once per face (the
                 // commutator of coarse point J reads faces J - 1 and J)
                 for (int J = tid; J < nH; J += nthr) vbs[J] = vavg(us, n, c, J, pad_at);
-                // --- DNS right-hand side (rhs!, src/Burgers.jl:103-109)
+                __syncthreads();
+                // --- closure commutator in FP64 at coarse point Jx: tau = Pc(s) - s_H(V u)
+                auto tau_at = [&](int Jx) {
+                    const double vb = vbs[Jx], va = vbs[Jx == 0 ? nH - 1 : Jx - 1];
+                    const int m0i = c * (Jx + 1) - 1 - c / 2;
+                    // m lies in [0, n) (c / 2 - 1 <= m0i, m0i + 1 <= n - c / 2);
+                    // point m - 1 = -1 is the ghost copy of point n - 1
+                    auto sfine = [&](int m) {
+                        const double a = (double)us[pad_at(m - 1)], b = (double)us[pad_at(m)];
+                        return -((a + b) * (a + b)) / 8 + div_by(visc * (b - a), h, rh);
+                    };
+                    const double fsu = (c & 1) ? sfine(m0i) : 0.5 * (sfine(m0i) + sfine(m0i + 1));
+                    const double svu = -((va + vb) * (va + vb)) / 8 + div_by(visc * (vb - va), H, rH);
+                    return (T)(fsu - svu);
+                };
+                // --- LES face fluxes, once per coarse face J (between coarse points
+                // J - 1 and J): exact closure fs = s_H(ul) + tau, no model fm = s_H(um).
+                // The right-hand side at J is (flux(J + 1) - flux(J)) / H.
+                T fs[PTL], fm[PTL];
+#pragma unroll
+                for (int q = 0; q < PTL; q++) {
+                    const int J = tid + q * nthr;
+                    fs[q] = T(0);
+                    fm[q] = T(0);
+                    if (J < nH) {
+                        const int Jm = J == 0 ? nH - 1 : J - 1;
+                        fs[q] = bstress(ul[Jm], ul[J], tv, iH) + tau_at(J);
+                        fm[q] = bstress(um[Jm], um[J], tv, iH);
+                        if ((tid & 63) == 0) {
+                            s_xs[(tid >> 6) * PTL + q] = fs[q];
+                            s_xm[(tid >> 6) * PTL + q] = fm[q];
+                        }
+                    }
+                }
+                // --- DNS right-hand side, after the LES ones so that it is not live
+                // beside the commutator's temporaries (rhs!, src/Burgers.jl:103-109)
                 T kd[PT];
 #pragma unroll
                 for (int q = 0; q < PT; q++) {
@@ -209,34 +260,27 @@ __global__ void __launch_bounds__(NT) k_burgers_persistent(const BInst *__restri
                     }
                 }
                 __syncthreads();
-                // --- closure commutator in FP64 at coarse point Jx: tau = Pc(s) - s_H(V u)
-                auto tau_at = [&](int Jx) {
-                    const double vb = vbs[Jx], va = vbs[Jx == 0 ? nH - 1 : Jx - 1];
-                    const int m0i = c * (Jx + 1) - 1 - c / 2;
-                    // m lies in [0, n) (c / 2 - 1 <= m0i, m0i + 1 <= n - c / 2);
-                    // point m - 1 = -1 is the ghost copy of point n - 1
-                    auto sfine = [&](int m) {
-                        const double a = (double)us[pad_at(m - 1)], b = (double)us[pad_at(m)];
-                        return -((a + b) * (a + b)) / 8 + visc * (b - a) / h;
-                    };
-                    const double fsu = (c & 1) ? sfine(m0i) : 0.5 * (sfine(m0i) + sfine(m0i + 1));
-                    const double svu = -((va + vb) * (va + vb)) / 8 + visc * (vb - va) / H;
-                    return (T)(fsu - svu);
-                };
-                // --- LES right-hand sides: exact closure and no model
+                // --- LES right-hand sides: flux(J + 1) is lane + 1's, except for the
+                // last lane of a wave (lane 0 of the next wave, or thread 0 at the
+                // next q) and for J = nH - 1 (face nH is face 0: thread 0, q = 0)
                 T kl[PTL], km[PTL];
 #pragma unroll
                 for (int q = 0; q < PTL; q++) {
                     const int J = tid + q * nthr;
+                    T ns = __shfl_down(fs[q], 1, 64), nm = __shfl_down(fm[q], 1, 64);
                     if (J < nH) {
-                        const int Jm = J == 0 ? nH - 1 : J - 1, Jp = J + 1 == nH ? 0 : J + 1;
-                        const T s0 = bstress(ul[Jm], ul[J], tv, iH) + tau_at(J);
-                        const T s1 = bstress(ul[J], ul[Jp], tv, iH) + tau_at(Jp);
-                        kl[q] = (s1 - s0) * iH;
-                        km[q] = (bstress(um[J], um[Jp], tv, iH) - bstress(um[Jm], um[J], tv, iH)) * iH;
+                        if (J + 1 == nH) {
+                            ns = s_xs[0];
+                            nm = s_xm[0];
+                        } else if ((tid & 63) == 63) {
+                            const int x = tid + 1 == NT ? q + 1 : ((tid + 1) >> 6) * PTL + q;
+                            ns = s_xs[x];
+                            nm = s_xm[x];
+                        }
+                        kl[q] = (ns - fs[q]) * iH;
+                        km[q] = (nm - fm[q]) * iH;
                     }
                 }
-                __syncthreads();
                 // --- RK4 bookkeeping (rk4_timestep!, src/Burgers.jl:120-142)
                 const T bw = (st == 0 || st == 3) ? T(1) : T(2);
                 const T a_next = st == 2 ? tdt : tdt / 2;
