@@ -57,7 +57,7 @@ __device__ __forceinline__ double vavg(const T *u, int n, int c, int J, At at) {
     const int h = c / 2;
     double s = 0.0;
     if (c & 1) {
-#pragma unroll 4
+#pragma unroll 8
         for (int t = -h; t <= h; t++) {
             int m = f + t;
             m = m < 0 ? m + n : (m >= n ? m - n : m);
@@ -65,7 +65,7 @@ __device__ __forceinline__ double vavg(const T *u, int n, int c, int J, At at) {
         }
         return s / c;
     }
-#pragma unroll 4
+#pragma unroll 8
     for (int t = -h + 1; t <= h - 1; t++) {
         int m = f + t;
         m = m < 0 ? m + n : (m >= n ? m - n : m);
