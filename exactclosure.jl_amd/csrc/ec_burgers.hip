@@ -50,7 +50,8 @@ __global__ void k_burgers_ic(const BInst *__restrict__ inst, int count, int kmax
 
 // Volume-average filter of the DNS face values at coarse face J (odd c:
 // top-hat of c faces; even c: trapezoid over c+1 faces).  `at` maps a point
-// index in [0, n) to its storage index.
+// index in [0, n) to its storage index.  A window starts at f - h >= c / 2 - 1
+// >= 0 and ends at f + h < 2 n, so only its top end wraps, once.
 template <typename T, typename At>
 __device__ __forceinline__ double vavg(const T *u, int n, int c, int J, At at) {
     const int f = c * (J + 1) - 1;
@@ -60,7 +61,7 @@ __device__ __forceinline__ double vavg(const T *u, int n, int c, int J, At at) {
 #pragma unroll 8
         for (int t = -h; t <= h; t++) {
             int m = f + t;
-            m = m < 0 ? m + n : (m >= n ? m - n : m);
+            m = m >= n ? m - n : m;
             s += (double)u[at(m)];
         }
         return s / c;
@@ -68,12 +69,11 @@ __device__ __forceinline__ double vavg(const T *u, int n, int c, int J, At at) {
 #pragma unroll 8
     for (int t = -h + 1; t <= h - 1; t++) {
         int m = f + t;
-        m = m < 0 ? m + n : (m >= n ? m - n : m);
+        m = m >= n ? m - n : m;
         s += (double)u[at(m)];
     }
     int a = f - h, b = f + h;
-    a = a < 0 ? a + n : (a >= n ? a - n : a);
-    b = b < 0 ? b + n : (b >= n ? b - n : b);
+    b = b >= n ? b - n : b;
     return (s + 0.5 * ((double)u[at(a)] + (double)u[at(b)])) / c;
 }
 
